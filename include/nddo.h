@@ -1,0 +1,155 @@
+/*
+ * nddo.h -- C ABI of the MI355X NDDO (MNDO / AM1 / PM3) ground-state SCF engine.
+ *
+ * Drop-in boundary for MolDS's ElectronicStructure path
+ *   src/base/ElectronicStructure.h:23-53   (SetMolecule, DoSCF, Get* accessors)
+ *   src/base/factories/ElectronicStructureFactory.cpp:58-95 (MNDO :70, AM1 :73, PM3 :79)
+ *   src/cndo/Cndo2.cpp:585-729             (Cndo2::DoSCF, the loop this library runs)
+ * Plain pointers and sizes only; no exception crosses the boundary.  Inputs are
+ * copied at submit; outputs are copied into caller-owned buffers.  One engine
+ * drives one HIP device.
+ */
+#ifndef MOLDS_AMD_NDDO_H
+#define MOLDS_AMD_NDDO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Theory selector (MolDS TheoryType MNDO / AM1 / PM3, src/base/Enums.h). */
+enum { NDDO_MNDO = 0, NDDO_AM1 = 1, NDDO_PM3 = 2 };
+/* Precision of the SCF iterations: FP64 everywhere, or mixed (FP32 two-electron
+ * contraction + FP64 eigensolve/DIIS/energy, finished with FP64 polish iterations). */
+enum { NDDO_FP64 = 0, NDDO_MIXED = 1 };
+
+/* Status codes.  The reference throws MolDSException for the same conditions:
+ * non-convergence (Cndo2.cpp:700-704), odd electron count (Cndo2.cpp:351-357),
+ * unsupported atom (Cndo2.cpp:359-376), LAPACK info != 0 (Lapack.cpp:152-159),
+ * heap cap (MallocerFreer.cpp:62-73). */
+enum {
+    NDDO_OK = 0,
+    NDDO_NOT_CONVERGED = -1,
+    NDDO_ODD_ELECTRONS = -2,
+    NDDO_UNSUPPORTED_ELEMENT = -3,
+    NDDO_ADMISSION_DENIED = -4,
+    NDDO_EIGENSOLVER_FAILED = -5,
+    NDDO_HIP_ERROR = -6,
+    NDDO_BAD_ARG = -7,
+    NDDO_PENDING = 1
+};
+
+/* SCF knobs of MolDS's Parameters (src/base/Parameters.cpp:132-144 defaults;
+ * input directives max_iter, rms_density, damping_thresh, damping_weight,
+ * diis_num_error_vect, diis_start_error, diis_end_error, InputParser.cpp:646-734). */
+typedef struct {
+    int max_iter;              /* maxIterationsSCF, default 100 */
+    double rms_density;        /* thresholdSCF, default 1e-8 */
+    double damping_thresh;     /* default 1.0 */
+    double damping_weight;     /* default 0.8 */
+    int diis_num_error_vect;   /* default 5 (0 disables DIIS), at most 16 */
+    double diis_start_error;   /* default 1e-2 */
+    double diis_end_error;     /* default 1e-8 */
+} nddo_scf_opts;
+
+/* Fills *o with the reference defaults. */
+void nddo_scf_opts_default(nddo_scf_opts* o);
+
+/* One SCF job: the molecule (atomic numbers H, C, N, O, F; coordinates in bohr,
+ * xyz interleaved) and its SCF options.  requires_guess = 1 is DoSCF(true):
+ * iteration 0 uses the guess Fock (Mndo.cpp:434-435, 511-513); requires_guess = 0
+ * is DoSCF(false): iteration 0 builds the Fock matrix from P0 (n*n, row-major),
+ * or from a zero density when P0 is NULL. */
+typedef struct {
+    int theory;
+    int precision;
+    int natoms;
+    const int* Z;
+    const double* xyz_bohr;
+    nddo_scf_opts opts;
+    int requires_guess;
+    const double* P0;
+} nddo_job_desc;
+
+/* Results.  Scalars are always filled; array pointers are caller-owned and may
+ * be NULL.  C holds the MO coefficients as rows (row i = MO i, ascending
+ * energy, AO order s, py, pz, px per atom), each row's sign chosen so that its
+ * element sum is >= 0, as Lapack::Dsyevd leaves MolDS's fockMatrix
+ * (Lapack.cpp:117-150). */
+typedef struct {
+    int status;
+    int nao;
+    int nocc;
+    int iterations;          /* SCF iterations run, including the converged one */
+    double energy_total;     /* elecSCFEnergy: electronic + core repulsion (Hartree) */
+    double energy_elec;
+    double energy_core;      /* coreRepulsionEnergy (Hartree) */
+    double heat_of_formation;/* MNDO only (Hartree); NaN for AM1 / PM3 (Am1.cpp:261-263) */
+    double final_rms;
+    double latency_s;        /* submit -> retire (served jobs), or wall time of the solve */
+    double* eps;             /* [nao] MO energies (Hartree), ascending */
+    double* C;               /* [nao*nao] MO rows */
+    double* P;               /* [nao*nao] density */
+    double* trajectory;      /* [max_iter*4]: rms, diis_error, diis_on, damping_on per iteration */
+} nddo_result;
+
+typedef struct nddo_engine nddo_engine;
+typedef int64_t nddo_job_id;
+
+typedef struct {
+    size_t hbm_budget_bytes;  /* admission budget for resident job state; 0 = 90% of free HBM */
+    int max_batch_jobs;       /* served batch capacity; 0 = 4096 */
+    int flags;                /* reserved, 0 */
+} nddo_engine_opts;
+
+int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out);
+int nddo_engine_destroy(nddo_engine* e);
+
+/* Synchronous single solve (what a DoSCF adapter calls).  Returns the job status. */
+int nddo_scf_once(nddo_engine* e, const nddo_job_desc* job, nddo_result* res);
+/* Synchronous batch: all jobs share one launch per SCF stage.  Returns NDDO_OK
+ * when the batch ran (per-job status in res[i].status) or a negative error. */
+int nddo_scf_batch(nddo_engine* e, int njobs, const nddo_job_desc* jobs, nddo_result* res);
+
+/* Serving: continuous batching.  submit copies the job and returns at once;
+ * jobs are admitted as device memory frees up and retired at convergence. */
+int nddo_submit(nddo_engine* e, const nddo_job_desc* job, nddo_job_id* id);
+int nddo_poll(nddo_engine* e, nddo_job_id id);                      /* NDDO_PENDING or the final status */
+int nddo_wait(nddo_engine* e, nddo_job_id id, double timeout_s);    /* blocks; NDDO_PENDING on timeout */
+int nddo_get_result(nddo_engine* e, nddo_job_id id, nddo_result* res);
+int nddo_release(nddo_engine* e, nddo_job_id id);
+
+/* Per-stage device timing (no counterpart in MolDS, whose only timer is the
+ * omp_get_wtime around DoSCF, Cndo2.cpp:587, 724-728).  With profiling on, the
+ * engine records hipEvents around every stage launch on its stream and
+ * accumulates milliseconds and launch counts per stage:
+ * 0 setup, 1 fock, 2 eigen, 3 density, 4 step (RMS/DIIS/damping), 5 energy. */
+enum { NDDO_STAGE_SETUP = 0, NDDO_STAGE_FOCK, NDDO_STAGE_EIGEN, NDDO_STAGE_DENSITY, NDDO_STAGE_STEP,
+       NDDO_STAGE_ENERGY, NDDO_STAGE_COUNT };
+int nddo_engine_profile(nddo_engine* e, int enable);
+int nddo_engine_stage_times(nddo_engine* e, double* ms, long long* launches, int reset);
+
+/* Text of the last error on the calling thread. */
+const char* nddo_last_error(void);
+
+/* Stage-level hooks (no counterpart in MolDS's public API; they expose the
+ * per-stage results the parity tests compare):
+ *   integrals: dense (mu nu|lam sig) [N][N][4][4][4][4] as MolDS's twoElecsTwoAtomCores
+ *              (Mndo.cpp:113-126), overlap S [n*n] (Cndo2.cpp:4030-4146), core Hamiltonian
+ *              H = F(P=0) [n*n], core repulsion energy.  Any output may be NULL.
+ *   fock:      F [n*n] from a given P (guess = 1 for the iteration-0 guess Fock).
+ *   eigh:      eigenvalues [n] ascending and eigenvector rows [n*n] of a symmetric F.
+ *   density:   P = 2 C_occ^T C_occ from MO rows C [n*n] (first nocc rows occupied). */
+int nddo_dbg_integrals(nddo_engine* e, int theory, int natoms, const int* Z, const double* xyz_bohr,
+                       double* g_dense, double* S, double* H, double* e_core);
+int nddo_dbg_fock(nddo_engine* e, int theory, int natoms, const int* Z, const double* xyz_bohr,
+                  const double* P, int guess, double* F);
+int nddo_dbg_eigh(nddo_engine* e, int n, const double* F, double* eps, double* C, int* sweeps);
+int nddo_dbg_density(nddo_engine* e, int n, int nocc, const double* C, double* P);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOLDS_AMD_NDDO_H */

@@ -1,0 +1,777 @@
+// Host side of the NDDO SCF engine: batch assembly, the SCF driver loop over
+// batched kernel launches, result extraction and the extern "C" boundary
+// declared in include/nddo.h.
+//
+// The driver mirrors Cndo2::DoSCF (src/cndo/Cndo2.cpp:585-729): per-geometry
+// setup (overlap, two-centre integrals, core repulsion), then per iteration
+// Fock -> eigensolve -> density -> convergence test -> DIIS -> damping, and at
+// convergence the electronic energy from a Fock matrix rebuilt with the final
+// density (Cndo2.cpp:732-754, 1324-1416).  All of it runs on the device; the
+// host only reads one counter per iteration to know when every job is done.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "nddo.h"
+#include "engine_internal.h"
+#include "launch.h"
+#include "nddo_device.h"
+#include "params.h"
+
+using namespace nddo;
+
+namespace {
+thread_local std::string g_err;
+}
+
+void nddo::set_error(const std::string& s) { g_err = s; }
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            nddo::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " in " #x); \
+            return NDDO_HIP_ERROR;                                                         \
+        }                                                                                  \
+    } while (0)
+
+namespace nddo {
+
+static const int kElemZ[kNumElem] = {1, 6, 7, 8, 9};
+
+int elem_slot(int Z) {
+    for (int i = 0; i < kNumElem; ++i)
+        if (kElemZ[i] == Z) return i;
+    return -1;
+}
+
+// Overlap integrand polynomials in prolate spheroidal coordinates
+// (Pople & Beveridge, Approximate MO Theory, App. B): for orbitals
+// (n_a, l_a) on A at z = -R/2 and (n_b, l_b) on B at z = +R/2, same |m|,
+//   r_a = R/2 (xi+eta), r_b = R/2 (xi-eta), z - z_A = R/2 (1 + xi eta),
+//   z - z_B = R/2 (xi eta - 1), rho^2 = (R/2)^2 (xi^2-1)(1-eta^2),
+//   dV = (R/2)^3 (xi^2 - eta^2) dxi deta dphi,
+// so S = N_a N_b (R/2)^(na+nb+1) * ang * sum_ij c_ij A_i(p) B_j(q).
+static void poly_mul(double a[5][5], const double b[5][5]) {
+    double r[5][5] = {};
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j)
+            if (a[i][j] != 0.0)
+                for (int k = 0; i + k < 5; ++k)
+                    for (int l = 0; j + l < 5; ++l) r[i + k][j + l] += a[i][j] * b[k][l];
+    std::memcpy(a, r, sizeof(r));
+}
+
+static void overlap_table(double* tab) {
+    std::memset(tab, 0, sizeof(double) * 32 * 25);
+    const double pi = 3.14159265358979323846;
+    double plus[5][5] = {}, minus[5][5] = {}, zA[5][5] = {}, zB[5][5] = {}, perp[5][5] = {}, vol[5][5] = {};
+    plus[1][0] = 1; plus[0][1] = 1;
+    minus[1][0] = 1; minus[0][1] = -1;
+    zA[0][0] = 1; zA[1][1] = 1;
+    zB[1][1] = 1; zB[0][0] = -1;
+    perp[2][0] = 1; perp[2][2] = -1; perp[0][0] = -1; perp[0][2] = 1;
+    vol[2][0] = 1; vol[0][2] = -1;
+    for (int na = 1; na <= 2; ++na)
+        for (int la = 0; la < na; ++la)
+            for (int nb = 1; nb <= 2; ++nb)
+                for (int lb = 0; lb < nb; ++lb)
+                    for (int m = 0; m <= std::min(la, lb); ++m) {
+                        double p[5][5] = {};
+                        p[0][0] = 1.0;
+                        for (int i = 0; i < na - 1 - la; ++i) poly_mul(p, plus);
+                        for (int i = 0; i < nb - 1 - lb; ++i) poly_mul(p, minus);
+                        double ang = (la ? std::sqrt(3.0 / (4 * pi)) : 1.0 / std::sqrt(4 * pi)) *
+                                     (lb ? std::sqrt(3.0 / (4 * pi)) : 1.0 / std::sqrt(4 * pi));
+                        if (m == 0) {
+                            if (la) poly_mul(p, zA);
+                            if (lb) poly_mul(p, zB);
+                            ang *= 2 * pi;
+                        } else {
+                            poly_mul(p, perp);
+                            ang *= pi;
+                        }
+                        poly_mul(p, vol);
+                        int cid = ((((na - 1) * 2 + la) * 2 + (nb - 1)) * 2 + lb) * 2 + m;
+                        for (int i = 0; i < 5; ++i)
+                            for (int j = 0; j < 5; ++j) tab[cid * 25 + i * 5 + j] = ang * p[i][j];
+                    }
+}
+
+size_t jacobi_lds_bytes(int n, bool a_in_lds) {
+    const int nh = (n + (n & 1)) / 2;
+    size_t b = (size_t)3 * nh * 8 + (size_t)(3 * nh + (nh & 1)) * 4;
+    if (a_in_lds) b += (size_t)n * n * 8;
+    return b;
+}
+
+// ------------------------------------------------------------------ batch
+Batch::~Batch() { release(); }
+
+void Batch::release() {
+    for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
+                     (void**)&d_ints, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
+                     (void**)&d_perm, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
+                     (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_sdbg}) {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+    }
+}
+
+int validate_job(const nddo_job_desc& d, int* nao, int* nocc) {
+    if (d.natoms <= 0 || !d.Z || !d.xyz_bohr || d.theory < 0 || d.theory > 2) {
+        set_error("bad job descriptor");
+        return NDDO_BAD_ARG;
+    }
+    if (d.opts.diis_num_error_vect < 0 || d.opts.diis_num_error_vect > NDDO_MAX_DIIS || d.opts.max_iter <= 0) {
+        set_error("bad SCF options (diis_num_error_vect must be 0..16, max_iter > 0)");
+        return NDDO_BAD_ARG;
+    }
+    int n = 0, ne = 0;
+    for (int a = 0; a < d.natoms; ++a) {
+        if (elem_slot(d.Z[a]) < 0) {
+            set_error("Error in Cndo2::CheckEnableAtomType: Non available atom is contained (Z=" +
+                      std::to_string(d.Z[a]) + ")");
+            return NDDO_UNSUPPORTED_ELEMENT;
+        }
+        ElemDev e;
+        elem_params(d.Z[a], d.theory, &e);
+        n += e.nao;
+        ne += (int)e.core;
+    }
+    if (ne % 2 != 0) {
+        set_error("Error in SetMolecule: Total number of valence electrons is odd. totalNumberValenceElectrons=" +
+                  std::to_string(ne));
+        return NDDO_ODD_ELECTRONS;
+    }
+    for (int a = 0; a < 3 * d.natoms; ++a)
+        if (!std::isfinite(d.xyz_bohr[a])) {
+            set_error("non-finite coordinate");
+            return NDDO_BAD_ARG;
+        }
+    *nao = n;
+    *nocc = ne / 2;
+    return NDDO_OK;
+}
+
+size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter) {
+    const size_t n2 = (size_t)nao * nao;
+    const size_t npairs = (size_t)natoms * (natoms - 1) / 2;
+    size_t b = (ARENA_FIXED + 2 * (size_t)diis_k) * n2 * 8;   // arena
+    b += npairs * (100 * 8 + 20 * 8 + 8 + sizeof(PairDev));   // integrals (upper bound), partials, ecore
+    b += (size_t)nao * 12 + (size_t)max_iter * 32 + sizeof(JobDev) + sizeof(JobState);
+    return b;
+}
+
+int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* elem_index_base) {
+    (void)elem_index_base;
+    atoms.clear(); pairs.clear(); jobs.clear(); tiles.clear(); list_lds.clear(); list_glb.clear();
+    src = which;
+    nslots = 0; nints = 0; arena_size = 0; nvec = 0; ntraj = 0;
+    maxn_lds = 0; maxn_glb = 0; max_iter_all = 0;
+    for (int jj = 0; jj < (int)which.size(); ++jj) {
+        const nddo_job_desc& d = descs[which[jj]];
+        (void)nd;
+        JobDev jb;
+        std::memset(&jb, 0, sizeof(jb));
+        jb.natoms = d.natoms;
+        jb.theory = d.theory;
+        jb.atom0 = (int)atoms.size();
+        int n = 0, ne = 0;
+        for (int a = 0; a < d.natoms; ++a) {
+            AtomDev at;
+            at.x = d.xyz_bohr[3 * a];
+            at.y = d.xyz_bohr[3 * a + 1];
+            at.z = d.xyz_bohr[3 * a + 2];
+            at.elem = d.theory * kNumElem + elem_slot(d.Z[a]);
+            at.job = jj;
+            at.ao = n;
+            at.local = a;
+            ElemDev e;
+            elem_params(d.Z[a], d.theory, &e);
+            n += e.nao;
+            ne += (int)e.core;
+            atoms.push_back(at);
+        }
+        jb.nao = n;
+        jb.nocc = ne / 2;
+        jb.npairs = d.natoms * (d.natoms - 1) / 2;
+        jb.pair0 = nslots;
+        for (int a = 0; a < d.natoms; ++a)
+            for (int b = a + 1; b < d.natoms; ++b) {
+                PairDev pr;
+                pr.a = jb.atom0 + a;
+                pr.b = jb.atom0 + b;
+                pr.job = jj;
+                pr.slot = (int)(nslots + (long long)a * d.natoms - (long long)a * (a + 1) / 2 + (b - a - 1));
+                pr.ioff = nints;
+                int na = atoms[pr.a].elem % kNumElem == 0 ? 1 : 10;
+                int nb = atoms[pr.b].elem % kNumElem == 0 ? 1 : 10;
+                nints += (long long)na * nb;
+                pairs.push_back(pr);
+            }
+        nslots += jb.npairs;
+        jb.diis_k = d.opts.diis_num_error_vect;
+        jb.arena = arena_size;
+        arena_size += (long long)(ARENA_FIXED + 2 * jb.diis_k) * n * n;
+        jb.vec0 = nvec;
+        jb.ivec0 = nvec;
+        nvec += n;
+        jb.tile0 = (int)tiles.size();
+        jb.tiles_per_dim = (n + 63) / 64;
+        for (int i0 = 0; i0 < n; i0 += 64)
+            for (int j0 = 0; j0 < n; j0 += 64) tiles.push_back({jj, i0, j0, 0});
+        jb.ntiles = (int)tiles.size() - jb.tile0;
+        jb.max_iter = d.opts.max_iter;
+        jb.requires_guess = d.requires_guess ? 1 : 0;
+        jb.precision = d.precision;
+        jb.rms_thresh = d.opts.rms_density;
+        jb.damp_thresh = d.opts.damping_thresh;
+        jb.damp_weight = d.opts.damping_weight;
+        jb.diis_start = d.opts.diis_start_error;
+        jb.diis_end = d.opts.diis_end_error;
+        jb.traj0 = ntraj;
+        ntraj += 4LL * d.opts.max_iter;
+        max_iter_all = std::max(max_iter_all, d.opts.max_iter);
+        jobs.push_back(jb);
+    }
+    return NDDO_OK;
+}
+
+int Batch::classify(size_t lds_max) {
+    list_lds.clear();
+    list_glb.clear();
+    maxn_lds = maxn_glb = 0;
+    std::vector<int> order(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) order[i] = (int)i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return jobs[a].nao > jobs[b].nao; });
+    for (int j : order) {
+        int n = jobs[j].nao;
+        // the Jacobi kernel also holds 8 KB of static LDS (its reduction buffer)
+        if (lds_max > 9216 && jacobi_lds_bytes(n, true) <= lds_max - 9216) {
+            list_lds.push_back(j);
+            maxn_lds = std::max(maxn_lds, n);
+        } else {
+            list_glb.push_back(j);
+            maxn_glb = std::max(maxn_glb, n);
+        }
+    }
+    return NDDO_OK;
+}
+
+int Batch::allocate(bool want_sdbg) {
+    release();
+    const size_t nj = std::max<size_t>(jobs.size(), 1);
+    HIPCHK(hipMalloc(&d_atoms, std::max<size_t>(atoms.size(), 1) * sizeof(AtomDev)));
+    HIPCHK(hipMalloc(&d_pairs, std::max<size_t>(pairs.size(), 1) * sizeof(PairDev)));
+    HIPCHK(hipMalloc(&d_jobs, nj * sizeof(JobDev)));
+    HIPCHK(hipMalloc(&d_state, nj * sizeof(JobState)));
+    HIPCHK(hipMalloc(&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(TileDev)));
+    HIPCHK(hipMalloc(&d_ints, std::max<long long>(nints, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&d_arena, std::max<long long>(arena_size, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&d_part, std::max<long long>(nslots, 1) * 20 * sizeof(double)));
+    HIPCHK(hipMalloc(&d_ecore, std::max<long long>(nslots, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&d_eps, std::max<long long>(nvec, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&d_perm, std::max<long long>(nvec, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&d_rmspart, std::max<size_t>(tiles.size(), 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&d_traj, std::max<long long>(ntraj, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&d_remaining, sizeof(int)));
+    HIPCHK(hipMalloc(&d_list_lds, std::max<size_t>(list_lds.size(), 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&d_list_glb, std::max<size_t>(list_glb.size(), 1) * sizeof(int)));
+    if (want_sdbg) HIPCHK(hipMalloc(&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double)));
+    return NDDO_OK;
+}
+
+int Batch::upload(hipStream_t st, const nddo_job_desc* descs) {
+    HIPCHK(hipMemcpyAsync(d_atoms, atoms.data(), atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, st));
+    if (!pairs.empty())
+        HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice, st));
+    if (!list_lds.empty())
+        HIPCHK(hipMemcpyAsync(d_list_lds, list_lds.data(), list_lds.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (!list_glb.empty())
+        HIPCHK(hipMemcpyAsync(d_list_glb, list_glb.data(), list_glb.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(d_arena, 0, std::max<long long>(arena_size, 1) * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(d_traj, 0, std::max<long long>(ntraj, 1) * sizeof(double), st));
+    if (d_sdbg) HIPCHK(hipMemsetAsync(d_sdbg, 0, std::max<long long>(arena_size, 1) * sizeof(double), st));
+    std::vector<JobState> init(jobs.size());
+    std::memset(init.data(), 0, init.size() * sizeof(JobState));
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        init[j].phase = PH_ACTIVE;
+        const nddo_job_desc& d = descs[src[j]];
+        if (!d.requires_guess && d.P0) {
+            const long long n = jobs[j].nao;
+            HIPCHK(hipMemcpyAsync(d_arena + jobs[j].arena + (long long)AR_P0 * n * n, d.P0, n * n * sizeof(double),
+                                  hipMemcpyHostToDevice, st));
+        }
+    }
+    HIPCHK(hipMemcpyAsync(d_state, init.data(), init.size() * sizeof(JobState), hipMemcpyHostToDevice, st));
+    int rem = (int)jobs.size();
+    HIPCHK(hipMemcpyAsync(d_remaining, &rem, sizeof(int), hipMemcpyHostToDevice, st));
+    return NDDO_OK;
+}
+
+void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
+    launch_setup(st, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, (int)jobs.size(),
+                 d_ints, d_arena, d_part, d_ecore, d_state, d_sdbg);
+}
+
+void Batch::launch_iteration(Engine* e, hipStream_t st) {
+    launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints,
+                d_arena, d_part);
+    launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
+                 jacobi_lds_bytes(maxn_lds, true), true);
+    launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
+                 jacobi_lds_bytes(maxn_glb, false), false);
+    launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
+    launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
+    launch_fock(st, 1, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints,
+                d_arena, d_part);
+    launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
+}
+
+int Batch::run_scf(Engine* e, hipStream_t st) {
+    const bool prof = e->profile;
+    if (prof) HIPCHK(hipEventRecord(e->ev[0], st));
+    launch_setup_stage(e, st);
+    HIPCHK(hipGetLastError());
+    if (prof) {
+        HIPCHK(hipEventRecord(e->ev[1], st));
+        HIPCHK(hipEventSynchronize(e->ev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+        e->stage_ms[NDDO_STAGE_SETUP] += ms;
+        e->stage_launches[NDDO_STAGE_SETUP] += 1;
+    }
+    int* h_rem = e->h_remaining;
+    for (int it = 0; it <= max_iter_all + 1; ++it) {
+        if (!prof) {
+            launch_iteration(e, st);
+        } else {
+            // the same launches as launch_iteration, with an event between stages
+            HIPCHK(hipEventRecord(e->ev[0], st));
+            launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
+                        d_ints, d_arena, d_part);
+            HIPCHK(hipEventRecord(e->ev[1], st));
+            launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
+                         jacobi_lds_bytes(maxn_lds, true), true);
+            launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
+                         jacobi_lds_bytes(maxn_glb, false), false);
+            HIPCHK(hipEventRecord(e->ev[2], st));
+            launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
+            HIPCHK(hipEventRecord(e->ev[3], st));
+            launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
+            HIPCHK(hipEventRecord(e->ev[4], st));
+            launch_fock(st, 1, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
+                        d_ints, d_arena, d_part);
+            launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
+            HIPCHK(hipEventRecord(e->ev[5], st));
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h_rem, d_remaining, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (prof) {
+            const int map[5] = {NDDO_STAGE_FOCK, NDDO_STAGE_EIGEN, NDDO_STAGE_DENSITY, NDDO_STAGE_STEP,
+                                NDDO_STAGE_ENERGY};
+            for (int k = 0; k < 5; ++k) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e->ev[k], e->ev[k + 1]));
+                e->stage_ms[map[k]] += ms;
+                e->stage_launches[map[k]] += 1;
+            }
+        }
+        if (*h_rem <= 0) break;
+    }
+    return NDDO_OK;
+}
+
+// Copies job jj's results into res (status already set by the caller on failure).
+int Batch::fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* res, double hof_offset) {
+    const JobDev& jb = jobs[jj];
+    JobState s;
+    HIPCHK(hipMemcpyAsync(&s, d_state + jj, sizeof(JobState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int n = jb.nao;
+    res->nao = n;
+    res->nocc = jb.nocc;
+    res->status = s.phase == PH_DONE ? s.status : NDDO_NOT_CONVERGED;
+    if (s.sweeps < 0 && res->status == NDDO_OK) res->status = NDDO_EIGENSOLVER_FAILED;
+    res->iterations = s.status == 0 && s.phase == PH_DONE ? s.iter + 1 : s.iter;
+    res->energy_core = s.e_core;
+    res->energy_elec = s.e_el;
+    res->energy_total = s.e_el + s.e_core;
+    res->heat_of_formation = d.theory == NDDO_MNDO ? res->energy_total + hof_offset : NAN;
+    res->final_rms = s.rms;
+    if (res->status != NDDO_OK) {
+        res->energy_elec = NAN;
+        res->energy_total = NAN;
+        res->heat_of_formation = NAN;
+    }
+    std::vector<double> eps(n);
+    std::vector<int> perm(n);
+    HIPCHK(hipMemcpyAsync(eps.data(), d_eps + jb.vec0, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(perm.data(), d_perm + jb.ivec0, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    const double* base = d_arena + jb.arena;
+    const long long nn = (long long)n * n;
+    std::vector<double> VT, P;
+    if (res->C) {
+        VT.resize(nn);
+        HIPCHK(hipMemcpyAsync(VT.data(), base + (long long)AR_C * nn, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    if (res->P) {
+        const int pk = AR_P0 + ((s.iter + 1) & 1);
+        HIPCHK(hipMemcpyAsync(res->P, base + (long long)pk * nn, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    if (res->trajectory)
+        HIPCHK(hipMemcpyAsync(res->trajectory, d_traj + jb.traj0, 4LL * jb.max_iter * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (res->eps) std::memcpy(res->eps, eps.data(), n * sizeof(double));
+    if (res->C) {
+        for (int r = 0; r < n; ++r) {
+            const double* src = &VT[(long long)perm[r] * n];
+            double sum = 0.0;
+            for (int i = 0; i < n; ++i) sum += src[i];
+            const double sg = sum < 0.0 ? -1.0 : 1.0;   // Lapack.cpp:139-150
+            for (int i = 0; i < n; ++i) res->C[(long long)r * n + i] = sg * src[i];
+        }
+    }
+    return NDDO_OK;
+}
+
+}  // namespace nddo
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char* nddo_last_error(void) { return g_err.c_str(); }
+
+void nddo_scf_opts_default(nddo_scf_opts* o) {
+    // src/base/Parameters.cpp:132-144
+    o->max_iter = 100;
+    o->rms_density = 1.0e-8;
+    o->damping_thresh = 1.0;
+    o->damping_weight = 0.8;
+    o->diis_num_error_vect = 5;
+    o->diis_start_error = 1.0e-2;
+    o->diis_end_error = 1.0e-8;
+}
+
+int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out) {
+    if (!out) { set_error("null engine pointer"); return NDDO_BAD_ARG; }
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (hip_device < 0 || hip_device >= ndev) {
+        set_error("no such HIP device");
+        return NDDO_BAD_ARG;
+    }
+    HIPCHK(hipSetDevice(hip_device));
+    Engine* e = new Engine();
+    e->device = hip_device;
+    if (opts) e->opts = *opts;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, hip_device));
+    e->lds_max = prop.sharedMemPerBlock;
+    e->num_cus = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    std::vector<ElemDev> el(3 * kNumElem);
+    for (int t = 0; t < 3; ++t)
+        for (int i = 0; i < kNumElem; ++i) elem_params(kElemZ[i], t, &el[t * kNumElem + i]);
+    HIPCHK(hipMalloc(&e->d_elems, el.size() * sizeof(ElemDev)));
+    HIPCHK(hipMemcpy(e->d_elems, el.data(), el.size() * sizeof(ElemDev), hipMemcpyHostToDevice));
+    double tab[32 * 25];
+    overlap_table(tab);
+    if (upload_overlap_table(tab) != 0) {
+        set_error("overlap table upload failed");
+        return NDDO_HIP_ERROR;
+    }
+    HIPCHK(hipHostMalloc((void**)&e->h_remaining, sizeof(int), hipHostMallocDefault));
+    for (int i = 0; i < 8; ++i) HIPCHK(hipEventCreate(&e->ev[i]));
+    size_t freeb = 0, totb = 0;
+    HIPCHK(hipMemGetInfo(&freeb, &totb));
+    e->hbm_budget = e->opts.hbm_budget_bytes ? e->opts.hbm_budget_bytes : (size_t)(0.9 * freeb);
+    e->start_server();
+    *out = reinterpret_cast<nddo_engine*>(e);
+    return NDDO_OK;
+}
+
+int nddo_engine_destroy(nddo_engine* eh) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    e->stop_server();
+    hipSetDevice(e->device);
+    if (e->d_elems) hipFree(e->d_elems);
+    if (e->h_remaining) hipHostFree(e->h_remaining);
+    for (int i = 0; i < 8; ++i)
+        if (e->ev[i]) hipEventDestroy(e->ev[i]);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return NDDO_OK;
+}
+
+int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_result* res) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || njobs < 0 || (njobs > 0 && (!descs || !res))) { set_error("bad arguments"); return NDDO_BAD_ARG; }
+    auto t0 = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<int> ok;
+    for (int i = 0; i < njobs; ++i) {
+        int n = 0, no = 0;
+        int s = validate_job(descs[i], &n, &no);
+        res[i].status = s;
+        res[i].nao = n;
+        res[i].nocc = no;
+        if (s == NDDO_OK) ok.push_back(i);
+    }
+    if (ok.empty()) return NDDO_OK;
+    Batch b;
+    b.build(descs, njobs, ok, nullptr);
+    b.classify(e->lds_max);
+    int rc = b.allocate(false);
+    if (rc) return rc;
+    rc = b.upload(e->stream, descs);
+    if (rc) return rc;
+    rc = b.run_scf(e, e->stream);
+    if (rc) return rc;
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int jj = 0; jj < (int)ok.size(); ++jj) {
+        const nddo_job_desc& d = descs[ok[jj]];
+        double hof = 0.0;
+        for (int a = 0; a < d.natoms; ++a) hof += mndo_atom_hof_offset(d.Z[a]);
+        rc = b.fetch(e->stream, jj, d, &res[ok[jj]], hof);
+        if (rc) return rc;
+        res[ok[jj]].latency_s = wall;
+    }
+    return NDDO_OK;
+}
+
+int nddo_engine_profile(nddo_engine* eh, int enable) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    e->profile = enable != 0;
+    return NDDO_OK;
+}
+
+int nddo_engine_stage_times(nddo_engine* eh, double* ms, long long* launches, int reset) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    for (int i = 0; i < NDDO_STAGE_COUNT; ++i) {
+        if (ms) ms[i] = e->stage_ms[i];
+        if (launches) launches[i] = e->stage_launches[i];
+        if (reset) { e->stage_ms[i] = 0.0; e->stage_launches[i] = 0; }
+    }
+    return NDDO_OK;
+}
+
+int nddo_scf_once(nddo_engine* e, const nddo_job_desc* job, nddo_result* res) {
+    int rc = nddo_scf_batch(e, 1, job, res);
+    if (rc) return rc;
+    if (res->status == NDDO_NOT_CONVERGED)
+        set_error("Error in Cndo2::DoSCF: SCF did not met convergence criterion. maxIterationsSCF=" +
+                  std::to_string(job->opts.max_iter));
+    return res->status;
+}
+
+// ------------------------------------------------------------------ stage hooks
+static int one_job(Engine* e, Batch& b, nddo_job_desc& d, int theory, int natoms, const int* Z, const double* xyz,
+                   bool sdbg) {
+    d.theory = theory;
+    d.precision = NDDO_FP64;
+    d.natoms = natoms;
+    d.Z = Z;
+    d.xyz_bohr = xyz;
+    nddo_scf_opts_default(&d.opts);
+    d.opts.diis_num_error_vect = 0;
+    d.requires_guess = 1;
+    d.P0 = nullptr;
+    int n = 0, no = 0;
+    int s = validate_job(d, &n, &no);
+    if (s) return s;
+    std::vector<int> which{0};
+    b.build(&d, 1, which, nullptr);
+    b.classify(e->lds_max);
+    int rc = b.allocate(sdbg);
+    if (rc) return rc;
+    return b.upload(e->stream, &d);
+}
+
+int nddo_dbg_integrals(nddo_engine* eh, int theory, int natoms, const int* Z, const double* xyz_bohr,
+                       double* g_dense, double* S, double* H, double* e_core) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    Batch b;
+    nddo_job_desc d;
+    int rc = one_job(e, b, d, theory, natoms, Z, xyz_bohr, S != nullptr);
+    if (rc) return rc;
+    b.launch_setup_stage(e, e->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const int n = b.jobs[0].nao;
+    const long long nn = (long long)n * n;
+    if (H) HIPCHK(hipMemcpy(H, b.d_arena + (long long)AR_H * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (S) {
+        HIPCHK(hipMemcpy(S, b.d_sdbg, nn * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) S[(long long)i * n + i] = 1.0;
+    }
+    if (e_core) {
+        JobState s;
+        HIPCHK(hipMemcpy(&s, b.d_state, sizeof(JobState), hipMemcpyDeviceToHost));
+        *e_core = s.e_core;
+    }
+    if (g_dense) {
+        std::vector<double> ints(std::max<long long>(b.nints, 1));
+        HIPCHK(hipMemcpy(ints.data(), b.d_ints, b.nints * sizeof(double), hipMemcpyDeviceToHost));
+        const int N = natoms;
+        std::memset(g_dense, 0, sizeof(double) * (size_t)N * N * 256);
+        static const int pk[10][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 1}, {1, 2}, {1, 3}, {2, 2}, {2, 3}, {3, 3}};
+        for (const PairDev& pr : b.pairs) {
+            const int a = pr.a, bb = pr.b;
+            const int nap = b.atoms[a].elem % kNumElem == 0 ? 1 : 10;
+            const int nbp = b.atoms[bb].elem % kNumElem == 0 ? 1 : 10;
+            for (int i = 0; i < nap; ++i)
+                for (int j = 0; j < nbp; ++j) {
+                    const double v = ints[pr.ioff + i * nbp + j];
+                    const int mu = pk[i][0], nu = pk[i][1], la = pk[j][0], sg = pk[j][1];
+                    auto G = [&](int A, int B, int p, int q, int r, int s) -> double& {
+                        return g_dense[((((((size_t)A * N + B) * 4 + p) * 4 + q) * 4 + r) * 4 + s)];
+                    };
+                    G(a, bb, mu, nu, la, sg) = v; G(a, bb, nu, mu, la, sg) = v;
+                    G(a, bb, mu, nu, sg, la) = v; G(a, bb, nu, mu, sg, la) = v;
+                    G(bb, a, la, sg, mu, nu) = v; G(bb, a, sg, la, mu, nu) = v;
+                    G(bb, a, la, sg, nu, mu) = v; G(bb, a, sg, la, nu, mu) = v;
+                }
+        }
+    }
+    return NDDO_OK;
+}
+
+int nddo_dbg_fock(nddo_engine* eh, int theory, int natoms, const int* Z, const double* xyz_bohr, const double* P,
+                  int guess, double* F) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !F) return NDDO_BAD_ARG;
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    Batch b;
+    nddo_job_desc d;
+    int rc = one_job(e, b, d, theory, natoms, Z, xyz_bohr, false);
+    if (rc) return rc;
+    const int n = b.jobs[0].nao;
+    const long long nn = (long long)n * n;
+    if (!guess) {
+        b.jobs[0].requires_guess = 0;
+        HIPCHK(hipMemcpy(b.d_jobs, b.jobs.data(), sizeof(JobDev), hipMemcpyHostToDevice));
+    }
+    if (P) HIPCHK(hipMemcpy(b.d_arena + (long long)AR_P0 * nn, P, nn * sizeof(double), hipMemcpyHostToDevice));
+    b.launch_setup_stage(e, e->stream);
+    launch_fock(e->stream, 0, b.d_pairs, (int)b.pairs.size(), b.d_atoms, (int)b.atoms.size(), e->d_elems, b.d_jobs,
+                b.d_state, b.d_ints, b.d_arena, b.d_part);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(F, b.d_arena + (long long)AR_F * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
+    return NDDO_OK;
+}
+
+// A pseudo job of n AOs and no atoms, used by the eigensolver and density hooks.
+static int matrix_job(Engine* e, Batch& b, int n, int nocc) {
+    b.atoms.clear(); b.pairs.clear(); b.jobs.clear(); b.tiles.clear();
+    JobDev jb;
+    std::memset(&jb, 0, sizeof(jb));
+    jb.nao = n;
+    jb.nocc = nocc;
+    jb.max_iter = 1;
+    jb.rms_thresh = 0.0;
+    jb.tile0 = 0;
+    for (int i0 = 0; i0 < n; i0 += 64)
+        for (int j0 = 0; j0 < n; j0 += 64) b.tiles.push_back({0, i0, j0, 0});
+    jb.ntiles = (int)b.tiles.size();
+    b.jobs.push_back(jb);
+    b.nslots = 0; b.nints = 0; b.nvec = n; b.ntraj = 4;
+    b.arena_size = (long long)ARENA_FIXED * n * n;
+    b.src = {0};
+    b.max_iter_all = 1;
+    b.classify(e->lds_max);
+    int rc = b.allocate(false);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(b.d_jobs, b.jobs.data(), sizeof(JobDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.d_tiles, b.tiles.data(), b.tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice));
+    if (!b.list_lds.empty()) HIPCHK(hipMemcpy(b.d_list_lds, b.list_lds.data(), sizeof(int), hipMemcpyHostToDevice));
+    if (!b.list_glb.empty()) HIPCHK(hipMemcpy(b.d_list_glb, b.list_glb.data(), sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(b.d_arena, 0, b.arena_size * sizeof(double)));
+    JobState s;
+    std::memset(&s, 0, sizeof(s));
+    s.phase = PH_ACTIVE;
+    HIPCHK(hipMemcpy(b.d_state, &s, sizeof(s), hipMemcpyHostToDevice));
+    return NDDO_OK;
+}
+
+int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* C, int* sweeps) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || n <= 0 || !F) return NDDO_BAD_ARG;
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    Batch b;
+    int rc = matrix_job(e, b, n, 0);
+    if (rc) return rc;
+    const long long nn = (long long)n * n;
+    HIPCHK(hipMemcpy(b.d_arena + (long long)AR_F * nn, F, nn * sizeof(double), hipMemcpyHostToDevice));
+    launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_lds, (int)b.list_lds.size(), b.maxn_lds, b.d_arena, b.d_eps,
+                 b.d_perm, jacobi_lds_bytes(b.maxn_lds, true), true);
+    launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_glb, (int)b.list_glb.size(), b.maxn_glb, b.d_arena, b.d_eps,
+                 b.d_perm, jacobi_lds_bytes(b.maxn_glb, false), false);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<int> perm(n);
+    std::vector<double> VT(nn);
+    if (eps) HIPCHK(hipMemcpy(eps, b.d_eps, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(perm.data(), b.d_perm, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(VT.data(), b.d_arena + (long long)AR_C * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
+    JobState s;
+    HIPCHK(hipMemcpy(&s, b.d_state, sizeof(s), hipMemcpyDeviceToHost));
+    if (sweeps) *sweeps = s.sweeps;
+    if (C) {
+        for (int r = 0; r < n; ++r) {
+            const double* src = &VT[(long long)perm[r] * n];
+            double sum = 0.0;
+            for (int i = 0; i < n; ++i) sum += src[i];
+            const double sg = sum < 0.0 ? -1.0 : 1.0;
+            for (int i = 0; i < n; ++i) C[(long long)r * n + i] = sg * src[i];
+        }
+    }
+    if (s.sweeps < 0) { set_error("Jacobi did not converge"); return NDDO_EIGENSOLVER_FAILED; }
+    return NDDO_OK;
+}
+
+int nddo_dbg_density(nddo_engine* eh, int n, int nocc, const double* C, double* P) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || n <= 0 || nocc < 0 || nocc > n || !C || !P) return NDDO_BAD_ARG;
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    Batch b;
+    int rc = matrix_job(e, b, n, nocc);
+    if (rc) return rc;
+    const long long nn = (long long)n * n;
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    HIPCHK(hipMemcpy(b.d_perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.d_arena + (long long)AR_C * nn, C, nn * sizeof(double), hipMemcpyHostToDevice));
+    launch_density(e->stream, b.d_tiles, (int)b.tiles.size(), b.d_jobs, b.d_state, b.d_arena, b.d_perm, b.d_rmspart);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(P, b.d_arena + (long long)AR_P1 * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
+    return NDDO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// Engine and batch internals shared by the engine and the serving loop.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "nddo.h"
+#include "nddo_device.h"
+
+namespace nddo {
+
+constexpr int kNumElem = 5;   // H, C, N, O, F
+
+void set_error(const std::string& s);
+int elem_slot(int Z);
+int validate_job(const nddo_job_desc& d, int* nao, int* nocc);
+size_t jacobi_lds_bytes(int n, bool a_in_lds);
+// Device bytes a job keeps resident (admission control; MolDS's MallocerFreer heap
+// cap, src/base/MallocerFreer.cpp:62-73, plays the same role on the host).
+size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter);
+
+struct Engine;
+
+// A set of jobs sharing one launch per SCF stage.
+struct Batch {
+    std::vector<AtomDev> atoms;
+    std::vector<PairDev> pairs;
+    std::vector<JobDev> jobs;
+    std::vector<TileDev> tiles;
+    std::vector<int> list_lds, list_glb, src;
+    int maxn_lds = 0, maxn_glb = 0, max_iter_all = 0;
+    long long nslots = 0, nints = 0, arena_size = 0, nvec = 0, ntraj = 0;
+
+    AtomDev* d_atoms = nullptr;
+    PairDev* d_pairs = nullptr;
+    JobDev* d_jobs = nullptr;
+    JobState* d_state = nullptr;
+    TileDev* d_tiles = nullptr;
+    double* d_ints = nullptr;
+    double* d_arena = nullptr;
+    double* d_part = nullptr;
+    double* d_ecore = nullptr;
+    double* d_eps = nullptr;
+    int* d_perm = nullptr;
+    double* d_rmspart = nullptr;
+    double* d_traj = nullptr;
+    int* d_remaining = nullptr;
+    int* d_list_lds = nullptr;
+    int* d_list_glb = nullptr;
+    double* d_sdbg = nullptr;
+
+    ~Batch();
+    void release();
+    int build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* unused);
+    int classify(size_t lds_max);
+    int allocate(bool want_sdbg);
+    int upload(hipStream_t st, const nddo_job_desc* descs);
+    void launch_setup_stage(Engine* e, hipStream_t st);
+    void launch_iteration(Engine* e, hipStream_t st);
+    int run_scf(Engine* e, hipStream_t st);
+    int fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* res, double hof_offset);
+};
+
+struct ServedJob;
+
+struct Engine {
+    int device = 0;
+    nddo_engine_opts opts{0, 0, 0};
+    hipStream_t stream = nullptr;
+    ElemDev* d_elems = nullptr;
+    size_t lds_max = 65536;
+    int num_cus = 0;
+    size_t hbm_budget = 0;
+    int* h_remaining = nullptr;
+    std::mutex run_mu;   // one synchronous batch at a time
+    bool profile = false;
+    double stage_ms[NDDO_STAGE_COUNT] = {};
+    long long stage_launches[NDDO_STAGE_COUNT] = {};
+    hipEvent_t ev[8] = {};
+
+    // serving state (server.cpp)
+    std::mutex smu;
+    std::condition_variable scv;
+    std::thread server;
+    bool stopping = false;
+    long long next_id = 1;
+    std::map<long long, ServedJob*> served;
+    std::deque<ServedJob*> queue;
+
+    void start_server();
+    void stop_server();
+};
+
+}  // namespace nddo

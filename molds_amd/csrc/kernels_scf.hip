@@ -1,0 +1,626 @@
+// SCF-loop kernels: Fock build, Jacobi eigensolver, density, convergence/DIIS
+// step and the electronic energy.  Every kernel works on a whole batch of jobs
+// and skips jobs whose phase does not match, so one launch per stage serves
+// molecules of uneven size at different points of their SCF.
+//
+// Reference path (MolDS): Cndo2::DoSCF loop (src/cndo/Cndo2.cpp:624-705),
+// CalcFockMatrix (:1526-1636) with Mndo::GetFockDiagElement /
+// GetFockOffDiagElement (src/mndo/Mndo.cpp:422-580), Lapack::Dsyevd
+// (src/wrappers/Lapack.cpp:102-161), CalcOrbitalElectronPopulation
+// (Cndo2.cpp:1702-1731), SatisfyConvergenceCriterion (:1472-1517),
+// DoDIIS (:920-1028), DoDamp (:1030-1061), CalcElecSCFEnergy (:1324-1416).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "launch.h"
+#include "nddo_device.h"
+
+namespace nddo {
+
+__device__ inline int pk_index(int a, int b) {
+    if (a > b) { int t = a; a = b; b = t; }
+    return a == 0 ? b : (a == 1 ? 3 + b : (a == 2 ? 5 + b : 9));
+}
+
+__device__ inline double* mat(double* arena, const JobDev& jb, int kind) {
+    return arena + jb.arena + (long long)kind * jb.nao * jb.nao;
+}
+
+__device__ inline long long tri_slot(int a, int b, int N) {   // a < b, job-local
+    return (long long)a * N - (long long)a * (a + 1) / 2 + (b - a - 1);
+}
+
+// One-centre Coulomb g and exchange h integrals (Mndo.cpp:583-639).
+__device__ inline double g1c(const ElemDev& E, int a, int b) {
+    if (a == 0 && b == 0) return E.Gss;
+    if (a == 0 || b == 0) return E.Gsp;
+    return a == b ? E.Gpp : E.Gpp2;
+}
+__device__ inline double h1c(const ElemDev& E, int a, int b) {
+    if (a == b) return g1c(E, a, b);
+    if (a == 0 || b == 0) return E.Hsp;
+    return E.Hpp;
+}
+
+// ------------------------------------------------------------------ Fock
+// Per pair (A<B): the off-diagonal block F_AB = beta_bar S - 1/2 sum P_AB (mu s|lam l)
+// written to both triangles, and the Coulomb partials of the pair to part[slot]:
+//   J_A[mu nu] = sum_{lam sig in B} P_lam,sig (mu nu|lam sig)   (slots 0..9)
+//   J_B[lam sig] = sum_{mu nu in A} P_mu,nu (mu nu|lam sig)      (slots 10..19)
+__global__ void __launch_bounds__(256) k_fock_pairs(
+    int mode, const PairDev* __restrict__ pairs, int npairs, const AtomDev* __restrict__ atoms,
+    const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs, const JobState* __restrict__ state,
+    const double* __restrict__ ints, double* __restrict__ arena, double* __restrict__ part) {
+    __shared__ double sI[4][100], sPab[4][16], sWa[4][10], sWb[4][10];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pidx = blockIdx.x * 4 + wave;
+    bool act = pidx < npairs;
+    PairDev pr;
+    JobDev jb;
+    int phase = -1, iter = 0;
+    if (act) {
+        pr = pairs[pidx];
+        jb = jobs[pr.job];
+        phase = state[pr.job].phase;
+        iter = state[pr.job].iter;
+        act = mode == 0 ? phase == PH_ACTIVE : phase == PH_ENERGY;
+    }
+    bool guess = false;
+    int nA = 0, nB = 0, nap = 0, nbp = 0, aoA = 0, aoB = 0, n = 0;
+    const double* P = nullptr;
+    if (act) {
+        const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
+        nA = elems[aA.elem].nao;
+        nB = elems[aB.elem].nao;
+        nap = nA == 4 ? 10 : 1;
+        nbp = nB == 4 ? 10 : 1;
+        aoA = aA.ao;
+        aoB = aB.ao;
+        n = jb.nao;
+        guess = mode == 0 && iter == 0 && jb.requires_guess;
+        int pk = mode == 0 ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1));
+        P = mat(arena, jb, pk);
+        if (!guess) {
+            for (int e = lane; e < nap * nbp; e += 64) sI[wave][e] = ints[pr.ioff + e];
+            if (lane < 16) {
+                int i = lane >> 2, j = lane & 3;
+                sPab[wave][lane] = (i < nA && j < nB) ? P[(long long)(aoA + i) * n + aoB + j] : 0.0;
+            } else if (lane < 26) {
+                int q = lane - 16;
+                if (q < nap) {
+                    int mu = q == 0 ? 0 : (q < 4 ? 0 : (q < 7 ? 1 : (q < 9 ? 2 : 3)));
+                    int nu = q < 4 ? q : (q < 7 ? q - 3 : (q < 9 ? q - 5 : 3));
+                    sWa[wave][q] = (mu == nu ? 1.0 : 2.0) * P[(long long)(aoA + mu) * n + aoA + nu];
+                }
+            } else if (lane >= 32 && lane < 42) {
+                int q = lane - 32;
+                if (q < nbp) {
+                    int mu = q == 0 ? 0 : (q < 4 ? 0 : (q < 7 ? 1 : (q < 9 ? 2 : 3)));
+                    int nu = q < 4 ? q : (q < 7 ? q - 3 : (q < 9 ? q - 5 : 3));
+                    sWb[wave][q] = (mu == nu ? 1.0 : 2.0) * P[(long long)(aoB + mu) * n + aoB + nu];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+    double* F = mat(arena, jb, AR_F);
+    const double* H = mat(arena, jb, AR_H);
+    if (lane < 10) {
+        if (!guess && lane < nap) {
+            double s = 0.0;
+            for (int j = 0; j < nbp; ++j) s += sI[wave][lane * nbp + j] * sWb[wave][j];
+            part[(long long)pr.slot * 20 + lane] = s;
+        }
+    } else if (lane >= 16 && lane < 26) {
+        int j = lane - 16;
+        if (!guess && j < nbp) {
+            double s = 0.0;
+            for (int i = 0; i < nap; ++i) s += sI[wave][i * nbp + j] * sWa[wave][i];
+            part[(long long)pr.slot * 20 + 10 + j] = s;
+        }
+    } else if (lane >= 32 && lane < 48) {
+        int mu = (lane - 32) >> 2, la = (lane - 32) & 3;
+        if (mu < nA && la < nB) {
+            long long ij = (long long)(aoA + mu) * n + aoB + la;
+            double v = H[ij];
+            if (!guess) {
+                double k = 0.0;
+                for (int nu = 0; nu < nA; ++nu)
+                    for (int sg = 0; sg < nB; ++sg)
+                        k += sPab[wave][nu * 4 + sg] * sI[wave][pk_index(mu, nu) * nbp + pk_index(la, sg)];
+                v -= 0.5 * k;
+            }
+            F[ij] = v;
+            F[(long long)(aoB + la) * n + aoA + mu] = v;
+        }
+    }
+}
+
+// Per atom: F_AA = H_AA + one-centre G(P_AA) + sum_B J partials (fixed-order
+// reduction over B); on the guess iteration F_AA = diag(U) (Mndo.cpp:422-488, 509-514).
+__global__ void __launch_bounds__(256) k_fock_atoms(
+    int mode, const AtomDev* __restrict__ atoms, int natoms, const ElemDev* __restrict__ elems,
+    const JobDev* __restrict__ jobs, const JobState* __restrict__ state, double* __restrict__ arena,
+    const double* __restrict__ part) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ai = blockIdx.x * 4 + wave;
+    if (ai >= natoms) return;
+    const AtomDev at = atoms[ai];
+    const JobState& st = state[at.job];
+    const int phase = st.phase, iter = st.iter;
+    if (mode == 0 ? phase != PH_ACTIVE : phase != PH_ENERGY) return;
+    const JobDev jb = jobs[at.job];
+    const ElemDev E = elems[at.elem];
+    const int n = jb.nao, N = jb.natoms, A = at.local, na = E.nao, np = na == 4 ? 10 : 1;
+    const bool guess = mode == 0 && iter == 0 && jb.requires_guess;
+    double* F = mat(arena, jb, AR_F);
+    const double* H = mat(arena, jb, AR_H);
+    const double* P = mat(arena, jb, mode == 0 ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1)));
+    if (guess) {
+        if (lane < 16) {
+            int mu = lane >> 2, nu = lane & 3;
+            if (mu < na && nu < na)
+                F[(long long)(at.ao + mu) * n + at.ao + nu] = mu == nu ? E.U[mu == 0 ? 0 : 1] : 0.0;
+        }
+        return;
+    }
+    double acc[10];
+    for (int i = 0; i < 10; ++i) acc[i] = 0.0;
+    for (int Bl = lane; Bl < N; Bl += 64) {
+        if (Bl == A) continue;
+        const double* p = A < Bl ? part + (jb.pair0 + tri_slot(A, Bl, N)) * 20
+                                 : part + (jb.pair0 + tri_slot(Bl, A, N)) * 20 + 10;
+        for (int i = 0; i < np; ++i) acc[i] += p[i];
+    }
+    for (int i = 0; i < np; ++i)
+        for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off);
+    if (lane < 16) {
+        int mu = lane >> 2, nu = lane & 3;
+        if (mu < na && nu < na) {
+            int pi = pk_index(mu, nu);
+            double j = 0.0;
+            for (int i = 0; i < np; ++i)
+                if (i == pi) j = acc[i];
+            const long long base = (long long)at.ao * n + at.ao;
+            double g;
+            if (mu == nu) {
+                g = 0.0;
+                for (int k = 0; k < na; ++k)
+                    g += P[base + (long long)k * n + k] * (g1c(E, mu, k) - 0.5 * h1c(E, mu, k));
+            } else {
+                g = (1.5 * h1c(E, mu, nu) - 0.5 * g1c(E, mu, nu)) * P[base + (long long)mu * n + nu];
+            }
+            const long long ij = base + (long long)mu * n + nu;
+            F[ij] = H[ij] + g + j;
+        }
+    }
+}
+
+void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
+                 const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
+                 double* arena, double* part) {
+    if (npairs > 0)
+        hipLaunchKernelGGL(k_fock_pairs, dim3((npairs + 3) / 4), dim3(256), 0, st, mode, pairs, npairs, atoms,
+                           elems, jobs, state, ints, arena, part);
+    hipLaunchKernelGGL(k_fock_atoms, dim3((natoms + 3) / 4), dim3(256), 0, st, mode, atoms, natoms, elems, jobs,
+                       state, arena, part);
+}
+
+// ------------------------------------------------------------------ Jacobi
+// Cyclic two-sided Jacobi with the round-robin (circle) ordering: every round
+// rotates n/2 disjoint index pairs at once; a 2x2 block of rows {p_i,q_i} and
+// columns {p_j,q_j} transforms as J_i^T B J_j, so each thread updates whole
+// 2x2 blocks of the upper block triangle and mirrors them.  Eigenvector rows
+// VT (= C, rows are MOs as MolDS stores them) accumulate VT <- J^T VT.
+// Rotation parameters: Golub & Van Loan, Matrix Computations, Alg. 8.5.1.
+// A sweep ends the solve when it rotates nothing (all |a_pq| <= thr).
+template <bool ALDS>
+__global__ void __launch_bounds__(1024) k_jacobi(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                                 const int* __restrict__ joblist, double* __restrict__ arena,
+                                                 double* __restrict__ eps, int* __restrict__ perm) {
+    extern __shared__ double dyn[];
+    __shared__ double red[1024];
+    const int job = joblist[blockIdx.x];
+    const JobDev jb = jobs[job];
+    if (state[job].phase != PH_ACTIVE) return;
+    const int n = jb.nao, npad = n + (n & 1), nh = npad / 2, m = npad - 1;
+    const int T = blockDim.x, tid = threadIdx.x;
+    double* sc = dyn;
+    double* ss = sc + nh;
+    double* stt = ss + nh;
+    int* sp = (int*)(stt + nh);
+    int* sq = sp + nh;
+    int* srot = sq + nh;
+    double* A = ALDS ? (double*)(srot + nh + (nh & 1)) : mat(arena, jb, AR_W);
+    const double* F = mat(arena, jb, AR_F);
+    double* VT = mat(arena, jb, AR_C);
+    const long long nn = (long long)n * n;
+
+    double fro = 0.0;
+    for (long long e = tid; e < nn; e += T) {
+        double v = F[e];
+        A[e] = v;
+        fro += v * v;
+        int i = (int)(e / n), j = (int)(e - (long long)i * n);
+        VT[e] = i == j ? 1.0 : 0.0;
+    }
+    red[tid] = fro;
+    __syncthreads();
+    for (int s = T / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double thr = 1e-15 * sqrt(red[0]);
+    __syncthreads();
+
+    int sweep = 0;
+    const int maxsweep = 60;
+    for (; sweep < maxsweep; ++sweep) {
+        int rotated = 0;
+        for (int r = 0; r < m; ++r) {
+            int flag = 0;
+            for (int k = tid; k < nh; k += T) {
+                int p, q;
+                if (k == 0) { p = r; q = m; }
+                else { p = (r + k) % m; q = (r - k + m) % m; }
+                if (p > q) { int t = p; p = q; q = t; }
+                double c = 1.0, s = 0.0, t = 0.0;
+                int rot = 0;
+                if (q < n) {
+                    double apq = A[(long long)p * n + q];
+                    if (fabs(apq) > thr) {
+                        double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
+                        double tau = (aqq - app) / (2.0 * apq);
+                        t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                        rot = 1;
+                    }
+                }
+                sp[k] = p; sq[k] = q; sc[k] = c; ss[k] = s; stt[k] = t; srot[k] = rot;
+                flag |= rot;
+            }
+            if (!__syncthreads_or(flag)) continue;
+            rotated = 1;
+            const long long nb = (long long)nh * (nh + 1) / 2;
+            for (long long b = tid; b < nb; b += T) {
+                // b enumerates (hi >= lo): b = hi*(hi+1)/2 + lo
+                long long hi = (long long)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+                while (hi * (hi + 1) / 2 > b) --hi;
+                while ((hi + 1) * (hi + 2) / 2 <= b) ++hi;
+                const int ki = (int)(b - hi * (hi + 1) / 2), kj = (int)hi;
+                const int ri = srot[ki], rj = srot[kj];
+                if (!ri && !rj) continue;
+                const int p1 = sp[ki], q1 = sq[ki], p2 = sp[kj], q2 = sq[kj];
+                if (ki == kj) {
+                    const double t = stt[ki];
+                    const double apq = A[(long long)p1 * n + q1];
+                    A[(long long)p1 * n + p1] -= t * apq;
+                    A[(long long)q1 * n + q1] += t * apq;
+                    A[(long long)p1 * n + q1] = 0.0;
+                    A[(long long)q1 * n + p1] = 0.0;
+                    continue;
+                }
+                const bool vq1 = q1 < n, vq2 = q2 < n;
+                const double c1 = sc[ki], s1 = ss[ki], c2 = sc[kj], s2 = ss[kj];
+                const double a = A[(long long)p1 * n + p2];
+                const double bq = vq2 ? A[(long long)p1 * n + q2] : 0.0;
+                const double cq = vq1 ? A[(long long)q1 * n + p2] : 0.0;
+                const double d = (vq1 && vq2) ? A[(long long)q1 * n + q2] : 0.0;
+                const double a1 = c1 * a - s1 * cq, b1 = c1 * bq - s1 * d;
+                const double c1r = s1 * a + c1 * cq, d1 = s1 * bq + c1 * d;
+                const double a2 = c2 * a1 - s2 * b1, b2 = s2 * a1 + c2 * b1;
+                const double c2r = c2 * c1r - s2 * d1, d2 = s2 * c1r + c2 * d1;
+                A[(long long)p1 * n + p2] = a2;
+                A[(long long)p2 * n + p1] = a2;
+                if (vq2) { A[(long long)p1 * n + q2] = b2; A[(long long)q2 * n + p1] = b2; }
+                if (vq1) { A[(long long)q1 * n + p2] = c2r; A[(long long)p2 * n + q1] = c2r; }
+                if (vq1 && vq2) { A[(long long)q1 * n + q2] = d2; A[(long long)q2 * n + q1] = d2; }
+            }
+            for (long long e = tid; e < (long long)nh * n; e += T) {
+                const int k = (int)(e / n), i = (int)(e - (long long)k * n);
+                if (!srot[k]) continue;
+                const int p = sp[k], q = sq[k];
+                const double c = sc[k], s = ss[k];
+                const double vp = VT[(long long)p * n + i], vq = VT[(long long)q * n + i];
+                VT[(long long)p * n + i] = c * vp - s * vq;
+                VT[(long long)q * n + i] = s * vp + c * vq;
+            }
+            __syncthreads();
+        }
+        if (!rotated) break;
+    }
+    // eigenvalues ascending (ties by index) and the row permutation
+    for (int p = tid; p < n; p += T) {
+        const double d = A[(long long)p * n + p];
+        int rank = 0;
+        for (int q = 0; q < n; ++q) {
+            const double e = A[(long long)q * n + q];
+            rank += (e < d) || (e == d && q < p);
+        }
+        eps[jb.vec0 + rank] = d;
+        perm[jb.ivec0 + rank] = p;
+    }
+    if (tid == 0) state[job].sweeps = sweep < maxsweep ? sweep + 1 : -1;
+}
+
+void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
+                  double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds) {
+    if (nlist <= 0) return;
+    int threads = max_n <= 64 ? 256 : (max_n <= 160 ? 512 : 1024);
+    if (a_in_lds) {
+        hipFuncSetAttribute((const void*)k_jacobi<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        hipLaunchKernelGGL(k_jacobi<true>, dim3(nlist), dim3(threads), lds_bytes, st, jobs, state, joblist, arena,
+                           eps, perm);
+    } else {
+        hipFuncSetAttribute((const void*)k_jacobi<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        hipLaunchKernelGGL(k_jacobi<false>, dim3(nlist), dim3(threads), lds_bytes, st, jobs, state, joblist, arena,
+                           eps, perm);
+    }
+}
+
+// ------------------------------------------------------------------ density
+// P = 2 C_occ^T C_occ (rows of C are MOs; occupied = the nocc lowest after the
+// eigenvalue sort), one 64x64 tile per workgroup; each tile also emits its
+// partial sum of (P - P_old)^2 for the RMS test.
+__global__ void __launch_bounds__(256) k_density(const TileDev* __restrict__ tiles, const JobDev* __restrict__ jobs,
+                                                 const JobState* __restrict__ state, double* __restrict__ arena,
+                                                 const int* __restrict__ perm, double* __restrict__ rmspart) {
+    const TileDev tl = tiles[blockIdx.x];
+    if (state[tl.job].phase != PH_ACTIVE) return;
+    const JobDev jb = jobs[tl.job];
+    const int iter = state[tl.job].iter;
+    const int n = jb.nao, nocc = jb.nocc;
+    const double* VT = mat(arena, jb, AR_C);
+    const double* Pin = mat(arena, jb, AR_P0 + (iter & 1));
+    double* Pout = mat(arena, jb, AR_P0 + ((iter + 1) & 1));
+    __shared__ double sA[16][68], sB[16][68];
+    __shared__ double red[256];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    double acc[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int o0 = 0; o0 < nocc; o0 += 16) {
+        for (int e = tid; e < 16 * 64; e += 256) {
+            const int kk = e >> 6, ii = e & 63, o = o0 + kk;
+            const int row = o < nocc ? perm[jb.ivec0 + o] : -1;
+            sA[kk][ii] = (row >= 0 && tl.i0 + ii < n) ? VT[(long long)row * n + tl.i0 + ii] : 0.0;
+            sB[kk][ii] = (row >= 0 && tl.j0 + ii < n) ? VT[(long long)row * n + tl.j0 + ii] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            double av[4], bv[4];
+            for (int a = 0; a < 4; ++a) av[a] = sA[kk][ty * 4 + a];
+            for (int b = 0; b < 4; ++b) bv[b] = sB[kk][tx * 4 + b];
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+        }
+        __syncthreads();
+    }
+    double loc = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        const int i = tl.i0 + ty * 4 + a;
+        if (i >= n) continue;
+        for (int b = 0; b < 4; ++b) {
+            const int j = tl.j0 + tx * 4 + b;
+            if (j >= n) continue;
+            const double v = 2.0 * acc[a][b];
+            const long long ij = (long long)i * n + j;
+            const double d = v - Pin[ij];
+            Pout[ij] = v;
+            loc += d * d;
+        }
+    }
+    red[tid] = loc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) rmspart[blockIdx.x] = red[0];
+}
+
+void launch_density(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, const JobState* state,
+                    double* arena, const int* perm, double* rmspart) {
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_density, dim3(ntiles), dim3(256), 0, st, tiles, jobs, state, arena, perm, rmspart);
+}
+
+// ------------------------------------------------------------------ step
+// RMS test, density-DIIS and damping for one job per workgroup
+// (Cndo2.cpp:663-695, 920-1061).  The DIIS ring keeps the reference's logical
+// order (oldest first) over physical slots starting at ring_start.
+__device__ bool small_solve(int m, double* a /* m*m row-major, destroyed */, double* b) {
+    for (int c = 0; c < m; ++c) {
+        int piv = c;
+        double best = fabs(a[c * m + c]);
+        for (int r = c + 1; r < m; ++r)
+            if (fabs(a[r * m + c]) > best) { best = fabs(a[r * m + c]); piv = r; }
+        if (best == 0.0) return false;
+        if (piv != c) {
+            for (int k = 0; k < m; ++k) { double t = a[c * m + k]; a[c * m + k] = a[piv * m + k]; a[piv * m + k] = t; }
+            double t = b[c]; b[c] = b[piv]; b[piv] = t;
+        }
+        for (int r = c + 1; r < m; ++r) {
+            double f = a[r * m + c] / a[c * m + c];
+            if (f == 0.0) continue;
+            for (int k = c; k < m; ++k) a[r * m + k] -= f * a[c * m + k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int r = m - 1; r >= 0; --r) {
+        double s = b[r];
+        for (int k = r + 1; k < m; ++k) s -= a[r * m + k] * b[k];
+        b[r] = s / a[r * m + r];
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                              double* __restrict__ arena, const double* __restrict__ rmspart,
+                                              double* __restrict__ traj, int* __restrict__ remaining) {
+    const int job = blockIdx.x;
+    JobState* st = &state[job];
+    if (st->phase != PH_ACTIVE) return;
+    const JobDev jb = jobs[job];
+    const int tid = threadIdx.x, T = blockDim.x;
+    const int n = jb.nao, iter = st->iter, k = jb.diis_k;
+    const long long nn = (long long)n * n;
+    __shared__ double sh_rms;
+    __shared__ double red[NDDO_MAX_DIIS + 1][256];
+    __shared__ double coef[NDDO_MAX_DIIS + 1];
+    __shared__ int diis_on, damp_on;
+    if (tid == 0) {
+        double s = 0.0;
+        for (int t = 0; t < jb.ntiles; ++t) s += rmspart[jb.tile0 + t];
+        sh_rms = sqrt(s / (double)nn);
+        double* tr = traj + jb.traj0 + 4LL * iter;
+        tr[0] = sh_rms;
+        tr[1] = st->diis_error;
+        tr[2] = st->diis_on;
+        tr[3] = st->damp_on;
+    }
+    __syncthreads();
+    const double rms = sh_rms;
+    if (rms < jb.rms_thresh) {
+        if (tid == 0) { st->rms = rms; st->phase = PH_ENERGY; }
+        return;
+    }
+    const bool guess = iter == 0 && jb.requires_guess;
+    const double* Pin = mat(arena, jb, AR_P0 + (iter & 1));
+    double* Pout = mat(arena, jb, AR_P0 + ((iter + 1) & 1));
+    if (!guess) {
+        if (k > 0) {
+            const int rs = (st->ring_start + 1) % k, slot_new = (rs + k - 1) % k;
+            double* RP = mat(arena, jb, AR_RING);
+            double* RE = RP + (long long)k * nn;
+            double part[NDDO_MAX_DIIS];
+            for (int i = 0; i < k; ++i) part[i] = 0.0;
+            double amax = 0.0;
+            for (long long e = tid; e < nn; e += T) {
+                const double pv = Pout[e], ev = pv - Pin[e];
+                RP[(long long)slot_new * nn + e] = pv;
+                RE[(long long)slot_new * nn + e] = ev;
+                amax = fmax(amax, fabs(ev));
+                for (int i = 0; i < k; ++i) {
+                    const int sl = (rs + i) % k;
+                    const double x = sl == slot_new ? ev : RE[(long long)sl * nn + e];
+                    part[i] += x * ev;
+                }
+            }
+            for (int i = 0; i < k; ++i) red[i][tid] = part[i];
+            red[k][tid] = amax;
+            __syncthreads();
+            for (int s = T / 2; s > 0; s >>= 1) {
+                if (tid < s) {
+                    for (int i = 0; i < k; ++i) red[i][tid] += red[i][tid + s];
+                    red[k][tid] = fmax(red[k][tid], red[k][tid + s]);
+                }
+                __syncthreads();
+            }
+            if (tid == 0) {
+                const int ld = NDDO_MAX_DIIS + 1;
+                double* B = st->B;
+                for (int a = 0; a < k - 1; ++a)
+                    for (int b = 0; b < k - 1; ++b) B[a * ld + b] = B[(a + 1) * ld + b + 1];
+                for (int i = 0; i < k; ++i) {
+                    B[(k - 1) * ld + i] = red[i][0];
+                    B[i * ld + (k - 1)] = red[i][0];
+                    B[i * ld + k] = -1.0;
+                    B[k * ld + i] = -1.0;
+                }
+                B[k * ld + k] = 0.0;
+                const double err = red[k][0];
+                st->diis_error = err;
+                int on = 0;
+                if (k <= iter && jb.diis_end < err && err < jb.diis_start) {
+                    double a[(NDDO_MAX_DIIS + 1) * (NDDO_MAX_DIIS + 1)], rhs[NDDO_MAX_DIIS + 1];
+                    for (int i = 0; i <= k; ++i) {
+                        rhs[i] = i == k ? -1.0 : 0.0;
+                        for (int j = 0; j <= k; ++j) a[i * (k + 1) + j] = B[i * ld + j];
+                    }
+                    on = small_solve(k + 1, a, rhs) ? 1 : 0;
+                    for (int i = 0; i <= k; ++i) coef[i] = rhs[i];
+                }
+                diis_on = on;
+                st->diis_on = on;
+                st->ring_start = rs;
+            }
+            __syncthreads();
+            if (diis_on) {
+                for (long long e = tid; e < nn; e += T) {
+                    double s = 0.0;
+                    for (int i = 0; i < k; ++i) s += coef[i] * RP[(long long)((rs + i) % k) * nn + e];
+                    Pout[e] = s;
+                }
+            }
+        }
+        const double w = jb.damp_weight;
+        const int don = (0.0 < w && jb.damp_thresh < rms) ? 1 : 0;
+        if (don)
+            for (long long e = tid; e < nn; e += T) Pout[e] = Pout[e] * (1.0 - w) + w * Pin[e];
+        if (tid == 0) st->damp_on = don;
+    }
+    if (tid == 0) {
+        st->iter = iter + 1;
+        if (iter + 1 >= jb.max_iter) {
+            st->phase = PH_DONE;
+            st->status = -1;   // NDDO_NOT_CONVERGED
+            st->rms = rms;
+            atomicSub(remaining, 1);
+        }
+    }
+}
+
+void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
+                 const double* rmspart, double* traj, int* remaining) {
+    hipLaunchKernelGGL(k_step, dim3(njobs), dim3(256), 0, st, jobs, state, arena, rmspart, traj, remaining);
+}
+
+// ------------------------------------------------------------------ energy
+// E_el = 1/2 sum_ij P_ij (F_ij + H_ij) with F built from the final P
+// (Cndo2.cpp:1348-1381); compensated summation.
+__global__ void __launch_bounds__(256) k_energy(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                                double* __restrict__ arena, int* __restrict__ remaining) {
+    const int job = blockIdx.x;
+    JobState* st = &state[job];
+    if (st->phase != PH_ENERGY) return;
+    const JobDev jb = jobs[job];
+    const int n = jb.nao, iter = st->iter, tid = threadIdx.x;
+    const long long nn = (long long)n * n;
+    const double* P = mat(arena, jb, AR_P0 + ((iter + 1) & 1));
+    const double* F = mat(arena, jb, AR_F);
+    const double* H = mat(arena, jb, AR_H);
+    double s = 0.0, c = 0.0;
+    for (long long e = tid; e < nn; e += blockDim.x) {
+        const double y = P[e] * (F[e] + H[e]) - c;
+        const double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __shared__ double ss[256], sc[256];
+    ss[tid] = s;
+    sc[tid] = c;
+    __syncthreads();
+    if (tid == 0) {
+        double S = 0.0, C = 0.0;
+        for (int i = 0; i < (int)blockDim.x; ++i) {
+            const double y = ss[i] - sc[i] - C;
+            const double t = S + y;
+            C = (t - S) - y;
+            S = t;
+        }
+        st->e_el = 0.5 * S;
+        st->phase = PH_DONE;
+        st->status = 0;
+        atomicSub(remaining, 1);
+    }
+}
+
+void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena, int* remaining) {
+    hipLaunchKernelGGL(k_energy, dim3(njobs), dim3(256), 0, st, jobs, state, arena, remaining);
+}
+
+}  // namespace nddo

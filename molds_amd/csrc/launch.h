@@ -1,0 +1,36 @@
+// Host-side launchers of the engine's HIP kernels (one translation unit per
+// kernel family; the engine only calls these).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nddo_device.h"
+
+namespace nddo {
+
+int upload_overlap_table(const double* table /* 32*25 */);
+
+void launch_setup(hipStream_t st, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
+                  const ElemDev* elems, const JobDev* jobs, int njobs, double* ints, double* arena,
+                  double* part, double* ecore, JobState* state, double* sdbg);
+
+// mode 0: SCF Fock for PH_ACTIVE jobs (guess on iteration 0 when requested);
+// mode 1: non-guess Fock of the final density for PH_ENERGY jobs.
+void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
+                 const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
+                 double* arena, double* part);
+
+// Eigen-decomposition of F for PH_ACTIVE jobs in joblist (one workgroup per job).
+// lds_ok: jobs whose n*n matrix fits the LDS budget take the LDS path.
+void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
+                  int max_n, double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds);
+
+void launch_density(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs,
+                    const JobState* state, double* arena, const int* perm, double* rmspart);
+
+void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
+                 const double* rmspart, double* traj, int* remaining);
+
+void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
+                   int* remaining);
+
+}  // namespace nddo

@@ -1,0 +1,86 @@
+// Shared host/device data layout of the NDDO SCF engine.
+//
+// A batch holds J jobs (molecules).  Everything is flattened over the batch:
+//   atoms  : AtomDev[natoms_total]      (job-local AO offsets)
+//   pairs  : PairDev[npairs_total]      one record per unordered atom pair A<B
+//   slots  : pair partial buffer indexed by  job.pair0 + tri(A,B)
+//   arena  : per job (ARENA_FIXED + 2k) * n^2 doubles, see ArenaKind
+// Matrices are row-major n x n with leading dimension n.
+#pragma once
+#include <stdint.h>
+
+#define NDDO_MAX_DIIS 16
+
+namespace nddo {
+
+// AO order inside an atom: s, py, pz, px (MolDS src/base/Enums.h OrbitalType).
+enum { ORB_S = 0, ORB_PY = 1, ORB_PZ = 2, ORB_PX = 3 };
+
+struct ElemDev {
+    double core;            // core charge Z
+    double U[2];            // U_ss, U_pp (Hartree)
+    double zeta[2];         // Slater exponents s, p
+    double beta[2];         // resonance beta_s, beta_p (Hartree)
+    double alpha;           // core-repulsion alpha (1/bohr)
+    double Gss, Gpp, Gsp, Gpp2, Hsp, Hpp;
+    double D[3];            // multipole separations D0 D1 D2 (bohr)
+    double rho[3];          // additive terms rho0 rho1 rho2 (bohr)
+    double K[4], L[4], M[4];// AM1/PM3 Gaussian core terms (a.u.)
+    int Z, nao, nshell, pad;
+};
+
+struct AtomDev {
+    double x, y, z;
+    int elem;     // index into the ElemDev table
+    int job;      // job index in the batch
+    int ao;       // job-local first AO
+    int local;    // job-local atom index
+};
+
+struct PairDev {
+    int a, b;          // global atom indices, job-local a < b
+    int job;
+    int slot;          // global pair slot (job.pair0 + tri(a_local, b_local))
+    long long ioff;    // offset of the packed integrals (doubles)
+};
+
+// Arena layout per job, in units of n*n doubles.
+enum ArenaKind {
+    AR_F = 0,      // Fock matrix
+    AR_P0 = 1,     // density ping
+    AR_P1 = 2,     // density pong
+    AR_C = 3,      // eigenvectors (columns), unsorted; order in perm[]
+    AR_W = 4,      // eigensolver work matrix
+    AR_H = 5,      // core Hamiltonian F(P = 0), full symmetric
+    AR_RING = 6    // DIIS ring: k stored densities then k stored errors
+};
+#define ARENA_FIXED 6
+
+enum JobPhase { PH_ACTIVE = 0, PH_ENERGY = 1, PH_DONE = 2, PH_IDLE = 3 };
+
+struct JobDev {
+    int natoms, nao, nocc, theory;
+    int atom0;            // first global atom
+    int npairs;           // natoms*(natoms-1)/2
+    long long pair0;      // first global pair slot
+    long long arena;      // offset (doubles) of this job's arena
+    long long vec0;       // offset (doubles) of per-job vectors of length n (eps)
+    long long ivec0;      // offset (ints) of per-job int vectors of length n (perm)
+    int tile0, ntiles;    // density tiles of this job in the tile list
+    int tiles_per_dim;
+    // SCF options (MolDS Parameters SCF knobs)
+    int max_iter, diis_k, requires_guess, precision;
+    double rms_thresh, damp_thresh, damp_weight, diis_start, diis_end;
+    long long traj0;      // offset (doubles) into the trajectory buffer: max_iter * 4
+};
+
+struct JobState {
+    int phase, iter, status, diis_on;
+    int damp_on, ring_start, warm, sweeps;
+    double rms, diis_error, e_el, e_core;
+    double B[(NDDO_MAX_DIIS + 1) * (NDDO_MAX_DIIS + 1)];  // DIIS bordered matrix
+};
+
+struct TileDev { int job, i0, j0, pad; };
+
+}  // namespace nddo

@@ -1,0 +1,309 @@
+"""ctypes binding of libnddo.so (include/nddo.h).
+
+This is the Python face of the engine: it mirrors MolDS's ElectronicStructure
+use (SetMolecule + DoSCF + getters, src/base/ElectronicStructure.h:23-53) through
+the C ABI.  There is no CPU fallback: if the HIP library is missing or no GPU
+is present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "lib", "libnddo.so")
+THEORY = {"mndo": 0, "am1": 1, "pm3": 2}
+PRECISION = {"fp64": 0, "mixed": 1}
+STATUS = {0: "OK", -1: "NOT_CONVERGED", -2: "ODD_ELECTRONS", -3: "UNSUPPORTED_ELEMENT",
+          -4: "ADMISSION_DENIED", -5: "EIGENSOLVER_FAILED", -6: "HIP_ERROR", -7: "BAD_ARG", 1: "PENDING"}
+ANG2AU = 1.0 / 0.5291772   # src/base/Parameters.cpp:48
+EV2AU = 0.03674903         # src/base/Parameters.cpp:45
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class ScfOpts(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int), ("rms_density", ctypes.c_double),
+                ("damping_thresh", ctypes.c_double), ("damping_weight", ctypes.c_double),
+                ("diis_num_error_vect", ctypes.c_int), ("diis_start_error", ctypes.c_double),
+                ("diis_end_error", ctypes.c_double)]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("theory", ctypes.c_int), ("precision", ctypes.c_int), ("natoms", ctypes.c_int),
+                ("Z", _ip), ("xyz_bohr", _dp), ("opts", ScfOpts), ("requires_guess", ctypes.c_int),
+                ("P0", _dp)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("nao", ctypes.c_int), ("nocc", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("energy_total", ctypes.c_double),
+                ("energy_elec", ctypes.c_double), ("energy_core", ctypes.c_double),
+                ("heat_of_formation", ctypes.c_double), ("final_rms", ctypes.c_double),
+                ("latency_s", ctypes.c_double), ("eps", _dp), ("C", _dp), ("P", _dp),
+                ("trajectory", _dp)]
+
+
+class EngineOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch_jobs", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+_lib = None
+
+
+def load_library(path=LIBPATH):
+    """Loads libnddo.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m molds_amd.build`")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.nddo_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(EngineOpts), ctypes.POINTER(vp)]
+    L.nddo_engine_destroy.argtypes = [vp]
+    L.nddo_scf_opts_default.argtypes = [ctypes.POINTER(ScfOpts)]
+    L.nddo_scf_once.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(Result)]
+    L.nddo_scf_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(JobDesc), ctypes.POINTER(Result)]
+    L.nddo_submit.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_int64)]
+    L.nddo_poll.argtypes = [vp, ctypes.c_int64]
+    L.nddo_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double]
+    L.nddo_get_result.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(Result)]
+    L.nddo_release.argtypes = [vp, ctypes.c_int64]
+    L.nddo_last_error.restype = ctypes.c_char_p
+    L.nddo_engine_profile.argtypes = [vp, ctypes.c_int]
+    L.nddo_engine_stage_times.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+    L.nddo_dbg_integrals.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp]
+    L.nddo_dbg_fock.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, _dp]
+    L.nddo_dbg_eigh.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _ip]
+    L.nddo_dbg_density.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    _lib = L
+    return L
+
+
+def last_error():
+    return load_library().nddo_last_error().decode()
+
+
+class NddoError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def default_opts(**kw):
+    o = ScfOpts()
+    load_library().nddo_scf_opts_default(ctypes.byref(o))
+    for k, v in kw.items():
+        if v is not None:
+            setattr(o, k, v)
+    return o
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+class Job:
+    """Keeps the numpy buffers a JobDesc points to alive."""
+
+    def __init__(self, theory, Z, xyz_bohr, precision="fp64", requires_guess=True, P0=None, **opts):
+        self.Z = np.ascontiguousarray(Z, dtype=np.int32)
+        self.xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+        self.P0 = None if P0 is None else np.ascontiguousarray(P0, dtype=np.float64)
+        self.opts = dict(opts)
+        self.desc = JobDesc()
+        d = self.desc
+        d.theory = THEORY[theory] if isinstance(theory, str) else int(theory)
+        d.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+        d.natoms = len(self.Z)
+        d.Z = self.Z.ctypes.data_as(_ip)
+        d.xyz_bohr = self.xyz.ctypes.data_as(_dp)
+        d.opts = default_opts(**opts)
+        d.requires_guess = 1 if requires_guess else 0
+        d.P0 = _p(self.P0)
+
+    @property
+    def max_iter(self):
+        return self.desc.opts.max_iter
+
+
+class ScfResult(dict):
+    __getattr__ = dict.__getitem__
+
+
+class Engine:
+    """One engine per HIP device (one process per GPU)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_jobs=0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        o = EngineOpts(hbm_budget_bytes, max_batch_jobs, 0)
+        rc = L.nddo_engine_create(device, ctypes.byref(o), ctypes.byref(h))
+        if rc != 0:
+            raise NddoError(rc, last_error())
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nddo_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --------------------------------------------------------------- SCF
+    def _result_buffers(self, job, want_C, want_P, want_traj):
+        n = self._nao(job)
+        bufs = dict(eps=np.zeros(n))
+        if want_C:
+            bufs["C"] = np.zeros((n, n))
+        if want_P:
+            bufs["P"] = np.zeros((n, n))
+        if want_traj:
+            bufs["traj"] = np.zeros((job.max_iter, 4))
+        r = Result()
+        r.eps = _p(bufs["eps"])
+        r.C = _p(bufs.get("C"))
+        r.P = _p(bufs.get("P"))
+        r.trajectory = _p(bufs.get("traj"))
+        return r, bufs
+
+    @staticmethod
+    def _nao(job):
+        return int(sum(1 if z == 1 else 4 for z in job.Z))
+
+    @staticmethod
+    def _pack(r, bufs, job):
+        out = ScfResult(status=r.status, status_name=STATUS.get(r.status, str(r.status)), nao=r.nao, nocc=r.nocc,
+                        iterations=r.iterations, energy=r.energy_total, e_el=r.energy_elec, e_core=r.energy_core,
+                        hof=r.heat_of_formation, rms=r.final_rms, latency_s=r.latency_s, eps=bufs["eps"],
+                        C=bufs.get("C"), P=bufs.get("P"))
+        if "traj" in bufs:
+            it = max(r.iterations, 0)
+            t = bufs["traj"][:it]
+            out["traj"] = [(i, t[i, 0], t[i, 1], bool(t[i, 2]), bool(t[i, 3])) for i in range(len(t))]
+        return out
+
+    def scf(self, theory, Z, xyz_bohr, precision="fp64", requires_guess=True, P0=None, want_C=True, want_P=True,
+            want_traj=True, **opts):
+        return self.scf_batch([Job(theory, Z, xyz_bohr, precision, requires_guess, P0, **opts)],
+                              want_C=want_C, want_P=want_P, want_traj=want_traj)[0]
+
+    def scf_batch(self, jobs, want_C=False, want_P=False, want_traj=False):
+        """All jobs share one launch per SCF stage.  Returns per-job results."""
+        descs = (JobDesc * len(jobs))(*[j.desc for j in jobs])
+        results = (Result * len(jobs))()
+        keep = []
+        for i, j in enumerate(jobs):
+            r, bufs = self._result_buffers(j, want_C, want_P, want_traj)
+            results[i] = r
+            keep.append(bufs)
+        rc = self.L.nddo_scf_batch(self.h, len(jobs), descs, results)
+        if rc != 0:
+            raise NddoError(rc, last_error())
+        return [self._pack(results[i], keep[i], jobs[i]) for i in range(len(jobs))]
+
+    # --------------------------------------------------------------- profiling
+    STAGES = ("setup", "fock", "eigen", "density", "step", "energy")
+
+    def profile(self, enable=True):
+        self.L.nddo_engine_profile(self.h, 1 if enable else 0)
+
+    def stage_times(self, reset=False):
+        ms = np.zeros(6)
+        n = (ctypes.c_longlong * 6)()
+        self.L.nddo_engine_stage_times(self.h, _p(ms), n, 1 if reset else 0)
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
+
+    # --------------------------------------------------------------- serving
+    def submit(self, job):
+        jid = ctypes.c_int64()
+        rc = self.L.nddo_submit(self.h, ctypes.byref(job.desc), ctypes.byref(jid))
+        if rc != 0:
+            raise NddoError(rc, last_error())
+        return jid.value
+
+    def poll(self, jid):
+        return self.L.nddo_poll(self.h, jid)
+
+    def wait(self, jid, timeout_s=60.0):
+        return self.L.nddo_wait(self.h, jid, timeout_s)
+
+    def get_result(self, jid, job, want_C=False, want_P=False, want_traj=False):
+        r, bufs = self._result_buffers(job, want_C, want_P, want_traj)
+        rc = self.L.nddo_get_result(self.h, jid, ctypes.byref(r))
+        if rc < 0 and rc not in (-1, -5):
+            raise NddoError(rc, last_error())
+        return self._pack(r, bufs, job)
+
+    def release(self, jid):
+        return self.L.nddo_release(self.h, jid)
+
+    # --------------------------------------------------------------- stage hooks
+    def dbg_integrals(self, theory, Z, xyz_bohr):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        N = len(Z)
+        n = int(sum(1 if z == 1 else 4 for z in Z))
+        g = np.zeros((N, N, 4, 4, 4, 4))
+        S = np.zeros((n, n))
+        H = np.zeros((n, n))
+        ec = ctypes.c_double()
+        rc = self.L.nddo_dbg_integrals(self.h, THEORY[theory], N, Z.ctypes.data_as(_ip), _p(xyz), _p(g), _p(S),
+                                       _p(H), ctypes.byref(ec))
+        if rc != 0:
+            raise NddoError(rc, last_error())
+        return g, S, H, ec.value
+
+    def dbg_fock(self, theory, Z, xyz_bohr, P, guess=False):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+        n = int(sum(1 if z == 1 else 4 for z in Z))
+        P = np.ascontiguousarray(P, dtype=np.float64) if P is not None else np.zeros((n, n))
+        F = np.zeros((n, n))
+        rc = self.L.nddo_dbg_fock(self.h, THEORY[theory], len(Z), Z.ctypes.data_as(_ip), _p(xyz), _p(P),
+                                  1 if guess else 0, _p(F))
+        if rc != 0:
+            raise NddoError(rc, last_error())
+        return F
+
+    def dbg_eigh(self, F):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        n = F.shape[0]
+        eps = np.zeros(n)
+        C = np.zeros((n, n))
+        sw = ctypes.c_int()
+        rc = self.L.nddo_dbg_eigh(self.h, n, _p(F), _p(eps), _p(C), ctypes.byref(sw))
+        if rc != 0:
+            raise NddoError(rc, last_error())
+        return eps, C, sw.value
+
+    def dbg_density(self, C, nocc):
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        n = C.shape[0]
+        P = np.zeros((n, n))
+        rc = self.L.nddo_dbg_density(self.h, n, nocc, _p(C), _p(P))
+        if rc != 0:
+            raise NddoError(rc, last_error())
+        return P
+
+
+def heat_of_formation_kcal(result):
+    """MNDO heat of formation in kcal/mol (Mndo.cpp:412-420 prints H/kcalMolin2AU)."""
+    h = result["hof"]
+    return h / 0.00159360175 if not math.isnan(h) else float("nan")

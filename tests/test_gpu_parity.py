@@ -1,0 +1,210 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle, through the C ABI.
+
+Tolerances: integrals, overlap, core Hamiltonian, Fock: 1e-12 relative to the
+largest element (FP64 everywhere, different but exact-in-theory formula order);
+total energy: 1e-6 eV vs the oracle (the north-star FP64 bar), 7 printed digits
+vs MolDS's goldens; density RMS vs the oracle: 1e-8.
+"""
+import numpy as np
+import pytest
+
+from conftest import case_xyz_bohr, golden_opts, load_goldens, oracle_opts
+
+pytestmark = pytest.mark.gpu
+
+EV = 1.0 / 0.03674903          # Hartree -> eV with MolDS's constant
+TOL_E_EV = 1e-6
+
+
+def _unique_cases():
+    seen, out = set(), []
+    for c in load_goldens():
+        key = (c["theory"], tuple(c["Z"]), tuple(map(tuple, c["xyz_angstrom"])), tuple(sorted(c["scf"].items())))
+        if key in seen:
+            continue
+        seen.add(key)
+        out.append(c)
+    return out
+
+
+MOLS = {
+    "c2h6_pm3": None,   # filled from the goldens
+}
+
+
+def _case(name):
+    return [c for c in load_goldens() if c["name"] == name][0]
+
+
+def _water(m, seed=5):
+    from molds_amd import workloads as W
+    Z, x = W.water_cluster(m, seed=seed, jitter=0.02)
+    return Z, W.to_bohr(x)
+
+
+def _decane():
+    from molds_amd import workloads as W
+    Z, x = W.decane()
+    return Z, W.to_bohr(x)
+
+
+GEOMS = [("pm3", "c2h6"), ("am1", "ch3f"), ("mndo", "water4"), ("pm3", "decane")]
+
+
+def _geom(tag):
+    if tag == "c2h6":
+        c = _case("c2h6_pm3")
+        return np.array(c["Z"]), case_xyz_bohr(c)
+    if tag == "ch3f":
+        c = _case("ch3f_am1")
+        return np.array(c["Z"]), case_xyz_bohr(c)
+    if tag == "water4":
+        return _water(4)
+    return _decane()
+
+
+@pytest.mark.parametrize("theory,tag", GEOMS)
+def test_setup_stage_matches_oracle(engine, theory, tag):
+    from oracle import scf as O
+    Z, xyz = _geom(tag)
+    g, S, H, ecore = engine.dbg_integrals(theory, Z, xyz)
+    mol = O.Molecule(theory, Z, xyz)
+    g0 = mol.integrals()
+    assert np.max(np.abs(g - g0)) <= 1e-12 * np.max(np.abs(g0))
+    S0 = mol.overlap()
+    assert np.max(np.abs(S - S0)) <= 1e-12
+    H0 = mol.fock(np.zeros_like(S0), guess=False)
+    assert np.max(np.abs(H - H0)) <= 1e-12 * np.max(np.abs(H0))
+    assert abs(ecore - mol.ecore) <= 1e-12 * abs(mol.ecore)
+
+
+@pytest.mark.parametrize("theory,tag", GEOMS)
+def test_fock_matches_oracle(engine, theory, tag):
+    from oracle import scf as O
+    Z, xyz = _geom(tag)
+    mol = O.Molecule(theory, Z, xyz)
+    n = mol.nao
+    rng = np.random.default_rng(7)
+    P = rng.standard_normal((n, n)) * 0.3
+    P = P + P.T
+    for guess in (True, False):
+        F = engine.dbg_fock(theory, Z, xyz, P, guess=guess)
+        F0 = mol.fock(P, guess=guess)
+        assert np.max(np.abs(F - F0)) <= 1e-12 * np.max(np.abs(F0)), guess
+    # linearity in P: F(P1 + P2) - F(P2) = F(P1) - F(0)
+    P2 = np.eye(n) * 0.5
+    F12 = engine.dbg_fock(theory, Z, xyz, P + P2)
+    F1 = engine.dbg_fock(theory, Z, xyz, P)
+    F2 = engine.dbg_fock(theory, Z, xyz, P2)
+    H = engine.dbg_fock(theory, Z, xyz, np.zeros((n, n)))
+    assert np.max(np.abs((F12 - F2) - (F1 - H))) <= 1e-12 * np.max(np.abs(F1))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 17, 62, 130, 201, 384])
+def test_eigh(engine, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    if n > 3:
+        A[0, :] = A[:, 0] = 0.0
+        A[0, 0] = A[1, 1]          # an exact degeneracy and a decoupled row
+    eps, C, sweeps = engine.dbg_eigh(A)
+    w = np.linalg.eigvalsh(A)
+    nrm = np.linalg.norm(A)
+    assert sweeps > 0
+    assert np.max(np.abs(eps - w)) <= 1e-13 * nrm
+    assert np.all(np.diff(eps) >= 0)
+    assert np.max(np.abs(C @ A - eps[:, None] * C)) <= 1e-12 * nrm
+    assert np.max(np.abs(C @ C.T - np.eye(n))) <= 1e-12
+    assert np.all(C.sum(axis=1) >= 0)
+
+
+@pytest.mark.parametrize("n,nocc", [(8, 4), (62, 31), (130, 50), (200, 133)])
+def test_density(engine, n, nocc):
+    rng = np.random.default_rng(n)
+    C, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    P = engine.dbg_density(C, nocc)
+    P0 = 2.0 * C[:nocc].T @ C[:nocc]
+    assert np.max(np.abs(P - P0)) <= 1e-13
+    assert abs(np.trace(P) - 2 * nocc) <= 1e-11
+
+
+def test_scf_goldens_batched(engine):
+    """All distinct MolDS golden cases in ONE batch (one launch per SCF stage)."""
+    from molds_amd.nddo import Job
+    cases = _unique_cases()
+    jobs = [Job(c["theory"], c["Z"], case_xyz_bohr(c), **golden_opts(c)) for c in cases]
+    res = engine.scf_batch(jobs, want_traj=True)
+    for c, r in zip(cases, res):
+        e = c["expect"]
+        assert r["status"] == 0, (c["name"], r["status_name"])
+        assert abs(r["energy"] - e["energy"]) <= 5.01e-7 * abs(e["energy"]), c["name"]
+        assert abs(r["e_core"] - e["ecore"]) <= 5.01e-7 * abs(e["ecore"]), c["name"]
+        mo = np.array(e["mo"])
+        assert np.max(np.abs(r["eps"] - mo) / np.maximum(np.abs(mo), 1e-30)) <= 5.01e-7, c["name"]
+        if c["theory"] == "mndo":
+            assert abs(r["hof"] - e["hof"]) <= 5.01e-7 * abs(e["hof"]), c["name"]
+        if c["scf"]["rms_density"] >= 1e-8:
+            assert r["iterations"] == len(e["iters"]), c["name"]
+            for (it, rms, de, don, dmp), g in zip(r["traj"], e["iters"]):
+                assert abs(rms - g[1]) <= 5.01e-7 * g[1] + 1e-300, (c["name"], it)
+                assert abs(de - g[2]) <= 5.01e-7 * abs(g[2]) + 1e-300, (c["name"], it)
+                assert don == g[3] and dmp == g[4], (c["name"], it)
+
+
+def _oracle_solve(theory, Z, xyz, **o):
+    from oracle import scf as O
+    return O.scf(theory, Z, xyz, **o)
+
+
+@pytest.mark.parametrize("theory,tag", [("pm3", "decane"), ("am1", "water8"), ("mndo", "water4")])
+def test_scf_fp64_vs_oracle(engine, theory, tag):
+    if tag == "water8":
+        Z, xyz = _water(8)
+    elif tag == "water4":
+        Z, xyz = _water(4)
+    else:
+        Z, xyz = _decane()
+    opts = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0,
+                damping_weight=0.0, max_iter=200)
+    r = engine.scf(theory, Z, xyz, **opts)
+    o = _oracle_solve(theory, Z, xyz, rms_density=1e-8, diis_k=5, diis_start=0.1, diis_end=0.0,
+                      damping_weight=0.0, max_iter=200)
+    assert r["status"] == 0
+    assert abs(r["energy"] - o["energy"]) * EV <= TOL_E_EV
+    assert abs(r["e_core"] - o["e_core"]) * EV <= 1e-9
+    n = r["nao"]
+    assert np.sqrt(np.mean((r["P"] - o["P"]) ** 2)) <= 1e-8
+    assert np.max(np.abs(r["eps"] - o["eps"])) <= 1e-6
+    assert r["iterations"] == o["iterations"]
+    # idempotency of the closed-shell density in the orthogonal NDDO basis: P^2 = 2P
+    P = r["P"]
+    assert np.max(np.abs(P @ P - 2 * P)) <= 1e-6
+    assert abs(np.trace(P) - 2 * r["nocc"]) <= 1e-8
+    assert n == o["nao"]
+
+
+def test_alone_equals_mixed(engine):
+    """A job's result is the same solved alone and inside an uneven batch."""
+    from molds_amd import workloads as W
+    from molds_amd.nddo import Job
+    mols = [W.alkane(3, seed=11), W.water_cluster(6, seed=12, jitter=0.02), W.alkane(7, seed=13),
+            W.water_cluster(3, seed=14, jitter=0.02), W.alkane(1, seed=15)]
+    jobs = [Job("pm3", Z, W.to_bohr(x), rms_density=1e-8) for Z, x in mols]
+    mixed = engine.scf_batch(jobs)
+    for j, m in zip(jobs, mixed):
+        alone = engine.scf_batch([j])[0]
+        assert alone["status"] == 0 and m["status"] == 0
+        assert abs(alone["energy"] - m["energy"]) * EV <= 1e-9
+        assert alone["iterations"] == m["iterations"]
+
+
+def test_error_statuses(engine):
+    from molds_amd.nddo import Job
+    odd = engine.scf_batch([Job("pm3", [6, 1, 1, 1], np.eye(4, 3) * 2.0)])[0]
+    assert odd["status_name"] == "ODD_ELECTRONS"
+    bad = engine.scf_batch([Job("pm3", [16, 1, 1], np.eye(3) * 2.5)])[0]
+    assert bad["status_name"] == "UNSUPPORTED_ELEMENT"
+    c = _case("c2h6_pm3")
+    nc = engine.scf_batch([Job("pm3", c["Z"], case_xyz_bohr(c), max_iter=3, rms_density=1e-10)])[0]
+    assert nc["status_name"] == "NOT_CONVERGED" and nc["iterations"] == 3
