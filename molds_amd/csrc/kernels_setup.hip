@@ -154,7 +154,10 @@ __device__ double overlap_local(const ElemDev& A, const ElemDev& B, int la, int 
     int na = A.nshell, nb = B.nshell;
     int cid = ((((na - 1) * 2 + la) * 2 + (nb - 1)) * 2 + lb) * 2 + m;
     const double* c = &c_ovl[cid * 25];
-    double p = 0.5 * (za * r + zb * r), q = 0.5 * (za * r - zb * r);
+    // alpha = zeta_a R and beta = zeta_b R are rounded separately (no FMA), so
+    // equal exponents give q == 0 exactly and take the B_k(0) closed form
+    const double al = __dmul_rn(za, r), be = __dmul_rn(zb, r);
+    double p = 0.5 * (al + be), q = 0.5 * (al - be);
     double Ak[5], Bk[5];
     for (int k = 0; k < 5; ++k) { Ak[k] = aux_A(k, p); Bk[k] = aux_B(k, q); }
     double s = 0.0;
