@@ -8,7 +8,7 @@ Workload of one step: the batched PM3 mix -- 4096 neutral closed-shell
 molecules, even index an all-trans alkane C_nH_{2n+2} with n ~ U{1..40}, odd
 index a water cluster (H2O)_m with m ~ U{2..64}, seeded geometries with
 N(0, 0.02 A) jitter -- solved to density RMS < 1e-8 (core-Hamiltonian guess,
-DIIS k = 5 / start 0.1 / end 0, no damping) in ONE batch: one launch per SCF
+DIIS k = 5 / start 0.1 / end 0, damping 0.7 above RMS 0.02) in ONE batch: one launch per SCF
 stage for all molecules.  Inputs are resident on the device before the timed
 region (each step re-uploads the geometry -- a few MB -- and rebuilds all
 integrals on the GPU: that setup is part of every solve).
@@ -33,8 +33,12 @@ sys.path.insert(0, HERE)
 METRIC = "PM3 SCF solves/s + p50/p99 latency, mixed vs FP64 GPU at |ΔE|≤1e-5 eV"
 EV = 1.0 / 0.03674903
 FP64_PEAK_TFLOPS = 78.6       # MI355X dense FP64 (matrix and vector)
+# MolDS SCF knobs for the mix: DIIS as test/ch3f_pm3.in; damping 0.7 while RMS > 0.02
+# keeps the randomly oriented water clusters out of occupation-swapping cycles
 SCF = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0,
-           damping_weight=0.0, max_iter=200)
+           damping_weight=0.7, damping_thresh=0.02, max_iter=100)
+CPU_SCF = dict(rms_density=1e-8, diis_k=5, diis_start=0.1, diis_end=0.0, damping_weight=0.7, damping_thresh=0.02,
+               max_iter=100)
 
 
 def log(*a):
@@ -73,16 +77,19 @@ def cpu_leg(mols, seconds):
     except Exception:
         threads = 1
     t0 = time.perf_counter()
-    done = 0
+    done = conv = 0
     for kind, size, Z, x in mols:
-        O.scf("pm3", Z, np.asarray(x) * O.ANG2AU, rms_density=1e-8, diis_k=5, diis_start=0.1, diis_end=0.0,
-              damping_weight=0.0, max_iter=200)
+        try:
+            O.scf("pm3", Z, np.asarray(x) * O.ANG2AU, **CPU_SCF)
+            conv += 1
+        except O.OracleError:
+            pass
         done += 1
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
     return dict(value=done / dt, unit="solves/s", cores=threads, kind="port",
-                sample=f"first {done} molecules of the same PM3 mix, solved one at a time "
+                sample=f"first {done} molecules of the same PM3 mix ({conv} converged), solved one at a time "
                        f"(oracle: C Fock/integrals + numpy LAPACK dsyevd, {dt:.1f} s)")
 
 

@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "lib", "obj")
 LIB = os.path.join(PKG, "lib", "libnddo.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["kernels_setup.hip", "kernels_scf.hip", "engine.cpp", "server.cpp", "params.cpp"]
+SOURCES = ["kernels_setup.hip", "kernels_scf.hip", "kernels_eig.hip", "engine.cpp", "server.cpp", "params.cpp"]
 # Packed FP32 instructions are disabled for all device code (hardware notes).
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-I" + CSRC, "-I" + os.path.join(ROOT, "include")]

@@ -105,6 +105,8 @@ static void overlap_table(double* tab) {
 }
 
 size_t jacobi_lds_bytes(int n, bool a_in_lds) {
+    if (heig_supported(n)) return heig_lds_bytes(n, a_in_lds);
+    if (n <= 512) return jacobi_rows_lds_bytes(n, a_in_lds);
     const int nh = (n + (n & 1)) / 2;
     size_t b = (size_t)3 * nh * 8 + (size_t)(3 * nh + (nh & 1)) * 4;
     if (a_in_lds) b += (size_t)n * n * 8;
@@ -253,8 +255,8 @@ int Batch::classify(size_t lds_max) {
     std::sort(order.begin(), order.end(), [&](int a, int b) { return jobs[a].nao > jobs[b].nao; });
     for (int j : order) {
         int n = jobs[j].nao;
-        // the Jacobi kernel also holds 8 KB of static LDS (its reduction buffer)
-        if (lds_max > 9216 && jacobi_lds_bytes(n, true) <= lds_max - 9216) {
+        // the Jacobi kernels also hold up to 8 KB of static LDS (reduction buffers)
+        if (n <= 512 && lds_max > 9216 && jacobi_lds_bytes(n, true) <= lds_max - 9216) {
             list_lds.push_back(j);
             maxn_lds = std::max(maxn_lds, n);
         } else {
@@ -326,6 +328,7 @@ void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
 void Batch::launch_iteration(Engine* e, hipStream_t st) {
     launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints,
                 d_arena, d_part);
+    launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
     launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
                  jacobi_lds_bytes(maxn_lds, true), true);
     launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
@@ -360,6 +363,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
                         d_ints, d_arena, d_part);
             HIPCHK(hipEventRecord(e->ev[1], st));
+            launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
             launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
                          jacobi_lds_bytes(maxn_lds, true), true);
             launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
@@ -741,6 +745,7 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
     JobState s;
     HIPCHK(hipMemcpy(&s, b.d_state, sizeof(s), hipMemcpyDeviceToHost));
     if (sweeps) *sweeps = s.sweeps;
+    for (int i = 0; i < 4; ++i) e->last_eig_prof[i] = s.prof[i];
     if (C) {
         for (int r = 0; r < n; ++r) {
             const double* src = &VT[(long long)perm[r] * n];
@@ -751,6 +756,18 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
         }
     }
     if (s.sweeps < 0) { set_error("Jacobi did not converge"); return NDDO_EIGENSOLVER_FAILED; }
+    return NDDO_OK;
+}
+
+int nddo_dbg_eigh_profile(nddo_engine* eh, double* ticks4, double* tick_hz) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !ticks4) return NDDO_BAD_ARG;
+    for (int i = 0; i < 4; ++i) ticks4[i] = e->last_eig_prof[i];
+    if (tick_hz) {
+        int khz = 0;
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device);
+        *tick_hz = 1e3 * khz;
+    }
     return NDDO_OK;
 }
 

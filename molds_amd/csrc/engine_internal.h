@@ -83,6 +83,7 @@ struct Engine {
     double stage_ms[NDDO_STAGE_COUNT] = {};
     long long stage_launches[NDDO_STAGE_COUNT] = {};
     hipEvent_t ev[8] = {};
+    double last_eig_prof[4] = {};
 
     // serving state (server.cpp)
     std::mutex smu;

@@ -237,13 +237,19 @@ __global__ void __launch_bounds__(1024) k_jacobi(const JobDev* __restrict__ jobs
     double* VT = mat(arena, jb, AR_C);
     const long long nn = (long long)n * n;
 
+    // cold start: VT = I.  Warm start: F already holds VT_prev F VT_prev^T
+    // (k_warm_gemm) and VT_prev is the initial accumulator, so the rotations
+    // found here compose with it into the eigenvectors of F.
+    const bool warm = state[job].warm != 0;
     double fro = 0.0;
     for (long long e = tid; e < nn; e += T) {
         double v = F[e];
         A[e] = v;
         fro += v * v;
-        int i = (int)(e / n), j = (int)(e - (long long)i * n);
-        VT[e] = i == j ? 1.0 : 0.0;
+        if (!warm) {
+            int i = (int)(e / n), j = (int)(e - (long long)i * n);
+            VT[e] = i == j ? 1.0 : 0.0;
+        }
     }
     red[tid] = fro;
     __syncthreads();
@@ -342,22 +348,310 @@ __global__ void __launch_bounds__(1024) k_jacobi(const JobDev* __restrict__ jobs
         eps[jb.vec0 + rank] = d;
         perm[jb.ivec0 + rank] = p;
     }
-    if (tid == 0) state[job].sweeps = sweep < maxsweep ? sweep + 1 : -1;
+    if (tid == 0) {
+        state[job].sweeps = sweep < maxsweep ? sweep + 1 : -1;
+        state[job].warm = 1;
+    }
+}
+
+// Row-pair streaming Jacobi.  Same rotation sequence as k_jacobi (round-robin
+// pairs, Golub-Van Loan parameters), but each wavefront owns one pair (p, q)
+// per round and rewrites rows p and q completely: element (i, j), with i in
+// pair P and j in pair Q, becomes sum over i' in P, j' in Q of
+// M_P[i][i'] M_Q[j][j'] A[i'][j'], where M = [[c, -s], [s, c]].  Rows are read
+// and written with coalesced accesses and no mirroring; the four products are
+// rounded separately and summed as [self-self + other-other] +
+// [self-other + other-self], which is the same expression for (i, j) and
+// (j, i), so A stays exactly symmetric.
+// (Ain and Aout alias in the LDS case, so they carry no __restrict__.)
+// In global memory the matrix ping-pongs between two arena buffers (AR_W and
+// AR_F, which is free once copied), so a round reads one and writes the other
+// and no wavefront ever waits on its own stores.  In LDS the update is in
+// place: a wavefront's LDS reads of rows p, q precede its writes in program
+// order, and the LDS unit serves a wavefront's instructions in order.
+template <int NPL>
+__device__ inline void jacobi_pair_rows(const double* Ain, double* Aout, int n, int lane,
+                                        int p, int q, const double* ca, const double* cb, const int* partner,
+                                        int rot, double t, double app, double aqq, double apq) {
+    const bool vq = q < n;
+    const double cap = ca[p], cbp = cb[p], caq = ca[q], cbq = cb[q];
+    double np_[NPL], nq_[NPL];
+#pragma unroll
+    for (int u = 0; u < NPL; ++u) {
+        const int j = lane + 64 * u;
+        np_[u] = 0.0;
+        nq_[u] = 0.0;
+        if (j < n) {
+            const int oj = partner[j];
+            const bool voj = oj < n;
+            const double cj = ca[j], dj = cb[j];
+            const double a_pj = Ain[(long long)p * n + j];
+            const double a_po = voj ? Ain[(long long)p * n + oj] : 0.0;
+            const double a_qj = vq ? Ain[(long long)q * n + j] : 0.0;
+            const double a_qo = (vq && voj) ? Ain[(long long)q * n + oj] : 0.0;
+            np_[u] = (__dmul_rn(cap * cj, a_pj) + __dmul_rn(cbp * dj, a_qo)) +
+                     (__dmul_rn(cap * dj, a_po) + __dmul_rn(cbp * cj, a_qj));
+            nq_[u] = (__dmul_rn(caq * cj, a_qj) + __dmul_rn(cbq * dj, a_po)) +
+                     (__dmul_rn(caq * dj, a_qo) + __dmul_rn(cbq * cj, a_pj));
+            if (rot) {
+                if (j == p) { np_[u] = app - t * apq; nq_[u] = 0.0; }
+                if (j == q) { np_[u] = 0.0; nq_[u] = aqq + t * apq; }
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NPL; ++u) {
+        const int j = lane + 64 * u;
+        if (j < n) {
+            Aout[(long long)p * n + j] = np_[u];
+            if (vq) Aout[(long long)q * n + j] = nq_[u];
+        }
+    }
+}
+
+template <bool ALDS, int NPL>
+__global__ void __launch_bounds__(1024) k_jacobi_rows(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                                      const int* __restrict__ joblist, double* __restrict__ arena,
+                                                      double* __restrict__ eps, int* __restrict__ perm) {
+    extern __shared__ double dyn[];
+    __shared__ double red[32];
+    const int job = joblist[blockIdx.x];
+    const JobDev jb = jobs[job];
+    if (state[job].phase != PH_ACTIVE) return;
+    const int n = jb.nao, npad = n + (n & 1), nh = npad / 2, m = npad - 1;
+    const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
+    double* ca = dyn;            // [npad] self coefficient of index i
+    double* cb = ca + npad;      // [npad] partner coefficient of index i
+    double* kt = cb + npad;      // [nh] per pair: t, a_pp, a_qq, a_pq
+    double* kapp = kt + nh;
+    double* kaqq = kapp + nh;
+    double* kapq = kaqq + nh;
+    int* partner = (int*)(kapq + nh);   // [npad]
+    int* kp = partner + npad;           // [nh]
+    int* kq = kp + nh;
+    int* krot = kq + nh;
+    const int nint = npad + 3 * nh;
+    double* F = mat(arena, jb, AR_F);
+    double* Abuf[2];
+    Abuf[0] = ALDS ? (double*)(krot + nh + (nint & 1)) : mat(arena, jb, AR_W);
+    Abuf[1] = ALDS ? Abuf[0] : F;
+    double* VT = mat(arena, jb, AR_C);
+    const long long nn = (long long)n * n;
+    const bool warm = state[job].warm != 0;
+
+    double fro = 0.0;
+    for (long long e = tid; e < nn; e += T) {
+        const double v = F[e];
+        Abuf[0][e] = v;
+        fro += v * v;
+        if (!warm) {
+            const int i = (int)(e / n), j = (int)(e - (long long)i * n);
+            VT[e] = i == j ? 1.0 : 0.0;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) fro += __shfl_xor(fro, off);
+    if (lane == 0) red[wave] = fro;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += red[w];
+        red[31] = s;
+    }
+    __syncthreads();
+    const double thr = 1e-15 * sqrt(red[31]);
+
+    int cur = 0;
+    int sweep = 0;
+    const int maxsweep = 60;
+    for (; sweep < maxsweep; ++sweep) {
+        int rotated = 0;
+        for (int r = 0; r < m; ++r) {
+            const double* A = Abuf[cur];
+            int flag = 0;
+            for (int k = tid; k < nh; k += T) {
+                int p, q;
+                if (k == 0) { p = r; q = m; }
+                else { p = (r + k) % m; q = (r - k + m) % m; }
+                if (p > q) { int t = p; p = q; q = t; }
+                double c = 1.0, sn = 0.0, t = 0.0, app = 0.0, aqq = 0.0, apq = 0.0;
+                int rot = 0;
+                if (q < n) {
+                    apq = A[(long long)p * n + q];
+                    app = A[(long long)p * n + p];
+                    aqq = A[(long long)q * n + q];
+                    if (fabs(apq) > thr) {
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        sn = t * c;
+                        rot = 1;
+                    }
+                }
+                kp[k] = p; kq[k] = q; krot[k] = rot;
+                kt[k] = t; kapp[k] = app; kaqq[k] = aqq; kapq[k] = apq;
+                partner[p] = q; partner[q] = p;
+                ca[p] = c; cb[p] = -sn;
+                ca[q] = c; cb[q] = sn;
+                flag |= rot;
+            }
+            if (!__syncthreads_or(flag)) continue;
+            rotated = 1;
+            double* Aout = Abuf[cur ^ 1];
+            for (int k = wave; k < nh; k += nw) {
+                const int p = kp[k], q = kq[k], rot = krot[k];
+                jacobi_pair_rows<NPL>(A, Aout, n, lane, p, q, ca, cb, partner, rot, kt[k], kapp[k], kaqq[k],
+                                      kapq[k]);
+                if (rot) {
+                    const double c = ca[p], sn = cb[q];
+                    for (int j = lane; j < n; j += 64) {
+                        const double vp = VT[(long long)p * n + j], vv = VT[(long long)q * n + j];
+                        VT[(long long)p * n + j] = c * vp - sn * vv;
+                        VT[(long long)q * n + j] = sn * vp + c * vv;
+                    }
+                }
+            }
+            cur ^= ALDS ? 0 : 1;
+            __syncthreads();
+        }
+        if (!rotated) break;
+    }
+    const double* A = Abuf[cur];
+    for (int p = tid; p < n; p += T) {
+        const double d = A[(long long)p * n + p];
+        int rank = 0;
+        for (int q = 0; q < n; ++q) {
+            const double e = A[(long long)q * n + q];
+            rank += (e < d) || (e == d && q < p);
+        }
+        eps[jb.vec0 + rank] = d;
+        perm[jb.ivec0 + rank] = p;
+    }
+    if (tid == 0) {
+        state[job].sweeps = sweep < maxsweep ? sweep + 1 : -1;
+        state[job].warm = 1;
+    }
+}
+
+template <bool ALDS, int NPL>
+static void launch_rows(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
+                        int threads, double* arena, double* eps, int* perm, size_t lds_bytes) {
+    hipFuncSetAttribute((const void*)k_jacobi_rows<ALDS, NPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds_bytes);
+    hipLaunchKernelGGL((k_jacobi_rows<ALDS, NPL>), dim3(nlist), dim3(threads), lds_bytes, st, jobs, state, joblist,
+                       arena, eps, perm);
+}
+
+size_t jacobi_rows_lds_bytes(int n, bool a_in_lds) {
+    const int npad = n + (n & 1), nh = npad / 2, nint = npad + 3 * nh;
+    size_t b = (size_t)(2 * npad + 4 * nh) * 8 + (size_t)(nint + (nint & 1)) * 4;
+    if (a_in_lds) b += (size_t)n * n * 8;
+    return b;
 }
 
 void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
                   double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds) {
     if (nlist <= 0) return;
-    int threads = max_n <= 64 ? 256 : (max_n <= 160 ? 512 : 1024);
-    if (a_in_lds) {
-        hipFuncSetAttribute((const void*)k_jacobi<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        hipLaunchKernelGGL(k_jacobi<true>, dim3(nlist), dim3(threads), lds_bytes, st, jobs, state, joblist, arena,
-                           eps, perm);
-    } else {
-        hipFuncSetAttribute((const void*)k_jacobi<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        hipLaunchKernelGGL(k_jacobi<false>, dim3(nlist), dim3(threads), lds_bytes, st, jobs, state, joblist, arena,
-                           eps, perm);
+    if (heig_supported(max_n)) {
+        launch_heig_class(st, jobs, state, joblist, nlist, max_n, arena, eps, perm, a_in_lds);
+        return;
     }
+    launch_jacobi(st, jobs, state, joblist, nlist, max_n, arena, eps, perm, lds_bytes, a_in_lds);
+}
+
+// Cyclic Jacobi (kept for matrices beyond the one-workgroup Householder path).
+void launch_jacobi(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
+                   double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds) {
+    if (nlist <= 0) return;
+    const int threads = max_n <= 32 ? 256 : (max_n <= 64 ? 512 : 1024);
+    if (max_n <= 512) {
+        const size_t b = jacobi_rows_lds_bytes(max_n, a_in_lds);
+        if (a_in_lds) {
+            if (max_n <= 64) launch_rows<true, 1>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
+            else if (max_n <= 128) launch_rows<true, 2>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
+            else launch_rows<true, 4>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
+        } else {
+            if (max_n <= 128) launch_rows<false, 2>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
+            else if (max_n <= 256) launch_rows<false, 4>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
+            else if (max_n <= 384) launch_rows<false, 6>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
+            else launch_rows<false, 8>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
+        }
+        return;
+    }
+    // beyond 512 AOs: the 2x2-block kernel (global memory)
+    hipFuncSetAttribute((const void*)k_jacobi<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    hipLaunchKernelGGL(k_jacobi<false>, dim3(nlist), dim3(1024), lds_bytes, st, jobs, state, joblist, arena, eps,
+                       perm);
+}
+
+// ------------------------------------------------------------------ warm start
+// Rotates F into the previous MO basis before the Jacobi solve:
+//   pass 0:  W = F VT^T        pass 1:  F <- VT W      (= VT F VT^T)
+// Once the SCF settles, F in that basis is nearly diagonal and the cyclic
+// Jacobi needs one or two sweeps instead of a cold solve.  64x64 output tile
+// per workgroup, 16-deep K slabs staged in LDS, 4x4 outputs per thread.
+template <int PASS>
+__global__ void __launch_bounds__(256) k_warm_gemm(const TileDev* __restrict__ tiles, const JobDev* __restrict__ jobs,
+                                                   const JobState* __restrict__ state, double* __restrict__ arena) {
+    const TileDev tl = tiles[blockIdx.x];
+    const JobState& st = state[tl.job];
+    if (st.phase != PH_ACTIVE || !st.warm) return;
+    const JobDev jb = jobs[tl.job];
+    const int n = jb.nao;
+    double* F = mat(arena, jb, AR_F);
+    double* W = mat(arena, jb, AR_W);
+    const double* VT = mat(arena, jb, AR_C);
+    // PASS 0: out W,  a(i,k) = F[i][k],  b(k,j) = VT[j][k]
+    // PASS 1: out F,  a(i,k) = VT[i][k], b(k,j) = W[k][j]
+    const double* Am = PASS == 0 ? F : VT;
+    const double* Bm = PASS == 0 ? VT : W;
+    double* Out = PASS == 0 ? W : F;
+    __shared__ double sA[16][68], sB[16][68];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    double acc[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        for (int e = tid; e < 16 * 64; e += 256) {
+            // a(i,k): rows i of a row-major matrix, contiguous in k
+            const int ii = e >> 4, kk = e & 15, i = tl.i0 + ii, k = k0 + kk;
+            sA[kk][ii] = (i < n && k < n) ? Am[(long long)i * n + k] : 0.0;
+            if (PASS == 0) {
+                const int j = tl.j0 + ii;
+                sB[kk][ii] = (j < n && k < n) ? Bm[(long long)j * n + k] : 0.0;
+            }
+        }
+        if (PASS == 1) {
+            for (int e = tid; e < 16 * 64; e += 256) {
+                const int kk = e >> 6, jj = e & 63, k = k0 + kk, j = tl.j0 + jj;
+                sB[kk][jj] = (j < n && k < n) ? Bm[(long long)k * n + j] : 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            double av[4], bv[4];
+            for (int a = 0; a < 4; ++a) av[a] = sA[kk][ty * 4 + a];
+            for (int b = 0; b < 4; ++b) bv[b] = sB[kk][tx * 4 + b];
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+        }
+        __syncthreads();
+    }
+    for (int a = 0; a < 4; ++a) {
+        const int i = tl.i0 + ty * 4 + a;
+        if (i >= n) continue;
+        for (int b = 0; b < 4; ++b) {
+            const int j = tl.j0 + tx * 4 + b;
+            if (j < n) Out[(long long)i * n + j] = acc[a][b];
+        }
+    }
+}
+
+void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, const JobState* state,
+                 double* arena) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_warm_gemm<0>, dim3(ntiles), dim3(256), 0, st, tiles, jobs, state, arena);
+    hipLaunchKernelGGL(k_warm_gemm<1>, dim3(ntiles), dim3(256), 0, st, tiles, jobs, state, arena);
 }
 
 // ------------------------------------------------------------------ density

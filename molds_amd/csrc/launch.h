@@ -19,8 +19,20 @@ void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, con
                  const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
                  double* arena, double* part);
 
+// Warm start: F <- VT F VT^T for PH_ACTIVE jobs whose VT holds a previous solve.
+void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, const JobState* state,
+                 double* arena);
+
 // Eigen-decomposition of F for PH_ACTIVE jobs in joblist (one workgroup per job).
 // lds_ok: jobs whose n*n matrix fits the LDS budget take the LDS path.
+size_t jacobi_rows_lds_bytes(int n, bool a_in_lds);
+// Householder + QL eigensolver (kernels_eig.hip), one workgroup per job, n <= 512.
+size_t heig_lds_bytes(int n, bool a_in_lds);
+bool heig_supported(int n);
+void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
+                       double* arena, double* eps, int* perm, bool a_in_lds);
+void launch_jacobi(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
+                   double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds);
 void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
                   int max_n, double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds);
 

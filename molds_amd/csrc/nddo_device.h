@@ -78,6 +78,7 @@ struct JobState {
     int phase, iter, status, diis_on;
     int damp_on, ring_start, warm, sweeps;
     double rms, diis_error, e_el, e_core;
+    double prof[4];   // eigensolver phase times of the last solve (device clock ticks)
     double B[(NDDO_MAX_DIIS + 1) * (NDDO_MAX_DIIS + 1)];  // DIIS bordered matrix
 };
 

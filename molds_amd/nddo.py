@@ -81,6 +81,7 @@ def load_library(path=LIBPATH):
     L.nddo_dbg_fock.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, _dp]
     L.nddo_dbg_eigh.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _ip]
     L.nddo_dbg_density.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    L.nddo_dbg_eigh_profile.argtypes = [vp, _dp, _dp]
     _lib = L
     return L
 
@@ -292,6 +293,12 @@ class Engine:
         if rc != 0:
             raise NddoError(rc, last_error())
         return eps, C, sw.value
+
+    def dbg_eigh_profile(self):
+        t = np.zeros(4)
+        hz = ctypes.c_double()
+        self.L.nddo_dbg_eigh_profile(self.h, _p(t), ctypes.byref(hz))
+        return dict(zip(("tridiag", "ql", "apply", "backtransform"), (t / hz.value * 1e3).tolist()))
 
     def dbg_density(self, C, nocc):
         C = np.ascontiguousarray(C, dtype=np.float64)
