@@ -119,7 +119,7 @@ Batch::~Batch() { release(); }
 void Batch::release() {
     for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
                      (void**)&d_ints, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
-                     (void**)&d_perm, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
+                     (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
                      (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_sdbg}) {
         if (*p) hipFree(*p);
         *p = nullptr;
@@ -281,6 +281,7 @@ int Batch::allocate(bool want_sdbg) {
     HIPCHK(hipMalloc(&d_ecore, std::max<long long>(nslots, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_eps, std::max<long long>(nvec, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_perm, std::max<long long>(nvec, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&d_tri, 3 * std::max<long long>(nvec, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_rmspart, std::max<size_t>(tiles.size(), 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_traj, std::max<long long>(ntraj, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_remaining, sizeof(int)));
@@ -329,9 +330,9 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
     launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints,
                 d_arena, d_part);
     launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
-    launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
+    launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri,
                  jacobi_lds_bytes(maxn_lds, true), true);
-    launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
+    launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri,
                  jacobi_lds_bytes(maxn_glb, false), false);
     launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
     launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
@@ -364,9 +365,9 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
                         d_ints, d_arena, d_part);
             HIPCHK(hipEventRecord(e->ev[1], st));
             launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
-            launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
+            launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri,
                          jacobi_lds_bytes(maxn_lds, true), true);
-            launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
+            launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri,
                          jacobi_lds_bytes(maxn_glb, false), false);
             HIPCHK(hipEventRecord(e->ev[2], st));
             launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
@@ -732,9 +733,9 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
     const long long nn = (long long)n * n;
     HIPCHK(hipMemcpy(b.d_arena + (long long)AR_F * nn, F, nn * sizeof(double), hipMemcpyHostToDevice));
     launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_lds, (int)b.list_lds.size(), b.maxn_lds, b.d_arena, b.d_eps,
-                 b.d_perm, jacobi_lds_bytes(b.maxn_lds, true), true);
+                 b.d_perm, b.d_tri, jacobi_lds_bytes(b.maxn_lds, true), true);
     launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_glb, (int)b.list_glb.size(), b.maxn_glb, b.d_arena, b.d_eps,
-                 b.d_perm, jacobi_lds_bytes(b.maxn_glb, false), false);
+                 b.d_perm, b.d_tri, jacobi_lds_bytes(b.maxn_glb, false), false);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
     std::vector<int> perm(n);

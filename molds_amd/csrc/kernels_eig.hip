@@ -1,18 +1,23 @@
-// FP64 symmetric eigensolver for the SCF loop, one workgroup per molecule:
-//   1. Householder tridiagonalization T = Q^T F Q (Golub & Van Loan, Alg. 8.3.1),
-//      with each step's rank-2 update fused with the next step's matrix-vector
-//      product, so the trailing block is read and written once per step;
-//   2. implicit QL with Wilkinson shifts on T (Bowdler, Martin, Reinsch &
-//      Wilkinson, Numer. Math. 11, 293 (1968)): one thread runs the scalar
-//      recurrence and records the Givens rotations of each step in an LDS ring,
-//      then every thread applies them to its own column of the eigenvector
-//      rows (coalesced: thread r owns element r of every row);
-//   3. back-transformation of the eigenvector rows by the stored reflectors,
-//      staged through LDS in blocks.
-// It replaces LAPACKE_dsyevd (src/wrappers/Lapack.cpp:102-161).  Output: rows of
-// AR_C are eigenvectors (unsorted), eps[] ascending and perm[] the row of each
-// rank, exactly as the density kernel consumes them.  Matrices up to the LDS
-// budget are tridiagonalized in LDS; larger ones in the arena (AR_W).
+// FP64 symmetric eigensolver of the SCF loop (replaces LAPACKE_dsyevd,
+// src/wrappers/Lapack.cpp:102-161) as three batched kernels, one workgroup per
+// molecule each, for matrices up to 512 AOs:
+//   1. k_tridiag        Householder tridiagonalization T = Q^T F Q (Golub & Van
+//                       Loan, Alg. 8.3.1); each step's rank-2 update is fused
+//                       with the next step's matrix-vector product, so the
+//                       trailing block is read and written once per step.
+//   2. k_dc             divide and conquer on T (Cuppen, Numer. Math. 36, 177
+//                       (1981)), bottom-up from 1x1 leaves, all merges of a
+//                       level at once: deflation as in Dongarra & Sorensen
+//                       (SIAM J. Sci. Stat. Comput. 8, s139 (1987)), secular
+//                       roots by a two-pole rational model with a bisection
+//                       safeguard, eigenvector components by the Loewner
+//                       formula of Gu & Eisenstat (SIAM J. Matrix Anal. Appl.
+//                       16, 172 (1995)), and the merge products Q_new = Q U on
+//                       FP64 MFMA (v_mfma_f64_16x16x4f64).
+//   3. k_backtransform  eigenvector rows <- rows * H_{n-3} ... H_0, reflectors
+//                       staged through LDS in blocks.
+// Output: rows of AR_C are the eigenvectors, already in ascending eigenvalue
+// order; eps[] ascending and perm[] = identity, as the density kernel reads them.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -23,14 +28,16 @@ namespace nddo {
 
 namespace {
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 __device__ inline double* amat(double* arena, const JobDev& jb, int kind) {
     return arena + jb.arena + (long long)kind * jb.nao * jb.nao;
 }
 
-constexpr int kRec = 2048;       // rotation record ring (doubles)
-constexpr int kBlk = 16;         // reflectors staged per back-transform block
+constexpr int kBlk = 16;                               // reflectors staged per back-transform block
+constexpr double kEps = 1.1102230246251565e-16;        // 2^-53, unit roundoff
 
-// Block-wide sum; every thread gets the result.  red has >= 17 doubles.
+// Block-wide sum; every thread gets the result.
 __device__ inline double block_sum(double v, double* red, int tid, int nw) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     __syncthreads();
@@ -43,56 +50,51 @@ __device__ inline double block_sum(double v, double* red, int tid, int nw) {
 
 }  // namespace
 
-size_t heig_lds_bytes(int n, bool a_in_lds) {
-    size_t d = 8 * (size_t)n + kRec;        // d e tau vcur vnext w p pnext + rotation ring
+size_t tridiag_lds_bytes(int n, bool a_in_lds) {
+    size_t d = 5 * (size_t)n;   // vcur vnext w p pnext
     if (a_in_lds) d += (size_t)n * n;
-    else d += (size_t)kBlk * n;             // reflector staging
     return d * 8;
 }
 
-template <bool ALDS, int NPL>
-__global__ void __launch_bounds__(1024) k_heig(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
-                                               const int* __restrict__ joblist, double* __restrict__ arena,
-                                               double* __restrict__ eps, int* __restrict__ perm) {
+size_t dc_lds_bytes(int n) { return (size_t)n * (12 * 8 + 9 * 4); }
+
+// ------------------------------------------------------------------ 1. tridiagonalize
+// Reflector k acts on indices k+1..n-1: H_k = I - tau_k v v^T with v[0] = 1,
+// kept in row k of W (columns k+1..n-1).  tri (per job, 3n): d[n], e[n], tau[n].
+template <bool ALDS>
+__global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                                  const int* __restrict__ joblist, double* __restrict__ arena,
+                                                  double* __restrict__ tri) {
     extern __shared__ double dyn[];
     __shared__ double red[32];
-    __shared__ int sh[4];
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
+    const unsigned long long t0 = wall_clock64();
     const int n = jb.nao;
     const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
-    double* d = dyn;
-    double* e = d + n;
-    double* tau = e + n;
-    double* vcur = tau + n;
+    double* vcur = dyn;
     double* vnext = vcur + n;
     double* w = vnext + n;
     double* p = w + n;
     double* pnext = p + n;
-    double* rec = pnext + n;
-    double* stage = rec + kRec;
-    double* A = ALDS ? stage : amat(arena, jb, AR_W);
+    double* W = amat(arena, jb, AR_W);
+    double* A = ALDS ? pnext + n : W;
     const double* F = amat(arena, jb, AR_F);
-    double* ZT = amat(arena, jb, AR_C);
+    double* d = tri + 3 * jb.vec0;
+    double* e = d + n;
+    double* tau = e + n;
     const long long nn = (long long)n * n;
-
-    const unsigned long long t0 = wall_clock64();
-    for (long long x = tid; x < nn; x += T) {
-        A[x] = F[x];
-        const int i = (int)(x / n), j = (int)(x - (long long)i * n);
-        ZT[x] = i == j ? 1.0 : 0.0;
-    }
-    for (int i = tid; i < n; i += T) e[i] = 0.0;
+    for (long long x = tid; x < nn; x += T) A[x] = F[x];
+    for (int i = tid; i < n; i += T) { e[i] = 0.0; tau[i] = 0.0; }
     __syncthreads();
-
-    // ---------------------------------------------------------------- 1. tridiagonalize
-    // Reflector k acts on indices k+1..n-1: H_k = I - tau_k v v^T, v[0] = 1,
-    // stored in row k of A (columns k+1..n-1).  Vectors below are indexed from
-    // k+1 (t = index - k - 1).
-    if (n >= 3) {
-        // reflector of row 0 and p = tau B v for B = A[1:, 1:]
-        {
+    if (n < 3) {
+        if (tid == 0) {
+            d[0] = A[0];
+            if (n == 2) { e[0] = A[1]; d[1] = A[3]; }
+        }
+    } else {
+        {   // reflector of row 0 and p = tau B v for B = A[1:, 1:]
             const int m = n - 1;
             double ss = 0.0;
             for (int t = 1 + tid; t < m; t += T) { const double x = A[1 + t]; ss += x * x; }
@@ -120,6 +122,7 @@ __global__ void __launch_bounds__(1024) k_heig(const JobDev* __restrict__ jobs, 
             }
             __syncthreads();
         }
+        // vectors below are indexed from k+1 (t = index - k - 1)
         for (int k = 0; k <= n - 3; ++k) {
             const int m = n - 1 - k;
             const double tk = tau[k];
@@ -129,25 +132,18 @@ __global__ void __launch_bounds__(1024) k_heig(const JobDev* __restrict__ jobs, 
             const double K = -0.5 * tk * pv;
             for (int t = tid; t < m; t += T) w[t] = p[t] + K * vcur[t];
             __syncthreads();
-            if (k == n - 3) {
-                // last trailing 2x2 block (rows/cols n-2, n-1)
+            if (k == n - 3) {   // last trailing 2x2 block (rows/cols n-2, n-1)
                 if (tid == 0) {
-                    const double a00 = A[(long long)(n - 2) * n + n - 2] - 2.0 * vcur[0] * w[0];
-                    const double a01 = A[(long long)(n - 2) * n + n - 1] - (vcur[0] * w[1] + w[0] * vcur[1]);
-                    const double a11 = A[(long long)(n - 1) * n + n - 1] - 2.0 * vcur[1] * w[1];
-                    d[n - 2] = a00;
-                    e[n - 2] = a01;
-                    d[n - 1] = a11;
+                    d[n - 2] = A[(long long)(n - 2) * n + n - 2] - 2.0 * vcur[0] * w[0];
+                    e[n - 2] = A[(long long)(n - 2) * n + n - 1] - (vcur[0] * w[1] + w[0] * vcur[1]);
+                    d[n - 1] = A[(long long)(n - 1) * n + n - 1] - 2.0 * vcur[1] * w[1];
                 }
-                __syncthreads();
                 break;
             }
-            // row k+1 (t = 0) of the updated trailing block
-            const long long r1 = (long long)(k + 1) * n;
+            const long long r1 = (long long)(k + 1) * n;   // row k+1 of the updated block
             for (int t = tid; t < m; t += T) A[r1 + k + 1 + t] -= vcur[0] * w[t] + w[0] * vcur[t];
             __syncthreads();
-            // reflector of row k+1: x = A[k+1][k+2 .. n-1]
-            const int m2 = m - 1;
+            const int m2 = m - 1;   // reflector of row k+1: x = A[k+1][k+2 .. n-1]
             double ss = 0.0;
             for (int t = 1 + tid; t < m2; t += T) { const double x = A[r1 + k + 2 + t]; ss += x * x; }
             ss = block_sum(ss, red, tid, nw);
@@ -185,227 +181,485 @@ __global__ void __launch_bounds__(1024) k_heig(const JobDev* __restrict__ jobs, 
             double* tmp = vcur; vcur = vnext; vnext = tmp;
             tmp = p; p = pnext; pnext = tmp;
         }
+        if (ALDS) {   // keep the reflectors (upper rows) for the back-transformation
+            __syncthreads();
+            for (long long x = tid; x < nn; x += T) W[x] = A[x];
+        }
+    }
+    if (tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
+}
+
+// ------------------------------------------------------------------ 2. divide and conquer
+// Root j (0-based) of the secular equation 1 + rho sum_t z_t^2 / (dl_t - x) = 0
+// for sorted dl[0..kk), rho > 0: x = dl[o] + tau with o the nearer pole, so
+// that dl_t - x = (dl_t - dl_o) - tau keeps its relative accuracy.
+__device__ void dc_secular(const double* dl, const double* zl, int kk, double rho, int j, int* orig,
+                           double* taup) {
+    int o;
+    double lo, hi;
+    if (j < kk - 1) {
+        const double mid = 0.5 * (dl[j + 1] - dl[j]);
+        double f = 0.0;
+        for (int t = 0; t < kk; ++t) f += zl[t] * zl[t] / ((dl[t] - dl[j]) - mid);
+        f = 1.0 + rho * f;
+        if (f >= 0.0) { o = j; lo = 0.0; hi = mid; }
+        else { o = j + 1; lo = -mid; hi = 0.0; }
     } else {
-        if (tid == 0) {
-            d[0] = A[0];
-            if (n == 2) { e[0] = A[1]; d[1] = A[3]; }
-        }
-        __syncthreads();
+        double zz = 0.0;
+        for (int t = 0; t < kk; ++t) zz += zl[t] * zl[t];
+        o = kk - 1; lo = 0.0; hi = rho * zz;
     }
-
-    const unsigned long long t1 = wall_clock64();
-    unsigned long long tql = 0, tapply = 0;
-    // ---------------------------------------------------------------- 2. implicit QL
-    // e[i] couples i and i+1; e[n-1] = 0.  Thread 0 runs the recurrence and
-    // records (m, count, c, s, ...) per QL step; the block applies a full ring.
-    if (tid == 0) { sh[0] = 0; sh[1] = 0; sh[2] = 0; sh[3] = 0; }   // l, done, failed, qlsteps
-    __syncthreads();
-    int l_state = 0;
-    while (true) {
-        const unsigned long long ta = wall_clock64();
-        if (tid == 0) {
-            int pos = 0;
-            int l = sh[0];
-            bool failed = false;
-            int steps = sh[3];
-            while (l < n) {
-                int it = 0;
-                bool deflated = false;
-                while (true) {
-                    int m = l;
-                    for (; m < n - 1; ++m) {
-                        const double dd = fabs(d[m]) + fabs(d[m + 1]);
-                        if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
-                    }
-                    if (m == l) { deflated = true; break; }
-                    // the ring's last slot holds its used length; n <= 512 keeps a step < kRec - 1
-                    if (pos + 2 + 2 * (m - l) > kRec - 1) break;     // ring full: apply first
-                    if (++it > 60) { failed = true; break; }
-                    ++steps;
-                    double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                    double r = hypot(g, 1.0);
-                    g = d[m] - d[l] + e[l] / (g + copysign(r, g));
-                    double s = 1.0, c = 1.0, pp = 0.0;
-                    int i = m - 1, cnt = 0;
-                    const int hdr = pos;
-                    pos += 2;
-                    bool underflow = false;
-                    // d[i+1] rides in a register (it is d[i] of the previous index);
-                    // e[i], d[i] are fetched one index ahead of their use
-                    double dip1 = d[m], ei = e[i], di = d[i];
-                    for (; i >= l; --i) {
-                        const double ein = i > l ? e[i - 1] : 0.0, din = i > l ? d[i - 1] : 0.0;
-                        const double f = s * ei, b = c * ei;
-                        r = sqrt(f * f + g * g);
-                        e[i + 1] = r;
-                        if (r == 0.0) { d[i + 1] = dip1 - pp; e[m] = 0.0; underflow = true; break; }
-                        const double ir = 1.0 / r;
-                        s = f * ir;
-                        c = g * ir;
-                        g = dip1 - pp;
-                        r = (di - g) * s + 2.0 * c * b;
-                        pp = s * r;
-                        d[i + 1] = g + pp;
-                        g = c * r - b;
-                        rec[pos] = c;
-                        rec[pos + 1] = s;
-                        pos += 2;
-                        ++cnt;
-                        dip1 = di;
-                        ei = ein;
-                        di = din;
-                    }
-                    rec[hdr] = (double)m;
-                    rec[hdr + 1] = (double)cnt;
-                    if (underflow) continue;
-                    d[l] -= pp;
-                    e[l] = g;
-                    e[m] = 0.0;
-                }
-                if (failed || !deflated) break;
-                ++l;
-            }
-            sh[0] = l;
-            sh[1] = (l >= n || failed) ? 1 : 0;
-            sh[2] = failed ? 1 : 0;
-            sh[3] = steps;
-            rec[kRec - 1] = (double)pos;   // used length (the ring never fills its last slot)
+    const double dO = dl[o];
+    double tau = 0.5 * (lo + hi);
+    for (int it = 0; it < 100; ++it) {
+        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, asum = 0.0;
+        for (int t = 0; t < kk; ++t) {
+            const double inv = 1.0 / ((dl[t] - dO) - tau);
+            const double term = zl[t] * zl[t] * inv, dterm = term * inv;
+            if (t <= j) { psi += term; dpsi += dterm; }
+            else { phi += term; dphi += dterm; }
+            asum += fabs(term);
         }
-        __syncthreads();
-        const unsigned long long tb = wall_clock64();
-        tql += tb - ta;
-        const int used = (int)rec[kRec - 1];
-        // apply the recorded rotations: thread r owns column r of the eigenvector rows
-        for (int r = tid; r < n; r += T) {
-            int q = 0;
-            while (q < used) {
-                const int m = (int)rec[q], cnt = (int)rec[q + 1];
-                q += 2;
-                double val = ZT[(long long)m * n + r];
-                double zi = cnt > 0 ? ZT[(long long)(m - 1) * n + r] : 0.0;
-                for (int t = 0; t < cnt; ++t) {
-                    const double c = rec[q + 2 * t], s = rec[q + 2 * t + 1];
-                    const int i = m - 1 - t;
-                    // fetch the next row's element before this step's store
-                    const double zn = t + 1 < cnt ? ZT[(long long)(i - 1) * n + r] : 0.0;
-                    ZT[(long long)(i + 1) * n + r] = s * zi + c * val;
-                    val = c * zi - s * val;
-                    zi = zn;
+        psi *= rho; dpsi *= rho; phi *= rho; dphi *= rho;
+        const double f = 1.0 + psi + phi;
+        if (fabs(f) <= 8.0 * kEps * (1.0 + rho * asum)) break;
+        if (f < 0.0) lo = tau;
+        else hi = tau;
+        double eta = 0.0;
+        bool have = false;
+        if (j < kk - 1) {
+            // psi ~ a1 + b1/(p_j - x), phi ~ a2 + b2/(p_j+1 - x), matched in value and slope at tau
+            const double u1 = (dl[j] - dO) - tau, u2 = (dl[j + 1] - dO) - tau;
+            const double b1 = u1 * u1 * dpsi, b2 = u2 * u2 * dphi;
+            const double c = 1.0 + (psi - b1 / u1) + (phi - b2 / u2);
+            const double qa = c, qb = -(c * (u1 + u2) + b1 + b2), qc = c * u1 * u2 + b1 * u2 + b2 * u1;
+            if (qa != 0.0) {
+                const double disc = qb * qb - 4.0 * qa * qc;
+                if (disc >= 0.0) {
+                    const double q = -0.5 * (qb + copysign(sqrt(disc), qb));
+                    const double e2 = q / qa;
+                    const double e1 = q != 0.0 ? qc / q : e2;
+                    eta = fabs(e1) < fabs(e2) ? e1 : e2;
+                    have = true;
                 }
-                ZT[(long long)(m - cnt) * n + r] = val;
-                q += 2 * cnt;
+            } else if (qb != 0.0) {
+                eta = -qc / qb;
+                have = true;
             }
+        } else {
+            const double u1 = -tau;   // the last pole is the origin
+            const double b1 = u1 * u1 * dpsi;
+            const double c = f - b1 / u1;
+            if (c != 0.0) { eta = u1 + b1 / c; have = true; }
         }
-        __syncthreads();
-        tapply += wall_clock64() - tb;
-        const int done = sh[1];
-        __syncthreads();   // all threads read the flag before thread 0 starts the next segment
-        if (done) break;
-        ++l_state;
+        if (have && fabs(eta) <= 2.0 * kEps * fabs(tau)) { tau += eta; break; }
+        const double nt = (have && lo < tau + eta && tau + eta < hi) ? tau + eta : 0.5 * (lo + hi);
+        if (fabs(nt - tau) <= 2.0 * kEps * fabs(nt) || hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi))) {
+            tau = nt;
+            break;
+        }
+        tau = nt;
     }
-    const bool failed = sh[2] != 0;
+    *orig = o;
+    *taup = tau;
+}
 
-    // ---------------------------------------------------------------- 3. back-transform
-    // row r of the eigenvectors <- row r * H_{n-3} * ... * H_0, reflectors in
-    // blocks of kBlk staged in LDS (or read in place when A lives in LDS);
-    // each wavefront holds one row in registers.
-    if (n >= 3) {
-        const int nref = n - 2;
-        for (int r0 = 0; r0 < n; r0 += nw) {
-            const int r = r0 + wave;
-            double row[NPL];
-#pragma unroll
-            for (int u = 0; u < NPL; ++u) {
-                const int j = lane + 64 * u;
-                row[u] = (r < n && j < n) ? ZT[(long long)r * n + j] : 0.0;
-            }
-            for (int kb = nref - 1; kb >= 0; kb -= kBlk) {
-                const int kl = kb - kBlk + 1 > 0 ? kb - kBlk + 1 : 0;   // block [kl, kb], applied kb first
-                const double* vb;
-                if (ALDS) {
-                    vb = A;
-                } else {
-                    __syncthreads();
-                    for (int x = tid; x < (kb - kl + 1) * n; x += T) {
-                        const int kk = kl + x / n, j = x % n;
-                        stage[x] = j > kk ? A[(long long)kk * n + j] : 0.0;
-                    }
-                    __syncthreads();
-                    vb = stage;
-                }
-                for (int k = kb; k >= kl; --k) {
-                    const double* v = ALDS ? vb + (long long)k * n : vb + (long long)(k - kl) * n;
-                    double s = 0.0;
-#pragma unroll
-                    for (int u = 0; u < NPL; ++u) {
-                        const int j = lane + 64 * u;
-                        if (j > k && j < n) s += row[u] * v[j];
-                    }
-                    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-                    s *= tau[k];
-#pragma unroll
-                    for (int u = 0; u < NPL; ++u) {
-                        const int j = lane + 64 * u;
-                        if (j > k && j < n) row[u] -= s * v[j];
-                    }
-                }
-            }
-            if (r < n) {
-#pragma unroll
-                for (int u = 0; u < NPL; ++u) {
-                    const int j = lane + 64 * u;
-                    if (j < n) ZT[(long long)r * n + j] = row[u];
-                }
-            }
+// Rows of QT (AR_C / AR_Q2, ping-pong) are eigenvectors of the current blocks;
+// the block [a, b) of level l only touches rows and columns a..b-1.  U (AR_U)
+// holds each merge's kk x kk eigenvector matrix in its diagonal block.
+__global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                             const int* __restrict__ joblist, double* __restrict__ arena,
+                                             const double* __restrict__ tri, double* __restrict__ eps,
+                                             int* __restrict__ perm) {
+    extern __shared__ double dyn[];
+    const int job = joblist[blockIdx.x];
+    const JobDev jb = jobs[job];
+    if (state[job].phase != PH_ACTIVE) return;
+    const unsigned long long t0 = wall_clock64();
+    const int n = jb.nao;
+    const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
+    double* Dc = dyn;            // sorted eigenvalues of each current block
+    double* ds = Dc + n;         // merged d, z in sorted order (per merge at [a, b))
+    double* zs = ds + n;
+    double* dl = zs + n;         // non-deflated first (kk of them), then deflated
+    double* zl = dl + n;
+    double* vals = zl + n;       // new eigenvalues: kk roots, then the deflated values
+    double* rtau = vals + n;     // root offset from its pole
+    double* zhat = rtau + n;
+    double* cnorm = zhat + n;    // inverse column norms of U
+    double* rc = cnorm + n;      // deflation rotations (c, s)
+    double* rs = rc + n;
+    double* rhoq = rs + n;       // per merge
+    int* col = (int*)(rhoq + n); // QT row of each sorted entry
+    int* flag = col + n;
+    int* cl = flag + n;          // QT row of each dl entry
+    int* rorig = cl + n;         // pole index (into dl) of each root
+    int* rank = rorig + n;       // position of each vals entry in the sorted block
+    int* r1 = rank + n;          // deflation rotation rows
+    int* r2 = r1 + n;
+    int* kkq = r2 + n;           // per merge: kk (-1: block carried to the next level)
+    int* nrot = kkq + n;
+    const double* d = tri + 3 * jb.vec0;
+    const double* e = d + n;
+    double* QC = amat(arena, jb, AR_C);
+    double* QX = amat(arena, jb, AR_Q2);
+    double* U = amat(arena, jb, AR_U);
+    int L = 0;
+    while ((1 << L) < n) ++L;
+    {
+        double* Q0 = (L & 1) ? QX : QC;
+        for (int i = tid; i < n; i += T) {
+            // T = diag(T_1, T_2) + |e| u u^T at every cut: both diagonal ends lose |e|
+            const double el = i > 0 ? fabs(e[i - 1]) : 0.0, er = i < n - 1 ? fabs(e[i]) : 0.0;
+            Dc[i] = d[i] - el - er;
+            Q0[(long long)i * n + i] = 1.0;
         }
     }
     __syncthreads();
-    // eigenvalues ascending (ties by index) and the row permutation
-    for (int pidx = tid; pidx < n; pidx += T) {
-        const double dv = d[pidx];
-        int rank = 0;
-        for (int q = 0; q < n; ++q) {
-            const double ev = d[q];
-            rank += (ev < dv) || (ev == dv && q < pidx);
+    for (int l = 1; l <= L; ++l) {
+        const int s = 1 << l, h = s >> 1, nm = (n + s - 1) >> l;
+        double* QO = ((L - l + 1) & 1) ? QX : QC;
+        double* QN = ((L - l) & 1) ? QX : QC;
+        // A: z = [last row of Q_1; sign(e) first row of Q_2] / sqrt(2); stable merge of the two sorted blocks
+        for (int i = tid; i < n; i += T) {
+            const int a = (i >> l) << l, m = min(a + h, n), b = min(a + s, n);
+            const long long ri = (long long)i * n;
+            if (m == b) {   // unpaired last block: carried as it is
+                for (int c = a; c < b; ++c) QN[ri + c] = QO[ri + c];
+                continue;
+            }
+            double z = i < m ? QO[ri + m - 1] : QO[ri + m];
+            if (i >= m && e[m - 1] < 0.0) z = -z;
+            z *= 0.70710678118654752440;
+            const double di = Dc[i];
+            int pos;
+            if (i < m) {
+                int lo = m, hi = b;   // right-block entries < di
+                while (lo < hi) { const int md = (lo + hi) >> 1; if (Dc[md] < di) lo = md + 1; else hi = md; }
+                pos = (i - a) + (lo - m);
+            } else {
+                int lo = a, hi = m;   // left-block entries <= di
+                while (lo < hi) { const int md = (lo + hi) >> 1; if (Dc[md] <= di) lo = md + 1; else hi = md; }
+                pos = (i - m) + (lo - a);
+            }
+            ds[a + pos] = di;
+            zs[a + pos] = z;
+            col[a + pos] = i;
         }
-        eps[jb.vec0 + rank] = dv;
-        perm[jb.ivec0 + rank] = pidx;
+        // the merged block of QO is diag(Q_1, Q_2): clear its two off-diagonal blocks
+        for (long long x = tid; x < ((long long)n << (l - 1)); x += T) {
+            const int i = (int)(x >> (l - 1)), c0 = (int)(x & (h - 1));
+            const int a = (i >> l) << l, m = min(a + h, n), b = min(a + s, n);
+            if (m == b) continue;
+            const int c = i < m ? m + c0 : a + c0;
+            if (c < (i < m ? b : m)) QO[(long long)i * n + c] = 0.0;
+        }
+        __syncthreads();
+        // B: deflation, one thread per merge (small |z| and close d pairs)
+        for (int q = tid; q < nm; q += T) {
+            const int a = q << l, m = min(a + h, n), b = min(a + s, n);
+            if (m == b) { kkq[q] = -1; continue; }
+            const double rho = 2.0 * fabs(e[m - 1]);
+            double dmax = 0.0, zmax = 0.0;
+            for (int j = a; j < b; ++j) { dmax = fmax(dmax, fabs(ds[j])); zmax = fmax(zmax, fabs(zs[j])); }
+            const double tol = 8.0 * kEps * fmax(dmax, zmax);
+            int nr = 0;
+            if (rho * zmax <= tol) {
+                for (int j = a; j < b; ++j) flag[j] = 1;
+            } else {
+                int pj = -1;
+                for (int j = a; j < b; ++j) {
+                    flag[j] = 0;
+                    if (rho * fabs(zs[j]) <= tol) { flag[j] = 1; continue; }
+                    if (pj < 0) { pj = j; continue; }
+                    double S = zs[pj], C = zs[j];
+                    const double tz = sqrt(C * C + S * S);
+                    const double t = ds[j] - ds[pj];
+                    C /= tz;
+                    S = -S / tz;
+                    if (fabs(t * C * S) <= tol) {
+                        zs[j] = tz;
+                        zs[pj] = 0.0;
+                        r1[a + nr] = col[pj];
+                        r2[a + nr] = col[j];
+                        rc[a + nr] = C;
+                        rs[a + nr] = S;
+                        ++nr;
+                        const double dp = ds[pj], dj = ds[j];
+                        ds[pj] = dp * C * C + dj * S * S;
+                        ds[j] = dp * S * S + dj * C * C;
+                        flag[pj] = 1;
+                    }
+                    pj = j;
+                }
+            }
+            int k = 0;
+            for (int j = a; j < b; ++j)
+                if (!flag[j]) { dl[a + k] = ds[j]; zl[a + k] = zs[j]; cl[a + k] = col[j]; ++k; }
+            int t = k;
+            for (int j = a; j < b; ++j)
+                if (flag[j]) { dl[a + t] = ds[j]; zl[a + t] = 0.0; cl[a + t] = col[j]; ++t; }
+            kkq[q] = k;
+            nrot[q] = nr;
+            rhoq[q] = rho;
+        }
+        __syncthreads();
+        // C: deflation rotations on the rows of QO (thread = column); D: secular roots
+        for (int i = tid; i < n; i += T) {
+            const int q = i >> l, a = q << l;
+            const int kk = kkq[q];
+            if (kk < 0) continue;
+            const int nr = nrot[q];
+            for (int t = 0; t < nr; ++t) {
+                const long long p1 = (long long)r1[a + t] * n + i, p2 = (long long)r2[a + t] * n + i;
+                const double x = QO[p1], y = QO[p2], C = rc[a + t], S = rs[a + t];
+                QO[p1] = C * x + S * y;
+                QO[p2] = C * y - S * x;
+            }
+            const int j = i - a;
+            if (j < kk) {
+                int o;
+                double tau;
+                dc_secular(dl + a, zl + a, kk, rhoq[q], j, &o, &tau);
+                rorig[i] = a + o;
+                rtau[i] = tau;
+                vals[i] = dl[a + o] + tau;
+            } else {
+                vals[i] = dl[i];
+            }
+        }
+        __syncthreads();
+        // E: Loewner z-hat (Gu & Eisenstat) and the rank of every new eigenvalue in its block
+        for (int i = tid; i < n; i += T) {
+            const int q = i >> l, a = q << l;
+            const int kk = kkq[q];
+            if (kk < 0) continue;
+            const int b = min(a + s, n), j = i - a;
+            if (j < kk) {
+                const double di = dl[i];
+                double wv = (di - dl[rorig[i]]) - rtau[i];
+                for (int t = 0; t < kk; ++t) {
+                    if (t == j) continue;
+                    const int ct = a + t;
+                    wv *= ((di - dl[rorig[ct]]) - rtau[ct]) / (di - dl[ct]);
+                }
+                zhat[i] = copysign(sqrt(fmax(-wv, 0.0)), zl[i]);
+            }
+            const double v = vals[i];
+            int rk = 0;
+            for (int t = a; t < b; ++t) {
+                const double u = vals[t];
+                rk += (u < v) || (u == v && t < i);
+            }
+            rank[i] = rk;
+        }
+        __syncthreads();
+        // F: inverse norms of the columns of U
+        for (int i = tid; i < n; i += T) {
+            const int q = i >> l, a = q << l;
+            const int kk = kkq[q];
+            if (i - a >= kk) continue;
+            const double dO = dl[rorig[i]], tj = rtau[i];
+            double s2 = 0.0;
+            for (int t = 0; t < kk; ++t) {
+                const double u = zhat[a + t] / ((dl[a + t] - dO) - tj);
+                s2 += u * u;
+            }
+            cnorm[i] = 1.0 / sqrt(s2);
+        }
+        __syncthreads();
+        // G: U[t][j] = zhat_t / (dl_t - lambda_j), columns normalized
+        const long long tot = (long long)n << l;
+        for (long long x = tid; x < tot; x += T) {
+            const int i = (int)(x >> l), j = (int)(x & (s - 1));
+            const int q = i >> l, a = q << l, kk = kkq[q];
+            if (i - a >= kk || j >= kk) continue;
+            const int cj = a + j;
+            U[(long long)i * n + cj] = zhat[i] / ((dl[i] - dl[rorig[cj]]) - rtau[cj]) * cnorm[cj];
+        }
+        __syncthreads();
+        // H: new eigenvector rows QN[a + rank] = U^T QO[cl]; deflated rows copied; sorted eigenvalues
+        const bool small = l < 5;
+        for (long long x = tid; x < tot; x += T) {
+            const int u = (int)(x >> l), rr = (int)(x & (s - 1));
+            const int q = u >> l, a = q << l, kk = kkq[q];
+            if (kk < 0) continue;
+            const int b = min(a + s, n), r = a + rr, j = u - a;
+            if (r >= b) continue;
+            double v;
+            if (j < kk) {
+                if (!small) continue;
+                v = 0.0;
+                for (int t = 0; t < kk; ++t) v += U[(long long)(a + t) * n + u] * QO[(long long)cl[a + t] * n + r];
+            } else {
+                v = QO[(long long)cl[u] * n + r];
+            }
+            QN[(long long)(a + rank[u]) * n + r] = v;
+        }
+        if (!small) {
+            // 32 x 32 output tiles (eigenvector index j x column r), one per wavefront,
+            // as 2 x 2 v_mfma_f64_16x16x4f64 blocks; k runs over the kk non-deflated rows
+            const int tpm = s >> 5;
+            const int ntile = nm * tpm * tpm;
+            const int lr = lane & 15, lk = lane >> 4;
+            for (int ti = wave; ti < ntile; ti += nw) {
+                const int q = ti / (tpm * tpm), rem = ti - q * tpm * tpm;
+                const int jt = rem / tpm, rt = rem - jt * tpm;
+                const int kk = kkq[q];
+                const int a = q << l, b = min(a + s, n);
+                const int j0 = jt * 32, r0 = a + rt * 32;
+                if (kk <= 0 || j0 >= kk || r0 >= b) continue;
+                dbl4 acc[2][2];
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
+                for (int k0 = 0; k0 < kk; k0 += 4) {
+                    const int t = k0 + lk;
+                    const bool tv = t < kk;
+                    const long long urow = (long long)(a + t) * n + a;
+                    const long long qrow = tv ? (long long)cl[a + t] * n : 0;
+                    double av[2], bv[2];
+#pragma unroll
+                    for (int x = 0; x < 2; ++x) {
+                        const int j = j0 + 16 * x + lr;
+                        av[x] = (tv && j < kk) ? U[urow + j] : 0.0;
+                        const int r = r0 + 16 * x + lr;
+                        bv[x] = (tv && r < b) ? QO[qrow + r] : 0.0;
+                    }
+#pragma unroll
+                    for (int x = 0; x < 2; ++x)
+#pragma unroll
+                        for (int y = 0; y < 2; ++y)
+                            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+                }
+                // D[i][c] of a 16x16 FP64 block sits in lane c + 16 (i % 4), element i / 4
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int j = j0 + 16 * x + 4 * v + lk, r = r0 + 16 * y + lr;
+                            if (j < kk && r < b) QN[(long long)(a + rank[a + j]) * n + r] = acc[x][y][v];
+                        }
+            }
+        }
+        for (int i = tid; i < n; i += T) {
+            const int q = i >> l, a = q << l;
+            if (kkq[q] >= 0) Dc[a + rank[i]] = vals[i];
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += T) {
+        eps[jb.vec0 + i] = Dc[i];
+        perm[jb.ivec0 + i] = i;
     }
     if (tid == 0) {
-        state[job].sweeps = failed ? -1 : (sh[3] > 0 ? sh[3] : 1);
+        state[job].sweeps = 1;
         state[job].warm = 0;
-        const unsigned long long t3 = wall_clock64();
-        state[job].prof[0] = (double)(t1 - t0);
-        state[job].prof[1] = (double)tql;
-        state[job].prof[2] = (double)tapply;
-        state[job].prof[3] = (double)(t3 - t1 - tql - tapply);
+        state[job].prof[1] = (double)(wall_clock64() - t0);
+        state[job].prof[2] = 0.0;
     }
 }
 
-template <bool ALDS, int NPL>
-static void launch_heig(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
-                        int threads, double* arena, double* eps, int* perm, size_t lds) {
-    hipFuncSetAttribute((const void*)k_heig<ALDS, NPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_heig<ALDS, NPL>), dim3(nlist), dim3(threads), lds, st, jobs, state, joblist, arena, eps,
-                       perm);
+// ------------------------------------------------------------------ 3. back-transform
+// row r of the eigenvectors <- row r * H_{n-3} * ... * H_0; reflectors (rows
+// of W) staged through LDS in blocks of kBlk; one row per wavefront, held in
+// registers.
+template <int NPL>
+__global__ void __launch_bounds__(1024) k_backtransform(const JobDev* __restrict__ jobs,
+                                                        JobState* __restrict__ state,
+                                                        const int* __restrict__ joblist, double* __restrict__ arena,
+                                                        const double* __restrict__ tri) {
+    extern __shared__ double stage[];
+    const int job = joblist[blockIdx.x];
+    const JobDev jb = jobs[job];
+    if (state[job].phase != PH_ACTIVE) return;
+    const unsigned long long t0 = wall_clock64();
+    const int n = jb.nao;
+    const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
+    const double* W = amat(arena, jb, AR_W);
+    double* ZT = amat(arena, jb, AR_C);
+    const double* tau = tri + 3 * jb.vec0 + 2 * n;
+    const int nref = n - 2;
+    for (int r0 = 0; r0 < n && nref > 0; r0 += nw) {
+        const int r = r0 + wave;
+        double row[NPL];
+#pragma unroll
+        for (int u = 0; u < NPL; ++u) {
+            const int j = lane + 64 * u;
+            row[u] = (r < n && j < n) ? ZT[(long long)r * n + j] : 0.0;
+        }
+        for (int kb = nref - 1; kb >= 0; kb -= kBlk) {
+            const int kl = kb - kBlk + 1 > 0 ? kb - kBlk + 1 : 0;   // block [kl, kb], applied kb first
+            __syncthreads();
+            for (int x = tid; x < (kb - kl + 1) * n; x += T) {
+                const int kk = kl + x / n, j = x % n;
+                stage[x] = j > kk ? W[(long long)kk * n + j] : 0.0;
+            }
+            __syncthreads();
+            for (int k = kb; k >= kl; --k) {
+                const double* v = stage + (long long)(k - kl) * n;
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < NPL; ++u) {
+                    const int j = lane + 64 * u;
+                    if (j > k && j < n) s += row[u] * v[j];
+                }
+                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+                s *= tau[k];
+#pragma unroll
+                for (int u = 0; u < NPL; ++u) {
+                    const int j = lane + 64 * u;
+                    if (j > k && j < n) row[u] -= s * v[j];
+                }
+            }
+        }
+        if (r < n) {
+#pragma unroll
+            for (int u = 0; u < NPL; ++u) {
+                const int j = lane + 64 * u;
+                if (j < n) ZT[(long long)r * n + j] = row[u];
+            }
+        }
+    }
+    if (tid == 0) state[job].prof[3] = (double)(wall_clock64() - t0);
 }
 
 bool heig_supported(int n) { return n <= 512; }
 
+size_t heig_lds_bytes(int n, bool a_in_lds) { return tridiag_lds_bytes(n, a_in_lds); }
+
+template <int NPL>
+static void launch_bt(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int threads,
+                      double* arena, const double* tri, size_t lds) {
+    hipFuncSetAttribute((const void*)k_backtransform<NPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_backtransform<NPL>), dim3(nlist), dim3(threads), lds, st, jobs, state, joblist, arena, tri);
+}
+
 void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
-                       double* arena, double* eps, int* perm, bool a_in_lds) {
+                       double* arena, double* eps, int* perm, double* tri, bool a_in_lds) {
     if (nlist <= 0) return;
     const int threads = max_n <= 32 ? 256 : (max_n <= 96 ? 512 : 1024);
-    const size_t lds = heig_lds_bytes(max_n, a_in_lds);
+    const size_t lt = tridiag_lds_bytes(max_n, a_in_lds);
     if (a_in_lds) {
-        if (max_n <= 64) launch_heig<true, 1>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, lds);
-        else launch_heig<true, 2>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, lds);
+        hipFuncSetAttribute((const void*)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
+        hipLaunchKernelGGL(k_tridiag<true>, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
     } else {
-        if (max_n <= 128) launch_heig<false, 2>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, lds);
-        else if (max_n <= 256) launch_heig<false, 4>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, lds);
-        else if (max_n <= 384) launch_heig<false, 6>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, lds);
-        else launch_heig<false, 8>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, lds);
+        hipFuncSetAttribute((const void*)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
+        hipLaunchKernelGGL(k_tridiag<false>, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
     }
+    const int tdc = max_n <= 64 ? 256 : (max_n <= 128 ? 512 : 1024);
+    const size_t ld = dc_lds_bytes(max_n);
+    hipFuncSetAttribute((const void*)k_dc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ld);
+    hipLaunchKernelGGL(k_dc, dim3(nlist), dim3(tdc), ld, st, jobs, state, joblist, arena, tri, eps, perm);
+    const size_t lb = (size_t)kBlk * max_n * 8;
+    if (max_n <= 64) launch_bt<1>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
+    else if (max_n <= 128) launch_bt<2>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
+    else if (max_n <= 256) launch_bt<4>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
+    else if (max_n <= 384) launch_bt<6>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
+    else launch_bt<8>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
 }
 
 }  // namespace nddo

@@ -549,10 +549,10 @@ size_t jacobi_rows_lds_bytes(int n, bool a_in_lds) {
 }
 
 void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
-                  double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds) {
+                  double* arena, double* eps, int* perm, double* tri, size_t lds_bytes, bool a_in_lds) {
     if (nlist <= 0) return;
     if (heig_supported(max_n)) {
-        launch_heig_class(st, jobs, state, joblist, nlist, max_n, arena, eps, perm, a_in_lds);
+        launch_heig_class(st, jobs, state, joblist, nlist, max_n, arena, eps, perm, tri, a_in_lds);
         return;
     }
     launch_jacobi(st, jobs, state, joblist, nlist, max_n, arena, eps, perm, lds_bytes, a_in_lds);

@@ -26,15 +26,16 @@ void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev*
 // Eigen-decomposition of F for PH_ACTIVE jobs in joblist (one workgroup per job).
 // lds_ok: jobs whose n*n matrix fits the LDS budget take the LDS path.
 size_t jacobi_rows_lds_bytes(int n, bool a_in_lds);
-// Householder + QL eigensolver (kernels_eig.hip), one workgroup per job, n <= 512.
+// Householder + divide-and-conquer eigensolver (kernels_eig.hip), one workgroup
+// per job, n <= 512; tri holds 3n doubles per job (d, e, tau) at 3 * vec0.
 size_t heig_lds_bytes(int n, bool a_in_lds);
 bool heig_supported(int n);
 void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
-                       double* arena, double* eps, int* perm, bool a_in_lds);
+                       double* arena, double* eps, int* perm, double* tri, bool a_in_lds);
 void launch_jacobi(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
                    double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds);
 void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
-                  int max_n, double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds);
+                  int max_n, double* arena, double* eps, int* perm, double* tri, size_t lds_bytes, bool a_in_lds);
 
 void launch_density(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs,
                     const JobState* state, double* arena, const int* perm, double* rmspart);

@@ -49,12 +49,14 @@ enum ArenaKind {
     AR_F = 0,      // Fock matrix
     AR_P0 = 1,     // density ping
     AR_P1 = 2,     // density pong
-    AR_C = 3,      // eigenvectors (columns), unsorted; order in perm[]
-    AR_W = 4,      // eigensolver work matrix
+    AR_C = 3,      // eigenvectors (rows); rank order in perm[]
+    AR_W = 4,      // eigensolver work matrix (Householder reflectors)
     AR_H = 5,      // core Hamiltonian F(P = 0), full symmetric
-    AR_RING = 6    // DIIS ring: k stored densities then k stored errors
+    AR_Q2 = 6,     // divide-and-conquer ping-pong partner of AR_C
+    AR_U = 7,      // divide-and-conquer merge eigenvectors
+    AR_RING = 8    // DIIS ring: k stored densities then k stored errors
 };
-#define ARENA_FIXED 6
+#define ARENA_FIXED 8
 
 enum JobPhase { PH_ACTIVE = 0, PH_ENERGY = 1, PH_DONE = 2, PH_IDLE = 3 };
 

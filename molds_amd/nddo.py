@@ -298,7 +298,7 @@ class Engine:
         t = np.zeros(4)
         hz = ctypes.c_double()
         self.L.nddo_dbg_eigh_profile(self.h, _p(t), ctypes.byref(hz))
-        return dict(zip(("tridiag", "ql", "apply", "backtransform"), (t / hz.value * 1e3).tolist()))
+        return dict(zip(("tridiag", "divide_conquer", "unused", "backtransform"), (t / hz.value * 1e3).tolist()))
 
     def dbg_density(self, C, nocc):
         C = np.ascontiguousarray(C, dtype=np.float64)

@@ -100,8 +100,12 @@ def test_fock_matches_oracle(engine, theory, tag):
     assert np.max(np.abs((F12 - F2) - (F1 - H))) <= 1e-12 * np.max(np.abs(F1))
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 8, 17, 62, 130, 201, 384])
+# sizes cover 1x1/2x2 corner cases, ragged divide-and-conquer trees (carried
+# blocks at several levels), the LDS/global tridiagonalization boundary near
+# 135 and the largest supported one-workgroup size
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 17, 33, 62, 96, 130, 136, 201, 257, 384, 512])
 def test_eigh(engine, n):
+    print(f"eigh n={n}", flush=True)
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n))
     A = A + A.T
