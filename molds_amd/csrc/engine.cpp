@@ -281,7 +281,8 @@ int Batch::allocate(bool want_sdbg) {
     HIPCHK(hipMalloc(&d_ecore, std::max<long long>(nslots, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_eps, std::max<long long>(nvec, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_perm, std::max<long long>(nvec, 1) * sizeof(int)));
-    HIPCHK(hipMalloc(&d_tri, 3 * std::max<long long>(nvec, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&d_tri, (EIG_WS_PER_AO * std::max<long long>(nvec, 1) + EIG_WS_PER_JOB * (long long)nj) *
+                                 sizeof(double)));
     HIPCHK(hipMalloc(&d_rmspart, std::max<size_t>(tiles.size(), 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_traj, std::max<long long>(ntraj, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_remaining, sizeof(int)));

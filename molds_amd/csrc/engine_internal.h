@@ -48,7 +48,7 @@ struct Batch {
     double* d_ecore = nullptr;
     double* d_eps = nullptr;
     int* d_perm = nullptr;
-    double* d_tri = nullptr;     // tridiagonal d, e, tau: 3 per vector element
+    double* d_tri = nullptr;     // eigensolver workspace (EIG_WS_PER_AO / EIG_WS_PER_JOB)
     double* d_rmspart = nullptr;
     double* d_traj = nullptr;
     int* d_remaining = nullptr;

@@ -14,8 +14,8 @@
 //                       formula of Gu & Eisenstat (SIAM J. Matrix Anal. Appl.
 //                       16, 172 (1995)), and the merge products Q_new = Q U on
 //                       FP64 MFMA (v_mfma_f64_16x16x4f64).
-//   3. k_backtransform  eigenvector rows <- rows * H_{n-3} ... H_0, reflectors
-//                       staged through LDS in blocks.
+//   3. k_wy_t, k_bt_wy  eigenvector rows <- rows * H_{n-3} ... H_0 in compact
+//                       WY blocks on FP64 MFMA, one workgroup per 16 rows.
 // Output: rows of AR_C are the eigenvectors, already in ascending eigenvalue
 // order; eps[] ascending and perm[] = identity, as the density kernel reads them.
 #include <hip/hip_runtime.h>
@@ -34,8 +34,14 @@ __device__ inline double* amat(double* arena, const JobDev& jb, int kind) {
     return arena + jb.arena + (long long)kind * jb.nao * jb.nao;
 }
 
-constexpr int kBlk = 16;                               // reflectors staged per back-transform block
 constexpr double kEps = 1.1102230246251565e-16;        // 2^-53, unit roundoff
+
+// Per-job eigensolver workspace (EIG_WS_PER_AO doubles per AO plus
+// EIG_WS_PER_JOB per job): d[n], e[n], tau[n], then the compact-WY T blocks
+// (16 x 16 per block of 16 reflectors, at most 16 (n + 14) doubles).
+__device__ inline double* eig_ws(double* ws, const JobDev& jb, int job) {
+    return ws + (long long)EIG_WS_PER_AO * jb.vec0 + (long long)EIG_WS_PER_JOB * job;
+}
 
 // Block-wide sum; every thread gets the result.
 __device__ inline double block_sum(double v, double* red, int tid, int nw) {
@@ -60,7 +66,7 @@ size_t dc_lds_bytes(int n) { return (size_t)n * (12 * 8 + 9 * 4); }
 
 // ------------------------------------------------------------------ 1. tridiagonalize
 // Reflector k acts on indices k+1..n-1: H_k = I - tau_k v v^T with v[0] = 1,
-// kept in row k of W (columns k+1..n-1).  tri (per job, 3n): d[n], e[n], tau[n].
+// kept in row k of W (columns k+1..n-1); d, e, tau go to the job's workspace.
 template <bool ALDS>
 __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
                                                   const int* __restrict__ joblist, double* __restrict__ arena,
@@ -81,7 +87,7 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
     double* W = amat(arena, jb, AR_W);
     double* A = ALDS ? pnext + n : W;
     const double* F = amat(arena, jb, AR_F);
-    double* d = tri + 3 * jb.vec0;
+    double* d = eig_ws(tri, jb, job);
     double* e = d + n;
     double* tau = e + n;
     const long long nn = (long long)n * n;
@@ -299,7 +305,7 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
     int* r2 = r1 + n;
     int* kkq = r2 + n;           // per merge: kk (-1: block carried to the next level)
     int* nrot = kkq + n;
-    const double* d = tri + 3 * jb.vec0;
+    const double* d = eig_ws(const_cast<double*>(tri), jb, job);
     const double* e = d + n;
     double* QC = amat(arena, jb, AR_C);
     double* QX = amat(arena, jb, AR_Q2);
@@ -564,79 +570,146 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
 }
 
 // ------------------------------------------------------------------ 3. back-transform
-// row r of the eigenvectors <- row r * H_{n-3} * ... * H_0; reflectors (rows
-// of W) staged through LDS in blocks of kBlk; one row per wavefront, held in
-// registers.
-template <int NPL>
-__global__ void __launch_bounds__(1024) k_backtransform(const JobDev* __restrict__ jobs,
-                                                        JobState* __restrict__ state,
-                                                        const int* __restrict__ joblist, double* __restrict__ arena,
-                                                        const double* __restrict__ tri) {
-    extern __shared__ double stage[];
+// Eigenvector rows x^T = y^T H_{n-3} ... H_0.  Reflectors are taken in blocks of
+// kBt (k0 .. k0+kBt-1); in compact WY form (Schreiber & Van Loan, SIAM J. Sci.
+// Stat. Comput. 10, 53 (1989)) H_k0 ... H_k0+kBt-1 = I - V T V^T with T upper
+// triangular, so a row block Z is updated as Z <- Z - (Z V) T^T V^T, applying
+// the blocks from the last one down.  V[j][c] = W[k0 + c][j] for j > k0 + c.
+constexpr int kBt = 16;
+
+// T of every reflector block: one wavefront per (job, block); T goes to the
+// job's eigensolver workspace.
+__global__ void __launch_bounds__(64) k_wy_t(const JobDev* __restrict__ jobs, const JobState* __restrict__ state,
+                                             const int* __restrict__ joblist, const double* __restrict__ arena,
+                                             double* __restrict__ tri) {
+    __shared__ double G[kBt * kBt];
+    __shared__ double Ts[kBt * kBt];
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
-    const unsigned long long t0 = wall_clock64();
-    const int n = jb.nao;
-    const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
-    const double* W = amat(arena, jb, AR_W);
-    double* ZT = amat(arena, jb, AR_C);
-    const double* tau = tri + 3 * jb.vec0 + 2 * n;
-    const int nref = n - 2;
-    for (int r0 = 0; r0 < n && nref > 0; r0 += nw) {
-        const int r = r0 + wave;
-        double row[NPL];
-#pragma unroll
-        for (int u = 0; u < NPL; ++u) {
-            const int j = lane + 64 * u;
-            row[u] = (r < n && j < n) ? ZT[(long long)r * n + j] : 0.0;
-        }
-        for (int kb = nref - 1; kb >= 0; kb -= kBlk) {
-            const int kl = kb - kBlk + 1 > 0 ? kb - kBlk + 1 : 0;   // block [kl, kb], applied kb first
-            __syncthreads();
-            for (int x = tid; x < (kb - kl + 1) * n; x += T) {
-                const int kk = kl + x / n, j = x % n;
-                stage[x] = j > kk ? W[(long long)kk * n + j] : 0.0;
-            }
-            __syncthreads();
-            for (int k = kb; k >= kl; --k) {
-                const double* v = stage + (long long)(k - kl) * n;
-                double s = 0.0;
-#pragma unroll
-                for (int u = 0; u < NPL; ++u) {
-                    const int j = lane + 64 * u;
-                    if (j > k && j < n) s += row[u] * v[j];
-                }
-                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-                s *= tau[k];
-#pragma unroll
-                for (int u = 0; u < NPL; ++u) {
-                    const int j = lane + 64 * u;
-                    if (j > k && j < n) row[u] -= s * v[j];
-                }
-            }
-        }
-        if (r < n) {
-#pragma unroll
-            for (int u = 0; u < NPL; ++u) {
-                const int j = lane + 64 * u;
-                if (j < n) ZT[(long long)r * n + j] = row[u];
-            }
-        }
+    const int n = jb.nao, nref = n - 2, k0 = blockIdx.y * kBt;
+    if (nref <= 0 || k0 >= nref) return;
+    const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+    const double* W = arena + jb.arena + (long long)AR_W * n * n;
+    double* ws = eig_ws(tri, jb, job);
+    const double* tau = ws + 2 * n;
+    const int nb = min(kBt, nref - k0);
+    // G = V^T V (16 x 16, K = j) on MFMA; rows/columns beyond nb stay zero
+    dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int c = k0 + lr;   // reflector of this lane's M (and N) index
+    for (int j0 = k0 + 1; j0 < n; j0 += 4) {
+        const int j = j0 + lk;
+        const double v = (lr < nb && j < n && j > c) ? W[(long long)c * n + j] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v, v, acc, 0, 0, 0);
     }
-    if (tid == 0) state[job].prof[3] = (double)(wall_clock64() - t0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) G[(4 * v + lk) * kBt + lr] = acc[v];
+    for (int x = lane; x < kBt * kBt; x += 64) Ts[x] = 0.0;
+    __syncthreads();
+    // T[i][i] = tau_i; T[0:i, i] = -tau_i T[0:i, 0:i] G[0:i, i]
+    for (int i = 0; i < nb; ++i) {
+        const double ti = tau[k0 + i];
+        double t = 0.0;
+        if (lane < i) {
+            for (int cc = lane; cc < i; ++cc) t += Ts[lane * kBt + cc] * G[cc * kBt + i];
+            t *= -ti;
+        }
+        __syncthreads();
+        if (lane < i) Ts[lane * kBt + i] = t;
+        if (lane == i) Ts[i * kBt + i] = ti;
+        __syncthreads();
+    }
+    double* Tout = ws + 3 * n + (long long)blockIdx.y * kBt * kBt;
+    for (int x = lane; x < kBt * kBt; x += 64) Tout[x] = Ts[x];
+}
+
+// One workgroup (4 wavefronts) per (job, chunk of 16 eigenvector rows); the
+// chunk and the current V block live in LDS (row stride n + 1).
+__global__ void __launch_bounds__(256) k_bt_wy(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                               const int* __restrict__ joblist, double* __restrict__ arena,
+                                               double* __restrict__ tri) {
+    extern __shared__ double dyn[];
+    __shared__ double Yp[4 * kBt * kBt];
+    __shared__ double Y2[kBt * kBt];
+    __shared__ double Ts[kBt * kBt];
+    const int job = joblist[blockIdx.x];
+    const JobDev jb = jobs[job];
+    if (state[job].phase != PH_ACTIVE) return;
+    const int n = jb.nao, nref = n - 2, r0 = blockIdx.y * 16;
+    if (nref <= 0 || r0 >= n) return;
+    const unsigned long long t0 = wall_clock64();
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    const int ld = n + 1;
+    double* Zc = dyn;              // 16 x ld
+    double* Vs = dyn + 16 * ld;    // 16 x ld: Vs[c][j] = V[j][c]
+    double* ZT = amat(arena, jb, AR_C);
+    const double* W = amat(arena, jb, AR_W);
+    const double* Tall = eig_ws(tri, jb, job) + 3 * n;
+    const int nr = min(16, n - r0);
+    for (int x = tid; x < 16 * n; x += 256) {
+        const int r = x / n, j = x - r * n;
+        Zc[r * ld + j] = r < nr ? ZT[(long long)(r0 + r) * n + j] : 0.0;
+    }
+    const int nblk = (nref + kBt - 1) / kBt;
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+        const int k0 = bi * kBt, nb = min(kBt, nref - k0);
+        const int jlo = ((k0 + 1) >> 4) << 4;   // first 16-column tile touched by this block
+        for (int x = tid; x < 16 * (n - jlo); x += 256) {
+            const int cc = x / (n - jlo), j = jlo + (x - cc * (n - jlo));
+            Vs[cc * ld + j] = (cc < nb && j > k0 + cc) ? W[(long long)(k0 + cc) * n + j] : 0.0;
+        }
+        for (int x = tid; x < kBt * kBt; x += 256) Ts[x] = Tall[(long long)bi * kBt * kBt + x];
+        __syncthreads();
+        // Y = Z V (16 x 16), K = j split over the 4 wavefronts
+        dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int j0 = jlo + 4 * wave; j0 < n; j0 += 16) {
+            const int j = j0 + lk;
+            const double a = j < n ? Zc[lr * ld + j] : 0.0;
+            const double b = j < n ? Vs[lr * ld + j] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Yp[wave * kBt * kBt + (4 * v + lk) * kBt + lr] = acc[v];
+        __syncthreads();
+        {   // Y2 = -(Y T^T): Y2[r][c] = -sum_{c' >= c} Y[r][c'] T[c][c']
+            const int r = tid >> 4, cc = tid & 15;
+            double s = 0.0;
+            for (int c2 = cc; c2 < kBt; ++c2) {
+                const int yi = r * kBt + c2;
+                const double y = Yp[yi] + Yp[kBt * kBt + yi] + Yp[2 * kBt * kBt + yi] + Yp[3 * kBt * kBt + yi];
+                s += y * Ts[cc * kBt + c2];
+            }
+            Y2[r * kBt + cc] = -s;
+        }
+        __syncthreads();
+        // Z[:, j] += Y2 V^T over 16-column tiles, spread over the wavefronts
+        for (int jt = jlo + 16 * wave; jt < n; jt += 64) {
+            const int j = jt + lr;
+            dbl4 z;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) z[v] = j < n ? Zc[(4 * v + lk) * ld + j] : 0.0;
+#pragma unroll
+            for (int c0 = 0; c0 < kBt; c0 += 4) {
+                const double a = Y2[lr * kBt + c0 + lk];
+                const double b = j < n ? Vs[(c0 + lk) * ld + j] : 0.0;
+                z = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, z, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (j < n) Zc[(4 * v + lk) * ld + j] = z[v];
+        }
+        __syncthreads();
+    }
+    for (int x = tid; x < nr * n; x += 256) {
+        const int r = x / n, j = x - r * n;
+        ZT[(long long)(r0 + r) * n + j] = Zc[r * ld + j];
+    }
+    if (tid == 0 && blockIdx.y == 0) state[job].prof[3] = (double)(wall_clock64() - t0);
 }
 
 bool heig_supported(int n) { return n <= 512; }
 
 size_t heig_lds_bytes(int n, bool a_in_lds) { return tridiag_lds_bytes(n, a_in_lds); }
-
-template <int NPL>
-static void launch_bt(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int threads,
-                      double* arena, const double* tri, size_t lds) {
-    hipFuncSetAttribute((const void*)k_backtransform<NPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_backtransform<NPL>), dim3(nlist), dim3(threads), lds, st, jobs, state, joblist, arena, tri);
-}
 
 void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
                        double* arena, double* eps, int* perm, double* tri, bool a_in_lds) {
@@ -654,12 +727,14 @@ void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, cons
     const size_t ld = dc_lds_bytes(max_n);
     hipFuncSetAttribute((const void*)k_dc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ld);
     hipLaunchKernelGGL(k_dc, dim3(nlist), dim3(tdc), ld, st, jobs, state, joblist, arena, tri, eps, perm);
-    const size_t lb = (size_t)kBlk * max_n * 8;
-    if (max_n <= 64) launch_bt<1>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
-    else if (max_n <= 128) launch_bt<2>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
-    else if (max_n <= 256) launch_bt<4>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
-    else if (max_n <= 384) launch_bt<6>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
-    else launch_bt<8>(st, jobs, state, joblist, nlist, threads, arena, tri, lb);
+    if (max_n >= 3) {
+        const int nblk = (max_n - 2 + kBt - 1) / kBt;
+        hipLaunchKernelGGL(k_wy_t, dim3(nlist, nblk), dim3(64), 0, st, jobs, state, joblist, arena, tri);
+        const size_t lb = (size_t)2 * 16 * (max_n + 1) * 8;
+        hipFuncSetAttribute((const void*)k_bt_wy, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+        hipLaunchKernelGGL(k_bt_wy, dim3(nlist, (max_n + 15) / 16), dim3(256), lb, st, jobs, state, joblist, arena,
+                           tri);
+    }
 }
 
 }  // namespace nddo

@@ -27,7 +27,7 @@ void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev*
 // lds_ok: jobs whose n*n matrix fits the LDS budget take the LDS path.
 size_t jacobi_rows_lds_bytes(int n, bool a_in_lds);
 // Householder + divide-and-conquer eigensolver (kernels_eig.hip), one workgroup
-// per job, n <= 512; tri holds 3n doubles per job (d, e, tau) at 3 * vec0.
+// per job, n <= 512; tri is the eigensolver workspace (EIG_WS_PER_AO, EIG_WS_PER_JOB).
 size_t heig_lds_bytes(int n, bool a_in_lds);
 bool heig_supported(int n);
 void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,

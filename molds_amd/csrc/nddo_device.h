@@ -58,6 +58,11 @@ enum ArenaKind {
 };
 #define ARENA_FIXED 8
 
+// Eigensolver workspace per job (kernels_eig.hip): d, e, tau and the compact-WY
+// T blocks, EIG_WS_PER_AO * n + EIG_WS_PER_JOB doubles.
+#define EIG_WS_PER_AO 19
+#define EIG_WS_PER_JOB 256
+
 enum JobPhase { PH_ACTIVE = 0, PH_ENERGY = 1, PH_DONE = 2, PH_IDLE = 3 };
 
 struct JobDev {
