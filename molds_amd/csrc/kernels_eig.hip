@@ -15,7 +15,7 @@
 //                       16, 172 (1995)), and the merge products Q_new = Q U on
 //                       FP64 MFMA (v_mfma_f64_16x16x4f64).
 //   3. k_wy_t, k_bt_wy  eigenvector rows <- rows * H_{n-3} ... H_0 in compact
-//                       WY blocks on FP64 MFMA, one workgroup per 16 rows.
+//                       WY blocks on FP64 MFMA, one workgroup per 64 rows.
 // Output: rows of AR_C are the eigenvectors, already in ascending eigenvalue
 // order; eps[] ascending and perm[] = identity, as the density kernel reads them.
 #include <hip/hip_runtime.h>
@@ -57,7 +57,7 @@ __device__ inline double block_sum(double v, double* red, int tid, int nw) {
 }  // namespace
 
 size_t tridiag_lds_bytes(int n, bool a_in_lds) {
-    size_t d = 5 * (size_t)n;   // vcur vnext w p pnext
+    size_t d = 4 * (size_t)n;   // vcur vnext p pnext
     if (a_in_lds) d += (size_t)n * n;
     return d * 8;
 }
@@ -72,7 +72,8 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
                                                   const int* __restrict__ joblist, double* __restrict__ arena,
                                                   double* __restrict__ tri) {
     extern __shared__ double dyn[];
-    __shared__ double red[32];
+    __shared__ double red[16], red2[16], sh_alpha;
+    __shared__ double part[1024];   // segment partials of A v (S * m2 <= blockDim.x)
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
@@ -81,8 +82,7 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
     const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
     double* vcur = dyn;
     double* vnext = vcur + n;
-    double* w = vnext + n;
-    double* p = w + n;
+    double* p = vnext + n;
     double* pnext = p + n;
     double* W = amat(arena, jb, AR_W);
     double* A = ALDS ? pnext + n : W;
@@ -99,98 +99,141 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
             d[0] = A[0];
             if (n == 2) { e[0] = A[1]; d[1] = A[3]; }
         }
-    } else {
-        {   // reflector of row 0 and p = tau B v for B = A[1:, 1:]
-            const int m = n - 1;
-            double ss = 0.0;
-            for (int t = 1 + tid; t < m; t += T) { const double x = A[1 + t]; ss += x * x; }
-            ss = block_sum(ss, red, tid, nw);
-            const double alpha = A[1];
-            __syncthreads();   // every wave has read alpha before A[1] is overwritten below
-            double beta = alpha, tk = 0.0, scale = 0.0;
-            if (ss > 0.0) {
-                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-                tk = (beta - alpha) / beta;
-                scale = 1.0 / (alpha - beta);
-            }
-            for (int t = tid; t < m; t += T) {
-                const double v = t == 0 ? 1.0 : (ss > 0.0 ? A[1 + t] * scale : 0.0);
-                vcur[t] = v;
-                A[1 + t] = v;
-            }
-            if (tid == 0) { d[0] = A[0]; e[0] = beta; tau[0] = tk; }
-            __syncthreads();
-            for (int i = 1 + wave; i < n; i += nw) {
-                double s = 0.0;
-                for (int j = 1 + lane; j < n; j += 64) s += A[(long long)i * n + j] * vcur[j - 1];
-                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-                if (lane == 0) p[i - 1] = tk * s;
-            }
-            __syncthreads();
+        if (tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
+        return;
+    }
+    // Each step k keeps v_k (vcur, indices k+1..n-1), p = tau_k A v_k (p), and
+    // per-wave partial sums of p.v in red; four barriers per step.
+    double tk;
+    {   // reflector of row 0, then p = tau_0 A[1:, 1:] v_0
+        double ss = 0.0;
+        for (int t = 2 + tid; t < n; t += T) { const double x = A[t]; ss += x * x; }
+        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+        if (lane == 0) red2[wave] = ss;
+        __syncthreads();
+        ss = 0.0;
+        for (int w = 0; w < nw; ++w) ss += red2[w];
+        const double alpha = A[1];
+        double beta = alpha, scale = 0.0;
+        tk = 0.0;
+        if (ss > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            tk = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
         }
-        // vectors below are indexed from k+1 (t = index - k - 1)
-        for (int k = 0; k <= n - 3; ++k) {
-            const int m = n - 1 - k;
-            const double tk = tau[k];
-            double pv = 0.0;
-            for (int t = tid; t < m; t += T) pv += p[t] * vcur[t];
-            pv = block_sum(pv, red, tid, nw);
-            const double K = -0.5 * tk * pv;
-            for (int t = tid; t < m; t += T) w[t] = p[t] + K * vcur[t];
-            __syncthreads();
-            if (k == n - 3) {   // last trailing 2x2 block (rows/cols n-2, n-1)
-                if (tid == 0) {
-                    d[n - 2] = A[(long long)(n - 2) * n + n - 2] - 2.0 * vcur[0] * w[0];
-                    e[n - 2] = A[(long long)(n - 2) * n + n - 1] - (vcur[0] * w[1] + w[0] * vcur[1]);
-                    d[n - 1] = A[(long long)(n - 1) * n + n - 1] - 2.0 * vcur[1] * w[1];
-                }
-                break;
+        for (int t = 1 + tid; t < n - 1; t += T) vcur[t] = ss > 0.0 ? A[1 + t] * scale : 0.0;
+        if (tid == 0) { vcur[0] = 1.0; d[0] = A[0]; e[0] = beta; tau[0] = tk; }
+        __syncthreads();
+        for (int t = 1 + tid; t < n - 1; t += T) A[1 + t] = vcur[t];
+        if (tid == 0) A[1] = 1.0;
+        double pv = 0.0;
+        for (int i = 1 + wave; i < n; i += nw) {
+            double s = 0.0;
+            for (int j = 1 + lane; j < n; j += 64) s += A[(long long)i * n + j] * vcur[j - 1];
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0) { p[i - 1] = tk * s; pv += tk * s * vcur[i - 1]; }
+        }
+        if (lane == 0) red[wave] = pv;
+        __syncthreads();
+    }
+    for (int k = 0; k <= n - 3; ++k) {
+        const int m = n - 1 - k;   // length of v_k
+        double pv = 0.0;
+        for (int w = 0; w < nw; ++w) pv += red[w];
+        const double K = -0.5 * tk * pv;   // w = p + K v
+        const double w0 = p[0] + K * vcur[0];
+        if (k == n - 3) {   // last trailing 2x2 block (rows/cols n-2, n-1)
+            if (tid == 0) {
+                const double w1 = p[1] + K * vcur[1];
+                d[n - 2] = A[(long long)(n - 2) * n + n - 2] - 2.0 * vcur[0] * w0;
+                e[n - 2] = A[(long long)(n - 2) * n + n - 1] - (vcur[0] * w1 + w0 * vcur[1]);
+                d[n - 1] = A[(long long)(n - 1) * n + n - 1] - 2.0 * vcur[1] * w1;
             }
-            const long long r1 = (long long)(k + 1) * n;   // row k+1 of the updated block
-            for (int t = tid; t < m; t += T) A[r1 + k + 1 + t] -= vcur[0] * w[t] + w[0] * vcur[t];
-            __syncthreads();
-            const int m2 = m - 1;   // reflector of row k+1: x = A[k+1][k+2 .. n-1]
-            double ss = 0.0;
-            for (int t = 1 + tid; t < m2; t += T) { const double x = A[r1 + k + 2 + t]; ss += x * x; }
-            ss = block_sum(ss, red, tid, nw);
-            const double alpha = A[r1 + k + 2];
-            __syncthreads();   // every wave has read alpha before it is overwritten below
-            double beta = alpha, tn = 0.0, scale = 0.0;
-            if (ss > 0.0) {
-                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-                tn = (beta - alpha) / beta;
-                scale = 1.0 / (alpha - beta);
-            }
-            for (int t = tid; t < m2; t += T) {
-                const double v = t == 0 ? 1.0 : (ss > 0.0 ? A[r1 + k + 2 + t] * scale : 0.0);
-                vnext[t] = v;
-                A[r1 + k + 2 + t] = v;
-            }
-            if (tid == 0) { d[k + 1] = A[r1 + k + 1]; e[k + 1] = beta; tau[k + 1] = tn; }
-            __syncthreads();
-            // fused pass over rows k+2 .. n-1: rank-2 update of step k, then p for step k+1
-            for (int i = k + 2 + wave; i < n; i += nw) {
-                const int ti = i - k - 1;
-                const double vi = vcur[ti], wi = w[ti];
-                const long long ri = (long long)i * n;
+            break;
+        }
+        // row k+1 of the updated block; its tail x = A[k+1][k+2 ..] defines the next reflector
+        const long long r1 = (long long)(k + 1) * n + k + 1;
+        double ss = 0.0;
+        for (int t = tid; t < m; t += T) {
+            const double a = A[r1 + t] - (vcur[0] * (p[t] + K * vcur[t]) + w0 * vcur[t]);
+            A[r1 + t] = a;
+            if (t >= 2) ss += a * a;
+            else if (t == 1) sh_alpha = a;
+            else d[k + 1] = a;
+        }
+        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+        if (lane == 0) red2[wave] = ss;
+        __syncthreads();   // 1
+        ss = 0.0;
+        for (int w = 0; w < nw; ++w) ss += red2[w];
+        const double alpha = sh_alpha;
+        double beta = alpha, tn = 0.0, scale = 0.0;
+        if (ss > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            tn = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        const int m2 = m - 1;   // length of v_{k+1}
+        for (int t = 1 + tid; t < m2; t += T) {
+            const double v = ss > 0.0 ? A[r1 + 1 + t] * scale : 0.0;
+            vnext[t] = v;
+            A[r1 + 1 + t] = v;
+        }
+        if (tid == 0) { vnext[0] = 1.0; A[r1 + 1] = 1.0; e[k + 1] = beta; tau[k + 1] = tn; }
+        __syncthreads();   // 2
+        // columns i = k+2 .. n-1 (A is kept symmetric, so column i is row i): rank-2
+        // update of step k and the partial sums of (A v_{k+1})_i over S segments of j
+        {
+            const int S = min(16, T / m2), L = (m2 + S - 1) / S;
+            const int seg = tid / m2, ci = tid - seg * m2;
+            if (seg < S) {
+                const int i = k + 2 + ci, ti = ci + 1;
+                const double vi = vcur[ti], wi = p[ti] + K * vi;
+                const int jb = k + 2 + seg * L, je = min(jb + L, n);
                 double s = 0.0;
-                for (int j = k + 2 + lane; j < n; j += 64) {
+                int j = jb;
+                for (; j + 3 < je; j += 4) {
+                    double a[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) a[u] = A[(long long)(j + u) * n + i];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int tj = j + u - k - 1;
+                        const double vj = vcur[tj];
+                        a[u] -= vj * wi + (p[tj] + K * vj) * vi;
+                        A[(long long)(j + u) * n + i] = a[u];
+                        s += a[u] * vnext[tj - 1];
+                    }
+                }
+                for (; j < je; ++j) {
                     const int tj = j - k - 1;
-                    const double a = A[ri + j] - (vi * w[tj] + wi * vcur[tj]);
-                    A[ri + j] = a;
+                    const double vj = vcur[tj];
+                    const double a = A[(long long)j * n + i] - (vj * wi + (p[tj] + K * vj) * vi);
+                    A[(long long)j * n + i] = a;
                     s += a * vnext[tj - 1];
                 }
-                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-                if (lane == 0) pnext[i - k - 2] = tn * s;
+                part[seg * m2 + ci] = s;
             }
-            __syncthreads();
-            double* tmp = vcur; vcur = vnext; vnext = tmp;
-            tmp = p; p = pnext; pnext = tmp;
+            __syncthreads();   // 3
+            pv = 0.0;
+            for (int t = tid; t < m2; t += T) {
+                double sp = 0.0;
+                for (int sg = 0; sg < S; ++sg) sp += part[sg * m2 + t];
+                const double pn = tn * sp;
+                pnext[t] = pn;
+                pv += pn * vnext[t];
+            }
+            for (int off = 32; off > 0; off >>= 1) pv += __shfl_xor(pv, off);
+            if (lane == 0) red[wave] = pv;
         }
-        if (ALDS) {   // keep the reflectors (upper rows) for the back-transformation
-            __syncthreads();
-            for (long long x = tid; x < nn; x += T) W[x] = A[x];
-        }
+        __syncthreads();   // 4
+        double* tmp = vcur; vcur = vnext; vnext = tmp;
+        tmp = p; p = pnext; pnext = tmp;
+        tk = tn;
+    }
+    if (ALDS) {   // keep the reflectors (upper rows) for the back-transformation
+        __syncthreads();
+        for (long long x = tid; x < nn; x += T) W[x] = A[x];
     }
     if (tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
 }
@@ -623,88 +666,142 @@ __global__ void __launch_bounds__(64) k_wy_t(const JobDev* __restrict__ jobs, co
     for (int x = lane; x < kBt * kBt; x += 64) Tout[x] = Ts[x];
 }
 
-// One workgroup (4 wavefronts) per (job, chunk of 16 eigenvector rows); the
-// chunk and the current V block live in LDS (row stride n + 1).
-__global__ void __launch_bounds__(256) k_bt_wy(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+// One workgroup (8 wavefronts) per (job, 64 eigenvector rows).  The rows are
+// held transposed, Z^T[j][r], as v_mfma_f64_16x16x4f64 accumulator tiles: wave
+// w owns the 16-column tiles tj = w, w + 8, ... for all four 16-row tiles.  In
+// that layout element v of a lane holds Z^T[16 tj + 4 v + lk][r0 + 16 rt + lr],
+// which is exactly the B operand of the k-step v of Y^T = V^T Z^T, so Z never
+// leaves registers.  Per block: Y^T = V^T Z^T (partials per wave, summed in
+// LDS), Y2^T = -T Y^T, Z^T += V Y2^T.
+template <int JT>
+__global__ void __launch_bounds__(512) k_bt_wy(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
                                                const int* __restrict__ joblist, double* __restrict__ arena,
                                                double* __restrict__ tri) {
     extern __shared__ double dyn[];
-    __shared__ double Yp[4 * kBt * kBt];
-    __shared__ double Y2[kBt * kBt];
-    __shared__ double Ts[kBt * kBt];
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
-    const int n = jb.nao, nref = n - 2, r0 = blockIdx.y * 16;
+    const int n = jb.nao, nref = n - 2, r0 = blockIdx.y * 64;
     if (nref <= 0 || r0 >= n) return;
     const unsigned long long t0 = wall_clock64();
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
     const int ld = n + 1;
-    double* Zc = dyn;              // 16 x ld
-    double* Vs = dyn + 16 * ld;    // 16 x ld: Vs[c][j] = V[j][c]
+    const int ntj = (n + 15) >> 4;
+    double* Vs = dyn;                 // 16 x ld: Vs[c][j] = V[j][c]
+    double* Yp = Vs + 16 * ld;        // 8 x 16 x 64 partial Y^T
+    double* Ys = Yp + 8 * 1024;       // 16 x 64: Y^T, then Y2^T
+    double* Ts = Ys + 1024;           // 16 x 16
     double* ZT = amat(arena, jb, AR_C);
     const double* W = amat(arena, jb, AR_W);
     const double* Tall = eig_ws(tri, jb, job) + 3 * n;
-    const int nr = min(16, n - r0);
-    for (int x = tid; x < 16 * n; x += 256) {
-        const int r = x / n, j = x - r * n;
-        Zc[r * ld + j] = r < nr ? ZT[(long long)(r0 + r) * n + j] : 0.0;
+    dbl4 acc[JT][4];
+#pragma unroll
+    for (int x = 0; x < JT; ++x) {
+        const int tj = wave + 8 * x;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            const int r = r0 + 16 * rt + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = 16 * tj + 4 * v + lk;
+                acc[x][rt][v] = (tj < ntj && r < n && j < n) ? ZT[(long long)r * n + j] : 0.0;
+            }
+        }
     }
     const int nblk = (nref + kBt - 1) / kBt;
     for (int bi = nblk - 1; bi >= 0; --bi) {
         const int k0 = bi * kBt, nb = min(kBt, nref - k0);
-        const int jlo = ((k0 + 1) >> 4) << 4;   // first 16-column tile touched by this block
-        for (int x = tid; x < 16 * (n - jlo); x += 256) {
-            const int cc = x / (n - jlo), j = jlo + (x - cc * (n - jlo));
+        const int tlo = (k0 + 1) >> 4;   // first column tile touched by this block
+        const int jlo = tlo << 4, span = n - jlo;
+        for (int x = tid; x < 16 * span; x += 512) {
+            const int cc = x / span, j = jlo + (x - cc * span);
             Vs[cc * ld + j] = (cc < nb && j > k0 + cc) ? W[(long long)(k0 + cc) * n + j] : 0.0;
         }
-        for (int x = tid; x < kBt * kBt; x += 256) Ts[x] = Tall[(long long)bi * kBt * kBt + x];
+        if (tid < kBt * kBt) Ts[tid] = Tall[(long long)bi * kBt * kBt + tid];
         __syncthreads();
-        // Y = Z V (16 x 16), K = j split over the 4 wavefronts
-        dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
-        for (int j0 = jlo + 4 * wave; j0 < n; j0 += 16) {
-            const int j = j0 + lk;
-            const double a = j < n ? Zc[lr * ld + j] : 0.0;
-            const double b = j < n ? Vs[lr * ld + j] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-        }
+        // partial Y^T[c][r] over this wave's column tiles
+        dbl4 y[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Yp[wave * kBt * kBt + (4 * v + lk) * kBt + lr] = acc[v];
-        __syncthreads();
-        {   // Y2 = -(Y T^T): Y2[r][c] = -sum_{c' >= c} Y[r][c'] T[c][c']
-            const int r = tid >> 4, cc = tid & 15;
-            double s = 0.0;
-            for (int c2 = cc; c2 < kBt; ++c2) {
-                const int yi = r * kBt + c2;
-                const double y = Yp[yi] + Yp[kBt * kBt + yi] + Yp[2 * kBt * kBt + yi] + Yp[3 * kBt * kBt + yi];
-                s += y * Ts[cc * kBt + c2];
+        for (int rt = 0; rt < 4; ++rt) y[rt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int x = 0; x < JT; ++x) {
+            const int tj = wave + 8 * x;
+            if (tj < tlo || tj >= ntj) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = 16 * tj + 4 * v + lk;
+                const double a = j < n ? Vs[lr * ld + j] : 0.0;
+#pragma unroll
+                for (int rt = 0; rt < 4; ++rt) y[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[x][rt][v], y[rt], 0, 0, 0);
             }
-            Y2[r * kBt + cc] = -s;
+        }
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Yp[wave * 1024 + (4 * v + lk) * 64 + 16 * rt + lr] = y[rt][v];
+        __syncthreads();
+        for (int x = tid; x < 1024; x += 512) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < 8; ++w) s += Yp[w * 1024 + x];
+            Ys[x] = s;
         }
         __syncthreads();
-        // Z[:, j] += Y2 V^T over 16-column tiles, spread over the wavefronts
-        for (int jt = jlo + 16 * wave; jt < n; jt += 64) {
-            const int j = jt + lr;
-            dbl4 z;
+        double y2[2];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) z[v] = j < n ? Zc[(4 * v + lk) * ld + j] : 0.0;
+        for (int u = 0; u < 2; ++u) {   // Y2^T[c][r] = -sum_{c2 >= c} T[c][c2] Y^T[c2][r]
+            const int x = tid + 512 * u, cc = x >> 6, r = x & 63;
+            double s = 0.0;
+            for (int c2 = cc; c2 < kBt; ++c2) s += Ts[cc * kBt + c2] * Ys[c2 * 64 + r];
+            y2[u] = -s;
+        }
+        __syncthreads();
+        Ys[tid] = y2[0];
+        Ys[tid + 512] = y2[1];
+        __syncthreads();
+        // Z^T[j][r] += sum_c V[j][c] Y2^T[c][r]
+#pragma unroll
+        for (int x = 0; x < JT; ++x) {
+            const int tj = wave + 8 * x;
+            if (tj < tlo || tj >= ntj) continue;
+            const int j = 16 * tj + lr;
 #pragma unroll
             for (int c0 = 0; c0 < kBt; c0 += 4) {
-                const double a = Y2[lr * kBt + c0 + lk];
-                const double b = j < n ? Vs[(c0 + lk) * ld + j] : 0.0;
-                z = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, z, 0, 0, 0);
-            }
+                const double a = j < n ? Vs[(c0 + lk) * ld + j] : 0.0;
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-                if (j < n) Zc[(4 * v + lk) * ld + j] = z[v];
+                for (int rt = 0; rt < 4; ++rt) {
+                    const double b = Ys[(c0 + lk) * 64 + 16 * rt + lr];
+                    acc[x][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[x][rt], 0, 0, 0);
+                }
+            }
         }
         __syncthreads();
     }
-    for (int x = tid; x < nr * n; x += 256) {
-        const int r = x / n, j = x - r * n;
-        ZT[(long long)(r0 + r) * n + j] = Zc[r * ld + j];
+#pragma unroll
+    for (int x = 0; x < JT; ++x) {
+        const int tj = wave + 8 * x;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            const int r = r0 + 16 * rt + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = 16 * tj + 4 * v + lk;
+                if (tj < ntj && r < n && j < n) ZT[(long long)r * n + j] = acc[x][rt][v];
+            }
+        }
     }
     if (tid == 0 && blockIdx.y == 0) state[job].prof[3] = (double)(wall_clock64() - t0);
+}
+
+static size_t bt_lds_bytes(int n) { return ((size_t)16 * (n + 1) + 8 * 1024 + 1024 + 256) * 8; }
+
+template <int JT>
+static void launch_bt(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
+                      double* arena, double* tri) {
+    const size_t lb = bt_lds_bytes(max_n);
+    hipFuncSetAttribute((const void*)k_bt_wy<JT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+    hipLaunchKernelGGL((k_bt_wy<JT>), dim3(nlist, (max_n + 63) / 64), dim3(512), lb, st, jobs, state, joblist, arena,
+                       tri);
 }
 
 bool heig_supported(int n) { return n <= 512; }
@@ -730,10 +827,10 @@ void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, cons
     if (max_n >= 3) {
         const int nblk = (max_n - 2 + kBt - 1) / kBt;
         hipLaunchKernelGGL(k_wy_t, dim3(nlist, nblk), dim3(64), 0, st, jobs, state, joblist, arena, tri);
-        const size_t lb = (size_t)2 * 16 * (max_n + 1) * 8;
-        hipFuncSetAttribute((const void*)k_bt_wy, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
-        hipLaunchKernelGGL(k_bt_wy, dim3(nlist, (max_n + 15) / 16), dim3(256), lb, st, jobs, state, joblist, arena,
-                           tri);
+        if (max_n <= 128) launch_bt<1>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else if (max_n <= 256) launch_bt<2>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else if (max_n <= 384) launch_bt<3>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else launch_bt<4>(st, jobs, state, joblist, nlist, max_n, arena, tri);
     }
 }
 
