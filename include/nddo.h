@@ -21,8 +21,10 @@ extern "C" {
 
 /* Theory selector (MolDS TheoryType MNDO / AM1 / PM3, src/base/Enums.h). */
 enum { NDDO_MNDO = 0, NDDO_AM1 = 1, NDDO_PM3 = 2 };
-/* Precision of the SCF iterations: FP64 everywhere, or mixed (FP32 two-electron
- * contraction + FP64 eigensolve/DIIS/energy, finished with FP64 polish iterations). */
+/* Precision of the SCF iterations: FP64 everywhere, or mixed: FP32 two-electron
+ * Fock contraction (FP32 copy of the integrals) and split-FP32 MFMA density while
+ * the density RMS is above max(rms_density, 1e-5), then FP64 polish iterations to
+ * convergence; the eigensolve, DIIS, energy and heat of formation stay FP64. */
 enum { NDDO_FP64 = 0, NDDO_MIXED = 1 };
 
 /* Status codes.  The reference throws MolDSException for the same conditions:
@@ -83,6 +85,8 @@ typedef struct {
     int nao;
     int nocc;
     int iterations;          /* SCF iterations run, including the converged one */
+    int fp64_iterations;     /* of these, iterations run in FP64 (NDDO_MIXED: the polish iterations) */
+    int reserved;
     double energy_total;     /* elecSCFEnergy: electronic + core repulsion (Hartree) */
     double energy_elec;
     double energy_core;      /* coreRepulsionEnergy (Hartree) */

@@ -118,7 +118,7 @@ Batch::~Batch() { release(); }
 
 void Batch::release() {
     for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
-                     (void**)&d_ints, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
+                     (void**)&d_ints, (void**)&d_ints32, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
                      (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
                      (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_sdbg}) {
         if (*p) hipFree(*p);
@@ -276,6 +276,9 @@ int Batch::allocate(bool want_sdbg) {
     HIPCHK(hipMalloc(&d_state, nj * sizeof(JobState)));
     HIPCHK(hipMalloc(&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(TileDev)));
     HIPCHK(hipMalloc(&d_ints, std::max<long long>(nints, 1) * sizeof(double)));
+    bool any_mixed = false;
+    for (const JobDev& jb : jobs) any_mixed |= jb.precision == NDDO_MIXED;
+    if (any_mixed) HIPCHK(hipMalloc(&d_ints32, std::max<long long>(nints, 1) * sizeof(float)));
     HIPCHK(hipMalloc(&d_arena, std::max<long long>(arena_size, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&d_part, std::max<long long>(nslots, 1) * 20 * sizeof(double)));
     HIPCHK(hipMalloc(&d_ecore, std::max<long long>(nslots, 1) * sizeof(double)));
@@ -309,6 +312,7 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs) {
     std::memset(init.data(), 0, init.size() * sizeof(JobState));
     for (size_t j = 0; j < jobs.size(); ++j) {
         init[j].phase = PH_ACTIVE;
+        init[j].fp32 = jobs[j].precision == NDDO_MIXED ? 1 : 0;
         const nddo_job_desc& d = descs[src[j]];
         if (!d.requires_guess && d.P0) {
             const long long n = jobs[j].nao;
@@ -325,10 +329,11 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs) {
 void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
     launch_setup(st, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, (int)jobs.size(),
                  d_ints, d_arena, d_part, d_ecore, d_state, d_sdbg);
+    if (d_ints32) launch_ints_f32(st, d_ints, d_ints32, nints);
 }
 
 void Batch::launch_iteration(Engine* e, hipStream_t st) {
-    launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints,
+    launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
                 d_arena, d_part);
     launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
     launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri,
@@ -337,7 +342,7 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
                  jacobi_lds_bytes(maxn_glb, false), false);
     launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
     launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
-    launch_fock(st, 1, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints,
+    launch_fock(st, 1, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
                 d_arena, d_part);
     launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
 }
@@ -363,7 +368,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             // the same launches as launch_iteration, with an event between stages
             HIPCHK(hipEventRecord(e->ev[0], st));
             launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
-                        d_ints, d_arena, d_part);
+                        d_ints, d_ints32, d_arena, d_part);
             HIPCHK(hipEventRecord(e->ev[1], st));
             launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
             launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri,
@@ -376,7 +381,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
             HIPCHK(hipEventRecord(e->ev[4], st));
             launch_fock(st, 1, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
-                        d_ints, d_arena, d_part);
+                        d_ints, d_ints32, d_arena, d_part);
             launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
             HIPCHK(hipEventRecord(e->ev[5], st));
         }
@@ -410,6 +415,8 @@ int Batch::fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* re
     res->status = s.phase == PH_DONE ? s.status : NDDO_NOT_CONVERGED;
     if (s.sweeps < 0 && res->status == NDDO_OK) res->status = NDDO_EIGENSOLVER_FAILED;
     res->iterations = s.status == 0 && s.phase == PH_DONE ? s.iter + 1 : s.iter;
+    res->fp64_iterations = jb.precision == NDDO_MIXED ? (s.fp32 ? 0 : res->iterations - s.switch_iter)
+                                                       : res->iterations;
     res->energy_core = s.e_core;
     res->energy_elec = s.e_el;
     res->energy_total = s.e_el + s.e_core;
@@ -683,7 +690,7 @@ int nddo_dbg_fock(nddo_engine* eh, int theory, int natoms, const int* Z, const d
     if (P) HIPCHK(hipMemcpy(b.d_arena + (long long)AR_P0 * nn, P, nn * sizeof(double), hipMemcpyHostToDevice));
     b.launch_setup_stage(e, e->stream);
     launch_fock(e->stream, 0, b.d_pairs, (int)b.pairs.size(), b.d_atoms, (int)b.atoms.size(), e->d_elems, b.d_jobs,
-                b.d_state, b.d_ints, b.d_arena, b.d_part);
+                b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(F, b.d_arena + (long long)AR_F * nn, nn * sizeof(double), hipMemcpyDeviceToHost));

@@ -43,6 +43,7 @@ struct Batch {
     JobState* d_state = nullptr;
     TileDev* d_tiles = nullptr;
     double* d_ints = nullptr;
+    float* d_ints32 = nullptr;   // FP32 copy of the integrals (only when a job is mixed-precision)
     double* d_arena = nullptr;
     double* d_part = nullptr;
     double* d_ecore = nullptr;

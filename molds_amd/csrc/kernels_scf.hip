@@ -47,11 +47,15 @@ __device__ inline double h1c(const ElemDev& E, int a, int b) {
 // written to both triangles, and the Coulomb partials of the pair to part[slot]:
 //   J_A[mu nu] = sum_{lam sig in B} P_lam,sig (mu nu|lam sig)   (slots 0..9)
 //   J_B[lam sig] = sum_{mu nu in A} P_mu,nu (mu nu|lam sig)      (slots 10..19)
+// Jobs in their FP32 (mixed-precision) iterations read the FP32 copy of the
+// integrals and contract in FP32; F_AB = H_AB - 1/2 K is assembled in FP64.
 __global__ void __launch_bounds__(256) k_fock_pairs(
     int mode, const PairDev* __restrict__ pairs, int npairs, const AtomDev* __restrict__ atoms,
     const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs, const JobState* __restrict__ state,
-    const double* __restrict__ ints, double* __restrict__ arena, double* __restrict__ part) {
+    const double* __restrict__ ints, const float* __restrict__ ints32, double* __restrict__ arena,
+    double* __restrict__ part) {
     __shared__ double sI[4][100], sPab[4][16], sWa[4][10], sWb[4][10];
+    __shared__ float fI[4][100], fPab[4][16], fWa[4][10], fWb[4][10];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int pidx = blockIdx.x * 4 + wave;
     bool act = pidx < npairs;
@@ -65,10 +69,11 @@ __global__ void __launch_bounds__(256) k_fock_pairs(
         iter = state[pr.job].iter;
         act = mode == 0 ? phase == PH_ACTIVE : phase == PH_ENERGY;
     }
-    bool guess = false;
+    bool guess = false, f32 = false;
     int nA = 0, nB = 0, nap = 0, nbp = 0, aoA = 0, aoB = 0, n = 0;
     const double* P = nullptr;
     if (act) {
+        f32 = mode == 0 && ints32 != nullptr && state[pr.job].fp32 != 0;
         const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
         nA = elems[aA.elem].nao;
         nB = elems[aB.elem].nao;
@@ -80,7 +85,27 @@ __global__ void __launch_bounds__(256) k_fock_pairs(
         guess = mode == 0 && iter == 0 && jb.requires_guess;
         int pk = mode == 0 ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1));
         P = mat(arena, jb, pk);
-        if (!guess) {
+        if (!guess && f32) {
+            for (int e = lane; e < nap * nbp; e += 64) fI[wave][e] = ints32[pr.ioff + e];
+            if (lane < 16) {
+                int i = lane >> 2, j = lane & 3;
+                fPab[wave][lane] = (i < nA && j < nB) ? (float)P[(long long)(aoA + i) * n + aoB + j] : 0.0f;
+            } else if (lane < 26) {
+                int q = lane - 16;
+                if (q < nap) {
+                    int mu = q == 0 ? 0 : (q < 4 ? 0 : (q < 7 ? 1 : (q < 9 ? 2 : 3)));
+                    int nu = q < 4 ? q : (q < 7 ? q - 3 : (q < 9 ? q - 5 : 3));
+                    fWa[wave][q] = (float)((mu == nu ? 1.0 : 2.0) * P[(long long)(aoA + mu) * n + aoA + nu]);
+                }
+            } else if (lane >= 32 && lane < 42) {
+                int q = lane - 32;
+                if (q < nbp) {
+                    int mu = q == 0 ? 0 : (q < 4 ? 0 : (q < 7 ? 1 : (q < 9 ? 2 : 3)));
+                    int nu = q < 4 ? q : (q < 7 ? q - 3 : (q < 9 ? q - 5 : 3));
+                    fWb[wave][q] = (float)((mu == nu ? 1.0 : 2.0) * P[(long long)(aoB + mu) * n + aoB + nu]);
+                }
+            }
+        } else if (!guess) {
             for (int e = lane; e < nap * nbp; e += 64) sI[wave][e] = ints[pr.ioff + e];
             if (lane < 16) {
                 int i = lane >> 2, j = lane & 3;
@@ -106,6 +131,35 @@ __global__ void __launch_bounds__(256) k_fock_pairs(
     if (!act) return;
     double* F = mat(arena, jb, AR_F);
     const double* H = mat(arena, jb, AR_H);
+    if (f32 && !guess) {
+        if (lane < 10) {
+            if (lane < nap) {
+                float s = 0.0f;
+                for (int j = 0; j < nbp; ++j) s += fI[wave][lane * nbp + j] * fWb[wave][j];
+                part[(long long)pr.slot * 20 + lane] = (double)s;
+            }
+        } else if (lane >= 16 && lane < 26) {
+            int j = lane - 16;
+            if (j < nbp) {
+                float s = 0.0f;
+                for (int i = 0; i < nap; ++i) s += fI[wave][i * nbp + j] * fWa[wave][i];
+                part[(long long)pr.slot * 20 + 10 + j] = (double)s;
+            }
+        } else if (lane >= 32 && lane < 48) {
+            int mu = (lane - 32) >> 2, la = (lane - 32) & 3;
+            if (mu < nA && la < nB) {
+                long long ij = (long long)(aoA + mu) * n + aoB + la;
+                float k = 0.0f;
+                for (int nu = 0; nu < nA; ++nu)
+                    for (int sg = 0; sg < nB; ++sg)
+                        k += fPab[wave][nu * 4 + sg] * fI[wave][pk_index(mu, nu) * nbp + pk_index(la, sg)];
+                const double v = H[ij] - 0.5 * (double)k;
+                F[ij] = v;
+                F[(long long)(aoB + la) * n + aoA + mu] = v;
+            }
+        }
+        return;
+    }
     if (lane < 10) {
         if (!guess && lane < nap) {
             double s = 0.0;
@@ -199,10 +253,10 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
 
 void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
                  const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
-                 double* arena, double* part) {
+                 const float* ints32, double* arena, double* part) {
     if (npairs > 0)
         hipLaunchKernelGGL(k_fock_pairs, dim3((npairs + 3) / 4), dim3(256), 0, st, mode, pairs, npairs, atoms,
-                           elems, jobs, state, ints, arena, part);
+                           elems, jobs, state, ints, ints32, arena, part);
     hipLaunchKernelGGL(k_fock_atoms, dim3((natoms + 3) / 4), dim3(256), 0, st, mode, atoms, natoms, elems, jobs,
                        state, arena, part);
 }
@@ -658,6 +712,71 @@ void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev*
 // P = 2 C_occ^T C_occ (rows of C are MOs; occupied = the nocc lowest after the
 // eigenvalue sort), one 64x64 tile per workgroup; each tile also emits its
 // partial sum of (P - P_old)^2 for the RMS test.
+
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Mixed precision: every operand split as x = hi + lo (hi = fp32(x),
+// lo = fp32(x - hi)) and P ~ 2 (A_hi B_hi + A_hi B_lo + A_lo B_hi) on
+// v_mfma_f32_16x16x4f32 with FP32 accumulation; each wavefront owns a 32x32
+// quadrant of the tile (2 x 2 MFMA blocks).
+__device__ void density_tile_split_f32(const TileDev& tl, const JobDev& jb, const double* VT, const int* perm,
+                                       const double* Pin, double* Pout, double* red, double* rmspart) {
+    const int n = jb.nao, nocc = jb.nocc;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    const int ib = tl.i0 + 32 * (wave >> 1), jbb = tl.j0 + 32 * (wave & 1);
+    flt4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = flt4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int o0 = 0; o0 < nocc; o0 += 4) {
+        const int o = o0 + lk;
+        const long long row = o < nocc ? (long long)perm[jb.ivec0 + o] * n : -1;
+        float ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int i = ib + 16 * x + lr, j = jbb + 16 * x + lr;
+            const double a = (row >= 0 && i < n) ? VT[row + i] : 0.0;
+            const double b = (row >= 0 && j < n) ? VT[row + j] : 0.0;
+            ah[x] = (float)a;
+            al[x] = (float)(a - (double)ah[x]);
+            bh[x] = (float)b;
+            bl[x] = (float)(b - (double)bh[x]);
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[x], bh[y], acc[x][y], 0, 0, 0);
+            }
+    }
+    // D[i][j] of a 16x16 FP32 block sits in lane j + 16 (i / 4), element i % 4
+    double loc = 0.0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = ib + 16 * x + 4 * lk + v, j = jbb + 16 * y + lr;
+                if (i < n && j < n) {
+                    const double pv = 2.0 * (double)acc[x][y][v];
+                    const long long ij = (long long)i * n + j;
+                    const double dd = pv - Pin[ij];
+                    Pout[ij] = pv;
+                    loc += dd * dd;
+                }
+            }
+    red[tid] = loc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) rmspart[blockIdx.x] = red[0];
+}
 __global__ void __launch_bounds__(256) k_density(const TileDev* __restrict__ tiles, const JobDev* __restrict__ jobs,
                                                  const JobState* __restrict__ state, double* __restrict__ arena,
                                                  const int* __restrict__ perm, double* __restrict__ rmspart) {
@@ -672,6 +791,10 @@ __global__ void __launch_bounds__(256) k_density(const TileDev* __restrict__ til
     __shared__ double sA[16][68], sB[16][68];
     __shared__ double red[256];
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    if (state[tl.job].fp32) {
+        density_tile_split_f32(tl, jb, VT, perm, Pin, Pout, red, rmspart);
+        return;
+    }
     double acc[4][4];
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
@@ -778,9 +901,18 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
     }
     __syncthreads();
     const double rms = sh_rms;
-    if (rms < jb.rms_thresh) {
+    const bool in32 = st->fp32 != 0;
+    if (rms < jb.rms_thresh && !in32) {
         if (tid == 0) { st->rms = rms; st->phase = PH_ENERGY; }
         return;
+    }
+    // a mixed-precision job leaves its FP32 iterations once the density RMS is
+    // below max(rms_density, 1e-5) (well above the FP32 noise floor of the Fock
+    // contraction) or after half its iteration budget; the FP64 polish
+    // iterations that follow decide convergence
+    if (in32 && tid == 0 && (rms < fmax(jb.rms_thresh, 1e-5) || 2 * (iter + 1) >= jb.max_iter)) {
+        st->fp32 = 0;
+        st->switch_iter = iter + 1;
     }
     const bool guess = iter == 0 && jb.requires_guess;
     const double* Pin = mat(arena, jb, AR_P0 + (iter & 1));

@@ -16,6 +16,8 @@
 //     sum_ij q_i q_j / sqrt(|R_B + d_j - d_i|^2 + (rho_A + rho_B)^2),
 // which equals the closed forms (52)-(66) term by term.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 
 #include "nddo_device.h"
@@ -376,6 +378,18 @@ void launch_setup(hipStream_t st, const PairDev* pairs, int npairs, const AtomDe
     hipLaunchKernelGGL(k_atom_hcore, dim3((natoms + 3) / 4), dim3(256), 0, st, atoms, natoms, elems, jobs,
                        part, arena);
     hipLaunchKernelGGL(k_ecore_reduce, dim3(njobs), dim3(256), 0, st, jobs, ecore, state);
+}
+
+__global__ void __launch_bounds__(256) k_ints_f32(const double* __restrict__ ints, float* __restrict__ ints32,
+                                                 long long nints) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < nints; i += (long long)gridDim.x * 256)
+        ints32[i] = (float)ints[i];
+}
+
+void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long long nints) {
+    if (nints <= 0) return;
+    const long long blocks = std::min<long long>((nints + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_ints_f32, dim3((unsigned)blocks), dim3(256), 0, st, ints, ints32, nints);
 }
 
 int upload_overlap_table(const double* table /* 32*25 */) {

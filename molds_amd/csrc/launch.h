@@ -13,11 +13,16 @@ void launch_setup(hipStream_t st, const PairDev* pairs, int npairs, const AtomDe
                   const ElemDev* elems, const JobDev* jobs, int njobs, double* ints, double* arena,
                   double* part, double* ecore, JobState* state, double* sdbg);
 
-// mode 0: SCF Fock for PH_ACTIVE jobs (guess on iteration 0 when requested);
-// mode 1: non-guess Fock of the final density for PH_ENERGY jobs.
+// FP32 copy of the packed integrals for mixed-precision jobs.
+void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long long nints);
+
+// mode 0: SCF Fock for PH_ACTIVE jobs (guess on iteration 0 when requested;
+// jobs in their FP32 iterations contract ints32, which may be null when no job
+// of the batch is mixed-precision);
+// mode 1: non-guess FP64 Fock of the final density for PH_ENERGY jobs.
 void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
                  const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
-                 double* arena, double* part);
+                 const float* ints32, double* arena, double* part);
 
 // Warm start: F <- VT F VT^T for PH_ACTIVE jobs whose VT holds a previous solve.
 void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, const JobState* state,

@@ -84,6 +84,8 @@ struct JobDev {
 struct JobState {
     int phase, iter, status, diis_on;
     int damp_on, ring_start, warm, sweeps;
+    int fp32;          // mixed-precision job still in its FP32 iterations
+    int switch_iter;   // first FP64 (polish) iteration of a mixed-precision job
     double rms, diis_error, e_el, e_core;
     double prof[4];   // eigensolver phase times of the last solve (device clock ticks)
     double B[(NDDO_MAX_DIIS + 1) * (NDDO_MAX_DIIS + 1)];  // DIIS bordered matrix

@@ -41,7 +41,8 @@ class JobDesc(ctypes.Structure):
 
 class Result(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("nao", ctypes.c_int), ("nocc", ctypes.c_int),
-                ("iterations", ctypes.c_int), ("energy_total", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("fp64_iterations", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("energy_total", ctypes.c_double),
                 ("energy_elec", ctypes.c_double), ("energy_core", ctypes.c_double),
                 ("heat_of_formation", ctypes.c_double), ("final_rms", ctypes.c_double),
                 ("latency_s", ctypes.c_double), ("eps", _dp), ("C", _dp), ("P", _dp),
@@ -191,7 +192,7 @@ class Engine:
     @staticmethod
     def _pack(r, bufs, job):
         out = ScfResult(status=r.status, status_name=STATUS.get(r.status, str(r.status)), nao=r.nao, nocc=r.nocc,
-                        iterations=r.iterations, energy=r.energy_total, e_el=r.energy_elec, e_core=r.energy_core,
+                        iterations=r.iterations, fp64_iterations=r.fp64_iterations, energy=r.energy_total, e_el=r.energy_elec, e_core=r.energy_core,
                         hof=r.heat_of_formation, rms=r.final_rms, latency_s=r.latency_s, eps=bufs["eps"],
                         C=bufs.get("C"), P=bufs.get("P"))
         if "traj" in bufs:
