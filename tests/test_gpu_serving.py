@@ -1,0 +1,74 @@
+"""Served mode (continuous batching): requests submitted while others are on
+the GPU are admitted at the next SCF-iteration boundary, retire at their own
+convergence, and give the same result as when solved alone (total energy
+within 1e-9 eV, same iteration count, density within 1e-10)."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EV = 1.0 / 0.03674903
+
+
+def _job(kind, size, seed, precision="fp64", rms=1e-8, damping=0.7):
+    from molds_amd import workloads as W
+    from molds_amd.nddo import Job
+    Z, x = W.alkane(size, seed=seed) if kind == "alkane" else W.water_cluster(size, seed=seed, jitter=0.02)
+    return Job("pm3", Z, W.to_bohr(x), precision=precision, rms_density=rms, diis_num_error_vect=5,
+               diis_start_error=0.1, diis_end_error=0.0, damping_weight=damping, damping_thresh=0.02, max_iter=200)
+
+
+def test_served_matches_alone_and_overlaps(engine):
+    print("served overlap", flush=True)
+    # the big request needs far more SCF iterations than the small later ones
+    big = _job("water", 64, 31, rms=1e-10)
+    first = [big, _job("alkane", 9, 32), _job("water", 12, 33, "mixed")]
+    second = [_job("alkane", 1, 34, damping=0.0), _job("alkane", 2, 35, "mixed", damping=0.0),
+              _job("water", 2, 36, damping=0.0)]
+    t_sub, ids = {}, {}
+    for j in first:
+        ids[id(j)] = engine.submit(j)
+        t_sub[id(j)] = time.perf_counter()
+    time.sleep(0.005)
+    for j in second:
+        ids[id(j)] = engine.submit(j)
+        t_sub[id(j)] = time.perf_counter()
+    res = {}
+    for j in first + second:
+        st = engine.wait(ids[id(j)], 60.0)
+        assert st == 0, st
+        res[id(j)] = engine.get_result(ids[id(j)], j, want_P=True)
+        assert engine.release(ids[id(j)]) == 0
+    done = {k: t_sub[k] + r["latency_s"] for k, r in res.items()}
+    # the small requests submitted after the big one joined it on the GPU and
+    # retired at their own convergence, before it
+    assert max(r["iterations"] for r in (res[id(j)] for j in second)) + 3 < res[id(big)]["iterations"]
+    assert max(done[id(j)] for j in second) < done[id(big)]
+    for j in first + second:
+        r = res[id(j)]
+        alone = engine.scf_batch([j], want_P=True)[0]
+        assert r["status"] == 0 and alone["status"] == 0
+        assert abs(r["energy"] - alone["energy"]) * EV <= 1e-9
+        assert r["iterations"] == alone["iterations"]
+        assert np.max(np.abs(r["P"] - alone["P"])) <= 1e-10
+        assert r["latency_s"] > 0.0
+
+
+def test_served_statuses(engine):
+    from molds_amd.nddo import Engine, Job
+    odd = Job("pm3", [6, 1, 1, 1], np.eye(4, 3) * 2.0)
+    jid = engine.submit(odd)
+    assert engine.poll(jid) == -2          # ODD_ELECTRONS, known at submit
+    assert engine.release(jid) == 0
+    assert engine.poll(jid) == -7          # BAD_ARG: released
+    small = Engine(0, hbm_budget_bytes=1 << 20)
+    try:
+        jid = small.submit(_job("water", 20, 41))
+        assert small.wait(jid, 10.0) == -4  # ADMISSION_DENIED
+    finally:
+        small.close()
+    # a queued or running request can be released before it finishes
+    jid = engine.submit(_job("water", 30, 42))
+    assert engine.release(jid) == 0
