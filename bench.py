@@ -1,26 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X NDDO SCF engine on its headline workload.
 
-Metric (north star): PM3 SCF solves/s with p50/p99 latency, mixed precision vs
-the FP64 GPU path, at |dE| <= 1e-5 eV.
+Metric: PM3 SCF solves/s with p50/p99 latency, mixed precision vs the FP64
+GPU path, at |dE| <= 1e-5 eV.
 
-Workload of one step: the batched PM3 mix -- 4096 neutral closed-shell
+Step (the timed unit): the batched PM3 mix -- 4096 neutral closed-shell
 molecules, even index an all-trans alkane C_nH_{2n+2} with n ~ U{1..40}, odd
 index a water cluster (H2O)_m with m ~ U{2..64}, seeded geometries with
 N(0, 0.02 A) jitter -- solved to density RMS < 1e-8 (core-Hamiltonian guess,
-DIIS k = 5 / start 0.1 / end 0, damping 0.7 above RMS 0.02) in ONE batch: one launch per SCF
-stage for all molecules.  Inputs are resident on the device before the timed
-region (each step re-uploads the geometry -- a few MB -- and rebuilds all
-integrals on the GPU: that setup is part of every solve).
+DIIS k = 5 / start 0.1 / end 0, damping 0.7 above RMS 0.02) in ONE batch: one
+launch per SCF stage for all molecules.  The geometry (a few MB) is uploaded
+at the start of each step and every integral is rebuilt on the GPU inside the
+timed region.
+
+Beside it, in the same run:
+  * served legs: Poisson arrivals (default 1000 requests/s, 2000 requests,
+    sizes log-uniform on 5..256 atoms over the alkane / water generators,
+    retired at density RMS < 1e-7) through the continuous-batching request
+    loop, mixed and FP64, with p50/p99 latency (submit -> retire);
+  * the plain FP64 GPU path: the same 4096-molecule mix solved one molecule at
+    a time (nddo_scf_once), on a bounded prefix;
+  * the CPU oracle on a bounded prefix of the same mix (cpu_baseline).
 
     python bench.py [--steps K] [--warmup W] [--count N] [--gpus 1]
 
-Prints ONE JSON line.  `value` is the solves/s of the fastest precision mode
-whose energies agree with the FP64 GPU path within 1e-5 eV.
+Prints ONE JSON line.  `value` is the batched solves/s of the fastest
+precision mode whose energies agree with the FP64 GPU path within 1e-5 eV.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -37,6 +45,7 @@ FP64_PEAK_TFLOPS = 78.6       # MI355X dense FP64 (matrix and vector)
 # keeps the randomly oriented water clusters out of occupation-swapping cycles
 SCF = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0,
            damping_weight=0.7, damping_thresh=0.02, max_iter=100)
+SERVED_SCF = dict(SCF, rms_density=1e-7)
 CPU_SCF = dict(rms_density=1e-8, diis_k=5, diis_start=0.1, diis_end=0.0, damping_weight=0.7, damping_thresh=0.02,
                max_iter=100)
 
@@ -45,14 +54,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_jobs(count, precision):
+def nao_of(Z):
+    return int(sum(1 if z == 1 else 4 for z in Z))
+
+
+def make_jobs(mols, precision, opts=SCF):
     from molds_amd import workloads as W
     from molds_amd.nddo import Job
-    mols = list(W.batch_mix(count))
-    return mols, [Job("pm3", Z, W.to_bohr(x), precision=precision, **SCF) for _, _, Z, x in mols]
+    return [Job("pm3", Z, W.to_bohr(x), precision=precision, **opts) for _, _, Z, x in mols]
 
 
-def gpu_leg(engine, jobs, steps, warmup):
+def batched_leg(engine, jobs, steps, warmup):
     for _ in range(warmup):
         engine.scf_batch(jobs)
     engine.profile(True)
@@ -65,6 +77,53 @@ def gpu_leg(engine, jobs, steps, warmup):
     st = engine.stage_times(reset=True)
     engine.profile(False)
     return res, wall, st
+
+
+def served_leg(engine, rate, count, precision, seed=3):
+    """Poisson arrivals through nddo_submit; returns throughput and latency percentiles."""
+    from molds_amd import workloads as W
+    from molds_amd.nddo import Job
+    reqs = [(t, Job("pm3", Z, W.to_bohr(x), precision=precision, **SERVED_SCF))
+            for t, _, _, Z, x in W.poisson_requests(rate, count, seed=seed, lo=5, hi=256)]
+    ids = []
+    t0 = time.perf_counter()
+    for t, job in reqs:
+        now = time.perf_counter() - t0
+        if t > now:
+            time.sleep(t - now)
+        ids.append((engine.submit(job), time.perf_counter() - t0, job))
+    lat, done_at, ok, fp64_it, it = [], [], 0, [], []
+    for jid, ts, job in ids:
+        engine.wait(jid, 300.0)
+        r = engine.get_result(jid, job)
+        engine.release(jid)
+        if r["status"] == 0:
+            ok += 1
+        lat.append(r["latency_s"])
+        done_at.append(ts + r["latency_s"])
+        it.append(r["iterations"])
+        fp64_it.append(r["fp64_iterations"])
+    span = max(done_at)
+    lat = np.array(lat)
+    return dict(precision=precision, rate_per_s=rate, requests=count, converged=ok,
+                solves_per_s=round(count / span, 3), span_s=round(span, 3),
+                p50_s=round(float(np.percentile(lat, 50)), 5), p99_s=round(float(np.percentile(lat, 99)), 5),
+                mean_iterations=round(float(np.mean(it)), 2), mean_fp64_iterations=round(float(np.mean(fp64_it)), 2),
+                sizes="log-uniform 5..256 atoms, alkane / water clusters", rms_density=SERVED_SCF["rms_density"])
+
+
+def plain_leg(engine, jobs, seconds):
+    """The plain FP64 GPU path: one molecule per nddo_scf_once call, in order."""
+    t0 = time.perf_counter()
+    done = 0
+    for j in jobs:
+        engine.scf_batch([j])
+        done += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return dict(solves_per_s=round(done / dt, 3), sample=f"first {done} molecules of the mix, one nddo_scf_once "
+                                                         f"each, FP64 ({dt:.1f} s)")
 
 
 def cpu_leg(mols, seconds):
@@ -88,7 +147,7 @@ def cpu_leg(mols, seconds):
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
-    return dict(value=done / dt, unit="solves/s", cores=threads, kind="port",
+    return dict(value=round(done / dt, 3), unit="solves/s", cores=threads, kind="port",
                 sample=f"first {done} molecules of the same PM3 mix ({conv} converged), solved one at a time "
                        f"(oracle: C Fock/integrals + numpy LAPACK dsyevd, {dt:.1f} s)")
 
@@ -96,40 +155,47 @@ def cpu_leg(mols, seconds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--served-rate", type=float, default=1000.0)
+    ap.add_argument("--served-count", type=int, default=2000)
+    ap.add_argument("--plain-seconds", type=float, default=8.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
-              f"(the batched workload shards by molecule across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the batched and served "
+              f"workloads shard by molecule across GPUs, one engine per GPU)", file=sys.stderr)
         sys.exit(2)
 
+    from molds_amd import workloads as W
     from molds_amd.nddo import Engine
     engine = Engine(0)
     t = time.perf_counter()
-    mols, jobs64 = make_jobs(args.count, "fp64")
+    mols = list(W.batch_mix(args.count))
+    nao = np.array([nao_of(m[2]) for m in mols], dtype=float)
     log(f"built {len(mols)} molecules in {time.perf_counter() - t:.1f} s")
-    nao = np.array([sum(1 if z == 1 else 4 for z in m[2]) for m in mols])
 
     legs = {}
     for mode in args.modes.split(","):
-        jobs = jobs64 if mode == "fp64" else make_jobs(args.count, mode)[1]
-        res, wall, st = gpu_leg(engine, jobs, args.steps, args.warmup)
+        jobs = make_jobs(mols, mode)
+        res, wall, st = batched_leg(engine, jobs, args.steps, args.warmup)
         ok = np.array([r["status"] == 0 for r in res])
-        iters = np.array([r["iterations"] for r in res])
+        iters = np.array([r["iterations"] for r in res], dtype=float)
         legs[mode] = dict(res=res, wall=wall, stages=st, ok=ok, iters=iters,
                           solves_per_s=len(jobs) * args.steps / wall, ms_per_step=1e3 * wall / args.steps)
         log(f"{mode}: {legs[mode]['solves_per_s']:.1f} solves/s, {legs[mode]['ms_per_step']:.1f} ms/step, "
             f"converged {ok.sum()}/{len(ok)}, mean iters {iters.mean():.1f}, stages {st}")
+        del jobs
 
     base = legs["fp64"]
     e64 = np.array([r["energy"] for r in base["res"]])
     best_mode, best = "fp64", base
-    max_de = {}
+    max_de = {"fp64": 0.0}
     for mode, lg in legs.items():
         if mode == "fp64":
             continue
@@ -139,13 +205,26 @@ def main():
         if max_de[mode] <= 1e-5 and lg["solves_per_s"] > best["solves_per_s"]:
             best_mode, best = mode, lg
 
-    # roofline of the dominant kernel (the eigensolver): algorithmic FP64 flops
-    # (10/3) n^3 per eigensolve (one per SCF iteration per molecule) over the
-    # measured eigen-stage time on the engine's stream
-    eig_ms, eig_launches = best["stages"]["eigen"]
-    flops = args.steps * float(np.sum(best["iters"] * (10.0 / 3.0) * nao.astype(float) ** 3))
-    achieved = flops / (eig_ms * 1e-3) / 1e12 if eig_ms > 0 else 0.0
-    lat = [best["wall"] / args.steps] * len(best["res"])
+    # roofline of the dominant kernel of the step: the tridiagonalization of
+    # the matrices that do not fit LDS (k_tridiag<false>), timed with hipEvents
+    # on the engine's stream around its launches; algorithmic work (4/3) n^3
+    # FP64 flops per eigensolve, one eigensolve per SCF iteration per molecule
+    info = engine.info()
+    glb = nao > info["lds_tridiag_max_n"]
+    ms_tri, launches_tri = best["stages"]["tridiag_glb"]
+    flops = args.steps * float(np.sum(best["iters"][glb] * (4.0 / 3.0) * nao[glb] ** 3))
+    achieved = flops / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
+    kernels = {k: round(v[0] / args.steps, 3) for k, v in best["stages"].items()}
+
+    served = {}
+    if not args.no_served:
+        for prec in ("mixed", "fp64"):
+            served[prec] = served_leg(engine, args.served_rate, args.served_count, prec)
+            log(f"served {prec}: {served[prec]}")
+    plain = None
+    if not args.no_plain:
+        plain = plain_leg(engine, make_jobs(mols, "fp64"), args.plain_seconds)
+        log(f"plain fp64: {plain}")
 
     out = {
         "metric": METRIC,
@@ -161,7 +240,7 @@ def main():
         "dtype": "f64" if best_mode == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"batched PM3 SCF, {args.count} molecules: 50% all-trans alkanes C_nH_2n+2 "
-                               f"n~U[1,40], 50% water clusters (H2O)_m m~U[2,64], seeded jitter, "
+                               f"n~U[1,40], 50% water clusters (H2O)_m m~U[2,64], seeded N(0,0.02 A) jitter, "
                                f"density RMS < 1e-8, one launch per SCF stage",
                    "molecules": args.count, "mode": best_mode},
         "fp64_solves_per_s": round(base["solves_per_s"], 3),
@@ -170,12 +249,22 @@ def main():
                       "max_abs_dE_vs_fp64_eV": max_de.get(m, 0.0),
                       "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in lg["stages"].items()}}
                   for m, lg in legs.items()},
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                      "note": "batched mode: every molecule retires with its batch"},
-        "roofline": {"bound": "mfma", "kernel": "k_jacobi (FP64 eigensolve)", "achieved": round(achieved, 4),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 6),
-                     "traffic": None},
+        "latency_s": ({"p50": served["mixed"]["p50_s"], "p99": served["mixed"]["p99_s"],
+                       "source": "served mixed-precision leg (submit -> retire)"} if served else None),
+        "served": served or None,
+        "plain_fp64_gpu": plain,
+        "speedup_vs_plain_fp64": ({"batched": round(best["solves_per_s"] / plain["solves_per_s"], 2)}
+                                  if plain else None),
+        "roofline": {"bound": "mfma", "kernel": "k_tridiag<false> (Householder tridiagonalization, device memory)",
+                     "achieved": round(achieved, 5), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None,
+                     "avg_launch_ms": round(ms_tri / max(launches_tri, 1), 4), "launches": launches_tri,
+                     "lds_tridiag_max_n": info["lds_tridiag_max_n"],
+                     "kernel_ms_per_step": kernels},
     }
+    if plain and served:
+        out["speedup_vs_plain_fp64"]["served_mixed"] = round(served["mixed"]["solves_per_s"] / plain["solves_per_s"],
+                                                              2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(mols, args.cpu_seconds)
     engine.close()

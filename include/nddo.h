@@ -129,11 +129,20 @@ int nddo_release(nddo_engine* e, nddo_job_id id);
  * omp_get_wtime around DoSCF, Cndo2.cpp:587, 724-728).  With profiling on, the
  * engine records hipEvents around every stage launch on its stream and
  * accumulates milliseconds and launch counts per stage:
- * 0 setup, 1 fock, 2 eigen, 3 density, 4 step (RMS/DIIS/damping), 5 energy. */
+ * 0 setup, 1 fock, 2 eigen, 3 density, 4 step (RMS/DIIS/damping), 5 energy;
+ * and the parts of the eigen stage: 6 tridiagonalization of the matrices kept
+ * in LDS, 7 tridiagonalization of the matrices in device memory (or the Jacobi
+ * solver beyond 512 AOs), 8 divide and conquer, 9 back-transformation. */
 enum { NDDO_STAGE_SETUP = 0, NDDO_STAGE_FOCK, NDDO_STAGE_EIGEN, NDDO_STAGE_DENSITY, NDDO_STAGE_STEP,
-       NDDO_STAGE_ENERGY, NDDO_STAGE_COUNT };
+       NDDO_STAGE_ENERGY, NDDO_STAGE_TRIDIAG_LDS, NDDO_STAGE_TRIDIAG_GLB, NDDO_STAGE_DC,
+       NDDO_STAGE_BACKTRANSFORM, NDDO_STAGE_COUNT };
 int nddo_engine_profile(nddo_engine* e, int enable);
 int nddo_engine_stage_times(nddo_engine* e, double* ms, long long* launches, int reset);
+
+/* Device facts the engine plans with: LDS bytes per workgroup, compute units,
+ * the HBM admission budget (bytes), and the largest AO count whose matrix the
+ * eigensolver keeps in LDS for its tridiagonalization. */
+int nddo_engine_info(nddo_engine* e, size_t* lds_bytes, int* num_cus, size_t* hbm_budget, int* lds_tridiag_max_n);
 
 /* Text of the last error on the calling thread. */
 const char* nddo_last_error(void);

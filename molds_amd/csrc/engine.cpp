@@ -366,36 +366,54 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             launch_iteration(e, st);
         } else {
             // the same launches as launch_iteration, with an event between stages
+            // and between the parts of the eigensolver
             HIPCHK(hipEventRecord(e->ev[0], st));
             launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
                         d_ints, d_ints32, d_arena, d_part);
             HIPCHK(hipEventRecord(e->ev[1], st));
             launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
-            launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri,
-                         jacobi_lds_bytes(maxn_lds, true), true);
-            launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri,
-                         jacobi_lds_bytes(maxn_glb, false), false);
+            const bool hl = heig_supported(maxn_lds), hg = heig_supported(maxn_glb);
+            if (hl) launch_heig_part(0, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
+                                     d_perm, d_tri, true);
             HIPCHK(hipEventRecord(e->ev[2], st));
-            launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
+            if (hg) launch_heig_part(0, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps,
+                                     d_perm, d_tri, false);
+            if (!hl) launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
+                                  d_tri, jacobi_lds_bytes(maxn_lds, true), true);
+            if (!hg) launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
+                                  d_tri, jacobi_lds_bytes(maxn_glb, false), false);
             HIPCHK(hipEventRecord(e->ev[3], st));
+            for (int part = 1; part < 3; ++part) {
+                if (hl) launch_heig_part(part, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena,
+                                         d_eps, d_perm, d_tri, true);
+                if (hg) launch_heig_part(part, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena,
+                                         d_eps, d_perm, d_tri, false);
+                HIPCHK(hipEventRecord(e->ev[3 + part], st));
+            }
+            launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
+            HIPCHK(hipEventRecord(e->ev[6], st));
             launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
-            HIPCHK(hipEventRecord(e->ev[4], st));
+            HIPCHK(hipEventRecord(e->ev[7], st));
             launch_fock(st, 1, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
                         d_ints, d_ints32, d_arena, d_part);
             launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
-            HIPCHK(hipEventRecord(e->ev[5], st));
+            HIPCHK(hipEventRecord(e->ev[8], st));
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(h_rem, d_remaining, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (prof) {
-            const int map[5] = {NDDO_STAGE_FOCK, NDDO_STAGE_EIGEN, NDDO_STAGE_DENSITY, NDDO_STAGE_STEP,
-                                NDDO_STAGE_ENERGY};
-            for (int k = 0; k < 5; ++k) {
+            // (stage, first event, last event)
+            const int map[9][3] = {{NDDO_STAGE_FOCK, 0, 1},         {NDDO_STAGE_EIGEN, 1, 5},
+                                   {NDDO_STAGE_TRIDIAG_LDS, 1, 2},  {NDDO_STAGE_TRIDIAG_GLB, 2, 3},
+                                   {NDDO_STAGE_DC, 3, 4},           {NDDO_STAGE_BACKTRANSFORM, 4, 5},
+                                   {NDDO_STAGE_DENSITY, 5, 6},      {NDDO_STAGE_STEP, 6, 7},
+                                   {NDDO_STAGE_ENERGY, 7, 8}};
+            for (int k = 0; k < 9; ++k) {
                 float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, e->ev[k], e->ev[k + 1]));
-                e->stage_ms[map[k]] += ms;
-                e->stage_launches[map[k]] += 1;
+                HIPCHK(hipEventElapsedTime(&ms, e->ev[map[k][1]], e->ev[map[k][2]]));
+                e->stage_ms[map[k][0]] += ms;
+                e->stage_launches[map[k][0]] += 1;
             }
         }
         if (*h_rem <= 0) break;
@@ -507,7 +525,7 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
         return NDDO_HIP_ERROR;
     }
     HIPCHK(hipHostMalloc((void**)&e->h_remaining, sizeof(int), hipHostMallocDefault));
-    for (int i = 0; i < 8; ++i) HIPCHK(hipEventCreate(&e->ev[i]));
+    for (int i = 0; i < 16; ++i) HIPCHK(hipEventCreate(&e->ev[i]));
     size_t freeb = 0, totb = 0;
     HIPCHK(hipMemGetInfo(&freeb, &totb));
     e->hbm_budget = e->opts.hbm_budget_bytes ? e->opts.hbm_budget_bytes : (size_t)(0.9 * freeb);
@@ -523,7 +541,7 @@ int nddo_engine_destroy(nddo_engine* eh) {
     hipSetDevice(e->device);
     if (e->d_elems) hipFree(e->d_elems);
     if (e->h_remaining) hipHostFree(e->h_remaining);
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 16; ++i)
         if (e->ev[i]) hipEventDestroy(e->ev[i]);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
@@ -563,6 +581,20 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
         rc = b.fetch(e->stream, jj, d, &res[ok[jj]], hof);
         if (rc) return rc;
         res[ok[jj]].latency_s = wall;
+    }
+    return NDDO_OK;
+}
+
+int nddo_engine_info(nddo_engine* eh, size_t* lds_bytes, int* num_cus, size_t* hbm_budget, int* lds_tridiag_max_n) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    if (lds_bytes) *lds_bytes = e->lds_max;
+    if (num_cus) *num_cus = e->num_cus;
+    if (hbm_budget) *hbm_budget = e->hbm_budget;
+    if (lds_tridiag_max_n) {
+        int n = 0;
+        while (n < 512 && e->lds_max > 9216 && jacobi_lds_bytes(n + 1, true) <= e->lds_max - 9216) ++n;
+        *lds_tridiag_max_n = n;
     }
     return NDDO_OK;
 }

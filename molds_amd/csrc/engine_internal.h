@@ -84,7 +84,7 @@ struct Engine {
     bool profile = false;
     double stage_ms[NDDO_STAGE_COUNT] = {};
     long long stage_launches[NDDO_STAGE_COUNT] = {};
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[16] = {};
     double last_eig_prof[4] = {};
 
     // serving state (server.cpp)

@@ -808,23 +808,25 @@ bool heig_supported(int n) { return n <= 512; }
 
 size_t heig_lds_bytes(int n, bool a_in_lds) { return tridiag_lds_bytes(n, a_in_lds); }
 
-void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
-                       double* arena, double* eps, int* perm, double* tri, bool a_in_lds) {
+void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
+                      int max_n, double* arena, double* eps, int* perm, double* tri, bool a_in_lds) {
     if (nlist <= 0) return;
-    const int threads = max_n <= 32 ? 256 : (max_n <= 96 ? 512 : 1024);
-    const size_t lt = tridiag_lds_bytes(max_n, a_in_lds);
-    if (a_in_lds) {
-        hipFuncSetAttribute((const void*)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
-        hipLaunchKernelGGL(k_tridiag<true>, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
-    } else {
-        hipFuncSetAttribute((const void*)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
-        hipLaunchKernelGGL(k_tridiag<false>, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
-    }
-    const int tdc = max_n <= 64 ? 256 : (max_n <= 128 ? 512 : 1024);
-    const size_t ld = dc_lds_bytes(max_n);
-    hipFuncSetAttribute((const void*)k_dc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ld);
-    hipLaunchKernelGGL(k_dc, dim3(nlist), dim3(tdc), ld, st, jobs, state, joblist, arena, tri, eps, perm);
-    if (max_n >= 3) {
+    if (part == 0) {
+        const int threads = max_n <= 32 ? 256 : (max_n <= 96 ? 512 : 1024);
+        const size_t lt = tridiag_lds_bytes(max_n, a_in_lds);
+        if (a_in_lds) {
+            hipFuncSetAttribute((const void*)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
+            hipLaunchKernelGGL(k_tridiag<true>, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
+        } else {
+            hipFuncSetAttribute((const void*)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
+            hipLaunchKernelGGL(k_tridiag<false>, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
+        }
+    } else if (part == 1) {
+        const int tdc = max_n <= 64 ? 256 : (max_n <= 128 ? 512 : 1024);
+        const size_t ld = dc_lds_bytes(max_n);
+        hipFuncSetAttribute((const void*)k_dc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ld);
+        hipLaunchKernelGGL(k_dc, dim3(nlist), dim3(tdc), ld, st, jobs, state, joblist, arena, tri, eps, perm);
+    } else if (max_n >= 3) {
         const int nblk = (max_n - 2 + kBt - 1) / kBt;
         hipLaunchKernelGGL(k_wy_t, dim3(nlist, nblk), dim3(64), 0, st, jobs, state, joblist, arena, tri);
         if (max_n <= 128) launch_bt<1>(st, jobs, state, joblist, nlist, max_n, arena, tri);
@@ -832,6 +834,12 @@ void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, cons
         else if (max_n <= 384) launch_bt<3>(st, jobs, state, joblist, nlist, max_n, arena, tri);
         else launch_bt<4>(st, jobs, state, joblist, nlist, max_n, arena, tri);
     }
+}
+
+void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
+                       double* arena, double* eps, int* perm, double* tri, bool a_in_lds) {
+    for (int part = 0; part < 3; ++part)
+        launch_heig_part(part, st, jobs, state, joblist, nlist, max_n, arena, eps, perm, tri, a_in_lds);
 }
 
 }  // namespace nddo

@@ -37,6 +37,9 @@ size_t heig_lds_bytes(int n, bool a_in_lds);
 bool heig_supported(int n);
 void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
                        double* arena, double* eps, int* perm, double* tri, bool a_in_lds);
+// One part of it: 0 tridiagonalization, 1 divide and conquer, 2 back-transformation.
+void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
+                      int max_n, double* arena, double* eps, int* perm, double* tri, bool a_in_lds);
 void launch_jacobi(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
                    double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds);
 void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,

@@ -78,6 +78,8 @@ def load_library(path=LIBPATH):
     L.nddo_last_error.restype = ctypes.c_char_p
     L.nddo_engine_profile.argtypes = [vp, ctypes.c_int]
     L.nddo_engine_stage_times.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+    L.nddo_engine_info.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int)]
     L.nddo_dbg_integrals.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp]
     L.nddo_dbg_fock.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, _dp]
     L.nddo_dbg_eigh.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _ip]
@@ -221,14 +223,20 @@ class Engine:
         return [self._pack(results[i], keep[i], jobs[i]) for i in range(len(jobs))]
 
     # --------------------------------------------------------------- profiling
-    STAGES = ("setup", "fock", "eigen", "density", "step", "energy")
+    STAGES = ("setup", "fock", "eigen", "density", "step", "energy", "tridiag_lds", "tridiag_glb", "divide_conquer",
+              "backtransform")
+
+    def info(self):
+        lds, cus, budget, nmax = ctypes.c_size_t(), ctypes.c_int(), ctypes.c_size_t(), ctypes.c_int()
+        self.L.nddo_engine_info(self.h, ctypes.byref(lds), ctypes.byref(cus), ctypes.byref(budget), ctypes.byref(nmax))
+        return dict(lds_bytes=lds.value, num_cus=cus.value, hbm_budget=budget.value, lds_tridiag_max_n=nmax.value)
 
     def profile(self, enable=True):
         self.L.nddo_engine_profile(self.h, 1 if enable else 0)
 
     def stage_times(self, reset=False):
-        ms = np.zeros(6)
-        n = (ctypes.c_longlong * 6)()
+        ms = np.zeros(len(self.STAGES))
+        n = (ctypes.c_longlong * len(self.STAGES))()
         self.L.nddo_engine_stage_times(self.h, _p(ms), n, 1 if reset else 0)
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
 

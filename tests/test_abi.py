@@ -51,3 +51,23 @@ def test_engine_without_gpu_fails_loudly():
     from molds_amd.nddo import Engine, NddoError
     with pytest.raises(NddoError):
         Engine(0)
+
+
+def test_stage_names_match_header_enum():
+    """The binding's stage list has one name per NDDO_STAGE_* entry, in order."""
+    from molds_amd.nddo import Engine
+    txt = open(HEADER).read()
+    body = re.search(r"enum\s*\{\s*(NDDO_STAGE_SETUP[^}]*)\}", txt).group(1)
+    names = [t.strip().split("=")[0].strip() for t in body.split(",") if t.strip()]
+    assert names[-1] == "NDDO_STAGE_COUNT"
+    assert len(Engine.STAGES) == len(names) - 1
+    assert [n[len("NDDO_STAGE_"):].lower() for n in names[:-1]][:6] == list(Engine.STAGES[:6])
+
+
+def test_result_struct_matches_header():
+    """ctypes nddo_result fields follow the header's declaration order."""
+    from molds_amd.nddo import Result
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\} nddo_result;", txt).group(1)
+    decl = [re.findall(r"([A-Za-z_0-9]+)\s*;", line)[0] for line in body.split("\n") if ";" in line]
+    assert [f[0] for f in Result._fields_] == decl
