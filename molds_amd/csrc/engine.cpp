@@ -138,7 +138,7 @@ int validate_job(const nddo_job_desc& d, int* nao, int* nocc) {
     int n = 0, ne = 0;
     for (int a = 0; a < d.natoms; ++a) {
         if (elem_slot(d.Z[a]) < 0) {
-            set_error("Error in Cndo2::CheckEnableAtomType: Non available atom is contained (Z=" +
+            set_error("Error in cndo::Cndo2::CheckEnableAtomType: Non available atom is contained.\n(Z=" +
                       std::to_string(d.Z[a]) + ")");
             return NDDO_UNSUPPORTED_ELEMENT;
         }
@@ -148,14 +148,24 @@ int validate_job(const nddo_job_desc& d, int* nao, int* nocc) {
         ne += (int)e.core;
     }
     if (ne % 2 != 0) {
-        set_error("Error in SetMolecule: Total number of valence electrons is odd. totalNumberValenceElectrons=" +
-                  std::to_string(ne));
+        set_error("Error in cndo::Cndo2::SetMolecule: Total number of valence electrons is odd. "
+                  "totalNumberValenceElectrons=" + std::to_string(ne));
         return NDDO_ODD_ELECTRONS;
     }
     for (int a = 0; a < 3 * d.natoms; ++a)
         if (!std::isfinite(d.xyz_bohr[a])) {
             set_error("non-finite coordinate");
             return NDDO_BAD_ARG;
+        }
+    // coincident atoms have no diatomic frame (R = 0 in every pair formula)
+    for (int a = 0; a < d.natoms; ++a)
+        for (int b = a + 1; b < d.natoms; ++b) {
+            const double dx = d.xyz_bohr[3 * a] - d.xyz_bohr[3 * b], dy = d.xyz_bohr[3 * a + 1] - d.xyz_bohr[3 * b + 1],
+                         dz = d.xyz_bohr[3 * a + 2] - d.xyz_bohr[3 * b + 2];
+            if (dx * dx + dy * dy + dz * dz < 1e-12) {
+                set_error("atoms " + std::to_string(a) + " and " + std::to_string(b) + " coincide");
+                return NDDO_BAD_ARG;
+            }
         }
     *nao = n;
     *nocc = ne / 2;
@@ -623,7 +633,7 @@ int nddo_scf_once(nddo_engine* e, const nddo_job_desc* job, nddo_result* res) {
     int rc = nddo_scf_batch(e, 1, job, res);
     if (rc) return rc;
     if (res->status == NDDO_NOT_CONVERGED)
-        set_error("Error in Cndo2::DoSCF: SCF did not met convergence criterion. maxIterationsSCF=" +
+        set_error("Error in cndo::Cndo2::DoSCF: SCF did not met convergence criterion. maxIterationsSCF=" +
                   std::to_string(job->opts.max_iter));
     return res->status;
 }

@@ -313,6 +313,13 @@ __device__ void dc_secular(const double* dl, const double* zl, int kk, double rh
     *taup = tau;
 }
 
+// Strict total order on (value, index) with NaN placed after +inf, so that the
+// positions computed from it are always a permutation of the block.
+__device__ inline bool dc_before(double u, int iu, double v, int iv) {
+    const double ku = isnan(u) ? INFINITY : u, kv = isnan(v) ? INFINITY : v;
+    return ku < kv || (ku == kv && iu < iv);
+}
+
 // Rows of QT (AR_C / AR_Q2, ping-pong) are eigenvectors of the current blocks;
 // the block [a, b) of level l only touches rows and columns a..b-1.  U (AR_U)
 // holds each merge's kk x kk eigenvector matrix in its diagonal block.
@@ -353,6 +360,17 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
     double* QC = amat(arena, jb, AR_C);
     double* QX = amat(arena, jb, AR_Q2);
     double* U = amat(arena, jb, AR_U);
+    // a non-finite tridiagonal (from a non-finite Fock matrix) fails the solve
+    int bad = 0;
+    for (int i = tid; i < n; i += T) bad |= !isfinite(d[i]) || (i < n - 1 && !isfinite(e[i]));
+    if (__syncthreads_or(bad)) {
+        for (int i = tid; i < n; i += T) {
+            eps[jb.vec0 + i] = 0.0;
+            perm[jb.ivec0 + i] = i;
+        }
+        if (tid == 0) state[job].sweeps = -1;
+        return;
+    }
     int L = 0;
     while ((1 << L) < n) ++L;
     {
@@ -381,16 +399,10 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             if (i >= m && e[m - 1] < 0.0) z = -z;
             z *= 0.70710678118654752440;
             const double di = Dc[i];
-            int pos;
-            if (i < m) {
-                int lo = m, hi = b;   // right-block entries < di
-                while (lo < hi) { const int md = (lo + hi) >> 1; if (Dc[md] < di) lo = md + 1; else hi = md; }
-                pos = (i - a) + (lo - m);
-            } else {
-                int lo = a, hi = m;   // left-block entries <= di
-                while (lo < hi) { const int md = (lo + hi) >> 1; if (Dc[md] <= di) lo = md + 1; else hi = md; }
-                pos = (i - m) + (lo - a);
-            }
+            // position in the merged order: the stable merge of the two sorted blocks
+            // (left before right on ties), counted directly so it is a permutation
+            int pos = 0;
+            for (int t = a; t < b; ++t) pos += dc_before(Dc[t], t, di, i);
             ds[a + pos] = di;
             zs[a + pos] = z;
             col[a + pos] = i;
@@ -496,10 +508,7 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             }
             const double v = vals[i];
             int rk = 0;
-            for (int t = a; t < b; ++t) {
-                const double u = vals[t];
-                rk += (u < v) || (u == v && t < i);
-            }
+            for (int t = a; t < b; ++t) rk += dc_before(vals[t], t, v, i);
             rank[i] = rk;
         }
         __syncthreads();

@@ -900,6 +900,15 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
         tr[3] = st->damp_on;
     }
     __syncthreads();
+    if (st->sweeps < 0) {   // the eigensolver rejected this job's Fock matrix
+        if (tid == 0) {
+            st->phase = PH_DONE;
+            st->status = -5;   // NDDO_EIGENSOLVER_FAILED
+            st->rms = sh_rms;
+            atomicSub(remaining, 1);
+        }
+        return;
+    }
     const double rms = sh_rms;
     const bool in32 = st->fp32 != 0;
     if (rms < jb.rms_thresh && !in32) {

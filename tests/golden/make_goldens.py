@@ -4,8 +4,8 @@ Reads the reference test inputs (test/<case>.in) and golden outputs
 (test/<case>.dat) as text and keeps only data: the SCF knobs, the geometry,
 and the printed results of the ground-state SCF (iteration lines, MO energies,
 electronic energy incl. core repulsion, core repulsion, MNDO heat of formation,
-Mulliken charges).  Run in the build container only (the GPU box has no
-reference tree):  python tests/golden/make_goldens.py /root/reference/test
+Mulliken charges).  Run where the reference tree is available (the machines
+that run the GPU tests have none):  python tests/golden/make_goldens.py /root/reference/test
 """
 import glob
 import json

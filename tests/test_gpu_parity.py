@@ -2,7 +2,7 @@
 
 Tolerances: integrals, overlap, core Hamiltonian, Fock: 1e-12 relative to the
 largest element (FP64 everywhere, different but exact-in-theory formula order);
-total energy: 1e-6 eV vs the oracle (the north-star FP64 bar), 7 printed digits
+total energy: 1e-6 eV vs the oracle (the FP64 parity bar), 7 printed digits
 vs MolDS's goldens; density RMS vs the oracle: 1e-8.
 """
 import numpy as np
@@ -212,3 +212,15 @@ def test_error_statuses(engine):
     c = _case("c2h6_pm3")
     nc = engine.scf_batch([Job("pm3", c["Z"], case_xyz_bohr(c), max_iter=3, rms_density=1e-10)])[0]
     assert nc["status_name"] == "NOT_CONVERGED" and nc["iterations"] == 3
+
+
+def test_degenerate_geometry_rejected(engine):
+    """Coincident atoms and non-finite coordinates are rejected on the host
+    (BAD_ARG), before any kernel sees them."""
+    from molds_amd.nddo import Job
+    xyz = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [1.8, 0.0, 0.0]])
+    r = engine.scf_batch([Job("pm3", [8, 1, 1], xyz)])[0]
+    assert r["status_name"] == "BAD_ARG"
+    xyz[2, 1] = np.nan
+    r = engine.scf_batch([Job("pm3", [8, 1, 1], xyz)])[0]
+    assert r["status_name"] == "BAD_ARG"
