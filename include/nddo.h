@@ -161,6 +161,9 @@ int nddo_dbg_fock(nddo_engine* e, int theory, int natoms, const int* Z, const do
                   const double* P, int guess, double* F);
 int nddo_dbg_eigh(nddo_engine* e, int n, const double* F, double* eps, double* C, int* sweeps);
 int nddo_dbg_density(nddo_engine* e, int n, int nocc, const double* C, double* P);
+/* Serving scheduler's host wall time in seconds: admission + cohort setup,
+ * launches, waiting for the device, retirement; and cohort iterations run. */
+int nddo_dbg_server_stats(nddo_engine* e, double* seconds4, long long* iterations, int reset);
 /* Device-clock ticks of the last nddo_dbg_eigh phases: tridiagonalization,
  * tridiagonal divide and conquer, (unused, 0), back-transformation; tick rate in Hz. */
 int nddo_dbg_eigh_profile(nddo_engine* e, double* ticks4, double* tick_hz);

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -121,9 +122,23 @@ void Batch::release() {
                      (void**)&d_ints, (void**)&d_ints32, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
                      (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
                      (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_sdbg}) {
-        if (*p) hipFree(*p);
+        if (*p) {
+            if (alloc_stream) hipFreeAsync(*p, alloc_stream);
+            else hipFree(*p);
+        }
         *p = nullptr;
     }
+}
+
+// Stream-ordered allocations (served cohorts) do not synchronize the device.
+int Batch::dalloc(void** p, size_t bytes) {
+    const hipError_t err = alloc_stream ? hipMallocAsync(p, bytes, alloc_stream) : hipMalloc(p, bytes);
+    if (err != hipSuccess) {
+        set_error(std::string("HIP error ") + hipGetErrorString(err) + " in device allocation");
+        *p = nullptr;
+        return NDDO_HIP_ERROR;
+    }
+    return NDDO_OK;
 }
 
 int validate_job(const nddo_job_desc& d, int* nao, int* nocc) {
@@ -279,42 +294,67 @@ int Batch::classify(size_t lds_max) {
 
 int Batch::allocate(bool want_sdbg) {
     release();
+#define ACHK(x)                      \
+    do {                             \
+        const int rc_ = (x);         \
+        if (rc_ != NDDO_OK) return rc_; \
+    } while (0)
     const size_t nj = std::max<size_t>(jobs.size(), 1);
-    HIPCHK(hipMalloc(&d_atoms, std::max<size_t>(atoms.size(), 1) * sizeof(AtomDev)));
-    HIPCHK(hipMalloc(&d_pairs, std::max<size_t>(pairs.size(), 1) * sizeof(PairDev)));
-    HIPCHK(hipMalloc(&d_jobs, nj * sizeof(JobDev)));
-    HIPCHK(hipMalloc(&d_state, nj * sizeof(JobState)));
-    HIPCHK(hipMalloc(&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(TileDev)));
-    HIPCHK(hipMalloc(&d_ints, std::max<long long>(nints, 1) * sizeof(double)));
+    ACHK(dalloc((void**)&d_atoms, std::max<size_t>(atoms.size(), 1) * sizeof(AtomDev)));
+    ACHK(dalloc((void**)&d_pairs, std::max<size_t>(pairs.size(), 1) * sizeof(PairDev)));
+    ACHK(dalloc((void**)&d_jobs, nj * sizeof(JobDev)));
+    ACHK(dalloc((void**)&d_state, nj * sizeof(JobState)));
+    ACHK(dalloc((void**)&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(TileDev)));
+    ACHK(dalloc((void**)&d_ints, std::max<long long>(nints, 1) * sizeof(double)));
     bool any_mixed = false;
     for (const JobDev& jb : jobs) any_mixed |= jb.precision == NDDO_MIXED;
-    if (any_mixed) HIPCHK(hipMalloc(&d_ints32, std::max<long long>(nints, 1) * sizeof(float)));
-    HIPCHK(hipMalloc(&d_arena, std::max<long long>(arena_size, 1) * sizeof(double)));
-    HIPCHK(hipMalloc(&d_part, std::max<long long>(nslots, 1) * 20 * sizeof(double)));
-    HIPCHK(hipMalloc(&d_ecore, std::max<long long>(nslots, 1) * sizeof(double)));
-    HIPCHK(hipMalloc(&d_eps, std::max<long long>(nvec, 1) * sizeof(double)));
-    HIPCHK(hipMalloc(&d_perm, std::max<long long>(nvec, 1) * sizeof(int)));
-    HIPCHK(hipMalloc(&d_tri, (EIG_WS_PER_AO * std::max<long long>(nvec, 1) + EIG_WS_PER_JOB * (long long)nj) *
+    if (any_mixed) ACHK(dalloc((void**)&d_ints32, std::max<long long>(nints, 1) * sizeof(float)));
+    ACHK(dalloc((void**)&d_arena, std::max<long long>(arena_size, 1) * sizeof(double)));
+    ACHK(dalloc((void**)&d_part, std::max<long long>(nslots, 1) * 20 * sizeof(double)));
+    ACHK(dalloc((void**)&d_ecore, std::max<long long>(nslots, 1) * sizeof(double)));
+    ACHK(dalloc((void**)&d_eps, std::max<long long>(nvec, 1) * sizeof(double)));
+    ACHK(dalloc((void**)&d_perm, std::max<long long>(nvec, 1) * sizeof(int)));
+    ACHK(dalloc((void**)&d_tri, (EIG_WS_PER_AO * std::max<long long>(nvec, 1) + EIG_WS_PER_JOB * (long long)nj) *
                                  sizeof(double)));
-    HIPCHK(hipMalloc(&d_rmspart, std::max<size_t>(tiles.size(), 1) * sizeof(double)));
-    HIPCHK(hipMalloc(&d_traj, std::max<long long>(ntraj, 1) * sizeof(double)));
-    HIPCHK(hipMalloc(&d_remaining, sizeof(int)));
-    HIPCHK(hipMalloc(&d_list_lds, std::max<size_t>(list_lds.size(), 1) * sizeof(int)));
-    HIPCHK(hipMalloc(&d_list_glb, std::max<size_t>(list_glb.size(), 1) * sizeof(int)));
-    if (want_sdbg) HIPCHK(hipMalloc(&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double)));
+    ACHK(dalloc((void**)&d_rmspart, std::max<size_t>(tiles.size(), 1) * sizeof(double)));
+    ACHK(dalloc((void**)&d_traj, std::max<long long>(ntraj, 1) * sizeof(double)));
+    ACHK(dalloc((void**)&d_remaining, sizeof(int)));
+    ACHK(dalloc((void**)&d_list_lds, std::max<size_t>(list_lds.size(), 1) * sizeof(int)));
+    ACHK(dalloc((void**)&d_list_glb, std::max<size_t>(list_glb.size(), 1) * sizeof(int)));
+    if (want_sdbg) ACHK(dalloc((void**)&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double)));
     return NDDO_OK;
+#undef ACHK
 }
 
-int Batch::upload(hipStream_t st, const nddo_job_desc* descs) {
-    HIPCHK(hipMemcpyAsync(d_atoms, atoms.data(), atoms.size() * sizeof(AtomDev), hipMemcpyHostToDevice, st));
-    if (!pairs.empty())
-        HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(PairDev), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice, st));
-    if (!list_lds.empty())
-        HIPCHK(hipMemcpyAsync(d_list_lds, list_lds.data(), list_lds.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    if (!list_glb.empty())
-        HIPCHK(hipMemcpyAsync(d_list_glb, list_glb.data(), list_glb.size() * sizeof(int), hipMemcpyHostToDevice, st));
+// Staging bytes upload() needs; independent of classify(): every job sits in
+// exactly one of the two eigensolver lists.
+size_t Batch::upload_bytes() const {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return up(atoms.size() * sizeof(AtomDev)) + up(pairs.size() * sizeof(PairDev)) + up(jobs.size() * sizeof(JobDev)) +
+           up(tiles.size() * sizeof(TileDev)) + 2 * up(jobs.size() * sizeof(int)) + up(jobs.size() * sizeof(JobState)) +
+           256;
+}
+
+// With a pinned staging buffer (stage, at least upload_bytes()), every copy is
+// asynchronous for the host; without one the copies come from pageable memory.
+int Batch::upload(hipStream_t st, const nddo_job_desc* descs, char* stage) {
+    size_t off = 0;
+    const size_t cap = stage ? upload_bytes() : 0;
+    auto h2d = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        if (!bytes) return hipSuccess;
+        if (!stage) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+        if (off + bytes > cap) return hipErrorInvalidValue;   // staging smaller than upload_bytes()
+        std::memcpy(stage + off, src, bytes);
+        const hipError_t err = hipMemcpyAsync(dst, stage + off, bytes, hipMemcpyHostToDevice, st);
+        off += (bytes + 255) & ~(size_t)255;
+        return err;
+    };
+    HIPCHK(h2d(d_atoms, atoms.data(), atoms.size() * sizeof(AtomDev)));
+    HIPCHK(h2d(d_pairs, pairs.data(), pairs.size() * sizeof(PairDev)));
+    HIPCHK(h2d(d_jobs, jobs.data(), jobs.size() * sizeof(JobDev)));
+    HIPCHK(h2d(d_tiles, tiles.data(), tiles.size() * sizeof(TileDev)));
+    HIPCHK(h2d(d_list_lds, list_lds.data(), list_lds.size() * sizeof(int)));
+    HIPCHK(h2d(d_list_glb, list_glb.data(), list_glb.size() * sizeof(int)));
     HIPCHK(hipMemsetAsync(d_arena, 0, std::max<long long>(arena_size, 1) * sizeof(double), st));
     HIPCHK(hipMemsetAsync(d_traj, 0, std::max<long long>(ntraj, 1) * sizeof(double), st));
     if (d_sdbg) HIPCHK(hipMemsetAsync(d_sdbg, 0, std::max<long long>(arena_size, 1) * sizeof(double), st));
@@ -330,9 +370,10 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs) {
                                   hipMemcpyHostToDevice, st));
         }
     }
-    HIPCHK(hipMemcpyAsync(d_state, init.data(), init.size() * sizeof(JobState), hipMemcpyHostToDevice, st));
-    int rem = (int)jobs.size();
-    HIPCHK(hipMemcpyAsync(d_remaining, &rem, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(h2d(d_state, init.data(), init.size() * sizeof(JobState)));
+    const int rem = (int)jobs.size();
+    HIPCHK(h2d(d_remaining, &rem, sizeof(int)));
+    if (!stage) HIPCHK(hipStreamSynchronize(st));   // pageable sources live on this stack frame
     return NDDO_OK;
 }
 
@@ -342,9 +383,13 @@ void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
     if (d_ints32) launch_ints_f32(st, d_ints, d_ints32, nints);
 }
 
+// One SCF iteration of every job: the Fock launch also rebuilds F from the final
+// density for jobs that converged in the previous iteration, whose energy is
+// then evaluated (Cndo2.cpp:1324-1416) before the eigensolve of the others.
 void Batch::launch_iteration(Engine* e, hipStream_t st) {
-    launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
+    launch_fock(st, 2, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
                 d_arena, d_part);
+    launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
     launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
     launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri,
                  jacobi_lds_bytes(maxn_lds, true), true);
@@ -352,9 +397,6 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
                  jacobi_lds_bytes(maxn_glb, false), false);
     launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
     launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
-    launch_fock(st, 1, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
-                d_arena, d_part);
-    launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
 }
 
 int Batch::run_scf(Engine* e, hipStream_t st) {
@@ -378,8 +420,10 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             // the same launches as launch_iteration, with an event between stages
             // and between the parts of the eigensolver
             HIPCHK(hipEventRecord(e->ev[0], st));
-            launch_fock(st, 0, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
+            launch_fock(st, 2, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
                         d_ints, d_ints32, d_arena, d_part);
+            HIPCHK(hipEventRecord(e->ev[9], st));
+            launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
             HIPCHK(hipEventRecord(e->ev[1], st));
             launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
             const bool hl = heig_supported(maxn_lds), hg = heig_supported(maxn_glb);
@@ -404,21 +448,17 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             HIPCHK(hipEventRecord(e->ev[6], st));
             launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
             HIPCHK(hipEventRecord(e->ev[7], st));
-            launch_fock(st, 1, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
-                        d_ints, d_ints32, d_arena, d_part);
-            launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
-            HIPCHK(hipEventRecord(e->ev[8], st));
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(h_rem, d_remaining, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (prof) {
             // (stage, first event, last event)
-            const int map[9][3] = {{NDDO_STAGE_FOCK, 0, 1},         {NDDO_STAGE_EIGEN, 1, 5},
+            const int map[9][3] = {{NDDO_STAGE_FOCK, 0, 9},         {NDDO_STAGE_EIGEN, 1, 5},
                                    {NDDO_STAGE_TRIDIAG_LDS, 1, 2},  {NDDO_STAGE_TRIDIAG_GLB, 2, 3},
                                    {NDDO_STAGE_DC, 3, 4},           {NDDO_STAGE_BACKTRANSFORM, 4, 5},
                                    {NDDO_STAGE_DENSITY, 5, 6},      {NDDO_STAGE_STEP, 6, 7},
-                                   {NDDO_STAGE_ENERGY, 7, 8}};
+                                   {NDDO_STAGE_ENERGY, 9, 1}};
             for (int k = 0; k < 9; ++k) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, e->ev[map[k][1]], e->ev[map[k][2]]));
@@ -431,12 +471,9 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
     return NDDO_OK;
 }
 
-// Copies job jj's results into res (status already set by the caller on failure).
-int Batch::fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* res, double hof_offset) {
-    const JobDev& jb = jobs[jj];
-    JobState s;
-    HIPCHK(hipMemcpyAsync(&s, d_state + jj, sizeof(JobState), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+// Scalar results of a job from its device state (shared by the synchronous
+// fetch and the serving scheduler).
+void fill_scalars(const JobDev& jb, const JobState& s, int theory, double hof_offset, nddo_result* res) {
     const int n = jb.nao;
     res->nao = n;
     res->nocc = jb.nocc;
@@ -448,13 +485,34 @@ int Batch::fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* re
     res->energy_core = s.e_core;
     res->energy_elec = s.e_el;
     res->energy_total = s.e_el + s.e_core;
-    res->heat_of_formation = d.theory == NDDO_MNDO ? res->energy_total + hof_offset : NAN;
+    res->heat_of_formation = theory == NDDO_MNDO ? res->energy_total + hof_offset : NAN;
     res->final_rms = s.rms;
     if (res->status != NDDO_OK) {
         res->energy_elec = NAN;
         res->energy_total = NAN;
         res->heat_of_formation = NAN;
     }
+}
+
+// C rows in rank order with MolDS's sign rule (Lapack.cpp:139-150): each row's element sum >= 0.
+void order_and_phase(int n, const double* VT, const int* perm, double* C) {
+    for (int r = 0; r < n; ++r) {
+        const double* src = &VT[(long long)perm[r] * n];
+        double sum = 0.0;
+        for (int i = 0; i < n; ++i) sum += src[i];
+        const double sg = sum < 0.0 ? -1.0 : 1.0;
+        for (int i = 0; i < n; ++i) C[(long long)r * n + i] = sg * src[i];
+    }
+}
+
+// Copies job jj's results into res (status already set by the caller on failure).
+int Batch::fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* res, double hof_offset) {
+    const JobDev& jb = jobs[jj];
+    JobState s;
+    HIPCHK(hipMemcpyAsync(&s, d_state + jj, sizeof(JobState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int n = jb.nao;
+    fill_scalars(jb, s, d.theory, hof_offset, res);
     std::vector<double> eps(n);
     std::vector<int> perm(n);
     HIPCHK(hipMemcpyAsync(eps.data(), d_eps + jb.vec0, n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -475,15 +533,7 @@ int Batch::fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* re
                               hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (res->eps) std::memcpy(res->eps, eps.data(), n * sizeof(double));
-    if (res->C) {
-        for (int r = 0; r < n; ++r) {
-            const double* src = &VT[(long long)perm[r] * n];
-            double sum = 0.0;
-            for (int i = 0; i < n; ++i) sum += src[i];
-            const double sg = sum < 0.0 ? -1.0 : 1.0;   // Lapack.cpp:139-150
-            for (int i = 0; i < n; ++i) res->C[(long long)r * n + i] = sg * src[i];
-        }
-    }
+    if (res->C) order_and_phase(n, VT.data(), perm.data(), res->C);
     return NDDO_OK;
 }
 
@@ -508,6 +558,10 @@ void nddo_scf_opts_default(nddo_scf_opts* o) {
 int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out) {
     if (!out) { set_error("null engine pointer"); return NDDO_BAD_ARG; }
     *out = nullptr;
+    // The serving scheduler runs up to 16 cohorts on streams of their own; ask
+    // HIP for as many hardware queues (its default is 4) unless the caller chose.
+    // Only effective when this is the process's first HIP call.
+    setenv("GPU_MAX_HW_QUEUES", "16", 0);
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (hip_device < 0 || hip_device >= ndev) {

@@ -24,8 +24,13 @@ size_t jacobi_lds_bytes(int n, bool a_in_lds);
 // Device bytes a job keeps resident (admission control; MolDS's MallocerFreer heap
 // cap, src/base/MallocerFreer.cpp:62-73, plays the same role on the host).
 size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter);
+// Device bytes a retired served job keeps for its arrays until nddo_get_result.
+size_t result_bytes(int n, int max_iter);
 
 struct Engine;
+
+void fill_scalars(const JobDev& jb, const JobState& s, int theory, double hof_offset, nddo_result* res);
+void order_and_phase(int n, const double* VT, const int* perm, double* C);
 
 // A set of jobs sharing one launch per SCF stage.
 struct Batch {
@@ -56,13 +61,16 @@ struct Batch {
     int* d_list_lds = nullptr;
     int* d_list_glb = nullptr;
     double* d_sdbg = nullptr;
+    hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream
 
     ~Batch();
+    int dalloc(void** p, size_t bytes);
     void release();
     int build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* unused);
     int classify(size_t lds_max);
     int allocate(bool want_sdbg);
-    int upload(hipStream_t st, const nddo_job_desc* descs);
+    size_t upload_bytes() const;
+    int upload(hipStream_t st, const nddo_job_desc* descs, char* stage = nullptr);
     void launch_setup_stage(Engine* e, hipStream_t st);
     void launch_iteration(Engine* e, hipStream_t st);
     int run_scf(Engine* e, hipStream_t st);
@@ -95,6 +103,9 @@ struct Engine {
     long long next_id = 1;
     std::map<long long, ServedJob*> served;
     std::deque<ServedJob*> queue;
+    // scheduler thread wall time: admission + cohort setup, launches, waiting, retirement
+    double server_s[4] = {};
+    long long server_turns = 0;   // cohort iterations completed
 
     void start_server();
     void stop_server();

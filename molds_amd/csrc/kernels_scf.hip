@@ -67,13 +67,14 @@ __global__ void __launch_bounds__(256) k_fock_pairs(
         jb = jobs[pr.job];
         phase = state[pr.job].phase;
         iter = state[pr.job].iter;
-        act = mode == 0 ? phase == PH_ACTIVE : phase == PH_ENERGY;
+        act = (mode != 1 && phase == PH_ACTIVE) || (mode != 0 && phase == PH_ENERGY);
     }
     bool guess = false, f32 = false;
     int nA = 0, nB = 0, nap = 0, nbp = 0, aoA = 0, aoB = 0, n = 0;
     const double* P = nullptr;
     if (act) {
-        f32 = mode == 0 && ints32 != nullptr && state[pr.job].fp32 != 0;
+        const bool scf = phase == PH_ACTIVE;   // else the final-density Fock of the energy
+        f32 = scf && ints32 != nullptr && state[pr.job].fp32 != 0;
         const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
         nA = elems[aA.elem].nao;
         nB = elems[aB.elem].nao;
@@ -82,8 +83,8 @@ __global__ void __launch_bounds__(256) k_fock_pairs(
         aoA = aA.ao;
         aoB = aB.ao;
         n = jb.nao;
-        guess = mode == 0 && iter == 0 && jb.requires_guess;
-        int pk = mode == 0 ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1));
+        guess = scf && iter == 0 && jb.requires_guess;
+        int pk = scf ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1));
         P = mat(arena, jb, pk);
         if (!guess && f32) {
             for (int e = lane; e < nap * nbp; e += 64) fI[wave][e] = ints32[pr.ioff + e];
@@ -203,14 +204,15 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
     const AtomDev at = atoms[ai];
     const JobState& st = state[at.job];
     const int phase = st.phase, iter = st.iter;
-    if (mode == 0 ? phase != PH_ACTIVE : phase != PH_ENERGY) return;
+    const bool scf = phase == PH_ACTIVE;   // else the final-density Fock of the energy
+    if (!((mode != 1 && scf) || (mode != 0 && phase == PH_ENERGY))) return;
     const JobDev jb = jobs[at.job];
     const ElemDev E = elems[at.elem];
     const int n = jb.nao, N = jb.natoms, A = at.local, na = E.nao, np = na == 4 ? 10 : 1;
-    const bool guess = mode == 0 && iter == 0 && jb.requires_guess;
+    const bool guess = scf && iter == 0 && jb.requires_guess;
     double* F = mat(arena, jb, AR_F);
     const double* H = mat(arena, jb, AR_H);
-    const double* P = mat(arena, jb, mode == 0 ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1)));
+    const double* P = mat(arena, jb, scf ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1)));
     if (guess) {
         if (lane < 16) {
             int mu = lane >> 2, nu = lane & 3;

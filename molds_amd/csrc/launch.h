@@ -19,7 +19,8 @@ void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long lon
 // mode 0: SCF Fock for PH_ACTIVE jobs (guess on iteration 0 when requested;
 // jobs in their FP32 iterations contract ints32, which may be null when no job
 // of the batch is mixed-precision);
-// mode 1: non-guess FP64 Fock of the final density for PH_ENERGY jobs.
+// mode 1: non-guess FP64 Fock of the final density for PH_ENERGY jobs;
+// mode 2: both in one launch (what the SCF loop uses).
 void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
                  const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
                  const float* ints32, double* arena, double* part);

@@ -1,23 +1,29 @@
 // Serving loop of the NDDO engine: continuous batching at SCF-iteration
 // granularity.
 //
-// nddo_submit copies a request and queues it.  One scheduler thread owns a
-// stream of its own and a set of resident cohorts; a cohort is a Batch (the
-// same device layout and kernels as nddo_scf_batch) of requests admitted
-// together.  Every turn of the scheduler
-//   1. admits queued requests as a new cohort when the HBM budget (the sum of
-//      job_footprint_bytes over resident jobs) and the cohort limit allow,
-//      runs its setup stage (integrals, H, core repulsion),
-//   2. launches one SCF iteration of every resident cohort (each job keeps
-//      its geometry, integrals, density and DIIS ring resident in its
-//      cohort's arena),
-//   3. reads back every job's phase (one strided copy per cohort) and
-//      retires each job the turn it finishes: its results are copied out,
-//      its latency (submit -> retire) recorded and waiters woken,
-//   4. frees a cohort's device memory once its last job has retired.
-// Requests that arrive while others are on the GPU therefore start at the
-// next iteration boundary and run interleaved with the resident ones; a
-// finished job costs its cohort nothing more than an early exit per kernel.
+// nddo_submit copies a request and queues it.  One scheduler thread keeps a
+// set of resident cohorts; a cohort is a Batch (the same device layout and
+// kernels as nddo_scf_batch) of requests admitted together, grouped by size,
+// with a stream, an event and pinned buffers from the scheduler's pools, so
+// the kernels of different cohorts overlap on the device.  The loop:
+//   1. admits queued requests as new cohorts (one per size class) when the HBM
+//      budget (the sum of job_footprint_bytes over resident jobs) and the pool
+//      allow; uploads through pinned staging and runs the setup stage;
+//   2. launches the next SCF iteration of every cohort that is not running one
+//      (each job keeps its geometry, integrals, density and DIIS ring resident
+//      in its cohort's arena), then copies the jobs' scalar state (phase,
+//      iteration, RMS, energies) to pinned memory and records the cohort event;
+//   3. polls the events; for a cohort whose iteration finished, every job that
+//      reached its end is retired at once: its arrays (eps, C, P, trajectory)
+//      are copied device-to-device into a result buffer from the
+//      stream-ordered pool, its scalars come from the pinned copy, and its
+//      latency (submit -> retire) is recorded -- the scheduler never waits on
+//      a host copy;
+//   4. returns a cohort's memory to the pool once its last job has retired.
+// nddo_get_result copies a retired job's arrays to the host in the caller's
+// thread.  A request that arrives while others are on the GPU starts at once
+// in a new cohort and runs beside the resident ones; a finished job costs its
+// cohort nothing more than an early exit per kernel.
 //
 // MolDS has no serving layer (one DoSCF per ElectronicStructure object,
 // src/cndo/Cndo2.cpp:585-729); admission control plays the role of the host
@@ -25,6 +31,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 
@@ -35,6 +42,21 @@
 namespace nddo {
 
 using Clock = std::chrono::steady_clock;
+
+namespace {
+
+// Leading part of JobState the scheduler reads every iteration: the int
+// fields and rms, diis_error, e_el, e_core.
+constexpr size_t kScalarBytes = offsetof(JobState, prof);
+
+struct EventRef {
+    hipEvent_t ev = nullptr;
+    ~EventRef() {
+        if (ev) hipEventDestroy(ev);
+    }
+};
+
+}  // namespace
 
 struct ServedJob {
     long long id = 0;
@@ -47,15 +69,25 @@ struct ServedJob {
     bool done = false;
     bool released = false;   // released by the caller before it finished
     nddo_result res{};
+    // arrays of a retired job: on the device until the first nddo_get_result
+    std::mutex mu;
+    double* d_res = nullptr;   // eps[n] | VT[n*n] | P[n*n] | traj[4*max_iter] | perm[n] (int)
+    hipStream_t res_stream = nullptr;
+    std::shared_ptr<EventRef> res_ev;
+    bool on_host = false;
     std::vector<double> eps, C, P, traj;
 };
 
 struct Cohort {
     Batch b;
+    int slot = -1;   // index into the scheduler's stream / event / pinned-buffer pools
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    char* h_state = nullptr;   // pinned: kScalarBytes per job
+    bool inflight = false;
     std::vector<nddo_job_desc> descs;
     std::vector<ServedJob*> members;
     std::vector<char> retired;
-    int* h_phase = nullptr;   // pinned, one int per job
     int live = 0;
     size_t bytes = 0;
     int turns = 0;
@@ -63,49 +95,123 @@ struct Cohort {
 
 namespace {
 
-constexpr int kMaxCohorts = 8;
+double secs_since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+constexpr int kMaxCohorts = 16;
+constexpr int kSizeClasses = 4;   // AO count <= 64, <= 137 (eigensolver matrix in LDS), <= 256, larger
+
+int size_class(int nao) { return nao <= 64 ? 0 : (nao <= 137 ? 1 : (nao <= 256 ? 2 : 3)); }
 
 int admit_limit(const Engine* e) { return e->opts.max_batch_jobs > 0 ? e->opts.max_batch_jobs : 4096; }
 
-void retire(Engine* e, Cohort* c, int jj, hipStream_t st) {
-    ServedJob* j = c->members[jj];
-    nddo_result& r = j->res;
-    r.eps = j->eps.data();
-    r.C = j->C.data();
-    r.P = j->P.data();
-    r.trajectory = j->traj.data();
-    const int rc = c->b.fetch(st, jj, c->descs[jj], &r, j->hof_offset);
-    if (rc != NDDO_OK) r.status = rc;
+}  // namespace
+
+size_t result_bytes(int n, int max_iter) {
+    return ((size_t)n + 2 * (size_t)n * n + 4 * (size_t)max_iter) * sizeof(double) + (size_t)n * sizeof(int);
+}
+
+namespace {
+
+// Retires the finished jobs `done` of cohort c: device-to-device copies of
+// their arrays, one event after them, then the scalars and the wake-up.
+void retire_jobs(Engine* e, Cohort* c, const std::vector<int>& done) {
+    std::shared_ptr<EventRef> ev;
+    std::vector<double*> bufs(done.size(), nullptr);
+    for (size_t k = 0; k < done.size(); ++k) {
+        const int jj = done[k];
+        ServedJob* j = c->members[jj];
+        if (!j || j->released) continue;
+        JobState s;
+        std::memcpy(&s, c->h_state + (size_t)jj * kScalarBytes, kScalarBytes);
+        const JobDev& jb = c->b.jobs[jj];
+        const long long n = jb.nao, nn = n * n;
+        double* d = nullptr;
+        if (hipMallocAsync((void**)&d, result_bytes((int)n, jb.max_iter), c->st) != hipSuccess) continue;
+        const double* base = c->b.d_arena + jb.arena;
+        const int pk = AR_P0 + ((s.iter + 1) & 1);
+        hipMemcpyAsync(d, c->b.d_eps + jb.vec0, n * sizeof(double), hipMemcpyDeviceToDevice, c->st);
+        hipMemcpyAsync(d + n, base + (long long)AR_C * nn, nn * sizeof(double), hipMemcpyDeviceToDevice, c->st);
+        hipMemcpyAsync(d + n + nn, base + (long long)pk * nn, nn * sizeof(double), hipMemcpyDeviceToDevice, c->st);
+        hipMemcpyAsync(d + n + 2 * nn, c->b.d_traj + jb.traj0, 4LL * jb.max_iter * sizeof(double),
+                       hipMemcpyDeviceToDevice, c->st);
+        hipMemcpyAsync(d + n + 2 * nn + 4LL * jb.max_iter, c->b.d_perm + jb.ivec0, n * sizeof(int),
+                       hipMemcpyDeviceToDevice, c->st);
+        bufs[k] = d;
+        if (!ev) {
+            ev = std::make_shared<EventRef>();
+            if (hipEventCreateWithFlags(&ev->ev, hipEventDisableTiming) != hipSuccess) ev->ev = nullptr;
+        }
+    }
+    if (ev && ev->ev) hipEventRecord(ev->ev, c->st);
     std::lock_guard<std::mutex> lk(e->smu);
-    j->t_done = Clock::now();
-    r.latency_s = std::chrono::duration<double>(j->t_done - j->t_submit).count();
-    j->done = true;
-    c->retired[jj] = 1;
-    --c->live;
-    if (j->released) {
-        e->served.erase(j->id);
-        delete j;
-        c->members[jj] = nullptr;
+    const auto now = Clock::now();
+    for (size_t k = 0; k < done.size(); ++k) {
+        const int jj = done[k];
+        ServedJob* j = c->members[jj];
+        c->retired[jj] = 1;
+        --c->live;
+        if (!j) continue;
+        if (j->released) {   // dropped by the caller while it ran
+            if (bufs[k]) hipFreeAsync(bufs[k], c->st);
+            e->served.erase(j->id);
+            delete j;
+            c->members[jj] = nullptr;
+            continue;
+        }
+        JobState s;
+        std::memcpy(&s, c->h_state + (size_t)jj * kScalarBytes, kScalarBytes);
+        nddo_result& r = j->res;
+        fill_scalars(c->b.jobs[jj], s, j->desc.theory, j->hof_offset, &r);
+        if (!bufs[k] && r.status == NDDO_OK) r.status = NDDO_HIP_ERROR;
+        r.latency_s = std::chrono::duration<double>(now - j->t_submit).count();
+        j->d_res = bufs[k];
+        j->res_stream = c->st;
+        j->res_ev = ev;
+        j->t_done = now;
+        j->done = true;
     }
     e->scv.notify_all();
 }
 
-// Builds, uploads and sets up a cohort from the admitted jobs.
-int start_cohort(Engine* e, Cohort* c, hipStream_t st) {
+// Pinned upload staging of one scheduler slot; grows on demand.
+struct Staging {
+    char* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t need, hipStream_t st) {
+        if (need <= cap) return NDDO_OK;
+        const size_t want = std::max(need, 2 * cap);
+        if (p) {
+            hipStreamSynchronize(st);   // the slot's previous upload has been consumed
+            hipHostFree(p);
+        }
+        p = nullptr;
+        cap = 0;
+        if (hipHostMalloc((void**)&p, want, hipHostMallocDefault) != hipSuccess) return NDDO_HIP_ERROR;
+        cap = want;
+        return NDDO_OK;
+    }
+};
+
+// Builds, uploads and sets up a cohort from the admitted jobs on its own
+// stream; its device memory comes from the stream-ordered pool and its upload
+// goes through the slot's pinned staging.
+int start_cohort(Engine* e, Cohort* c, Staging& stage) {
+    hipStream_t st = c->st;
+    c->b.alloc_stream = st;
     std::vector<int> which(c->members.size());
     for (size_t i = 0; i < which.size(); ++i) which[i] = (int)i;
     c->descs.resize(c->members.size());
     for (size_t i = 0; i < c->members.size(); ++i) c->descs[i] = c->members[i]->desc;
     c->b.build(c->descs.data(), (int)c->descs.size(), which, nullptr);
     c->b.classify(e->lds_max);
-    int rc = c->b.allocate(false);
+    int rc = stage.ensure(c->b.upload_bytes(), st);
     if (rc) return rc;
-    rc = c->b.upload(st, c->descs.data());
+    rc = c->b.allocate(false);
+    if (rc) return rc;
+    rc = c->b.upload(st, c->descs.data(), stage.p);
     if (rc) return rc;
     c->b.launch_setup_stage(e, st);
     if (hipGetLastError() != hipSuccess) return NDDO_HIP_ERROR;
-    if (hipHostMalloc((void**)&c->h_phase, c->members.size() * sizeof(int), hipHostMallocDefault) != hipSuccess)
-        return NDDO_HIP_ERROR;
     c->retired.assign(c->members.size(), 0);
     c->live = (int)c->members.size();
     return NDDO_OK;
@@ -128,10 +234,31 @@ void fail_cohort(Engine* e, Cohort* c, int rc) {
     e->scv.notify_all();
 }
 
-void free_cohort(Cohort* c) {
-    if (c->h_phase) hipHostFree(c->h_phase);
-    c->h_phase = nullptr;
-    c->b.release();
+// Returns the cohort's device memory to the pool (stream-ordered frees); the
+// stream, event and pinned buffers go back to the scheduler's pools.
+void free_cohort(Cohort* c) { c->b.release(); }
+
+// Copies a retired job's arrays to the host (once), applying MolDS's row order
+// and sign rule to C.
+int materialize(ServedJob* j) {
+    if (j->on_host || !j->d_res) return NDDO_OK;
+    const int n = j->res.nao, mi = j->desc.opts.max_iter;
+    const size_t nn = (size_t)n * n;
+    if (j->res_ev && j->res_ev->ev) hipEventSynchronize(j->res_ev->ev);
+    else hipStreamSynchronize(j->res_stream);
+    std::vector<char> h(result_bytes(n, mi));
+    if (hipMemcpy(h.data(), j->d_res, h.size(), hipMemcpyDeviceToHost) != hipSuccess) return NDDO_HIP_ERROR;
+    const double* hd = (const double*)h.data();
+    j->eps.assign(hd, hd + n);
+    j->C.resize(nn);
+    order_and_phase(n, hd + n, (const int*)(hd + n + 2 * nn + 4 * (size_t)mi), j->C.data());
+    j->P.assign(hd + n + nn, hd + n + 2 * nn);
+    j->traj.assign(hd + n + 2 * nn, hd + n + 2 * nn + 4 * (size_t)mi);
+    hipFreeAsync(j->d_res, j->res_stream);
+    j->d_res = nullptr;
+    j->res_ev.reset();
+    j->on_host = true;
+    return NDDO_OK;
 }
 
 }  // namespace
@@ -140,68 +267,135 @@ void Engine::start_server() {
     stopping = false;
     server = std::thread([this]() {
         hipSetDevice(device);
-        hipStream_t st = nullptr;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return;
+        // keep freed cohort memory in the stream-ordered pool instead of returning it
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        const int cap = admit_limit(this);
+        std::vector<hipStream_t> spool(kMaxCohorts, nullptr);
+        std::vector<hipEvent_t> epool(kMaxCohorts, nullptr);
+        std::vector<char*> hstate(kMaxCohorts, nullptr);
+        std::vector<Staging> stage(kMaxCohorts);
+        std::vector<int> free_slots;
+        for (int k = kMaxCohorts - 1; k >= 0; --k) {
+            if (hipStreamCreateWithFlags(&spool[k], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&epool[k], hipEventDisableTiming) != hipSuccess ||
+                hipHostMalloc((void**)&hstate[k], (size_t)cap * kScalarBytes, hipHostMallocDefault) != hipSuccess)
+                continue;
+            free_slots.push_back(k);
+        }
         std::vector<std::unique_ptr<Cohort>> cohorts;
         size_t resident = 0;
         for (;;) {
-            // 1. admission
-            std::unique_ptr<Cohort> fresh;
+            auto t_phase = Clock::now();
+            // 1. admission: queued requests are grouped by size (AO count) so that a
+            //    cohort's iteration time is set by molecules of like size
+            std::vector<std::unique_ptr<Cohort>> fresh;
             {
                 std::unique_lock<std::mutex> lk(smu);
                 if (cohorts.empty()) scv.wait(lk, [&] { return stopping || !queue.empty(); });
                 if (stopping) break;
-                if (!queue.empty() && (int)cohorts.size() < kMaxCohorts) {
-                    fresh.reset(new Cohort());
-                    const int cap = admit_limit(this);
-                    while (!queue.empty() && (int)fresh->members.size() < cap) {
-                        ServedJob* j = queue.front();
-                        if (resident + fresh->bytes + j->footprint > hbm_budget) break;
-                        queue.pop_front();
-                        fresh->members.push_back(j);
-                        fresh->bytes += j->footprint;
+                std::unique_ptr<Cohort> bucket[kSizeClasses];
+                size_t nb = 0, admitted = 0;
+                for (auto it = queue.begin(); it != queue.end();) {
+                    ServedJob* j = *it;
+                    const int k = size_class(j->res.nao);
+                    if (!bucket[k]) {
+                        if (nb >= free_slots.size()) { ++it; continue; }
+                        bucket[k].reset(new Cohort());
+                        ++nb;
                     }
-                    if (fresh->members.empty()) fresh.reset();
+                    Cohort* c = bucket[k].get();
+                    if ((int)c->members.size() >= cap || resident + admitted + j->footprint > hbm_budget) {
+                        ++it;
+                        continue;
+                    }
+                    c->members.push_back(j);
+                    c->bytes += j->footprint;
+                    admitted += j->footprint;
+                    it = queue.erase(it);
                 }
+                for (auto& b : bucket)
+                    if (b && !b->members.empty()) fresh.push_back(std::move(b));
             }
-            if (fresh) {
-                const int rc = start_cohort(this, fresh.get(), st);
+            for (auto& f : fresh) {
+                const int k = free_slots.back();
+                free_slots.pop_back();
+                f->slot = k;
+                f->st = spool[k];
+                f->ev = epool[k];
+                f->h_state = hstate[k];
+                const int rc = start_cohort(this, f.get(), stage[k]);
                 if (rc != NDDO_OK) {
-                    fail_cohort(this, fresh.get(), rc);
-                    free_cohort(fresh.get());
+                    fail_cohort(this, f.get(), rc);
+                    free_cohort(f.get());
+                    free_slots.push_back(k);
                 } else {
-                    resident += fresh->bytes;
-                    cohorts.push_back(std::move(fresh));
+                    resident += f->bytes;
+                    cohorts.push_back(std::move(f));
                 }
             }
+            server_s[0] += secs_since(t_phase);
+            t_phase = Clock::now();
             if (cohorts.empty()) continue;
-            // 2. one SCF iteration of every resident cohort, then the job phases
+            // 2. the next SCF iteration of every idle cohort, then its jobs' scalar state
             for (auto& c : cohorts) {
-                c->b.launch_iteration(this, st);
-                hipMemcpy2DAsync(c->h_phase, sizeof(int), c->b.d_state, sizeof(JobState), sizeof(int),
-                                 c->members.size(), hipMemcpyDeviceToHost, st);
+                if (c->inflight || c->live <= 0) continue;
+                c->b.launch_iteration(this, c->st);
+                hipMemcpy2DAsync(c->h_state, kScalarBytes, c->b.d_state, sizeof(JobState), kScalarBytes,
+                                 c->members.size(), hipMemcpyDeviceToHost, c->st);
+                hipEventRecord(c->ev, c->st);
+                c->inflight = true;
                 ++c->turns;
             }
-            const hipError_t err = hipStreamSynchronize(st);
-            // 3. retire finished jobs; 4. free finished cohorts
-            for (auto& c : cohorts) {
-                if (err != hipSuccess) {
-                    fail_cohort(this, c.get(), NDDO_HIP_ERROR);
-                    c->live = 0;
-                    continue;
+            server_s[1] += secs_since(t_phase);
+            t_phase = Clock::now();
+            // 3. wait until some cohort finished its iteration (or new requests can be admitted)
+            std::vector<Cohort*> ready;
+            for (;;) {
+                for (auto& c : cohorts) {
+                    if (!c->inflight) continue;
+                    const hipError_t q = hipEventQuery(c->ev);
+                    if (q == hipSuccess) {
+                        ready.push_back(c.get());
+                    } else if (q != hipErrorNotReady) {
+                        fail_cohort(this, c.get(), NDDO_HIP_ERROR);
+                        c->live = 0;
+                        c->inflight = false;
+                    }
                 }
-                for (size_t jj = 0; jj < c->members.size(); ++jj)
-                    if (!c->retired[jj] && c->h_phase[jj] == PH_DONE) retire(this, c.get(), (int)jj, st);
-                // every job ends within max_iter + 1 turns (k_step marks NOT_CONVERGED)
-                if (c->live > 0 && c->turns > c->b.max_iter_all + 2) {
-                    for (size_t jj = 0; jj < c->members.size(); ++jj)
-                        if (!c->retired[jj]) retire(this, c.get(), (int)jj, st);
+                if (!ready.empty()) break;
+                {
+                    std::lock_guard<std::mutex> lk(smu);
+                    if (stopping || (!queue.empty() && !free_slots.empty())) break;
                 }
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
             }
+            server_s[2] += secs_since(t_phase);
+            t_phase = Clock::now();
+            server_turns += (long long)ready.size();
+            for (Cohort* c : ready) {
+                c->inflight = false;
+                if (c->live <= 0) continue;
+                std::vector<int> done;
+                for (size_t jj = 0; jj < c->members.size(); ++jj) {
+                    if (c->retired[jj]) continue;
+                    int phase;
+                    std::memcpy(&phase, c->h_state + jj * kScalarBytes + offsetof(JobState, phase), sizeof(int));
+                    // every job ends within max_iter + 1 iterations (k_step marks NOT_CONVERGED)
+                    if (phase == PH_DONE || c->turns > c->b.max_iter_all + 2) done.push_back((int)jj);
+                }
+                if (!done.empty()) retire_jobs(this, c, done);
+            }
+            server_s[3] += secs_since(t_phase);
+            // 4. free finished cohorts
             for (size_t i = 0; i < cohorts.size();) {
-                if (cohorts[i]->live <= 0) {
+                if (cohorts[i]->live <= 0 && !cohorts[i]->inflight) {
                     resident -= cohorts[i]->bytes;
                     free_cohort(cohorts[i].get());
+                    free_slots.push_back(cohorts[i]->slot);
                     cohorts.erase(cohorts.begin() + i);
                 } else {
                     ++i;
@@ -209,10 +403,26 @@ void Engine::start_server() {
             }
         }
         for (auto& c : cohorts) {
+            hipStreamSynchronize(c->st);
             fail_cohort(this, c.get(), NDDO_BAD_ARG);
             free_cohort(c.get());
         }
-        hipStreamDestroy(st);
+        for (int k = 0; k < kMaxCohorts; ++k)
+            if (spool[k]) hipStreamSynchronize(spool[k]);
+        {   // results not yet copied to the host go back to the pool now
+            std::lock_guard<std::mutex> lk(smu);
+            for (auto& kv : served)
+                if (kv.second->d_res) {
+                    hipFree(kv.second->d_res);
+                    kv.second->d_res = nullptr;
+                }
+        }
+        for (int k = 0; k < kMaxCohorts; ++k) {
+            if (spool[k]) hipStreamDestroy(spool[k]);
+            if (epool[k]) hipEventDestroy(epool[k]);
+            if (hstate[k]) hipHostFree(hstate[k]);
+            if (stage[k].p) hipHostFree(stage[k].p);
+        }
     });
 }
 
@@ -236,6 +446,18 @@ using nddo::ServedJob;
 
 extern "C" {
 
+int nddo_dbg_server_stats(nddo_engine* eh, double* seconds4, long long* iterations, int reset) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    for (int i = 0; i < 4; ++i) {
+        if (seconds4) seconds4[i] = e->server_s[i];
+        if (reset) e->server_s[i] = 0.0;
+    }
+    if (iterations) *iterations = e->server_turns;
+    if (reset) e->server_turns = 0;
+    return NDDO_OK;
+}
+
 int nddo_submit(nddo_engine* eh, const nddo_job_desc* d, nddo_job_id* id) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !d || !id) { nddo::set_error("bad arguments"); return NDDO_BAD_ARG; }
@@ -258,12 +480,9 @@ int nddo_submit(nddo_engine* eh, const nddo_job_desc* d, nddo_job_id* id) {
     j->res.nao = n;
     j->res.nocc = no;
     if (s == NDDO_OK) {
-        j->footprint = nddo::job_footprint_bytes(d->natoms, n, d->opts.diis_num_error_vect, d->opts.max_iter);
+        j->footprint = nddo::job_footprint_bytes(d->natoms, n, d->opts.diis_num_error_vect, d->opts.max_iter) +
+                       nddo::result_bytes(n, d->opts.max_iter);
         for (int a = 0; a < d->natoms; ++a) j->hof_offset += nddo::mndo_atom_hof_offset(d->Z[a]);
-        j->eps.resize(n);
-        j->C.resize((size_t)n * n);
-        j->P.resize((size_t)n * n);
-        j->traj.assign(4 * (size_t)d->opts.max_iter, 0.0);
         j->res.status = NDDO_PENDING;
     }
     std::lock_guard<std::mutex> lk(e->smu);
@@ -309,11 +528,17 @@ int nddo_wait(nddo_engine* eh, nddo_job_id id, double timeout_s) {
 int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !res) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->smu);
-    auto it = e->served.find(id);
-    if (it == e->served.end() || it->second->released) { nddo::set_error("unknown job id"); return NDDO_BAD_ARG; }
-    ServedJob* j = it->second;
-    if (!j->done) return NDDO_PENDING;
+    ServedJob* j = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(e->smu);
+        auto it = e->served.find(id);
+        if (it == e->served.end() || it->second->released) { nddo::set_error("unknown job id"); return NDDO_BAD_ARG; }
+        j = it->second;
+        if (!j->done) return NDDO_PENDING;
+    }
+    std::lock_guard<std::mutex> jl(j->mu);
+    hipSetDevice(e->device);
+    if (nddo::materialize(j) != NDDO_OK) { nddo::set_error("result copy failed"); return NDDO_HIP_ERROR; }
     double* eps = res->eps;
     double* C = res->C;
     double* P = res->P;
@@ -324,9 +549,7 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
     res->P = P;
     res->trajectory = tr;
     const size_t n = (size_t)std::max(j->res.nao, 0);
-    const bool have = j->res.status != NDDO_ODD_ELECTRONS && j->res.status != NDDO_UNSUPPORTED_ELEMENT &&
-                      j->res.status != NDDO_BAD_ARG && j->res.status != NDDO_ADMISSION_DENIED && !j->eps.empty();
-    if (have) {
+    if (j->on_host) {
         if (eps) std::memcpy(eps, j->eps.data(), n * sizeof(double));
         if (C) std::memcpy(C, j->C.data(), n * n * sizeof(double));
         if (P) std::memcpy(P, j->P.data(), n * n * sizeof(double));
@@ -338,22 +561,32 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
 int nddo_release(nddo_engine* eh, nddo_job_id id) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->smu);
-    auto it = e->served.find(id);
-    if (it == e->served.end() || it->second->released) { nddo::set_error("unknown job id"); return NDDO_BAD_ARG; }
-    ServedJob* j = it->second;
-    auto q = std::find(e->queue.begin(), e->queue.end(), j);
-    if (q != e->queue.end()) {   // not admitted yet: cancel
-        e->queue.erase(q);
+    ServedJob* j = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(e->smu);
+        auto it = e->served.find(id);
+        if (it == e->served.end() || it->second->released) { nddo::set_error("unknown job id"); return NDDO_BAD_ARG; }
+        j = it->second;
+        auto q = std::find(e->queue.begin(), e->queue.end(), j);
+        if (q != e->queue.end()) {   // not admitted yet: cancel
+            e->queue.erase(q);
+            e->served.erase(it);
+            delete j;
+            return NDDO_OK;
+        }
+        if (!j->done) {   // running: dropped when it retires
+            j->released = true;
+            return NDDO_OK;
+        }
         e->served.erase(it);
-        delete j;
-        return NDDO_OK;
     }
-    if (!j->done) {   // running: dropped when it retires
-        j->released = true;
-        return NDDO_OK;
+    {
+        std::lock_guard<std::mutex> jl(j->mu);
+        if (j->d_res) {
+            hipSetDevice(e->device);
+            hipFreeAsync(j->d_res, j->res_stream);   // ordered after the result copies
+        }
     }
-    e->served.erase(it);
     delete j;
     return NDDO_OK;
 }

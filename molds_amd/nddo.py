@@ -78,6 +78,7 @@ def load_library(path=LIBPATH):
     L.nddo_last_error.restype = ctypes.c_char_p
     L.nddo_engine_profile.argtypes = [vp, ctypes.c_int]
     L.nddo_engine_stage_times.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+    L.nddo_dbg_server_stats.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
     L.nddo_engine_info.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int),
                                    ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int)]
     L.nddo_dbg_integrals.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp]
@@ -225,6 +226,12 @@ class Engine:
     # --------------------------------------------------------------- profiling
     STAGES = ("setup", "fock", "eigen", "density", "step", "energy", "tridiag_lds", "tridiag_glb", "divide_conquer",
               "backtransform")
+
+    def server_stats(self, reset=False):
+        t = np.zeros(4)
+        it = ctypes.c_longlong()
+        self.L.nddo_dbg_server_stats(self.h, _p(t), ctypes.byref(it), 1 if reset else 0)
+        return dict(zip(("admit_setup_s", "launch_s", "wait_s", "retire_s"), t.tolist()), cohort_iterations=it.value)
 
     def info(self):
         lds, cus, budget, nmax = ctypes.c_size_t(), ctypes.c_int(), ctypes.c_size_t(), ctypes.c_int()
