@@ -167,6 +167,10 @@ int nddo_dbg_server_stats(nddo_engine* e, double* seconds4, long long* iteration
 /* Device-clock ticks of the last nddo_dbg_eigh phases: tridiagonalization,
  * tridiagonal divide and conquer, (unused, 0), back-transformation; tick rate in Hz. */
 int nddo_dbg_eigh_profile(nddo_engine* e, double* ticks4, double* tick_hz);
+/* Device-clock ticks of the divide-and-conquer phases of the last nddo_dbg_eigh,
+ * summed over the merge levels: merge order + z, deflation, rotations + secular
+ * roots, Loewner z + ranks, column norms, merge matrix U, merge products. */
+int nddo_dbg_eigh_dc_phases(nddo_engine* e, double* ticks7);
 
 #ifdef __cplusplus
 }

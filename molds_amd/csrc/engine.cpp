@@ -851,6 +851,8 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
     HIPCHK(hipMemcpy(&s, b.d_state, sizeof(s), hipMemcpyDeviceToHost));
     if (sweeps) *sweeps = s.sweeps;
     for (int i = 0; i < 4; ++i) e->last_eig_prof[i] = s.prof[i];
+    if (heig_supported(n))
+        HIPCHK(hipMemcpy(e->last_dc_phase, b.d_tri + 19LL * n + 224, 7 * sizeof(double), hipMemcpyDeviceToHost));
     if (C) {
         for (int r = 0; r < n; ++r) {
             const double* src = &VT[(long long)perm[r] * n];
@@ -873,6 +875,13 @@ int nddo_dbg_eigh_profile(nddo_engine* eh, double* ticks4, double* tick_hz) {
         hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device);
         *tick_hz = 1e3 * khz;
     }
+    return NDDO_OK;
+}
+
+int nddo_dbg_eigh_dc_phases(nddo_engine* eh, double* ticks7) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !ticks7) return NDDO_BAD_ARG;
+    for (int i = 0; i < 7; ++i) ticks7[i] = e->last_dc_phase[i];
     return NDDO_OK;
 }
 

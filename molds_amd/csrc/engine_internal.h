@@ -94,6 +94,7 @@ struct Engine {
     long long stage_launches[NDDO_STAGE_COUNT] = {};
     hipEvent_t ev[16] = {};
     double last_eig_prof[4] = {};
+    double last_dc_phase[7] = {};
 
     // serving state (server.cpp)
     std::mutex smu;

@@ -239,39 +239,63 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
 }
 
 // ------------------------------------------------------------------ 2. divide and conquer
+// 1/x from the hardware estimate (v_rcp_f64) refined by two Newton-Raphson
+// steps r <- r + r (1 - x r); the arguments here are differences of
+// eigenvalues, far from the denormal and overflow ranges.
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Sums of the secular function at x = dl[o] + tau: psi over poles t <= j,
+// phi over t > j, their derivatives and sum |terms| (all scaled by rho).
+struct SecSums {
+    double psi, dpsi, phi, dphi, asum;
+};
+
+__device__ inline SecSums dc_sums(const double* dl, const double* zl, int kk, double rho, int j, double dO,
+                                  double tau) {
+    double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, asum = 0.0;
+    for (int t = 0; t < kk; ++t) {
+        const double inv = rcp_nr((dl[t] - dO) - tau);
+        const double term = zl[t] * zl[t] * inv, dterm = term * inv;
+        if (t <= j) { psi += term; dpsi += dterm; }
+        else { phi += term; dphi += dterm; }
+        asum += fabs(term);
+    }
+    return SecSums{rho * psi, rho * dpsi, rho * phi, rho * dphi, asum};
+}
+
 // Root j (0-based) of the secular equation 1 + rho sum_t z_t^2 / (dl_t - x) = 0
 // for sorted dl[0..kk), rho > 0: x = dl[o] + tau with o the nearer pole, so
-// that dl_t - x = (dl_t - dl_o) - tau keeps its relative accuracy.
+// that dl_t - x = (dl_t - dl_o) - tau keeps its relative accuracy.  The first
+// evaluation, at the middle of (dl_j, dl_j+1), both picks the pole and starts
+// the iteration.
 __device__ void dc_secular(const double* dl, const double* zl, int kk, double rho, int j, int* orig,
                            double* taup) {
     int o;
-    double lo, hi;
+    double lo, hi, tau;
+    SecSums S;
     if (j < kk - 1) {
         const double mid = 0.5 * (dl[j + 1] - dl[j]);
-        double f = 0.0;
-        for (int t = 0; t < kk; ++t) f += zl[t] * zl[t] / ((dl[t] - dl[j]) - mid);
-        f = 1.0 + rho * f;
-        if (f >= 0.0) { o = j; lo = 0.0; hi = mid; }
-        else { o = j + 1; lo = -mid; hi = 0.0; }
+        S = dc_sums(dl, zl, kk, rho, j, dl[j], mid);
+        if (1.0 + S.psi + S.phi >= 0.0) { o = j; lo = 0.0; hi = mid; tau = mid; }
+        else { o = j + 1; lo = -mid; hi = 0.0; tau = -mid; }
     } else {
         double zz = 0.0;
         for (int t = 0; t < kk; ++t) zz += zl[t] * zl[t];
         o = kk - 1; lo = 0.0; hi = rho * zz;
+        tau = 0.5 * hi;
+        S = dc_sums(dl, zl, kk, rho, j, dl[o], tau);
     }
     const double dO = dl[o];
-    double tau = 0.5 * (lo + hi);
     for (int it = 0; it < 100; ++it) {
-        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, asum = 0.0;
-        for (int t = 0; t < kk; ++t) {
-            const double inv = 1.0 / ((dl[t] - dO) - tau);
-            const double term = zl[t] * zl[t] * inv, dterm = term * inv;
-            if (t <= j) { psi += term; dpsi += dterm; }
-            else { phi += term; dphi += dterm; }
-            asum += fabs(term);
-        }
-        psi *= rho; dpsi *= rho; phi *= rho; dphi *= rho;
-        const double f = 1.0 + psi + phi;
-        if (fabs(f) <= 8.0 * kEps * (1.0 + rho * asum)) break;
+        if (it > 0) S = dc_sums(dl, zl, kk, rho, j, dO, tau);
+        const double f = 1.0 + S.psi + S.phi;
+        if (fabs(f) <= 8.0 * kEps * (1.0 + rho * S.asum)) break;
         if (f < 0.0) lo = tau;
         else hi = tau;
         double eta = 0.0;
@@ -279,8 +303,8 @@ __device__ void dc_secular(const double* dl, const double* zl, int kk, double rh
         if (j < kk - 1) {
             // psi ~ a1 + b1/(p_j - x), phi ~ a2 + b2/(p_j+1 - x), matched in value and slope at tau
             const double u1 = (dl[j] - dO) - tau, u2 = (dl[j + 1] - dO) - tau;
-            const double b1 = u1 * u1 * dpsi, b2 = u2 * u2 * dphi;
-            const double c = 1.0 + (psi - b1 / u1) + (phi - b2 / u2);
+            const double b1 = u1 * u1 * S.dpsi, b2 = u2 * u2 * S.dphi;
+            const double c = 1.0 + (S.psi - b1 / u1) + (S.phi - b2 / u2);
             const double qa = c, qb = -(c * (u1 + u2) + b1 + b2), qc = c * u1 * u2 + b1 * u2 + b2 * u1;
             if (qa != 0.0) {
                 const double disc = qb * qb - 4.0 * qa * qc;
@@ -297,7 +321,7 @@ __device__ void dc_secular(const double* dl, const double* zl, int kk, double rh
             }
         } else {
             const double u1 = -tau;   // the last pole is the origin
-            const double b1 = u1 * u1 * dpsi;
+            const double b1 = u1 * u1 * S.dpsi;
             const double c = f - b1 / u1;
             if (c != 0.0) { eta = u1 + b1 / c; have = true; }
         }
@@ -383,6 +407,16 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
         }
     }
     __syncthreads();
+    // per-phase device clock ticks, summed over the levels (thread 0), kept in
+    // the spare tail of the job's eigensolver workspace for the profiling hook
+    unsigned long long tick[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long tlast = wall_clock64();
+#define DC_TICK(k)                                   \
+    if (tid == 0) {                                  \
+        const unsigned long long tnow = wall_clock64(); \
+        tick[k] += tnow - tlast;                     \
+        tlast = tnow;                                \
+    }
     for (int l = 1; l <= L; ++l) {
         const int s = 1 << l, h = s >> 1, nm = (n + s - 1) >> l;
         double* QO = ((L - l + 1) & 1) ? QX : QC;
@@ -416,55 +450,89 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             if (c < (i < m ? b : m)) QO[(long long)i * n + c] = 0.0;
         }
         __syncthreads();
-        // B: deflation, one thread per merge (small |z| and close d pairs)
-        for (int q = tid; q < nm; q += T) {
+        DC_TICK(0)
+        // B: deflation (small |z| and close d pairs), one wavefront per merge: the
+        // maxima and the compaction are wave-parallel, the scan itself (each pair
+        // depends on the previous decision) runs on lane 0 with the running pair
+        // in registers
+        for (int q = wave; q < nm; q += nw) {
             const int a = q << l, m = min(a + h, n), b = min(a + s, n);
-            if (m == b) { kkq[q] = -1; continue; }
+            if (m == b) {
+                if (lane == 0) kkq[q] = -1;
+                continue;
+            }
             const double rho = 2.0 * fabs(e[m - 1]);
             double dmax = 0.0, zmax = 0.0;
-            for (int j = a; j < b; ++j) { dmax = fmax(dmax, fabs(ds[j])); zmax = fmax(zmax, fabs(zs[j])); }
-            const double tol = 8.0 * kEps * fmax(dmax, zmax);
-            int nr = 0;
-            if (rho * zmax <= tol) {
-                for (int j = a; j < b; ++j) flag[j] = 1;
-            } else {
-                int pj = -1;
-                for (int j = a; j < b; ++j) {
-                    flag[j] = 0;
-                    if (rho * fabs(zs[j]) <= tol) { flag[j] = 1; continue; }
-                    if (pj < 0) { pj = j; continue; }
-                    double S = zs[pj], C = zs[j];
-                    const double tz = sqrt(C * C + S * S);
-                    const double t = ds[j] - ds[pj];
-                    C /= tz;
-                    S = -S / tz;
-                    if (fabs(t * C * S) <= tol) {
-                        zs[j] = tz;
-                        zs[pj] = 0.0;
-                        r1[a + nr] = col[pj];
-                        r2[a + nr] = col[j];
-                        rc[a + nr] = C;
-                        rs[a + nr] = S;
-                        ++nr;
-                        const double dp = ds[pj], dj = ds[j];
-                        ds[pj] = dp * C * C + dj * S * S;
-                        ds[j] = dp * S * S + dj * C * C;
-                        flag[pj] = 1;
-                    }
-                    pj = j;
-                }
+            for (int jj = a + lane; jj < b; jj += 64) { dmax = fmax(dmax, fabs(ds[jj])); zmax = fmax(zmax, fabs(zs[jj])); }
+            for (int off = 32; off > 0; off >>= 1) {
+                dmax = fmax(dmax, __shfl_xor(dmax, off));
+                zmax = fmax(zmax, __shfl_xor(zmax, off));
             }
+            const double tol = 8.0 * kEps * fmax(dmax, zmax);
+            if (rho * zmax <= tol) {
+                for (int jj = a + lane; jj < b; jj += 64) flag[jj] = 1;
+                if (lane == 0) nrot[q] = 0;
+            } else if (lane == 0) {
+                int pj = -1, nr = 0;
+                double zp = 0.0, dp = 0.0;
+                for (int jj = a; jj < b; ++jj) {
+                    double zj = zs[jj], dj = ds[jj];
+                    if (rho * fabs(zj) <= tol) { flag[jj] = 1; continue; }
+                    flag[jj] = 0;
+                    if (pj >= 0) {
+                        // |t c s| <= tol with c = zj / tz, s = -zp / tz, tz^2 = zj^2 + zp^2
+                        const double t = dj - dp, zz = zj * zj + zp * zp;
+                        if (fabs(t * zj * zp) <= tol * zz) {
+                            const double tz = sqrt(zz);
+                            const double C = zj / tz, S = -zp / tz;
+                            zs[jj] = tz;
+                            zs[pj] = 0.0;
+                            r1[a + nr] = col[pj];
+                            r2[a + nr] = col[jj];
+                            rc[a + nr] = C;
+                            rs[a + nr] = S;
+                            ++nr;
+                            ds[pj] = dp * C * C + dj * S * S;
+                            dj = dp * S * S + dj * C * C;
+                            ds[jj] = dj;
+                            zj = tz;
+                            flag[pj] = 1;
+                        }
+                    }
+                    pj = jj;
+                    zp = zj;
+                    dp = dj;
+                }
+                nrot[q] = nr;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // compaction: non-deflated first (kk of them), then the deflated ones
             int k = 0;
-            for (int j = a; j < b; ++j)
-                if (!flag[j]) { dl[a + k] = ds[j]; zl[a + k] = zs[j]; cl[a + k] = col[j]; ++k; }
-            int t = k;
-            for (int j = a; j < b; ++j)
-                if (flag[j]) { dl[a + t] = ds[j]; zl[a + t] = 0.0; cl[a + t] = col[j]; ++t; }
-            kkq[q] = k;
-            nrot[q] = nr;
-            rhoq[q] = rho;
+            for (int c0 = a; c0 < b; c0 += 64) {
+                const int jj = c0 + lane;
+                const bool keep = jj < b && !flag[jj];
+                const unsigned long long mk = __ballot(keep);
+                const int pre = __popcll(mk & ((1ull << lane) - 1ull));
+                if (keep) { dl[a + k + pre] = ds[jj]; zl[a + k + pre] = zs[jj]; cl[a + k + pre] = col[jj]; }
+                k += __popcll(mk);
+            }
+            int t2 = k;
+            for (int c0 = a; c0 < b; c0 += 64) {
+                const int jj = c0 + lane;
+                const bool dfl = jj < b && flag[jj];
+                const unsigned long long mk = __ballot(dfl);
+                const int pre = __popcll(mk & ((1ull << lane) - 1ull));
+                if (dfl) { dl[a + t2 + pre] = ds[jj]; zl[a + t2 + pre] = 0.0; cl[a + t2 + pre] = col[jj]; }
+                t2 += __popcll(mk);
+            }
+            if (lane == 0) {
+                kkq[q] = k;
+                rhoq[q] = rho;
+            }
         }
         __syncthreads();
+        DC_TICK(1)
         // C: deflation rotations on the rows of QO (thread = column); D: secular roots
         for (int i = tid; i < n; i += T) {
             const int q = i >> l, a = q << l;
@@ -490,6 +558,7 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             }
         }
         __syncthreads();
+        DC_TICK(2)
         // E: Loewner z-hat (Gu & Eisenstat) and the rank of every new eigenvalue in its block
         for (int i = tid; i < n; i += T) {
             const int q = i >> l, a = q << l;
@@ -512,6 +581,7 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             rank[i] = rk;
         }
         __syncthreads();
+        DC_TICK(3)
         // F: inverse norms of the columns of U
         for (int i = tid; i < n; i += T) {
             const int q = i >> l, a = q << l;
@@ -526,6 +596,7 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             cnorm[i] = 1.0 / sqrt(s2);
         }
         __syncthreads();
+        DC_TICK(4)
         // G: U[t][j] = zhat_t / (dl_t - lambda_j), columns normalized
         const long long tot = (long long)n << l;
         for (long long x = tid; x < tot; x += T) {
@@ -536,6 +607,7 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             U[(long long)i * n + cj] = zhat[i] / ((dl[i] - dl[rorig[cj]]) - rtau[cj]) * cnorm[cj];
         }
         __syncthreads();
+        DC_TICK(5)
         // H: new eigenvector rows QN[a + rank] = U^T QO[cl]; deflated rows copied; sorted eigenvalues
         const bool small = l < 5;
         for (long long x = tid; x < tot; x += T) {
@@ -608,7 +680,9 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             if (kkq[q] >= 0) Dc[a + rank[i]] = vals[i];
         }
         __syncthreads();
+        DC_TICK(6)
     }
+#undef DC_TICK
     for (int i = tid; i < n; i += T) {
         eps[jb.vec0 + i] = Dc[i];
         perm[jb.ivec0 + i] = i;
@@ -618,6 +692,8 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
         state[job].warm = 0;
         state[job].prof[1] = (double)(wall_clock64() - t0);
         state[job].prof[2] = 0.0;
+        double* ph = eig_ws(const_cast<double*>(tri), jb, job) + 19 * n + 224;
+        for (int k = 0; k < 7; ++k) ph[k] = (double)tick[k];
     }
 }
 

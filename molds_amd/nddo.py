@@ -86,6 +86,7 @@ def load_library(path=LIBPATH):
     L.nddo_dbg_eigh.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _ip]
     L.nddo_dbg_density.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.nddo_dbg_eigh_profile.argtypes = [vp, _dp, _dp]
+    L.nddo_dbg_eigh_dc_phases.argtypes = [vp, _dp]
     _lib = L
     return L
 
@@ -315,6 +316,14 @@ class Engine:
         hz = ctypes.c_double()
         self.L.nddo_dbg_eigh_profile(self.h, _p(t), ctypes.byref(hz))
         return dict(zip(("tridiag", "divide_conquer", "unused", "backtransform"), (t / hz.value * 1e3).tolist()))
+
+    def dbg_eigh_dc_phases(self):
+        t = np.zeros(7)
+        hz = ctypes.c_double()
+        self.L.nddo_dbg_eigh_profile(self.h, _p(np.zeros(4)), ctypes.byref(hz))
+        self.L.nddo_dbg_eigh_dc_phases(self.h, _p(t))
+        names = ("merge_z", "deflation", "rotations_secular", "loewner_rank", "norms", "build_U", "products")
+        return dict(zip(names, (t / hz.value * 1e3).tolist()))
 
     def dbg_density(self, C, nocc):
         C = np.ascontiguousarray(C, dtype=np.float64)
