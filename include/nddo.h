@@ -131,11 +131,12 @@ int nddo_release(nddo_engine* e, nddo_job_id id);
  * accumulates milliseconds and launch counts per stage:
  * 0 setup, 1 fock, 2 eigen, 3 density, 4 step (RMS/DIIS/damping), 5 energy;
  * and the parts of the eigen stage: 6 tridiagonalization of the matrices kept
- * in LDS, 7 tridiagonalization of the matrices in device memory (or the Jacobi
- * solver beyond 512 AOs), 8 divide and conquer, 9 back-transformation. */
+ * in LDS, 7 tridiagonalization of the matrices in device memory, 8 divide and
+ * conquer, 9 back-transformation (these four up to 512 AOs), 10 the
+ * device-wide block-Jacobi solver of the larger molecules. */
 enum { NDDO_STAGE_SETUP = 0, NDDO_STAGE_FOCK, NDDO_STAGE_EIGEN, NDDO_STAGE_DENSITY, NDDO_STAGE_STEP,
        NDDO_STAGE_ENERGY, NDDO_STAGE_TRIDIAG_LDS, NDDO_STAGE_TRIDIAG_GLB, NDDO_STAGE_DC,
-       NDDO_STAGE_BACKTRANSFORM, NDDO_STAGE_COUNT };
+       NDDO_STAGE_BACKTRANSFORM, NDDO_STAGE_BLOCK_JACOBI, NDDO_STAGE_COUNT };
 int nddo_engine_profile(nddo_engine* e, int enable);
 int nddo_engine_stage_times(nddo_engine* e, double* ms, long long* launches, int reset);
 

@@ -121,7 +121,8 @@ void Batch::release() {
     for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
                      (void**)&d_ints, (void**)&d_ints32, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
                      (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
-                     (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_sdbg}) {
+                     (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_list_xl, (void**)&d_xl_off,
+                     (void**)&d_xlws, (void**)&d_xlpart, (void**)&d_xlcoef, (void**)&d_bar, (void**)&d_sdbg}) {
         if (*p) {
             if (alloc_stream) hipFreeAsync(*p, alloc_stream);
             else hipFree(*p);
@@ -198,10 +199,11 @@ size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter) {
 
 int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* elem_index_base) {
     (void)elem_index_base;
-    atoms.clear(); pairs.clear(); jobs.clear(); tiles.clear(); list_lds.clear(); list_glb.clear();
+    atoms.clear(); pairs.clear(); jobs.clear(); tiles.clear(); list_lds.clear(); list_glb.clear(); list_xl.clear();
+    xl_off.clear();
     src = which;
     nslots = 0; nints = 0; arena_size = 0; nvec = 0; ntraj = 0;
-    maxn_lds = 0; maxn_glb = 0; max_iter_all = 0;
+    maxn_lds = 0; maxn_glb = 0; maxn_xl = 0; max_iter_all = 0;
     for (int jj = 0; jj < (int)which.size(); ++jj) {
         const nddo_job_desc& d = descs[which[jj]];
         (void)nd;
@@ -271,23 +273,43 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
     return NDDO_OK;
 }
 
-int Batch::classify(size_t lds_max) {
+// Eigensolver classes: the one-workgroup solvers with the matrix in LDS, the
+// same in device memory (up to 512 AOs), and the device-wide block Jacobi for
+// larger molecules (kernels_xl.hip).
+int Batch::classify(size_t lds_max, int num_cus) {
     list_lds.clear();
     list_glb.clear();
-    maxn_lds = maxn_glb = 0;
+    list_xl.clear();
+    xl_off.clear();
+    maxn_lds = maxn_glb = maxn_xl = 0;
+    xlws_size = 0;
     std::vector<int> order(jobs.size());
     for (size_t i = 0; i < jobs.size(); ++i) order[i] = (int)i;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return jobs[a].nao > jobs[b].nao; });
     for (int j : order) {
         int n = jobs[j].nao;
         // the Jacobi kernels also hold up to 8 KB of static LDS (reduction buffers)
+        jobs[j].xl = 0;
         if (n <= 512 && lds_max > 9216 && jacobi_lds_bytes(n, true) <= lds_max - 9216) {
             list_lds.push_back(j);
             maxn_lds = std::max(maxn_lds, n);
-        } else {
+        } else if (heig_supported(n)) {
             list_glb.push_back(j);
             maxn_glb = std::max(maxn_glb, n);
+        } else {
+            list_xl.push_back(j);
+            jobs[j].xl = (int)list_xl.size();
+            xl_off.push_back(xlws_size);
+            xlws_size += xl_ws_doubles(n);
+            maxn_xl = std::max(maxn_xl, n);
         }
+    }
+    xl_chunks = xl_nchunk(maxn_xl);
+    xl_grid = 1;
+    for (size_t g = 0; g < list_xl.size(); g += xl_max_jobs()) {
+        std::vector<int> ns;
+        for (size_t i = g; i < std::min(list_xl.size(), g + xl_max_jobs()); ++i) ns.push_back(jobs[list_xl[i]].nao);
+        xl_grid = std::max(xl_grid, ::nddo::xl_grid(num_cus, ns.data(), (int)ns.size()));
     }
     return NDDO_OK;
 }
@@ -321,18 +343,26 @@ int Batch::allocate(bool want_sdbg) {
     ACHK(dalloc((void**)&d_remaining, sizeof(int)));
     ACHK(dalloc((void**)&d_list_lds, std::max<size_t>(list_lds.size(), 1) * sizeof(int)));
     ACHK(dalloc((void**)&d_list_glb, std::max<size_t>(list_glb.size(), 1) * sizeof(int)));
+    if (!list_xl.empty()) {
+        ACHK(dalloc((void**)&d_list_xl, list_xl.size() * sizeof(int)));
+        ACHK(dalloc((void**)&d_xl_off, list_xl.size() * sizeof(long long)));
+        ACHK(dalloc((void**)&d_xlws, xlws_size * sizeof(double)));
+        ACHK(dalloc((void**)&d_xlpart, list_xl.size() * (size_t)xl_chunks * XL_REC * sizeof(double)));
+        ACHK(dalloc((void**)&d_xlcoef, nj * XL_COEF * sizeof(double)));
+        ACHK(dalloc((void**)&d_bar, 2 * sizeof(unsigned)));
+    }
     if (want_sdbg) ACHK(dalloc((void**)&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double)));
     return NDDO_OK;
 #undef ACHK
 }
 
 // Staging bytes upload() needs; independent of classify(): every job sits in
-// exactly one of the two eigensolver lists.
+// exactly one of the three eigensolver lists.
 size_t Batch::upload_bytes() const {
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     return up(atoms.size() * sizeof(AtomDev)) + up(pairs.size() * sizeof(PairDev)) + up(jobs.size() * sizeof(JobDev)) +
-           up(tiles.size() * sizeof(TileDev)) + 2 * up(jobs.size() * sizeof(int)) + up(jobs.size() * sizeof(JobState)) +
-           256;
+           up(tiles.size() * sizeof(TileDev)) + 3 * up(jobs.size() * sizeof(int)) +
+           up(jobs.size() * sizeof(long long)) + up(jobs.size() * sizeof(JobState)) + 256;
 }
 
 // With a pinned staging buffer (stage, at least upload_bytes()), every copy is
@@ -355,6 +385,10 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs, char* stage) {
     HIPCHK(h2d(d_tiles, tiles.data(), tiles.size() * sizeof(TileDev)));
     HIPCHK(h2d(d_list_lds, list_lds.data(), list_lds.size() * sizeof(int)));
     HIPCHK(h2d(d_list_glb, list_glb.data(), list_glb.size() * sizeof(int)));
+    if (!list_xl.empty()) {
+        HIPCHK(h2d(d_list_xl, list_xl.data(), list_xl.size() * sizeof(int)));
+        HIPCHK(h2d(d_xl_off, xl_off.data(), xl_off.size() * sizeof(long long)));
+    }
     HIPCHK(hipMemsetAsync(d_arena, 0, std::max<long long>(arena_size, 1) * sizeof(double), st));
     HIPCHK(hipMemsetAsync(d_traj, 0, std::max<long long>(ntraj, 1) * sizeof(double), st));
     if (d_sdbg) HIPCHK(hipMemsetAsync(d_sdbg, 0, std::max<long long>(arena_size, 1) * sizeof(double), st));
@@ -387,16 +421,35 @@ void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
 // density for jobs that converged in the previous iteration, whose energy is
 // then evaluated (Cndo2.cpp:1324-1416) before the eigensolve of the others.
 void Batch::launch_iteration(Engine* e, hipStream_t st) {
+    const int nxl = (int)list_xl.size();
     launch_fock(st, 2, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
                 d_arena, d_part);
-    launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
+    launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
+    launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
     launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
     launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri,
                  jacobi_lds_bytes(maxn_lds, true), true);
     launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri,
                  jacobi_lds_bytes(maxn_glb, false), false);
+    launch_xl_eigen_all(e, st);
     launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
-    launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
+    launch_xl_dots(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
+    launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining, d_xlpart, d_xlcoef,
+                xl_chunks);
+    launch_xl_apply(st, d_jobs, d_list_xl, nxl, d_arena, d_xlcoef, xl_chunks);
+}
+
+// The large-molecule eigensolves, at most xl_max_jobs() per persistent launch,
+// ordered after the previous such launch on any stream of this engine.
+void Batch::launch_xl_eigen_all(Engine* e, hipStream_t st) {
+    if (list_xl.empty()) return;
+    if (e->xl_chain) hipStreamWaitEvent(st, e->xl_chain, 0);
+    for (size_t g = 0; g < list_xl.size(); g += xl_max_jobs()) {
+        const int cnt = (int)std::min<size_t>(xl_max_jobs(), list_xl.size() - g);
+        launch_xl_eigen(st, xl_grid, d_jobs, d_state, d_list_xl + g, d_xl_off + g, cnt, d_arena, d_eps, d_perm, d_xlws,
+                        d_bar);
+    }
+    if (e->xl_chain) hipEventRecord(e->xl_chain, st);
 }
 
 int Batch::run_scf(Engine* e, hipStream_t st) {
@@ -423,7 +476,8 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             launch_fock(st, 2, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
                         d_ints, d_ints32, d_arena, d_part);
             HIPCHK(hipEventRecord(e->ev[9], st));
-            launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining);
+            launch_xl_energy(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
+            launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
             HIPCHK(hipEventRecord(e->ev[1], st));
             launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
             const bool hl = heig_supported(maxn_lds), hg = heig_supported(maxn_glb);
@@ -444,9 +498,14 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
                                          d_eps, d_perm, d_tri, false);
                 HIPCHK(hipEventRecord(e->ev[3 + part], st));
             }
+            launch_xl_eigen_all(e, st);
+            HIPCHK(hipEventRecord(e->ev[10], st));
             launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
             HIPCHK(hipEventRecord(e->ev[6], st));
-            launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining);
+            launch_xl_dots(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
+            launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining, d_xlpart,
+                        d_xlcoef, xl_chunks);
+            launch_xl_apply(st, d_jobs, d_list_xl, (int)list_xl.size(), d_arena, d_xlcoef, xl_chunks);
             HIPCHK(hipEventRecord(e->ev[7], st));
         }
         HIPCHK(hipGetLastError());
@@ -454,12 +513,12 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
         HIPCHK(hipStreamSynchronize(st));
         if (prof) {
             // (stage, first event, last event)
-            const int map[9][3] = {{NDDO_STAGE_FOCK, 0, 9},         {NDDO_STAGE_EIGEN, 1, 5},
-                                   {NDDO_STAGE_TRIDIAG_LDS, 1, 2},  {NDDO_STAGE_TRIDIAG_GLB, 2, 3},
-                                   {NDDO_STAGE_DC, 3, 4},           {NDDO_STAGE_BACKTRANSFORM, 4, 5},
-                                   {NDDO_STAGE_DENSITY, 5, 6},      {NDDO_STAGE_STEP, 6, 7},
-                                   {NDDO_STAGE_ENERGY, 9, 1}};
-            for (int k = 0; k < 9; ++k) {
+            const int map[10][3] = {{NDDO_STAGE_FOCK, 0, 9},         {NDDO_STAGE_EIGEN, 1, 10},
+                                    {NDDO_STAGE_TRIDIAG_LDS, 1, 2},  {NDDO_STAGE_TRIDIAG_GLB, 2, 3},
+                                    {NDDO_STAGE_DC, 3, 4},           {NDDO_STAGE_BACKTRANSFORM, 4, 5},
+                                    {NDDO_STAGE_BLOCK_JACOBI, 5, 10}, {NDDO_STAGE_DENSITY, 10, 6},
+                                    {NDDO_STAGE_STEP, 6, 7},         {NDDO_STAGE_ENERGY, 9, 1}};
+            for (int k = 0; k < 10; ++k) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, e->ev[map[k][1]], e->ev[map[k][2]]));
                 e->stage_ms[map[k][0]] += ms;
@@ -590,6 +649,7 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     }
     HIPCHK(hipHostMalloc((void**)&e->h_remaining, sizeof(int), hipHostMallocDefault));
     for (int i = 0; i < 16; ++i) HIPCHK(hipEventCreate(&e->ev[i]));
+    HIPCHK(hipEventCreateWithFlags(&e->xl_chain, hipEventDisableTiming));
     size_t freeb = 0, totb = 0;
     HIPCHK(hipMemGetInfo(&freeb, &totb));
     e->hbm_budget = e->opts.hbm_budget_bytes ? e->opts.hbm_budget_bytes : (size_t)(0.9 * freeb);
@@ -607,6 +667,7 @@ int nddo_engine_destroy(nddo_engine* eh) {
     if (e->h_remaining) hipHostFree(e->h_remaining);
     for (int i = 0; i < 16; ++i)
         if (e->ev[i]) hipEventDestroy(e->ev[i]);
+    if (e->xl_chain) hipEventDestroy(e->xl_chain);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
     return NDDO_OK;
@@ -630,7 +691,7 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
     if (ok.empty()) return NDDO_OK;
     Batch b;
     b.build(descs, njobs, ok, nullptr);
-    b.classify(e->lds_max);
+    b.classify(e->lds_max, e->num_cus);
     int rc = b.allocate(false);
     if (rc) return rc;
     rc = b.upload(e->stream, descs);
@@ -709,7 +770,7 @@ static int one_job(Engine* e, Batch& b, nddo_job_desc& d, int theory, int natoms
     if (s) return s;
     std::vector<int> which{0};
     b.build(&d, 1, which, nullptr);
-    b.classify(e->lds_max);
+    b.classify(e->lds_max, e->num_cus);
     int rc = b.allocate(sdbg);
     if (rc) return rc;
     return b.upload(e->stream, &d);
@@ -811,13 +872,18 @@ static int matrix_job(Engine* e, Batch& b, int n, int nocc) {
     b.arena_size = (long long)ARENA_FIXED * n * n;
     b.src = {0};
     b.max_iter_all = 1;
-    b.classify(e->lds_max);
+    b.classify(e->lds_max, e->num_cus);
     int rc = b.allocate(false);
     if (rc) return rc;
     HIPCHK(hipMemcpy(b.d_jobs, b.jobs.data(), sizeof(JobDev), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.d_tiles, b.tiles.data(), b.tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice));
     if (!b.list_lds.empty()) HIPCHK(hipMemcpy(b.d_list_lds, b.list_lds.data(), sizeof(int), hipMemcpyHostToDevice));
     if (!b.list_glb.empty()) HIPCHK(hipMemcpy(b.d_list_glb, b.list_glb.data(), sizeof(int), hipMemcpyHostToDevice));
+    if (!b.list_xl.empty()) {
+        HIPCHK(hipMemcpy(b.d_list_xl, b.list_xl.data(), sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.d_xl_off, b.xl_off.data(), sizeof(long long), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.d_jobs, b.jobs.data(), sizeof(JobDev), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemset(b.d_arena, 0, b.arena_size * sizeof(double)));
     JobState s;
     std::memset(&s, 0, sizeof(s));
@@ -840,6 +906,7 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
                  b.d_perm, b.d_tri, jacobi_lds_bytes(b.maxn_lds, true), true);
     launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_glb, (int)b.list_glb.size(), b.maxn_glb, b.d_arena, b.d_eps,
                  b.d_perm, b.d_tri, jacobi_lds_bytes(b.maxn_glb, false), false);
+    b.launch_xl_eigen_all(e, e->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
     std::vector<int> perm(n);

@@ -38,8 +38,11 @@ struct Batch {
     std::vector<PairDev> pairs;
     std::vector<JobDev> jobs;
     std::vector<TileDev> tiles;
-    std::vector<int> list_lds, list_glb, src;
-    int maxn_lds = 0, maxn_glb = 0, max_iter_all = 0;
+    std::vector<int> list_lds, list_glb, list_xl, src;
+    std::vector<long long> xl_off;   // per list_xl entry: offset in d_xlws
+    int maxn_lds = 0, maxn_glb = 0, maxn_xl = 0, max_iter_all = 0;
+    int xl_grid = 1, xl_chunks = 1;
+    long long xlws_size = 0;
     long long nslots = 0, nints = 0, arena_size = 0, nvec = 0, ntraj = 0;
 
     AtomDev* d_atoms = nullptr;
@@ -60,6 +63,13 @@ struct Batch {
     int* d_remaining = nullptr;
     int* d_list_lds = nullptr;
     int* d_list_glb = nullptr;
+    // large molecules (kernels_xl.hip)
+    int* d_list_xl = nullptr;
+    long long* d_xl_off = nullptr;
+    double* d_xlws = nullptr;
+    double* d_xlpart = nullptr;
+    double* d_xlcoef = nullptr;
+    unsigned* d_bar = nullptr;
     double* d_sdbg = nullptr;
     hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream
 
@@ -67,7 +77,8 @@ struct Batch {
     int dalloc(void** p, size_t bytes);
     void release();
     int build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* unused);
-    int classify(size_t lds_max);
+    int classify(size_t lds_max, int num_cus);
+    void launch_xl_eigen_all(Engine* e, hipStream_t st);
     int allocate(bool want_sdbg);
     size_t upload_bytes() const;
     int upload(hipStream_t st, const nddo_job_desc* descs, char* stage = nullptr);
@@ -93,6 +104,10 @@ struct Engine {
     double stage_ms[NDDO_STAGE_COUNT] = {};
     long long stage_launches[NDDO_STAGE_COUNT] = {};
     hipEvent_t ev[16] = {};
+    // completion of the last large-molecule eigensolve launched on any stream:
+    // those persistent launches run one at a time, so their workgroups are
+    // always resident together
+    hipEvent_t xl_chain = nullptr;
     double last_eig_prof[4] = {};
     double last_dc_phase[7] = {};
 

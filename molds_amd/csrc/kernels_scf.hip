@@ -879,12 +879,18 @@ __device__ bool small_solve(int m, double* a /* m*m row-major, destroyed */, dou
 
 __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
                                               double* __restrict__ arena, const double* __restrict__ rmspart,
-                                              double* __restrict__ traj, int* __restrict__ remaining) {
+                                              double* __restrict__ traj, int* __restrict__ remaining,
+                                              const double* __restrict__ xlpart, double* __restrict__ xlcoef,
+                                              int nchunk) {
     const int job = blockIdx.x;
     JobState* st = &state[job];
-    if (st->phase != PH_ACTIVE) return;
     const JobDev jb = jobs[job];
     const int tid = threadIdx.x, T = blockDim.x;
+    // large molecules: the element-wise work runs in k_xl_dots / k_xl_apply;
+    // nothing to apply unless the DIIS / damping decisions below say so
+    double* xc = jb.xl ? xlcoef + (long long)job * XL_COEF : nullptr;
+    if (xc && tid == 0) xc[XL_COEF_FLAGS] = 0.0;
+    if (st->phase != PH_ACTIVE) return;
     const int n = jb.nao, iter = st->iter, k = jb.diis_k;
     const long long nn = (long long)n * n;
     __shared__ double sh_rms;
@@ -936,7 +942,14 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
             double part[NDDO_MAX_DIIS];
             for (int i = 0; i < k; ++i) part[i] = 0.0;
             double amax = 0.0;
-            for (long long e = tid; e < nn; e += T) {
+            if (xc) {   // chunk partials of k_xl_dots, fixed order per thread
+                for (int c = tid; c < nchunk; c += T) {
+                    const double* rec = xlpart + ((long long)(jb.xl - 1) * nchunk + c) * XL_REC;
+                    for (int i = 0; i < k; ++i) part[i] += rec[i];
+                    amax = fmax(amax, rec[k]);
+                }
+            }
+            for (long long e = xc ? nn : tid; e < nn; e += T) {
                 const double pv = Pout[e], ev = pv - Pin[e];
                 RP[(long long)slot_new * nn + e] = pv;
                 RE[(long long)slot_new * nn + e] = ev;
@@ -986,7 +999,13 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
                 st->ring_start = rs;
             }
             __syncthreads();
-            if (diis_on) {
+            if (xc && tid == 0) {
+                for (int i = 0; i < k; ++i) xc[i] = coef[i];
+                xc[XL_COEF_FLAGS] = diis_on ? 1.0 : 0.0;
+                xc[XL_COEF_ITER] = (double)iter;
+                xc[XL_COEF_RS] = (double)rs;
+            }
+            if (diis_on && !xc) {
                 for (long long e = tid; e < nn; e += T) {
                     double s = 0.0;
                     for (int i = 0; i < k; ++i) s += coef[i] * RP[(long long)((rs + i) % k) * nn + e];
@@ -996,9 +1015,16 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
         }
         const double w = jb.damp_weight;
         const int don = (0.0 < w && jb.damp_thresh < rms) ? 1 : 0;
-        if (don)
+        if (don && !xc)
             for (long long e = tid; e < nn; e += T) Pout[e] = Pout[e] * (1.0 - w) + w * Pin[e];
-        if (tid == 0) st->damp_on = don;
+        if (tid == 0) {
+            st->damp_on = don;
+            if (xc && don) {
+                xc[XL_COEF_FLAGS] += 2.0;
+                xc[XL_COEF_ITER] = (double)iter;
+                if (k <= 0) xc[XL_COEF_RS] = 0.0;
+            }
+        }
     }
     if (tid == 0) {
         st->iter = iter + 1;
@@ -1012,15 +1038,18 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
 }
 
 void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
-                 const double* rmspart, double* traj, int* remaining) {
-    hipLaunchKernelGGL(k_step, dim3(njobs), dim3(256), 0, st, jobs, state, arena, rmspart, traj, remaining);
+                 const double* rmspart, double* traj, int* remaining, const double* xlpart, double* xlcoef,
+                 int nchunk) {
+    hipLaunchKernelGGL(k_step, dim3(njobs), dim3(256), 0, st, jobs, state, arena, rmspart, traj, remaining, xlpart,
+                       xlcoef, nchunk);
 }
 
 // ------------------------------------------------------------------ energy
 // E_el = 1/2 sum_ij P_ij (F_ij + H_ij) with F built from the final P
 // (Cndo2.cpp:1348-1381); compensated summation.
 __global__ void __launch_bounds__(256) k_energy(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
-                                                double* __restrict__ arena, int* __restrict__ remaining) {
+                                                double* __restrict__ arena, int* __restrict__ remaining,
+                                                const double* __restrict__ xlpart, int nchunk) {
     const int job = blockIdx.x;
     JobState* st = &state[job];
     if (st->phase != PH_ENERGY) return;
@@ -1031,7 +1060,16 @@ __global__ void __launch_bounds__(256) k_energy(const JobDev* __restrict__ jobs,
     const double* F = mat(arena, jb, AR_F);
     const double* H = mat(arena, jb, AR_H);
     double s = 0.0, c = 0.0;
-    for (long long e = tid; e < nn; e += blockDim.x) {
+    if (jb.xl) {   // compensated chunk partials of k_xl_energy, fixed order
+        for (int ch = tid; ch < nchunk; ch += blockDim.x) {
+            const double* rec = xlpart + ((long long)(jb.xl - 1) * nchunk + ch) * XL_REC;
+            const double y = (rec[0] - rec[1]) - c;
+            const double t = s + y;
+            c = (t - s) - y;
+            s = t;
+        }
+    }
+    for (long long e = jb.xl ? nn : tid; e < nn; e += blockDim.x) {
         const double y = P[e] * (F[e] + H[e]) - c;
         const double t = s + y;
         c = (t - s) - y;
@@ -1056,8 +1094,9 @@ __global__ void __launch_bounds__(256) k_energy(const JobDev* __restrict__ jobs,
     }
 }
 
-void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena, int* remaining) {
-    hipLaunchKernelGGL(k_energy, dim3(njobs), dim3(256), 0, st, jobs, state, arena, remaining);
+void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena, int* remaining,
+                   const double* xlpart, int nchunk) {
+    hipLaunchKernelGGL(k_energy, dim3(njobs), dim3(256), 0, st, jobs, state, arena, remaining, xlpart, nchunk);
 }
 
 }  // namespace nddo

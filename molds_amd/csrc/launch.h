@@ -49,10 +49,33 @@ void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int
 void launch_density(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs,
                     const JobState* state, double* arena, const int* perm, double* rmspart);
 
+// xlpart / xlcoef / nchunk: chunk partials and DIIS coefficients of the large
+// molecules (JobDev::xl > 0); unused by the others.
 void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
-                 const double* rmspart, double* traj, int* remaining);
+                 const double* rmspart, double* traj, int* remaining, const double* xlpart, double* xlcoef,
+                 int nchunk);
 
 void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
-                   int* remaining);
+                   int* remaining, const double* xlpart, int nchunk);
+
+// Large molecules (kernels_xl.hip): device-wide block-Jacobi eigensolver, one
+// persistent launch of `grid` workgroups (at most one per CU) over up to
+// xl_max_jobs() jobs; xloff[i] is job i's offset in xlws (xl_ws_doubles(n)
+// each); bar: two unsigned words, reset by the launch itself.
+long long xl_ws_doubles(int n);
+int xl_grid(int num_cus, const int* ns, int nlist);
+int xl_max_jobs();
+int xl_nchunk(int max_n);
+void launch_xl_eigen(hipStream_t st, int grid, const JobDev* jobs, JobState* state, const int* joblist,
+                     const long long* xloff, int nlist, double* arena, double* eps, int* perm, double* xlws,
+                     unsigned* bar);
+// Chunked DIIS dot products (before k_step) and DIIS / damping update (after
+// it), E_el partials (before k_energy); xlpart holds nlist * nchunk * XL_REC doubles.
+void launch_xl_dots(hipStream_t st, const JobDev* jobs, const JobState* state, const int* joblist, int nlist,
+                    double* arena, double* xlpart, int nchunk);
+void launch_xl_apply(hipStream_t st, const JobDev* jobs, const int* joblist, int nlist, double* arena,
+                     const double* xlcoef, int nchunk);
+void launch_xl_energy(hipStream_t st, const JobDev* jobs, const JobState* state, const int* joblist, int nlist,
+                      double* arena, double* xlpart, int nchunk);
 
 }  // namespace nddo

@@ -77,6 +77,7 @@ struct JobDev {
     int tiles_per_dim;
     // SCF options (MolDS Parameters SCF knobs)
     int max_iter, diis_k, requires_guess, precision;
+    int xl;               // 1 + position in the batch's large-molecule list (0: not large)
     double rms_thresh, damp_thresh, damp_weight, diis_start, diis_end;
     long long traj0;      // offset (doubles) into the trajectory buffer: max_iter * 4
 };
@@ -92,5 +93,14 @@ struct JobState {
 };
 
 struct TileDev { int job, i0, j0, pad; };
+
+// Large molecules (kernels_xl.hip): the chunked DIIS / energy reductions keep
+// NDDO_MAX_DIIS + 1 doubles per (job, chunk), and k_step leaves the DIIS
+// coefficients and what to apply for k_xl_apply in XL_COEF doubles per job.
+#define XL_REC (NDDO_MAX_DIIS + 1)
+#define XL_COEF (NDDO_MAX_DIIS + 4)
+#define XL_COEF_FLAGS (NDDO_MAX_DIIS + 1)   // bit 0: DIIS extrapolation, bit 1: damping
+#define XL_COEF_ITER (NDDO_MAX_DIIS + 2)
+#define XL_COEF_RS (NDDO_MAX_DIIS + 3)
 
 }  // namespace nddo

@@ -226,7 +226,7 @@ class Engine:
 
     # --------------------------------------------------------------- profiling
     STAGES = ("setup", "fock", "eigen", "density", "step", "energy", "tridiag_lds", "tridiag_glb", "divide_conquer",
-              "backtransform")
+              "backtransform", "block_jacobi")
 
     def server_stats(self, reset=False):
         t = np.zeros(4)

@@ -1,0 +1,105 @@
+"""Molecules above 512 AOs: the device-wide block-Jacobi eigensolver and the
+chunked DIIS / energy reductions (kernels_xl.hip).
+
+Tolerances are those of the one-workgroup paths: eigenvalues within 1e-13
+||A||_F of LAPACK, residual and orthonormality 1e-12; FP64 SCF within 1e-6 eV
+total energy and 1e-8 density RMS of the oracle; mixed precision within 1e-5 eV
+of FP64.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EV = 1.0 / 0.03674903
+SCF = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0,
+           damping_weight=0.0, max_iter=200)
+
+
+# 513: one row past the one-workgroup solvers; 600 / 1000: ragged last block
+# and an odd block count (a padding block in the round-robin); 1536: the
+# (H2O)_256 size
+@pytest.mark.parametrize("n", [513, 600, 1000, 1536])
+def test_eigh_large(engine, n):
+    print(f"eigh large n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    A[0, :] = A[:, 0] = 0.0
+    A[0, 0] = A[1, 1]          # an exact degeneracy and a decoupled row
+    t0 = time.perf_counter()
+    eps, C, sweeps = engine.dbg_eigh(A)
+    print(f"  {sweeps} sweeps, {time.perf_counter() - t0:.3f} s with transfers", flush=True)
+    w = np.linalg.eigvalsh(A)
+    nrm = np.linalg.norm(A)
+    assert sweeps > 0
+    assert np.max(np.abs(eps - w)) <= 1e-13 * nrm
+    assert np.all(np.diff(eps) >= 0)
+    assert np.max(np.abs(C @ A - eps[:, None] * C)) <= 1e-12 * nrm
+    assert np.max(np.abs(C @ C.T - np.eye(n))) <= 1e-12
+    assert np.all(C.sum(axis=1) >= 0)
+
+
+def test_eigh_large_clustered(engine):
+    """Tight eigenvalue clusters (like the orbital bands of a water cluster)."""
+    n = 700
+    rng = np.random.default_rng(7)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.repeat(rng.standard_normal(70), 10) + 1e-10 * rng.standard_normal(n)
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+    eps, C, sweeps = engine.dbg_eigh(A)
+    nrm = np.linalg.norm(A)
+    assert sweeps > 0
+    assert np.max(np.abs(eps - np.linalg.eigvalsh(A))) <= 1e-13 * nrm
+    assert np.max(np.abs(C @ A - eps[:, None] * C)) <= 1e-12 * nrm
+    assert np.max(np.abs(C @ C.T - np.eye(n))) <= 1e-12
+
+
+def _water(m, seed):
+    from molds_amd import workloads as W
+    Z, x = W.water_cluster(m, seed=seed, jitter=0.02)
+    return Z, W.to_bohr(x)
+
+
+def test_scf_large_fp64_and_mixed_vs_oracle(engine):
+    """(H2O)_90, 540 AOs, PM3: FP64 against the oracle, mixed against both."""
+    print("scf large (H2O)_90", flush=True)
+    from oracle import scf as O
+    Z, xyz = _water(90, 11)
+    t0 = time.perf_counter()
+    r64 = engine.scf("pm3", Z, xyz, precision="fp64", **SCF)
+    t1 = time.perf_counter()
+    rmx = engine.scf("pm3", Z, xyz, precision="mixed", **SCF)
+    t2 = time.perf_counter()
+    print(f"  fp64 {t1 - t0:.2f} s ({r64['iterations']} it), mixed {t2 - t1:.2f} s ({rmx['iterations']} it)", flush=True)
+    o = O.scf("pm3", Z, xyz, rms_density=1e-8, diis_k=5, diis_start=0.1, diis_end=0.0, damping_weight=0.0,
+              max_iter=200)
+    assert r64["status"] == 0 and rmx["status"] == 0
+    assert abs(r64["energy"] - o["energy"]) * EV <= 1e-6
+    assert np.sqrt(np.mean((r64["P"] - o["P"]) ** 2)) <= 1e-8
+    assert np.max(np.abs(r64["eps"] - o["eps"])) * EV <= 1e-6
+    assert abs(rmx["energy"] - r64["energy"]) * EV <= 1e-5
+    assert np.max(np.abs(rmx["eps"] - r64["eps"])) * EV <= 1e-5
+    assert 1 <= rmx["fp64_iterations"] < rmx["iterations"]
+
+
+def test_large_alone_equals_batch(engine):
+    """A large molecule gives the same result alone and batched with small ones
+    (its chunked reductions and block-Jacobi launch share the batch's stages)."""
+    from molds_amd import workloads as W
+    from molds_amd.nddo import Job
+    Z, xyz = _water(95, 12)
+    big = Job("pm3", Z, xyz, rms_density=1e-8)
+    small = [Job("pm3", *(lambda t: (t[0], W.to_bohr(t[1])))(W.alkane(6, seed=31)), rms_density=1e-8),
+             Job("am1", *_water(5, 32), rms_density=1e-8)]
+    alone = engine.scf_batch([big])[0]
+    mix = engine.scf_batch([small[0], big, small[1]])
+    assert alone["status"] == 0 and all(r["status"] == 0 for r in mix)
+    assert abs(alone["energy"] - mix[1]["energy"]) * EV <= 1e-9
+    assert alone["iterations"] == mix[1]["iterations"]
+    for j, r in zip(small, (mix[0], mix[2])):
+        s = engine.scf_batch([j])[0]
+        assert abs(s["energy"] - r["energy"]) * EV <= 1e-9
