@@ -920,6 +920,8 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
     for (int i = 0; i < 4; ++i) e->last_eig_prof[i] = s.prof[i];
     if (heig_supported(n))
         HIPCHK(hipMemcpy(e->last_dc_phase, b.d_tri + 19LL * n + 224, 7 * sizeof(double), hipMemcpyDeviceToHost));
+    else
+        HIPCHK(hipMemcpy(e->last_dc_phase, b.d_xlws + xl_dbg_offset(n), 9 * sizeof(double), hipMemcpyDeviceToHost));
     if (C) {
         for (int r = 0; r < n; ++r) {
             const double* src = &VT[(long long)perm[r] * n];

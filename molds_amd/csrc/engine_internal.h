@@ -109,7 +109,7 @@ struct Engine {
     // always resident together
     hipEvent_t xl_chain = nullptr;
     double last_eig_prof[4] = {};
-    double last_dc_phase[7] = {};
+    double last_dc_phase[9] = {};
 
     // serving state (server.cpp)
     std::mutex smu;

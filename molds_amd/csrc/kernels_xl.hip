@@ -40,9 +40,13 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 constexpr int BJ_B = 32;              // rows per block
 constexpr int BJ_P = 64;              // rows per pair block
-constexpr int BJ_LD = 65;             // LDS leading dimension of a 64 x 64 tile
+constexpr int BJ_LD = 68;             // LDS leading dimension of a 64 x 64 tile (2-way bank
+                                      // aliasing at most for the MFMA operand reads)
 constexpr int BJ_MAXSWEEP = 50;
-constexpr int BJ_INNER_MAXSWEEP = 40;
+// One cyclic sweep per pair block and round: diagonalizing the pair blocks
+// exactly buys no fewer outer sweeps (measured on random and near-diagonal
+// matrices) and costs several times more.
+constexpr int BJ_INNER_MAXSWEEP = 1;
 constexpr int BJ_MAXJOBS = 32;        // jobs per launch
 constexpr int BJ_MAXGRID = 1024;
 constexpr int BJ_T = 256;             // threads per workgroup
@@ -89,6 +93,25 @@ __device__ inline void bj_pair(int mp, int r, int k, int* p, int* q) {
 
 // Matrix row of local index a (0..63) of the pair block (p, q); >= n is padding.
 __device__ inline int bj_row(int p, int q, int a) { return a < BJ_B ? p * BJ_B + a : q * BJ_B + (a - BJ_B); }
+
+// Newton-Raphson refinements of the hardware reciprocal and reciprocal square
+// root estimates (v_rcp_f64, v_rsq_f64): two steps each, for arguments well
+// inside the normal range.
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ inline double rsq_nr(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e = fma(-h, r * r, 0.5);
+    r = fma(r, e, r);
+    e = fma(-h, r * r, 0.5);
+    return fma(r, e, r);
+}
 
 // Grid barrier.  bar[0] counts arrivals, bar[1] is the generation.  Returns
 // false when the wait ran out (the caller then abandons the solve).
@@ -153,68 +176,144 @@ struct BjShared {
     double red[BJ_T];
     double thr[BJ_MAXJOBS];
     int nsweep[BJ_MAXJOBS];
+    unsigned long long it[4];   // workgroup 0: pair-block load, rotations, write-back, phase-1 barrier wait
+    unsigned char bki[528], bkj[528];   // (row pair, column pair) of the 2 x 2 blocks, kj >= ki
+    int any[2];   // by round parity: a wave may still read round r's flag while wave 0 writes r + 1's
     int ok;
 };
 
 // Phase 1: exact eigen-decomposition of the 64 x 64 pair block k (round r):
 // S <- GT S GT^T diagonal; S written back to the matrix, GT to the workspace.
+// The 64 x 64 tile loops below have a fixed trip count per thread (thread t
+// owns column t % 64 of rows t / 64 + 4 i), fully unrolled so that all of a
+// thread's loads are in flight at once.
+#define BJ_ROWS 16   // rows per thread of a 64 x 64 tile
+
+// Phase 1: one cyclic Jacobi sweep of the 64 x 64 pair block (p, q):
+// S <- GT S GT^T written back to the matrix, GT to the workspace.
 __device__ void bj_inner(double* S, double* G, BjShared& sh, double* W, int n, int p, int q, double thr, double* gt,
                          double* flag, double* sweepflag) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, b = tid & 63, a0 = tid >> 6;
+    const bool clk = blockIdx.x == 0 && tid == 0;
+    unsigned long long c0 = clk ? wall_clock64() : 0ull;
+    const int rb = bj_row(p, q, b);
+    double v[BJ_ROWS];
+#pragma unroll
+    for (int i = 0; i < BJ_ROWS; ++i) {
+        const int ra = bj_row(p, q, a0 + 4 * i);
+        v[i] = (ra < n && rb < n) ? W[(long long)ra * n + rb] : 0.0;
+    }
     double offmax = 0.0;
-    for (int e = tid; e < BJ_P * BJ_P; e += BJ_T) {
-        const int a = e >> 6, b = e & 63;
-        const int ra = bj_row(p, q, a), rb = bj_row(p, q, b);
-        const double v = (ra < n && rb < n) ? W[(long long)ra * n + rb] : 0.0;
-        S[a * BJ_LD + b] = v;
+#pragma unroll
+    for (int i = 0; i < BJ_ROWS; ++i) {
+        const int a = a0 + 4 * i;
+        S[a * BJ_LD + b] = v[i];
         G[a * BJ_LD + b] = a == b ? 1.0 : 0.0;
-        if (a != b) offmax = fmax(offmax, fabs(v));
+        if (a != b) offmax = fmax(offmax, fabs(v[i]));
     }
     // the whole block is already diagonal to the threshold: nothing to rotate
     // (a NaN entry also lands here: fmax ignores it and the solve fails later
     // on the non-finite norm)
-    if (!__syncthreads_or(offmax > thr)) {
+    const int need = __syncthreads_or(offmax > thr);
+    if (clk) {
+        const unsigned long long c1 = wall_clock64();
+        sh.it[0] += c1 - c0;
+        c0 = c1;
+    }
+    if (!need) {
         if (tid == 0) *flag = 0.0;
         return;
     }
     for (int isw = 0; isw < BJ_INNER_MAXSWEEP; ++isw) {
         int any = 0;
         for (int r = 0; r < 63; ++r) {
-            int rot = 0;
-            if (tid < 32) {
+            if (tid < 64) {   // wave 0: the 32 rotations of this round
+                int rot = 0;
                 int pp, qq;
-                if (tid == 0) { pp = r; qq = 63; }
-                else { pp = (r + tid) % 63; qq = (r - tid + 63) % 63; }
+                const int kk = tid & 31;
+                if (kk == 0) { pp = r; qq = 63; }
+                else { pp = (r + kk) % 63; qq = (r - kk + 63) % 63; }
                 if (pp > qq) { const int t = pp; pp = qq; qq = t; }
                 double c = 1.0, s = 0.0, t = 0.0, u = 0.0;
                 const double apq = S[pp * BJ_LD + qq];
-                if (fabs(apq) > thr) {
+                if (tid < 32 && fabs(apq) > thr) {
+                    // t: the smaller root of t^2 + 2 tau t - 1 = 0, tau = (a_qq - a_pp) / (2 a_pq),
+                    // as 2 a_pq sign(d) / (|d| + sqrt(d^2 + 4 a_pq^2)) with d = a_qq - a_pp
                     const double app = S[pp * BJ_LD + pp], aqq = S[qq * BJ_LD + qq];
-                    const double tau = (aqq - app) / (2.0 * apq);
-                    t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    c = 1.0 / sqrt(1.0 + t * t);
-                    s = t * c;
-                    u = s / (1.0 + c);
+                    const double d = aqq - app, h = fma(d, d, 4.0 * apq * apq);
+                    if (h > 1e-290 && h < 1e290) {
+                        t = (d >= 0.0 ? 2.0 * apq : -2.0 * apq) * rcp_nr(fabs(d) + h * rsq_nr(h));
+                        c = rsq_nr(fma(t, t, 1.0));
+                        s = t * c;
+                        u = s * rcp_nr(1.0 + c);
+                    } else {   // extreme scales: the same rotation with IEEE division and square root
+                        const double tau = d / (2.0 * apq);
+                        t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                        u = s / (1.0 + c);
+                    }
                     rot = 1;
                 }
-                sh.sp[tid] = pp; sh.sq[tid] = qq; sh.sc[tid] = c; sh.ss[tid] = s; sh.st[tid] = t; sh.su[tid] = u;
-                sh.srot[tid] = rot;
+                if (tid < 32) {
+                    sh.sp[tid] = pp; sh.sq[tid] = qq; sh.sc[tid] = c; sh.ss[tid] = s; sh.st[tid] = t; sh.su[tid] = u;
+                    sh.srot[tid] = rot;
+                }
+                const bool anyr = __ballot(tid < 32 && rot) != 0ull;
+                if (tid == 0) sh.any[r & 1] = anyr ? 1 : 0;
             }
-            if (!__syncthreads_or(rot)) continue;
+            __syncthreads();
+            if (!sh.any[r & 1]) continue;
             any = 1;
-            // 2 x 2 blocks (rows {p1,q1}, columns {p2,q2}) of the upper block triangle
-            for (int b = tid; b < 528; b += BJ_T) {
-                int hi = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
-                while (hi * (hi + 1) / 2 > b) --hi;
-                while ((hi + 1) * (hi + 2) / 2 <= b) ++hi;
-                const int ki = b - hi * (hi + 1) / 2, kj = hi;
-                const int ri = sh.srot[ki], rj = sh.srot[kj];
-                if (!ri && !rj) continue;
-                const int p1 = sh.sp[ki], q1 = sh.sq[ki], p2 = sh.sp[kj], q2 = sh.sq[kj];
+            // 2 x 2 blocks (rows {p1,q1}, columns {p2,q2}) of the upper block triangle,
+            // three per thread; the blocks are disjoint, so every read goes first
+            int P1[3], Q1[3], P2[3], Q2[3], KI[3], KJ[3];
+            bool act[3];
+            double x0[3], x1[3], x2[3], x3[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int bb = tid + BJ_T * i;
+                const int ki = bb < 528 ? sh.bki[bb] : 0, kj = bb < 528 ? sh.bkj[bb] : 0;
+                KI[i] = ki;
+                KJ[i] = kj;
+                act[i] = bb < 528 && (sh.srot[min(ki, 31)] || sh.srot[min(kj, 31)]);
+                P1[i] = sh.sp[min(ki, 31)]; Q1[i] = sh.sq[min(ki, 31)];
+                P2[i] = sh.sp[min(kj, 31)]; Q2[i] = sh.sq[min(kj, 31)];
+                x0[i] = x1[i] = x2[i] = x3[i] = 0.0;
+                if (act[i]) {
+                    if (ki == kj) {
+                        x0[i] = S[P1[i] * BJ_LD + Q1[i]];
+                    } else {
+                        x0[i] = S[P1[i] * BJ_LD + P2[i]];
+                        x1[i] = S[P1[i] * BJ_LD + Q2[i]];
+                        x2[i] = S[Q1[i] * BJ_LD + P2[i]];
+                        x3[i] = S[Q1[i] * BJ_LD + Q2[i]];
+                    }
+                }
+            }
+            double aP[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) aP[i] = (act[i] && KI[i] == KJ[i]) ? S[P1[i] * BJ_LD + P1[i]] : 0.0;
+            double aQ[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) aQ[i] = (act[i] && KI[i] == KJ[i]) ? S[Q1[i] * BJ_LD + Q1[i]] : 0.0;
+            // GT rows of the rotations: GT <- J^T GT, eight (pair, column) entries per thread
+            double gp[8], gq[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = a0 + 4 * i;
+                gp[i] = G[sh.sp[k] * BJ_LD + b];
+                gq[i] = G[sh.sq[k] * BJ_LD + b];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                if (!act[i]) continue;
+                const int ki = KI[i], kj = KJ[i];
+                const int p1 = P1[i], q1 = Q1[i], p2 = P2[i], q2 = Q2[i];
                 if (ki == kj) {
-                    const double t = sh.st[ki], apq = S[p1 * BJ_LD + q1];
-                    S[p1 * BJ_LD + p1] -= t * apq;
-                    S[q1 * BJ_LD + q1] += t * apq;
+                    const double t = sh.st[ki], apq = x0[i];
+                    S[p1 * BJ_LD + p1] = aP[i] - t * apq;
+                    S[q1 * BJ_LD + q1] = aQ[i] + t * apq;
                     S[p1 * BJ_LD + q1] = 0.0;
                     S[q1 * BJ_LD + p1] = 0.0;
                     continue;
@@ -222,8 +321,7 @@ __device__ void bj_inner(double* S, double* G, BjShared& sh, double* W, int n, i
                 // rotations in Rutishauser's form x - s (y + u x), y + s (x - u y), which
                 // keeps them orthogonal to rounding also when 1 + t^2 rounds to 1
                 const double s1 = sh.ss[ki], u1 = sh.su[ki], s2 = sh.ss[kj], u2 = sh.su[kj];
-                const double a = S[p1 * BJ_LD + p2], bq = S[p1 * BJ_LD + q2];
-                const double cq = S[q1 * BJ_LD + p2], d = S[q1 * BJ_LD + q2];
+                const double a = x0[i], bq = x1[i], cq = x2[i], d = x3[i];
                 const double a1 = a - s1 * (cq + u1 * a), b1 = bq - s1 * (d + u1 * bq);
                 const double c1r = cq + s1 * (a - u1 * cq), d1 = d + s1 * (bq - u1 * d);
                 const double a2 = a1 - s2 * (b1 + u2 * a1), b2 = b1 + s2 * (a1 - u2 * b1);
@@ -233,41 +331,64 @@ __device__ void bj_inner(double* S, double* G, BjShared& sh, double* W, int n, i
                 S[q1 * BJ_LD + p2] = c2r; S[p2 * BJ_LD + q1] = c2r;
                 S[q1 * BJ_LD + q2] = d2; S[q2 * BJ_LD + q1] = d2;
             }
-            // rotation rows of GT: GT <- J^T GT
-            for (int e = tid; e < 32 * BJ_P; e += BJ_T) {
-                const int k = e >> 6, c = e & 63;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = a0 + 4 * i;
                 if (!sh.srot[k]) continue;
-                const int pp = sh.sp[k], qq = sh.sq[k];
                 const double sn = sh.ss[k], u = sh.su[k];
-                const double gp = G[pp * BJ_LD + c], gq = G[qq * BJ_LD + c];
-                G[pp * BJ_LD + c] = gp - sn * (gq + u * gp);
-                G[qq * BJ_LD + c] = gq + sn * (gp - u * gq);
+                G[sh.sp[k] * BJ_LD + b] = gp[i] - sn * (gq[i] + u * gp[i]);
+                G[sh.sq[k] * BJ_LD + b] = gq[i] + sn * (gp[i] - u * gq[i]);
             }
             __syncthreads();
         }
         if (!any) break;
+        // a sweep that would rotate nothing leaves S as it is, so checking every
+        // off-diagonal entry against the threshold now decides the same thing
+        double om = 0.0;
+#pragma unroll
+        for (int i = 0; i < BJ_ROWS; ++i) {
+            const int a = a0 + 4 * i;
+            if (a != b) om = fmax(om, fabs(S[a * BJ_LD + b]));
+        }
+        if (!__syncthreads_or(om > thr)) break;
     }
-    for (int e = tid; e < BJ_P * BJ_P; e += BJ_T) {
-        const int a = e >> 6, b = e & 63;
-        const int ra = bj_row(p, q, a), rb = bj_row(p, q, b);
+    if (clk) {
+        const unsigned long long c1 = wall_clock64();
+        sh.it[1] += c1 - c0;
+        c0 = c1;
+    }
+#pragma unroll
+    for (int i = 0; i < BJ_ROWS; ++i) {
+        const int a = a0 + 4 * i;
+        const int ra = bj_row(p, q, a);
         if (ra < n && rb < n) W[(long long)ra * n + rb] = S[a * BJ_LD + b];
-        gt[e] = G[a * BJ_LD + b];
+        gt[a * BJ_P + b] = G[a * BJ_LD + b];
     }
     if (tid == 0) {
         *flag = 1.0;
         *sweepflag = 1.0;
     }
+    if (clk) sh.it[2] += wall_clock64() - c0;
 }
 
 // Phase 2, off-diagonal tile (i, j), i < j:  A_ij <- GT_i A_ij GT_j^T, both triangles.
 __device__ void bj_tile(double* As, double* Bs, double* W, int n, int pi, int qi, int pj, int qj, bool fi, bool fj,
                         const double* gti, const double* gtj) {
-    const int tid = threadIdx.x;
-    for (int e = tid; e < BJ_P * BJ_P; e += BJ_T) {
-        const int a = e >> 6, b = e & 63;
-        const int ra = bj_row(pi, qi, a), rb = bj_row(pj, qj, b);
-        As[a * BJ_LD + b] = (ra < n && rb < n) ? W[(long long)ra * n + rb] : 0.0;
-        if (fj) Bs[a * BJ_LD + b] = gtj[e];
+    const int tid = threadIdx.x, b = tid & 63, a0 = tid >> 6;
+    {
+        const int rb = bj_row(pj, qj, b);
+        double v[BJ_ROWS], g[BJ_ROWS];
+#pragma unroll
+        for (int i = 0; i < BJ_ROWS; ++i) {
+            const int ra = bj_row(pi, qi, a0 + 4 * i);
+            v[i] = (ra < n && rb < n) ? W[(long long)ra * n + rb] : 0.0;
+            g[i] = fj ? gtj[(a0 + 4 * i) * BJ_P + b] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < BJ_ROWS; ++i) {
+            As[(a0 + 4 * i) * BJ_LD + b] = v[i];
+            Bs[(a0 + 4 * i) * BJ_LD + b] = g[i];
+        }
     }
     __syncthreads();
     dbl4 acc[4];
@@ -277,32 +398,49 @@ __device__ void bj_tile(double* As, double* Bs, double* W, int n, int pi, int qi
         bj_store(As, acc);
     }
     if (fi) {   // T = GT_i X
-        for (int e = tid; e < BJ_P * BJ_P; e += BJ_T) Bs[(e >> 6) * BJ_LD + (e & 63)] = gti[e];
+        double g[BJ_ROWS];
+#pragma unroll
+        for (int i = 0; i < BJ_ROWS; ++i) g[i] = gti[(a0 + 4 * i) * BJ_P + b];
+#pragma unroll
+        for (int i = 0; i < BJ_ROWS; ++i) Bs[(a0 + 4 * i) * BJ_LD + b] = g[i];
         __syncthreads();
         bj_gemm<false>(Bs, As, acc);
         __syncthreads();
         bj_store(As, acc);
     }
     __syncthreads();
-    for (int e = tid; e < BJ_P * BJ_P; e += BJ_T) {
-        const int a = e >> 6, b = e & 63;
-        const int ra = bj_row(pi, qi, a), rb = bj_row(pj, qj, b);
-        if (ra < n && rb < n) W[(long long)ra * n + rb] = As[a * BJ_LD + b];
-        // transposed copy: row (pj,qj)[a], column (pi,qi)[b]
-        const int ta = bj_row(pj, qj, a), tb = bj_row(pi, qi, b);
-        if (ta < n && tb < n) W[(long long)ta * n + tb] = As[b * BJ_LD + a];
+    {
+        const int rbj = bj_row(pj, qj, b), rbi = bj_row(pi, qi, b);
+#pragma unroll
+        for (int i = 0; i < BJ_ROWS; ++i) {
+            const int a = a0 + 4 * i;
+            const int ra = bj_row(pi, qi, a);
+            if (ra < n && rbj < n) W[(long long)ra * n + rbj] = As[a * BJ_LD + b];
+            // transposed copy: row (pj,qj)[a], column (pi,qi)[b]
+            const int ta = bj_row(pj, qj, a);
+            if (ta < n && rbi < n) W[(long long)ta * n + rbi] = As[b * BJ_LD + a];
+        }
     }
     __syncthreads();
 }
 
 // Phase 2, eigenvector tile: rows (pi, qi) x columns c0 .. c0 + 63 of VT <- GT_i VT.
 __device__ void bj_vt_tile(double* As, double* Bs, double* VT, int n, int pi, int qi, int c0, const double* gti) {
-    const int tid = threadIdx.x;
-    for (int e = tid; e < BJ_P * BJ_P; e += BJ_T) {
-        const int a = e >> 6, b = e & 63;
-        const int ra = bj_row(pi, qi, a), cb = c0 + b;
-        As[a * BJ_LD + b] = (ra < n && cb < n) ? VT[(long long)ra * n + cb] : 0.0;
-        Bs[a * BJ_LD + b] = gti[e];
+    const int tid = threadIdx.x, b = tid & 63, a0 = tid >> 6;
+    {
+        const int cb = c0 + b;
+        double v[BJ_ROWS], g[BJ_ROWS];
+#pragma unroll
+        for (int i = 0; i < BJ_ROWS; ++i) {
+            const int ra = bj_row(pi, qi, a0 + 4 * i);
+            v[i] = (ra < n && cb < n) ? VT[(long long)ra * n + cb] : 0.0;
+            g[i] = gti[(a0 + 4 * i) * BJ_P + b];
+        }
+#pragma unroll
+        for (int i = 0; i < BJ_ROWS; ++i) {
+            As[(a0 + 4 * i) * BJ_LD + b] = v[i];
+            Bs[(a0 + 4 * i) * BJ_LD + b] = g[i];
+        }
     }
     __syncthreads();
     dbl4 acc[4];
@@ -311,10 +449,10 @@ __device__ void bj_vt_tile(double* As, double* Bs, double* VT, int n, int pi, in
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int a = 16 * w + 4 * v + lk, c = c0 + 16 * cb + li;
+        for (int vv = 0; vv < 4; ++vv) {
+            const int a = 16 * w + 4 * vv + lk, c = c0 + 16 * cb + li;
             const int ra = bj_row(pi, qi, a);
-            if (ra < n && c < n) VT[(long long)ra * n + c] = acc[cb][v];
+            if (ra < n && c < n) VT[(long long)ra * n + c] = acc[cb][vv];
         }
     __syncthreads();
 }
@@ -344,6 +482,17 @@ __global__ void __launch_bounds__(BJ_T) k_bjac(const JobDev* __restrict__ jobs, 
     const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x;
     const long long gt0 = (long long)wg * BJ_T + tid, gstride = (long long)G * BJ_T;
     unsigned gen = 0;   // bar[] is reset to zero before every launch
+    // device clock ticks seen by workgroup 0: phase 1 (pair blocks, up to the
+    // barrier), phase 2 (tiles, up to the barrier), the rest; rounds run
+    unsigned long long tk[3] = {0, 0, 0}, tl = wall_clock64(), nrounds = 0;
+    const unsigned long long cyc0 = clock64(), wal0 = tl;
+    if (tid < 4) sh.it[tid] = 0;
+    for (int hi = 0; hi < 32; ++hi)
+        for (int ki = tid; ki <= hi; ki += BJ_T) {
+            sh.bki[hi * (hi + 1) / 2 + ki] = (unsigned char)ki;
+            sh.bkj[hi * (hi + 1) / 2 + ki] = (unsigned char)hi;
+        }
+    __syncthreads();
     bool alive = true;   // false once a barrier wait ran out
 
     // jobs solved in this launch
@@ -423,7 +572,15 @@ __global__ void __launch_bounds__(BJ_T) k_bjac(const JobDev* __restrict__ jobs, 
                     __syncthreads();
                 }
             }
+            const unsigned long long tb = wall_clock64();
             alive = grid_sync(bar, G, &gen, &sh.ok);
+            if (wg == 0 && tid == 0) sh.it[3] += wall_clock64() - tb;
+            if (wg == 0 && tid == 0) {
+                const unsigned long long t = wall_clock64();
+                tk[0] += t - tl;
+                tl = t;
+                ++nrounds;
+            }
             if (!alive) break;
             // phase 2: off-diagonal tiles and eigenvector tiles
             c = 0;
@@ -463,6 +620,11 @@ __global__ void __launch_bounds__(BJ_T) k_bjac(const JobDev* __restrict__ jobs, 
                 c += nt;
             }
             alive = grid_sync(bar, G, &gen, &sh.ok);
+            if (wg == 0 && tid == 0) {
+                const unsigned long long t = wall_clock64();
+                tk[1] += t - tl;
+                tl = t;
+            }
         }
         if (!alive) break;
         for (int jl = 0; jl < nlist; ++jl) {
@@ -511,6 +673,19 @@ __global__ void __launch_bounds__(BJ_T) k_bjac(const JobDev* __restrict__ jobs, 
         if (wg == 0 && tid == 0) {
             state[job].sweeps = bad ? -1 : sh.nsweep[jl];
             state[job].warm = bad ? 0 : 1;
+            const unsigned long long t = wall_clock64();
+            state[job].prof[0] = (double)tk[0];
+            state[job].prof[1] = (double)tk[1];
+            state[job].prof[2] = (double)(tk[2] + (t - tl));
+            state[job].prof[3] = (double)nrounds;
+            // the same and the parts of workgroup 0's phase 1, for the profiling hook
+            double* dt = xlws + xloff[jl] + bj_ws_sweep(bj_geom(n).np, n) + BJ_MAXSWEEP;
+            for (int k = 0; k < 4; ++k) dt[k] = (double)sh.it[k];
+            dt[4] = (double)tk[0];
+            dt[5] = (double)tk[1];
+            dt[6] = (double)nrounds;
+            dt[7] = (double)(clock64() - cyc0);    // shader clock cycles over the solve
+            dt[8] = (double)(wall_clock64() - wal0);
         }
     }
 }
@@ -661,6 +836,11 @@ int xl_grid(int num_cus, const int* ns, int nlist) {
 }
 
 int xl_max_jobs() { return BJ_MAXJOBS; }
+
+long long xl_dbg_offset(int n) {
+    const int m = (n + BJ_B - 1) / BJ_B, np = (m + (m & 1)) / 2;
+    return bj_ws_sweep(np, n) + BJ_MAXSWEEP;
+}
 
 int xl_nchunk(int max_n) {
     const long long nn = (long long)max_n * max_n;

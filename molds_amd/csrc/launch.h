@@ -65,6 +65,8 @@ void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* stat
 long long xl_ws_doubles(int n);
 int xl_grid(int num_cus, const int* ns, int nlist);
 int xl_max_jobs();
+// Offset in a job's workspace of 7 doubles of phase clock ticks of its last solve.
+long long xl_dbg_offset(int n);
 int xl_nchunk(int max_n);
 void launch_xl_eigen(hipStream_t st, int grid, const JobDev* jobs, JobState* state, const int* joblist,
                      const long long* xloff, int nlist, double* arena, double* eps, int* perm, double* xlws,
