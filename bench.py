@@ -79,12 +79,15 @@ def batched_leg(engine, jobs, steps, warmup):
     return res, wall, st
 
 
-def served_leg(engine, rate, count, precision, seed=3):
+SERVED_LO, SERVED_HI = 5, 600   # request sizes (atoms), log-uniform
+
+
+def served_leg(engine, rate, count, precision, seed=3, hi=SERVED_HI):
     """Poisson arrivals through nddo_submit; returns throughput and latency percentiles."""
     from molds_amd import workloads as W
     from molds_amd.nddo import Job
     reqs = [(t, Job("pm3", Z, W.to_bohr(x), precision=precision, **SERVED_SCF))
-            for t, _, _, Z, x in W.poisson_requests(rate, count, seed=seed, lo=5, hi=256)]
+            for t, _, _, Z, x in W.poisson_requests(rate, count, seed=seed, lo=SERVED_LO, hi=hi)]
     ids = []
     t0 = time.perf_counter()
     for t, job in reqs:
@@ -109,7 +112,8 @@ def served_leg(engine, rate, count, precision, seed=3):
                 solves_per_s=round(count / span, 3), span_s=round(span, 3),
                 p50_s=round(float(np.percentile(lat, 50)), 5), p99_s=round(float(np.percentile(lat, 99)), 5),
                 mean_iterations=round(float(np.mean(it)), 2), mean_fp64_iterations=round(float(np.mean(fp64_it)), 2),
-                sizes="log-uniform 5..256 atoms, alkane / water clusters", rms_density=SERVED_SCF["rms_density"])
+                sizes=f"log-uniform {SERVED_LO}..{hi} atoms, alkane / water clusters",
+                rms_density=SERVED_SCF["rms_density"])
 
 
 def headline_latency(served, rate):

@@ -105,8 +105,15 @@ typedef int64_t nddo_job_id;
 typedef struct {
     size_t hbm_budget_bytes;  /* admission budget for resident job state; 0 = 90% of free HBM */
     int max_batch_jobs;       /* served batch capacity; 0 = 4096 */
-    int flags;                /* reserved, 0 */
+    int flags;                /* NDDO_ENGINE_* bits, 0 = defaults */
 } nddo_engine_opts;
+
+/* Eigensolver placement.  By default a molecule of up to 1200 AOs gets one
+ * workgroup (Householder + divide and conquer), which gives the most solves/s
+ * when many molecules share the GPU; larger ones use the device-wide block-Jacobi
+ * solver.  With this flag every molecule above 512 AOs uses the device-wide
+ * solver: lower latency when only a few large molecules are in flight. */
+#define NDDO_ENGINE_DEVICE_WIDE_EIGEN 1
 
 int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out);
 int nddo_engine_destroy(nddo_engine* e);

@@ -276,7 +276,7 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
 // Eigensolver classes: the one-workgroup solvers with the matrix in LDS, the
 // same in device memory (up to 512 AOs), and the device-wide block Jacobi for
 // larger molecules (kernels_xl.hip).
-int Batch::classify(size_t lds_max, int num_cus) {
+int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao) {
     list_lds.clear();
     list_glb.clear();
     list_xl.clear();
@@ -293,7 +293,7 @@ int Batch::classify(size_t lds_max, int num_cus) {
         if (n <= 512 && lds_max > 9216 && jacobi_lds_bytes(n, true) <= lds_max - 9216) {
             list_lds.push_back(j);
             maxn_lds = std::max(maxn_lds, n);
-        } else if (heig_supported(n)) {
+        } else if (heig_supported(n) && n < xl_min_nao) {
             list_glb.push_back(j);
             maxn_glb = std::max(maxn_glb, n);
         } else {
@@ -635,6 +635,7 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     HIPCHK(hipGetDeviceProperties(&prop, hip_device));
     e->lds_max = prop.sharedMemPerBlock;
     e->num_cus = prop.multiProcessorCount;
+    e->xl_min_nao = (e->opts.flags & NDDO_ENGINE_DEVICE_WIDE_EIGEN) ? 513 : 1201;
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     std::vector<ElemDev> el(3 * kNumElem);
     for (int t = 0; t < 3; ++t)
@@ -691,7 +692,7 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
     if (ok.empty()) return NDDO_OK;
     Batch b;
     b.build(descs, njobs, ok, nullptr);
-    b.classify(e->lds_max, e->num_cus);
+    b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     int rc = b.allocate(false);
     if (rc) return rc;
     rc = b.upload(e->stream, descs);
@@ -770,7 +771,7 @@ static int one_job(Engine* e, Batch& b, nddo_job_desc& d, int theory, int natoms
     if (s) return s;
     std::vector<int> which{0};
     b.build(&d, 1, which, nullptr);
-    b.classify(e->lds_max, e->num_cus);
+    b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     int rc = b.allocate(sdbg);
     if (rc) return rc;
     return b.upload(e->stream, &d);
@@ -872,7 +873,7 @@ static int matrix_job(Engine* e, Batch& b, int n, int nocc) {
     b.arena_size = (long long)ARENA_FIXED * n * n;
     b.src = {0};
     b.max_iter_all = 1;
-    b.classify(e->lds_max, e->num_cus);
+    b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     int rc = b.allocate(false);
     if (rc) return rc;
     HIPCHK(hipMemcpy(b.d_jobs, b.jobs.data(), sizeof(JobDev), hipMemcpyHostToDevice));
@@ -918,7 +919,7 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
     HIPCHK(hipMemcpy(&s, b.d_state, sizeof(s), hipMemcpyDeviceToHost));
     if (sweeps) *sweeps = s.sweeps;
     for (int i = 0; i < 4; ++i) e->last_eig_prof[i] = s.prof[i];
-    if (heig_supported(n))
+    if (b.list_xl.empty())
         HIPCHK(hipMemcpy(e->last_dc_phase, b.d_tri + 19LL * n + 224, 7 * sizeof(double), hipMemcpyDeviceToHost));
     else
         HIPCHK(hipMemcpy(e->last_dc_phase, b.d_xlws + xl_dbg_offset(n), 9 * sizeof(double), hipMemcpyDeviceToHost));

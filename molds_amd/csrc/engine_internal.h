@@ -77,7 +77,8 @@ struct Batch {
     int dalloc(void** p, size_t bytes);
     void release();
     int build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* unused);
-    int classify(size_t lds_max, int num_cus);
+    // xl_min_nao: smallest molecule of the device-wide eigensolver class
+    int classify(size_t lds_max, int num_cus, int xl_min_nao);
     void launch_xl_eigen_all(Engine* e, hipStream_t st);
     int allocate(bool want_sdbg);
     size_t upload_bytes() const;
@@ -97,6 +98,7 @@ struct Engine {
     ElemDev* d_elems = nullptr;
     size_t lds_max = 65536;
     int num_cus = 0;
+    int xl_min_nao = 1201;   // 513 with NDDO_ENGINE_DEVICE_WIDE_EIGEN
     size_t hbm_budget = 0;
     int* h_remaining = nullptr;
     std::mutex run_mu;   // one synchronous batch at a time

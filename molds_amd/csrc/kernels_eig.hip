@@ -64,6 +64,10 @@ size_t tridiag_lds_bytes(int n, bool a_in_lds) {
 
 size_t dc_lds_bytes(int n) { return (size_t)n * (12 * 8 + 9 * 4); }
 
+// Largest matrix of the one-workgroup path: the divide-and-conquer kernel keeps
+// 132 bytes of LDS per row (158 KB at 1200 AOs).
+constexpr int kHeigMaxN = 1200;
+
 // ------------------------------------------------------------------ 1. tridiagonalize
 // Reflector k acts on indices k+1..n-1: H_k = I - tau_k v v^T with v[0] = 1,
 // kept in row k of W (columns k+1..n-1); d, e, tau go to the job's workspace.
@@ -73,7 +77,9 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
                                                   double* __restrict__ tri) {
     extern __shared__ double dyn[];
     __shared__ double red[16], red2[16], sh_alpha;
-    __shared__ double part[1024];   // segment partials of A v (S * m2 <= blockDim.x)
+    // segment partials of A v (S * m2 <= max(blockDim.x, m2)); the LDS variant
+    // serves at most 137 AOs and keeps its static LDS small
+    __shared__ double part[ALDS ? 1024 : 2048];
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
@@ -184,9 +190,11 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
         // columns i = k+2 .. n-1 (A is kept symmetric, so column i is row i): rank-2
         // update of step k and the partial sums of (A v_{k+1})_i over S segments of j
         {
-            const int S = min(16, T / m2), L = (m2 + S - 1) / S;
-            const int seg = tid / m2, ci = tid - seg * m2;
-            if (seg < S) {
+            // S segments of j per column; past T columns every thread takes whole
+            // columns, several of them
+            const int S = max(1, min(16, T / m2)), L = (m2 + S - 1) / S;
+            for (int cc = tid; cc < S * m2; cc += T) {
+                const int seg = cc / m2, ci = cc - seg * m2;
                 const int i = k + 2 + ci, ti = ci + 1;
                 const double vi = vcur[ti], wi = p[ti] + K * vi;
                 const int jb = k + 2 + seg * L, je = min(jb + L, n);
@@ -751,40 +759,44 @@ __global__ void __launch_bounds__(64) k_wy_t(const JobDev* __restrict__ jobs, co
     for (int x = lane; x < kBt * kBt; x += 64) Tout[x] = Ts[x];
 }
 
-// One workgroup (8 wavefronts) per (job, 64 eigenvector rows).  The rows are
+// One workgroup (8 wavefronts) per (job, 16 RT eigenvector rows).  The rows are
 // held transposed, Z^T[j][r], as v_mfma_f64_16x16x4f64 accumulator tiles: wave
-// w owns the 16-column tiles tj = w, w + 8, ... for all four 16-row tiles.  In
+// w owns the 16-column tiles tj = w, w + 8, ... for all RT 16-row tiles.  In
 // that layout element v of a lane holds Z^T[16 tj + 4 v + lk][r0 + 16 rt + lr],
 // which is exactly the B operand of the k-step v of Y^T = V^T Z^T, so Z never
 // leaves registers.  Per block: Y^T = V^T Z^T (partials per wave, summed in
-// LDS), Y2^T = -T Y^T, Z^T += V Y2^T.
-template <int JT>
+// LDS), Y2^T = -T Y^T, Z^T += V Y2^T.  The block's 16 reflectors are staged in
+// LDS (VG = false, up to 512 AOs: RT = 4, 64 rows) or read from device memory
+// (VG = true, larger molecules: RT = 2, 32 rows, which keeps the row block in
+// registers and the LDS footprint small).
+template <int JT, int RT, bool VG>
 __global__ void __launch_bounds__(512) k_bt_wy(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
                                                const int* __restrict__ joblist, double* __restrict__ arena,
                                                double* __restrict__ tri) {
+    constexpr int R = 16 * RT;   // rows per workgroup
     extern __shared__ double dyn[];
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
-    const int n = jb.nao, nref = n - 2, r0 = blockIdx.y * 64;
+    const int n = jb.nao, nref = n - 2, r0 = blockIdx.y * R;
     if (nref <= 0 || r0 >= n) return;
     const unsigned long long t0 = wall_clock64();
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
     const int ld = n + 1;
     const int ntj = (n + 15) >> 4;
-    double* Vs = dyn;                 // 16 x ld: Vs[c][j] = V[j][c]
-    double* Yp = Vs + 16 * ld;        // 8 x 16 x 64 partial Y^T
-    double* Ys = Yp + 8 * 1024;       // 16 x 64: Y^T, then Y2^T
-    double* Ts = Ys + 1024;           // 16 x 16
+    double* Vs = dyn;                           // 16 x ld: Vs[c][j] = V[j][c] (VG = false)
+    double* Yp = VG ? dyn : Vs + 16 * ld;       // 8 x 16 x R partial Y^T
+    double* Ys = Yp + 8 * 16 * R;               // 16 x R: Y^T, then Y2^T
+    double* Ts = Ys + 16 * R;                   // 16 x 16
     double* ZT = amat(arena, jb, AR_C);
     const double* W = amat(arena, jb, AR_W);
     const double* Tall = eig_ws(tri, jb, job) + 3 * n;
-    dbl4 acc[JT][4];
+    dbl4 acc[JT][RT];
 #pragma unroll
     for (int x = 0; x < JT; ++x) {
         const int tj = wave + 8 * x;
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
+        for (int rt = 0; rt < RT; ++rt) {
             const int r = r0 + 16 * rt + lr;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -797,52 +809,63 @@ __global__ void __launch_bounds__(512) k_bt_wy(const JobDev* __restrict__ jobs, 
     for (int bi = nblk - 1; bi >= 0; --bi) {
         const int k0 = bi * kBt, nb = min(kBt, nref - k0);
         const int tlo = (k0 + 1) >> 4;   // first column tile touched by this block
-        const int jlo = tlo << 4, span = n - jlo;
-        for (int x = tid; x < 16 * span; x += 512) {
-            const int cc = x / span, j = jlo + (x - cc * span);
-            Vs[cc * ld + j] = (cc < nb && j > k0 + cc) ? W[(long long)(k0 + cc) * n + j] : 0.0;
+        // V[j][c] of the block: row k0 + c of W from column k0 + c + 1 on (v[0] = 1 is stored)
+        auto vget = [&](int c, int j) -> double {
+            if (VG) return (c < nb && j > k0 + c && j < n) ? W[(long long)(k0 + c) * n + j] : 0.0;
+            return j < n ? Vs[c * ld + j] : 0.0;
+        };
+        if (!VG) {
+            const int jlo = tlo << 4, span = n - jlo;
+            for (int x = tid; x < 16 * span; x += 512) {
+                const int cc = x / span, j = jlo + (x - cc * span);
+                Vs[cc * ld + j] = (cc < nb && j > k0 + cc) ? W[(long long)(k0 + cc) * n + j] : 0.0;
+            }
         }
         if (tid < kBt * kBt) Ts[tid] = Tall[(long long)bi * kBt * kBt + tid];
         __syncthreads();
         // partial Y^T[c][r] over this wave's column tiles
-        dbl4 y[4];
+        dbl4 y[RT];
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) y[rt] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int rt = 0; rt < RT; ++rt) y[rt] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int x = 0; x < JT; ++x) {
             const int tj = wave + 8 * x;
             if (tj < tlo || tj >= ntj) continue;
+            double a[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int j = 16 * tj + 4 * v + lk;
-                const double a = j < n ? Vs[lr * ld + j] : 0.0;
+            for (int v = 0; v < 4; ++v) a[v] = vget(lr, 16 * tj + 4 * v + lk);
 #pragma unroll
-                for (int rt = 0; rt < 4; ++rt) y[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[x][rt][v], y[rt], 0, 0, 0);
-            }
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt)
+                    y[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[v], acc[x][rt][v], y[rt], 0, 0, 0);
         }
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
+        for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Yp[wave * 1024 + (4 * v + lk) * 64 + 16 * rt + lr] = y[rt][v];
+            for (int v = 0; v < 4; ++v) Yp[wave * 16 * R + (4 * v + lk) * R + 16 * rt + lr] = y[rt][v];
         __syncthreads();
-        for (int x = tid; x < 1024; x += 512) {
-            double s = 0.0;
+        for (int x = tid; x < 16 * R; x += 512) {
+            double sum = 0.0;
 #pragma unroll
-            for (int w = 0; w < 8; ++w) s += Yp[w * 1024 + x];
-            Ys[x] = s;
+            for (int w = 0; w < 8; ++w) sum += Yp[w * 16 * R + x];
+            Ys[x] = sum;
         }
         __syncthreads();
-        double y2[2];
+        constexpr int NY = (16 * R + 511) / 512;
+        double y2[NY];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {   // Y2^T[c][r] = -sum_{c2 >= c} T[c][c2] Y^T[c2][r]
-            const int x = tid + 512 * u, cc = x >> 6, r = x & 63;
-            double s = 0.0;
-            for (int c2 = cc; c2 < kBt; ++c2) s += Ts[cc * kBt + c2] * Ys[c2 * 64 + r];
-            y2[u] = -s;
+        for (int u = 0; u < NY; ++u) {   // Y2^T[c][r] = -sum_{c2 >= c} T[c][c2] Y^T[c2][r]
+            const int x = tid + 512 * u, cc = x / R, r = x % R;
+            double sum = 0.0;
+            if (x < 16 * R)
+                for (int c2 = cc; c2 < kBt; ++c2) sum += Ts[cc * kBt + c2] * Ys[c2 * R + r];
+            y2[u] = -sum;
         }
         __syncthreads();
-        Ys[tid] = y2[0];
-        Ys[tid + 512] = y2[1];
+#pragma unroll
+        for (int u = 0; u < NY; ++u)
+            if (tid + 512 * u < 16 * R) Ys[tid + 512 * u] = y2[u];
         __syncthreads();
         // Z^T[j][r] += sum_c V[j][c] Y2^T[c][r]
 #pragma unroll
@@ -850,15 +873,16 @@ __global__ void __launch_bounds__(512) k_bt_wy(const JobDev* __restrict__ jobs, 
             const int tj = wave + 8 * x;
             if (tj < tlo || tj >= ntj) continue;
             const int j = 16 * tj + lr;
+            double a[4];
 #pragma unroll
-            for (int c0 = 0; c0 < kBt; c0 += 4) {
-                const double a = j < n ? Vs[(c0 + lk) * ld + j] : 0.0;
+            for (int q = 0; q < 4; ++q) a[q] = vget(4 * q + lk, j);
 #pragma unroll
-                for (int rt = 0; rt < 4; ++rt) {
-                    const double b = Ys[(c0 + lk) * 64 + 16 * rt + lr];
-                    acc[x][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[x][rt], 0, 0, 0);
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    const double b = Ys[(4 * q + lk) * R + 16 * rt + lr];
+                    acc[x][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b, acc[x][rt], 0, 0, 0);
                 }
-            }
         }
         __syncthreads();
     }
@@ -866,7 +890,7 @@ __global__ void __launch_bounds__(512) k_bt_wy(const JobDev* __restrict__ jobs, 
     for (int x = 0; x < JT; ++x) {
         const int tj = wave + 8 * x;
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
+        for (int rt = 0; rt < RT; ++rt) {
             const int r = r0 + 16 * rt + lr;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -878,18 +902,22 @@ __global__ void __launch_bounds__(512) k_bt_wy(const JobDev* __restrict__ jobs, 
     if (tid == 0 && blockIdx.y == 0) state[job].prof[3] = (double)(wall_clock64() - t0);
 }
 
-static size_t bt_lds_bytes(int n) { return ((size_t)16 * (n + 1) + 8 * 1024 + 1024 + 256) * 8; }
-
-template <int JT>
-static void launch_bt(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
-                      double* arena, double* tri) {
-    const size_t lb = bt_lds_bytes(max_n);
-    hipFuncSetAttribute((const void*)k_bt_wy<JT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
-    hipLaunchKernelGGL((k_bt_wy<JT>), dim3(nlist, (max_n + 63) / 64), dim3(512), lb, st, jobs, state, joblist, arena,
-                       tri);
+// ------------------------------------------------------------------ host side
+static size_t bt_lds_bytes(int n, int rt, bool vg) {
+    const size_t r = 16 * (size_t)rt;
+    return ((vg ? 0 : (size_t)16 * (n + 1)) + 8 * 16 * r + 16 * r + 256) * 8;
 }
 
-bool heig_supported(int n) { return n <= 512; }
+template <int JT, int RT, bool VG>
+static void launch_bt(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
+                      double* arena, double* tri) {
+    const size_t lb = bt_lds_bytes(max_n, RT, VG);
+    hipFuncSetAttribute((const void*)k_bt_wy<JT, RT, VG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+    hipLaunchKernelGGL((k_bt_wy<JT, RT, VG>), dim3(nlist, (max_n + 16 * RT - 1) / (16 * RT)), dim3(512), lb, st, jobs,
+                       state, joblist, arena, tri);
+}
+
+bool heig_supported(int n) { return n <= kHeigMaxN; }
 
 size_t heig_lds_bytes(int n, bool a_in_lds) { return tridiag_lds_bytes(n, a_in_lds); }
 
@@ -914,10 +942,13 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
     } else if (max_n >= 3) {
         const int nblk = (max_n - 2 + kBt - 1) / kBt;
         hipLaunchKernelGGL(k_wy_t, dim3(nlist, nblk), dim3(64), 0, st, jobs, state, joblist, arena, tri);
-        if (max_n <= 128) launch_bt<1>(st, jobs, state, joblist, nlist, max_n, arena, tri);
-        else if (max_n <= 256) launch_bt<2>(st, jobs, state, joblist, nlist, max_n, arena, tri);
-        else if (max_n <= 384) launch_bt<3>(st, jobs, state, joblist, nlist, max_n, arena, tri);
-        else launch_bt<4>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        if (max_n <= 128) launch_bt<1, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else if (max_n <= 256) launch_bt<2, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else if (max_n <= 384) launch_bt<3, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else if (max_n <= 512) launch_bt<4, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else if (max_n <= 768) launch_bt<6, 2, true>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else if (max_n <= 1024) launch_bt<8, 2, true>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        else launch_bt<10, 1, true>(st, jobs, state, joblist, nlist, max_n, arena, tri);
     }
 }
 

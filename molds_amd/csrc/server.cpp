@@ -98,9 +98,16 @@ namespace {
 double secs_since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
 
 constexpr int kMaxCohorts = 16;
-constexpr int kSizeClasses = 4;   // AO count <= 64, <= 137 (eigensolver matrix in LDS), <= 256, larger
+// AO count <= 64, <= 137 (eigensolver matrix in LDS), <= 256, <= 512, up to
+// the device-wide eigensolver's threshold (one workgroup per molecule), larger
+// (device-wide block Jacobi, whose launches of different cohorts run one after
+// another; at most xl_max_jobs() molecules per cohort)
+constexpr int kSizeClasses = 6;
 
-int size_class(int nao) { return nao <= 64 ? 0 : (nao <= 137 ? 1 : (nao <= 256 ? 2 : 3)); }
+int size_class(int nao, int xl_min_nao) {
+    if (nao >= xl_min_nao) return 5;
+    return nao <= 64 ? 0 : (nao <= 137 ? 1 : (nao <= 256 ? 2 : (nao <= 512 ? 3 : 4)));
+}
 
 int admit_limit(const Engine* e) { return e->opts.max_batch_jobs > 0 ? e->opts.max_batch_jobs : 4096; }
 
@@ -203,7 +210,7 @@ int start_cohort(Engine* e, Cohort* c, Staging& stage) {
     c->descs.resize(c->members.size());
     for (size_t i = 0; i < c->members.size(); ++i) c->descs[i] = c->members[i]->desc;
     c->b.build(c->descs.data(), (int)c->descs.size(), which, nullptr);
-    c->b.classify(e->lds_max, e->num_cus);
+    c->b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     int rc = stage.ensure(c->b.upload_bytes(), st);
     if (rc) return rc;
     rc = c->b.allocate(false);
@@ -301,14 +308,15 @@ void Engine::start_server() {
                 size_t nb = 0, admitted = 0;
                 for (auto it = queue.begin(); it != queue.end();) {
                     ServedJob* j = *it;
-                    const int k = size_class(j->res.nao);
+                    const int k = size_class(j->res.nao, xl_min_nao);
                     if (!bucket[k]) {
                         if (nb >= free_slots.size()) { ++it; continue; }
                         bucket[k].reset(new Cohort());
                         ++nb;
                     }
                     Cohort* c = bucket[k].get();
-                    if ((int)c->members.size() >= cap || resident + admitted + j->footprint > hbm_budget) {
+                    const int kcap = k == kSizeClasses - 1 ? std::min(cap, xl_max_jobs()) : cap;
+                    if ((int)c->members.size() >= kcap || resident + admitted + j->footprint > hbm_budget) {
                         ++it;
                         continue;
                     }

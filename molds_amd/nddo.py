@@ -145,10 +145,12 @@ class ScfResult(dict):
 class Engine:
     """One engine per HIP device (one process per GPU)."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_jobs=0):
+    DEVICE_WIDE_EIGEN = 1   # NDDO_ENGINE_DEVICE_WIDE_EIGEN
+
+    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_jobs=0, flags=0):
         L = load_library()
         h = ctypes.c_void_p()
-        o = EngineOpts(hbm_budget_bytes, max_batch_jobs, 0)
+        o = EngineOpts(hbm_budget_bytes, max_batch_jobs, flags)
         rc = L.nddo_engine_create(device, ctypes.byref(o), ctypes.byref(h))
         if rc != 0:
             raise NddoError(rc, last_error())

@@ -46,3 +46,13 @@ def engine():
     e = Engine(0)
     yield e
     e.close()
+
+
+@pytest.fixture(scope="session")
+def engine_dw():
+    """An engine that solves every molecule above 512 AOs with the device-wide
+    block-Jacobi eigensolver (NDDO_ENGINE_DEVICE_WIDE_EIGEN)."""
+    from molds_amd.nddo import Engine
+    e = Engine(0, flags=Engine.DEVICE_WIDE_EIGEN)
+    yield e
+    e.close()

@@ -1,10 +1,12 @@
-"""Molecules above 512 AOs: the device-wide block-Jacobi eigensolver and the
-chunked DIIS / energy reductions (kernels_xl.hip).
+"""Molecules above 512 AOs: the one-workgroup Householder + divide-and-conquer
+solver (up to 1200 AOs), the device-wide block-Jacobi eigensolver (above 1200,
+or above 512 with NDDO_ENGINE_DEVICE_WIDE_EIGEN), and the chunked DIIS / energy
+reductions of the device-wide class (kernels_eig.hip, kernels_xl.hip).
 
-Tolerances are those of the one-workgroup paths: eigenvalues within 1e-13
-||A||_F of LAPACK, residual and orthonormality 1e-12; FP64 SCF within 1e-6 eV
-total energy and 1e-8 density RMS of the oracle; mixed precision within 1e-5 eV
-of FP64.
+Tolerances are those of the smaller sizes: eigenvalues within 1e-13 ||A||_F
+of LAPACK, residual and orthonormality 1e-12; FP64 SCF within 1e-6 eV total
+energy and 1e-8 density RMS of the oracle; mixed precision within 1e-5 eV of
+FP64.
 """
 import time
 
@@ -18,20 +20,11 @@ SCF = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_e
            damping_weight=0.0, max_iter=200)
 
 
-# 513: one row past the one-workgroup solvers; 600 / 1000: ragged last block
-# and an odd block count (a padding block in the round-robin); 1536: the
-# (H2O)_256 size
-@pytest.mark.parametrize("n", [513, 600, 1000, 1536])
-def test_eigh_large(engine, n):
-    print(f"eigh large n={n}", flush=True)
-    rng = np.random.default_rng(n)
-    A = rng.standard_normal((n, n))
-    A = A + A.T
-    A[0, :] = A[:, 0] = 0.0
-    A[0, 0] = A[1, 1]          # an exact degeneracy and a decoupled row
+def _check_eigh(e, A, label):
+    n = A.shape[0]
     t0 = time.perf_counter()
-    eps, C, sweeps = engine.dbg_eigh(A)
-    print(f"  {sweeps} sweeps, {time.perf_counter() - t0:.3f} s with transfers", flush=True)
+    eps, C, sweeps = e.dbg_eigh(A)
+    print(f"  {label} n={n}: {sweeps} sweeps, {time.perf_counter() - t0:.3f} s with transfers", flush=True)
     w = np.linalg.eigvalsh(A)
     nrm = np.linalg.norm(A)
     assert sweeps > 0
@@ -42,7 +35,34 @@ def test_eigh_large(engine, n):
     assert np.all(C.sum(axis=1) >= 0)
 
 
-def test_eigh_large_clustered(engine):
+def _random_sym(n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    A[0, :] = A[:, 0] = 0.0
+    A[0, 0] = A[1, 1]          # an exact degeneracy and a decoupled row
+    return A
+
+
+# one workgroup: 513 (past the 512-AO kernels), 777 / 1000 (back-transform
+# variants, trailing block wider than the workgroup past 1025), 1200 (the
+# largest); device-wide: 1536 (the (H2O)_256 size)
+@pytest.mark.parametrize("n", [513, 777, 1000, 1100, 1200, 1536])
+def test_eigh_large(engine, n):
+    print(f"eigh large n={n}", flush=True)
+    _check_eigh(engine, _random_sym(n), "default")
+
+
+# the device-wide solver at sizes the one-workgroup path also covers: ragged
+# last block, an odd block count (a padding block in the round-robin)
+@pytest.mark.parametrize("n", [513, 600, 1000])
+def test_eigh_device_wide(engine_dw, n):
+    print(f"eigh device-wide n={n}", flush=True)
+    _check_eigh(engine_dw, _random_sym(n), "device-wide")
+
+
+@pytest.mark.parametrize("which", ["default", "device-wide"])
+def test_eigh_large_clustered(engine, engine_dw, which):
     """Tight eigenvalue clusters (like the orbital bands of a water cluster)."""
     n = 700
     rng = np.random.default_rng(7)
@@ -50,12 +70,7 @@ def test_eigh_large_clustered(engine):
     lam = np.repeat(rng.standard_normal(70), 10) + 1e-10 * rng.standard_normal(n)
     A = (Q * lam) @ Q.T
     A = 0.5 * (A + A.T)
-    eps, C, sweeps = engine.dbg_eigh(A)
-    nrm = np.linalg.norm(A)
-    assert sweeps > 0
-    assert np.max(np.abs(eps - np.linalg.eigvalsh(A))) <= 1e-13 * nrm
-    assert np.max(np.abs(C @ A - eps[:, None] * C)) <= 1e-12 * nrm
-    assert np.max(np.abs(C @ C.T - np.eye(n))) <= 1e-12
+    _check_eigh(engine if which == "default" else engine_dw, A, which)
 
 
 def _water(m, seed):
@@ -64,15 +79,17 @@ def _water(m, seed):
     return Z, W.to_bohr(x)
 
 
-def test_scf_large_fp64_and_mixed_vs_oracle(engine):
+@pytest.mark.parametrize("which", ["default", "device-wide"])
+def test_scf_large_fp64_and_mixed_vs_oracle(engine, engine_dw, which):
     """(H2O)_90, 540 AOs, PM3: FP64 against the oracle, mixed against both."""
-    print("scf large (H2O)_90", flush=True)
+    print(f"scf large (H2O)_90 {which}", flush=True)
     from oracle import scf as O
+    e = engine if which == "default" else engine_dw
     Z, xyz = _water(90, 11)
     t0 = time.perf_counter()
-    r64 = engine.scf("pm3", Z, xyz, precision="fp64", **SCF)
+    r64 = e.scf("pm3", Z, xyz, precision="fp64", **SCF)
     t1 = time.perf_counter()
-    rmx = engine.scf("pm3", Z, xyz, precision="mixed", **SCF)
+    rmx = e.scf("pm3", Z, xyz, precision="mixed", **SCF)
     t2 = time.perf_counter()
     print(f"  fp64 {t1 - t0:.2f} s ({r64['iterations']} it), mixed {t2 - t1:.2f} s ({rmx['iterations']} it)", flush=True)
     o = O.scf("pm3", Z, xyz, rms_density=1e-8, diis_k=5, diis_start=0.1, diis_end=0.0, damping_weight=0.0,
@@ -86,20 +103,23 @@ def test_scf_large_fp64_and_mixed_vs_oracle(engine):
     assert 1 <= rmx["fp64_iterations"] < rmx["iterations"]
 
 
-def test_large_alone_equals_batch(engine):
-    """A large molecule gives the same result alone and batched with small ones
-    (its chunked reductions and block-Jacobi launch share the batch's stages)."""
+@pytest.mark.parametrize("which", ["default", "device-wide"])
+def test_large_alone_equals_batch(engine, engine_dw, which):
+    """Large molecules give the same result alone and batched together with
+    small ones (with the device-wide solver: several in one persistent launch,
+    of different sizes and iteration counts)."""
     from molds_amd import workloads as W
     from molds_amd.nddo import Job
-    Z, xyz = _water(95, 12)
-    big = Job("pm3", Z, xyz, rms_density=1e-8)
+    e = engine if which == "default" else engine_dw
+    Zc, xc = W.alkane(90, seed=33)
+    big = [Job("pm3", *_water(95, 12), rms_density=1e-8), Job("pm3", Zc, W.to_bohr(xc), rms_density=1e-8),
+           Job("am1", *_water(88, 13), rms_density=1e-8)]
     small = [Job("pm3", *(lambda t: (t[0], W.to_bohr(t[1])))(W.alkane(6, seed=31)), rms_density=1e-8),
              Job("am1", *_water(5, 32), rms_density=1e-8)]
-    alone = engine.scf_batch([big])[0]
-    mix = engine.scf_batch([small[0], big, small[1]])
-    assert alone["status"] == 0 and all(r["status"] == 0 for r in mix)
-    assert abs(alone["energy"] - mix[1]["energy"]) * EV <= 1e-9
-    assert alone["iterations"] == mix[1]["iterations"]
-    for j, r in zip(small, (mix[0], mix[2])):
-        s = engine.scf_batch([j])[0]
-        assert abs(s["energy"] - r["energy"]) * EV <= 1e-9
+    mix = e.scf_batch([small[0], big[0], big[1], small[1], big[2]])
+    assert all(r["status"] == 0 for r in mix)
+    for j, r in zip(small + big, (mix[0], mix[3], mix[1], mix[2], mix[4])):
+        alone = e.scf_batch([j])[0]
+        assert alone["status"] == 0
+        assert abs(alone["energy"] - r["energy"]) * EV <= 1e-9
+        assert alone["iterations"] == r["iterations"]
