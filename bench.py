@@ -14,10 +14,12 @@ at the start of each step and every integral is rebuilt on the GPU inside the
 timed region.
 
 Beside it, in the same run:
-  * served legs: Poisson arrivals swept over 50..2000 requests/s (2 s of
-    arrivals each, sizes log-uniform on 5..256 atoms over the alkane / water
-    generators, retired at density RMS < 1e-7) through the continuous-batching
-    request loop, mixed and FP64, with p50/p99 latency (submit -> retire);
+  * served legs: Poisson arrivals at 50..2000 requests/s (2 s of arrivals,
+    at most 400 requests per leg, sizes log-uniform on 5..600 atoms over the
+    alkane / water generators, retired at density RMS < 1e-7) through the
+    continuous-batching request loop, mixed and FP64, with p50/p99 latency
+    (submit -> retire);
+  * (H2O)_256 AM1 (1536 AOs), one molecule, FP64 and mixed precision;
   * the plain FP64 GPU path: the same 4096-molecule mix solved one molecule at
     a time (nddo_scf_once), on a bounded prefix;
   * the CPU oracle on a bounded prefix of the same mix (cpu_baseline).
@@ -116,6 +118,30 @@ def served_leg(engine, rate, count, precision, seed=3, hi=SERVED_HI):
                 rms_density=SERVED_SCF["rms_density"])
 
 
+def large_leg(engine):
+    """(H2O)_256, AM1, 1536 AOs, one molecule: FP64 and mixed precision, seconds
+    per solve and the mixed-vs-FP64 differences."""
+    from molds_amd import workloads as W
+    Z, x = W.water_cluster(256, seed=1, jitter=0.0)
+    xyz = W.to_bohr(x)
+    opts = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0,
+                damping_weight=0.0, max_iter=200)
+    out = {"workload": "(H2O)_256 AM1, 1536 AOs, cube at 1 g/cm3, density RMS < 1e-8"}
+    res = {}
+    for prec in ("fp64", "mixed"):
+        engine.scf("am1", Z, xyz, precision=prec, want_C=False, want_P=False, **opts)   # warm-up
+        t0 = time.perf_counter()
+        r = engine.scf("am1", Z, xyz, precision=prec, want_C=False, **opts)
+        res[prec] = r
+        out[prec] = dict(seconds=round(time.perf_counter() - t0, 4), status=r["status"], iterations=r["iterations"],
+                         fp64_iterations=r["fp64_iterations"], energy_hartree=r["energy"])
+    a, b = res["fp64"], res["mixed"]
+    out["mixed_vs_fp64"] = dict(abs_dE_eV=float(abs(a["energy"] - b["energy"]) * EV),
+                                max_abs_deps_eV=float(np.max(np.abs(a["eps"] - b["eps"])) * EV),
+                                density_rms=float(np.sqrt(np.mean((a["P"] - b["P"]) ** 2))))
+    return out
+
+
 def headline_latency(served, rate):
     """p50/p99 (submit -> retire) of the mixed-precision served leg at `rate`."""
     leg = min(served["mixed"], key=lambda g: abs(g["rate_per_s"] - rate))
@@ -170,8 +196,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--modes", default="fp64,mixed")
-    ap.add_argument("--served-rates", default="50,200,500,1000,2000")
+    ap.add_argument("--served-rates", default="50,500,2000", help="mixed-precision served legs (requests/s)")
+    ap.add_argument("--served-rates-fp64", default="500", help="FP64 served legs (requests/s)")
     ap.add_argument("--served-seconds", type=float, default=2.0, help="seconds of arrivals per served leg")
+    ap.add_argument("--served-max", type=int, default=400, help="at most this many requests per served leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the (H2O)_256 single-molecule leg")
     ap.add_argument("--latency-rate", type=float, default=500.0, help="served rate whose p50/p99 is the headline")
     ap.add_argument("--plain-seconds", type=float, default=8.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -230,13 +259,17 @@ def main():
 
     served = {}
     if not args.no_served:
-        for prec in ("mixed", "fp64"):
+        for prec, rates in (("mixed", args.served_rates), ("fp64", args.served_rates_fp64)):
             served[prec] = []
-            for rate in [float(r) for r in args.served_rates.split(",")]:
-                count = max(50, int(round(rate * args.served_seconds)))
+            for rate in [float(r) for r in rates.split(",") if r]:
+                count = min(args.served_max, max(50, int(round(rate * args.served_seconds))))
                 leg = served_leg(engine, rate, count, prec)
                 served[prec].append(leg)
                 log(f"served {prec} {rate:g}/s: {leg}")
+    large = None
+    if not args.no_large:
+        large = large_leg(engine)
+        log(f"large: {large}")
     plain = None
     if not args.no_plain:
         plain = plain_leg(engine, make_jobs(mols, "fp64"), args.plain_seconds)
@@ -267,6 +300,7 @@ def main():
                   for m, lg in legs.items()},
         "latency_s": (headline_latency(served, args.latency_rate) if served else None),
         "served": served or None,
+        "large_single": large,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain_fp64": ({"batched": round(best["solves_per_s"] / plain["solves_per_s"], 2)}
                                   if plain else None),
