@@ -114,6 +114,10 @@ typedef struct {
  * solver.  With this flag every molecule above 512 AOs uses the device-wide
  * solver: lower latency when only a few large molecules are in flight. */
 #define NDDO_ENGINE_DEVICE_WIDE_EIGEN 1
+/* Launch every SCF iteration stage by stage instead of replaying the
+ * iteration's hipGraph (captured once per batch or served cohort after its
+ * first iteration; batches with device-wide eigensolves always launch eagerly). */
+#define NDDO_ENGINE_NO_GRAPHS 2
 
 int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out);
 int nddo_engine_destroy(nddo_engine* e);
@@ -179,6 +183,8 @@ int nddo_dbg_eigh_profile(nddo_engine* e, double* ticks4, double* tick_hz);
  * summed over the merge levels: merge order + z, deflation, rotations + secular
  * roots, Loewner z + ranks, column norms, merge matrix U, merge products. */
 int nddo_dbg_eigh_dc_phases(nddo_engine* e, double* ticks7);
+/* Test hook: iteration graphs captured and replayed so far by this engine. */
+int nddo_dbg_graph_stats(nddo_engine* e, long long* captured, long long* replays);
 
 #ifdef __cplusplus
 }

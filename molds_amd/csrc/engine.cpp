@@ -118,6 +118,10 @@ size_t jacobi_lds_bytes(int n, bool a_in_lds) {
 Batch::~Batch() { release(); }
 
 void Batch::release() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    iterations_launched = 0;
+    graph_off = false;
     for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
                      (void**)&d_ints, (void**)&d_ints32, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
                      (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
@@ -439,6 +443,36 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
     launch_xl_apply(st, d_jobs, d_list_xl, nxl, d_arena, d_xlcoef, xl_chunks);
 }
 
+void Batch::iterate(Engine* e, hipStream_t st) {
+    const bool eager = !e->use_graphs || !list_xl.empty() || graph_off || iterations_launched == 0;
+    ++iterations_launched;
+    if (eager) {
+        launch_iteration(e, st);
+        return;
+    }
+    if (!gexec) {
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            launch_iteration(e, st);
+            ok = hipStreamEndCapture(st, &g) == hipSuccess && g;
+        }
+        if (ok) ok = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) hipGraphDestroy(g);
+        if (ok) e->graphs_captured++;
+        if (!ok) {   // capture unavailable: this batch launches eagerly from now on
+            if (gexec) hipGraphExecDestroy(gexec);
+            gexec = nullptr;
+            graph_off = true;
+            hipGetLastError();
+            launch_iteration(e, st);
+            return;
+        }
+    }
+    hipGraphLaunch(gexec, st);
+    e->graph_replays++;
+}
+
 // The large-molecule eigensolves, at most xl_max_jobs() per persistent launch,
 // ordered after the previous such launch on any stream of this engine.
 void Batch::launch_xl_eigen_all(Engine* e, hipStream_t st) {
@@ -468,7 +502,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
     int* h_rem = e->h_remaining;
     for (int it = 0; it <= max_iter_all + 1; ++it) {
         if (!prof) {
-            launch_iteration(e, st);
+            iterate(e, st);
         } else {
             // the same launches as launch_iteration, with an event between stages
             // and between the parts of the eigensolver
@@ -636,6 +670,7 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     e->lds_max = prop.sharedMemPerBlock;
     e->num_cus = prop.multiProcessorCount;
     e->xl_min_nao = (e->opts.flags & NDDO_ENGINE_DEVICE_WIDE_EIGEN) ? 513 : 1201;
+    e->use_graphs = !(e->opts.flags & NDDO_ENGINE_NO_GRAPHS);
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     std::vector<ElemDev> el(3 * kNumElem);
     for (int t = 0; t < 3; ++t)
@@ -945,6 +980,14 @@ int nddo_dbg_eigh_profile(nddo_engine* eh, double* ticks4, double* tick_hz) {
         hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device);
         *tick_hz = 1e3 * khz;
     }
+    return NDDO_OK;
+}
+
+int nddo_dbg_graph_stats(nddo_engine* eh, long long* captured, long long* replays) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    if (captured) *captured = e->graphs_captured.load();
+    if (replays) *replays = e->graph_replays.load();
     return NDDO_OK;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -85,6 +86,13 @@ struct Batch {
     int upload(hipStream_t st, const nddo_job_desc* descs, char* stage = nullptr);
     void launch_setup_stage(Engine* e, hipStream_t st);
     void launch_iteration(Engine* e, hipStream_t st);
+    // One SCF iteration: the first eagerly, the next ones as replays of a
+    // hipGraph captured from the second (unless graphs are off or the batch
+    // has device-wide eigensolves, whose launches are chained across streams).
+    void iterate(Engine* e, hipStream_t st);
+    hipGraphExec_t gexec = nullptr;
+    int iterations_launched = 0;
+    bool graph_off = false;
     int run_scf(Engine* e, hipStream_t st);
     int fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* res, double hof_offset);
 };
@@ -99,6 +107,8 @@ struct Engine {
     size_t lds_max = 65536;
     int num_cus = 0;
     int xl_min_nao = 1201;   // 513 with NDDO_ENGINE_DEVICE_WIDE_EIGEN
+    bool use_graphs = true;  // false with NDDO_ENGINE_NO_GRAPHS
+    std::atomic<long long> graphs_captured{0}, graph_replays{0};
     size_t hbm_budget = 0;
     int* h_remaining = nullptr;
     std::mutex run_mu;   // one synchronous batch at a time

@@ -351,7 +351,7 @@ void Engine::start_server() {
             // 2. the next SCF iteration of every idle cohort, then its jobs' scalar state
             for (auto& c : cohorts) {
                 if (c->inflight || c->live <= 0) continue;
-                c->b.launch_iteration(this, c->st);
+                c->b.iterate(this, c->st);
                 hipMemcpy2DAsync(c->h_state, kScalarBytes, c->b.d_state, sizeof(JobState), kScalarBytes,
                                  c->members.size(), hipMemcpyDeviceToHost, c->st);
                 hipEventRecord(c->ev, c->st);

@@ -87,6 +87,7 @@ def load_library(path=LIBPATH):
     L.nddo_dbg_density.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.nddo_dbg_eigh_profile.argtypes = [vp, _dp, _dp]
     L.nddo_dbg_eigh_dc_phases.argtypes = [vp, _dp]
+    L.nddo_dbg_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     _lib = L
     return L
 
@@ -146,6 +147,7 @@ class Engine:
     """One engine per HIP device (one process per GPU)."""
 
     DEVICE_WIDE_EIGEN = 1   # NDDO_ENGINE_DEVICE_WIDE_EIGEN
+    NO_GRAPHS = 2           # NDDO_ENGINE_NO_GRAPHS
 
     def __init__(self, device=0, hbm_budget_bytes=0, max_batch_jobs=0, flags=0):
         L = load_library()
@@ -318,6 +320,11 @@ class Engine:
         hz = ctypes.c_double()
         self.L.nddo_dbg_eigh_profile(self.h, _p(t), ctypes.byref(hz))
         return dict(zip(("tridiag", "divide_conquer", "unused", "backtransform"), (t / hz.value * 1e3).tolist()))
+
+    def graph_stats(self):
+        c, r = ctypes.c_longlong(), ctypes.c_longlong()
+        self.L.nddo_dbg_graph_stats(self.h, ctypes.byref(c), ctypes.byref(r))
+        return {"captured": c.value, "replays": r.value}
 
     def dbg_eigh_dc_phases(self):
         t = np.zeros(7)
