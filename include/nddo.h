@@ -118,6 +118,9 @@ typedef struct {
  * iteration's hipGraph (captured once per batch or served cohort after its
  * first iteration; batches with device-wide eigensolves always launch eagerly). */
 #define NDDO_ENGINE_NO_GRAPHS 2
+/* Keep the default launch shapes instead of timing the candidates at engine
+ * creation (see nddo_engine_tuning). */
+#define NDDO_ENGINE_NO_AUTOTUNE 4
 
 int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out);
 int nddo_engine_destroy(nddo_engine* e);
@@ -184,6 +187,13 @@ int nddo_dbg_eigh_profile(nddo_engine* e, double* ticks4, double* tick_hz);
  * roots, Loewner z + ranks, column norms, merge matrix U, merge products. */
 int nddo_dbg_eigh_dc_phases(nddo_engine* e, double* ticks7);
 /* Test hook: iteration graphs captured and replayed so far by this engine. */
+/* The launch shapes the engine's autotuner picked for its device (timed on
+ * synthetic work at engine creation): atom pairs per workgroup of the pair
+ * Fock kernel (1, 2, 4 or 8) and the back-transformation rows per workgroup
+ * (32 or 64) for n <= 128 / 256 / 384 / 512.  No counterpart in MolDS. */
+int nddo_engine_tuning(nddo_engine* e, int* fock_wpb, int* bt_rows4);
+/* Test hook: force the launch shapes. */
+int nddo_dbg_set_tuning(nddo_engine* e, int fock_wpb, const int* bt_rows4);
 int nddo_dbg_graph_stats(nddo_engine* e, long long* captured, long long* replays);
 
 #ifdef __cplusplus

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cstdlib>
 #include <chrono>
 #include <cmath>
@@ -648,6 +649,8 @@ void nddo_scf_opts_default(nddo_scf_opts* o) {
     o->diis_end_error = 1.0e-8;
 }
 
+static int autotune(Engine* e);
+
 int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out) {
     if (!out) { set_error("null engine pointer"); return NDDO_BAD_ARG; }
     *out = nullptr;
@@ -689,6 +692,10 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     size_t freeb = 0, totb = 0;
     HIPCHK(hipMemGetInfo(&freeb, &totb));
     e->hbm_budget = e->opts.hbm_budget_bytes ? e->opts.hbm_budget_bytes : (size_t)(0.9 * freeb);
+    if (!(e->opts.flags & NDDO_ENGINE_NO_AUTOTUNE) && autotune(e) != NDDO_OK) {
+        set_tuning(Tuning());   // keep the defaults when the tuning runs fail
+        hipGetLastError();
+    }
     e->start_server();
     *out = reinterpret_cast<nddo_engine*>(e);
     return NDDO_OK;
@@ -891,40 +898,199 @@ int nddo_dbg_fock(nddo_engine* eh, int theory, int natoms, const int* Z, const d
 }
 
 // A pseudo job of n AOs and no atoms, used by the eigensolver and density hooks.
-static int matrix_job(Engine* e, Batch& b, int n, int nocc) {
+// K jobs of one n x n matrix each (the stage-level test hooks and the autotuner).
+static int matrix_batch(Engine* e, Batch& b, int n, int nocc, int K) {
     b.atoms.clear(); b.pairs.clear(); b.jobs.clear(); b.tiles.clear();
-    JobDev jb;
-    std::memset(&jb, 0, sizeof(jb));
-    jb.nao = n;
-    jb.nocc = nocc;
-    jb.max_iter = 1;
-    jb.rms_thresh = 0.0;
-    jb.tile0 = 0;
-    for (int i0 = 0; i0 < n; i0 += 64)
-        for (int j0 = 0; j0 < n; j0 += 64) b.tiles.push_back({0, i0, j0, 0});
-    jb.ntiles = (int)b.tiles.size();
-    b.jobs.push_back(jb);
-    b.nslots = 0; b.nints = 0; b.nvec = n; b.ntraj = 4;
-    b.arena_size = (long long)ARENA_FIXED * n * n;
-    b.src = {0};
+    for (int k = 0; k < K; ++k) {
+        JobDev jb;
+        std::memset(&jb, 0, sizeof(jb));
+        jb.nao = n;
+        jb.nocc = nocc;
+        jb.max_iter = 1;
+        jb.rms_thresh = 0.0;
+        jb.arena = (long long)k * ARENA_FIXED * n * n;
+        jb.vec0 = (long long)k * n;
+        jb.ivec0 = (long long)k * n;
+        jb.traj0 = 4LL * k;
+        jb.tile0 = (int)b.tiles.size();
+        for (int i0 = 0; i0 < n; i0 += 64)
+            for (int j0 = 0; j0 < n; j0 += 64) b.tiles.push_back({k, i0, j0, 0});
+        jb.ntiles = (int)b.tiles.size() - jb.tile0;
+        b.jobs.push_back(jb);
+    }
+    b.nslots = 0; b.nints = 0; b.nvec = (long long)K * n; b.ntraj = 4LL * K;
+    b.arena_size = (long long)K * ARENA_FIXED * n * n;
+    b.src.assign(K, 0);
     b.max_iter_all = 1;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     int rc = b.allocate(false);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(b.d_jobs, b.jobs.data(), sizeof(JobDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.d_jobs, b.jobs.data(), K * sizeof(JobDev), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.d_tiles, b.tiles.data(), b.tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice));
-    if (!b.list_lds.empty()) HIPCHK(hipMemcpy(b.d_list_lds, b.list_lds.data(), sizeof(int), hipMemcpyHostToDevice));
-    if (!b.list_glb.empty()) HIPCHK(hipMemcpy(b.d_list_glb, b.list_glb.data(), sizeof(int), hipMemcpyHostToDevice));
+    if (!b.list_lds.empty())
+        HIPCHK(hipMemcpy(b.d_list_lds, b.list_lds.data(), b.list_lds.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!b.list_glb.empty())
+        HIPCHK(hipMemcpy(b.d_list_glb, b.list_glb.data(), b.list_glb.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!b.list_xl.empty()) {
-        HIPCHK(hipMemcpy(b.d_list_xl, b.list_xl.data(), sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b.d_xl_off, b.xl_off.data(), sizeof(long long), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b.d_jobs, b.jobs.data(), sizeof(JobDev), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.d_list_xl, b.list_xl.data(), b.list_xl.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.d_xl_off, b.xl_off.data(), b.xl_off.size() * sizeof(long long), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemset(b.d_arena, 0, b.arena_size * sizeof(double)));
-    JobState s;
-    std::memset(&s, 0, sizeof(s));
-    s.phase = PH_ACTIVE;
-    HIPCHK(hipMemcpy(b.d_state, &s, sizeof(s), hipMemcpyHostToDevice));
+    std::vector<JobState> s(K);
+    std::memset(s.data(), 0, K * sizeof(JobState));
+    for (auto& x : s) x.phase = PH_ACTIVE;
+    HIPCHK(hipMemcpy(b.d_state, s.data(), K * sizeof(JobState), hipMemcpyHostToDevice));
+    return NDDO_OK;
+}
+
+static int matrix_job(Engine* e, Batch& b, int n, int nocc) { return matrix_batch(e, b, n, nocc, 1); }
+
+// ------------------------------------------------------------------ autotuner
+// Picks the launch shapes of Tuning (launch.h) when the engine is created, by
+// timing every candidate with hipEvents on synthetic work: the workgroup size
+// of the pair Fock kernel on water lattices (all three atom-pair classes, a
+// non-guess iteration), and the row block of the back-transformation on
+// random symmetric matrices of each size class.  Every candidate computes the
+// same numbers; the choice only moves time.
+static int time_ms(Engine* e, const std::function<void()>& f, int reps, float* best) {
+    *best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+        HIPCHK(hipEventRecord(e->ev[14], e->stream));
+        f();
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e->ev[15], e->stream));
+        HIPCHK(hipEventSynchronize(e->ev[15]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[14], e->ev[15]));
+        *best = std::min(*best, ms);
+    }
+    return NDDO_OK;
+}
+
+static int autotune(Engine* e) {
+    Tuning t;
+    set_tuning(t);
+    const double ang = 1.0 / 0.52917721092;
+    {   // pair Fock: 48 cubic water lattices of 8, 27 and 64 molecules (2.9 A apart)
+        std::vector<std::vector<int>> Z(48);
+        std::vector<std::vector<double>> X(48);
+        std::vector<nddo_job_desc> d(48);
+        for (int k = 0; k < 48; ++k) {
+            const int m = 2 + k % 3;
+            for (int i = 0; i < m * m * m; ++i) {
+                const double o[3] = {2.9 * (i % m), 2.9 * ((i / m) % m), 2.9 * (i / (m * m))};
+                const double h[2][3] = {{0.757, 0.586, 0.0}, {-0.757, 0.586, 0.0}};
+                Z[k].push_back(8);
+                for (int c = 0; c < 3; ++c) X[k].push_back(o[c] * ang);
+                for (int a = 0; a < 2; ++a) {
+                    Z[k].push_back(1);
+                    for (int c = 0; c < 3; ++c) X[k].push_back((o[c] + h[a][c]) * ang);
+                }
+            }
+            std::memset(&d[k], 0, sizeof(nddo_job_desc));
+            d[k].theory = NDDO_PM3;
+            d[k].precision = NDDO_FP64;
+            d[k].natoms = (int)Z[k].size();
+            d[k].Z = Z[k].data();
+            d[k].xyz_bohr = X[k].data();
+            d[k].requires_guess = 1;
+            nddo_scf_opts_default(&d[k].opts);
+        }
+        std::vector<int> which(48);
+        for (int k = 0; k < 48; ++k) which[k] = k;
+        Batch b;
+        int rc = b.build(d.data(), 48, which, nullptr);
+        if (rc == NDDO_OK) rc = b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+        if (rc == NDDO_OK) rc = b.allocate(false);
+        if (rc == NDDO_OK) rc = b.upload(e->stream, d.data());
+        if (rc != NDDO_OK) return rc;
+        b.launch_setup_stage(e, e->stream);
+        std::vector<JobState> st(48);
+        std::memset(st.data(), 0, st.size() * sizeof(JobState));
+        for (auto& x : st) { x.phase = PH_ACTIVE; x.iter = 1; }   // a regular (non-guess) Fock build
+        HIPCHK(hipMemcpyAsync(b.d_state, st.data(), st.size() * sizeof(JobState), hipMemcpyHostToDevice, e->stream));
+        float best = 1e30f;
+        for (int w : {1, 2, 4, 8}) {
+            Tuning c = t;
+            c.fock_wpb = w;
+            set_tuning(c);
+            float ms = 0.f;
+            rc = time_ms(e, [&] {
+                for (int r = 0; r < 4; ++r)
+                    launch_fock(e->stream, 0, b.d_pairs, (int)b.pairs.size(), b.d_atoms, (int)b.atoms.size(),
+                                e->d_elems, b.d_jobs, b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
+            }, 3, &ms);
+            if (rc != NDDO_OK) return rc;
+            if (ms < best) { best = ms; t.fock_wpb = w; }
+        }
+        set_tuning(t);
+    }
+    // back-transformation row block per size class: 32 random symmetric matrices each
+    const int sizes[4] = {128, 256, 384, 512};
+    std::vector<double> F;
+    for (int cls = 0; cls < 4; ++cls) {
+        const int n = sizes[cls], K = 32;
+        Batch b;
+        int rc = matrix_batch(e, b, n, n / 2, K);
+        if (rc != NDDO_OK) return rc;
+        F.assign((size_t)n * n, 0.0);
+        unsigned long long x = 0x9e3779b97f4a7c15ull * (cls + 1);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j <= i; ++j) {
+                x ^= x << 13; x ^= x >> 7; x ^= x << 17;   // xorshift64
+                const double v = (double)(x >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+                F[(size_t)i * n + j] = F[(size_t)j * n + i] = v;
+            }
+        for (int k = 0; k < K; ++k)
+            HIPCHK(hipMemcpy(b.d_arena + b.jobs[k].arena + (long long)AR_F * n * n, F.data(), F.size() * sizeof(double),
+                             hipMemcpyHostToDevice));
+        const bool lds = !b.list_lds.empty();
+        const int* list = lds ? b.d_list_lds : b.d_list_glb;
+        const int nl = (int)(lds ? b.list_lds.size() : b.list_glb.size());
+        for (int part = 0; part < 2; ++part)
+            launch_heig_part(part, e->stream, b.d_jobs, b.d_state, list, nl, n, b.d_arena, b.d_eps, b.d_perm, b.d_tri,
+                             lds);
+        float best = 1e30f;
+        for (int rows : {64, 32}) {
+            Tuning c = t;
+            c.bt_rows[cls] = rows;
+            set_tuning(c);
+            float ms = 0.f;
+            rc = time_ms(e, [&] {
+                launch_heig_part(2, e->stream, b.d_jobs, b.d_state, list, nl, n, b.d_arena, b.d_eps, b.d_perm, b.d_tri,
+                                 lds);
+            }, 3, &ms);
+            if (rc != NDDO_OK) return rc;
+            if (ms < best) { best = ms; t.bt_rows[cls] = rows; }
+        }
+        set_tuning(t);
+    }
+    return NDDO_OK;
+}
+
+int nddo_engine_tuning(nddo_engine* eh, int* fock_wpb, int* bt_rows4) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    const Tuning t = get_tuning();
+    if (fock_wpb) *fock_wpb = t.fock_wpb;
+    if (bt_rows4)
+        for (int i = 0; i < 4; ++i) bt_rows4[i] = t.bt_rows[i];
+    return NDDO_OK;
+}
+
+int nddo_dbg_set_tuning(nddo_engine* eh, int fock_wpb, const int* bt_rows4) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !bt_rows4 || !(fock_wpb == 1 || fock_wpb == 2 || fock_wpb == 4 || fock_wpb == 8)) return NDDO_BAD_ARG;
+    Tuning t;
+    t.fock_wpb = fock_wpb;
+    for (int i = 0; i < 4; ++i) {
+        if (bt_rows4[i] != 32 && bt_rows4[i] != 64) return NDDO_BAD_ARG;
+        t.bt_rows[i] = bt_rows4[i];
+    }
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    set_tuning(t);
     return NDDO_OK;
 }
 

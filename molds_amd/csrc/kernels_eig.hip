@@ -942,10 +942,22 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
     } else if (max_n >= 3) {
         const int nblk = (max_n - 2 + kBt - 1) / kBt;
         hipLaunchKernelGGL(k_wy_t, dim3(nlist, nblk), dim3(64), 0, st, jobs, state, joblist, arena, tri);
-        if (max_n <= 128) launch_bt<1, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
-        else if (max_n <= 256) launch_bt<2, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
-        else if (max_n <= 384) launch_bt<3, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
-        else if (max_n <= 512) launch_bt<4, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        const Tuning tu = get_tuning();
+        const int cls = max_n <= 128 ? 0 : (max_n <= 256 ? 1 : (max_n <= 384 ? 2 : 3));
+        const bool r32 = max_n <= 512 && tu.bt_rows[cls] == 32;
+        if (max_n <= 128) {
+            if (r32) launch_bt<1, 2, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+            else launch_bt<1, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        } else if (max_n <= 256) {
+            if (r32) launch_bt<2, 2, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+            else launch_bt<2, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        } else if (max_n <= 384) {
+            if (r32) launch_bt<3, 2, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+            else launch_bt<3, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        } else if (max_n <= 512) {
+            if (r32) launch_bt<4, 2, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+            else launch_bt<4, 4, false>(st, jobs, state, joblist, nlist, max_n, arena, tri);
+        }
         else if (max_n <= 768) launch_bt<6, 2, true>(st, jobs, state, joblist, nlist, max_n, arena, tri);
         else if (max_n <= 1024) launch_bt<8, 2, true>(st, jobs, state, joblist, nlist, max_n, arena, tri);
         else launch_bt<10, 1, true>(st, jobs, state, joblist, nlist, max_n, arena, tri);

@@ -22,6 +22,19 @@ __device__ inline int pk_index(int a, int b) {
     return a == 0 ? b : (a == 1 ? 3 + b : (a == 2 ? 5 + b : 9));
 }
 
+namespace {
+constexpr int kMaxDevices = 64;
+Tuning g_tuning[kMaxDevices];
+int current_device() {
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxDevices) d = 0;
+    return d;
+}
+}  // namespace
+
+Tuning get_tuning() { return g_tuning[current_device()]; }
+void set_tuning(const Tuning& t) { g_tuning[current_device()] = t; }
+
 __device__ inline double* mat(double* arena, const JobDev& jb, int kind) {
     return arena + jb.arena + (long long)kind * jb.nao * jb.nao;
 }
@@ -49,15 +62,18 @@ __device__ inline double h1c(const ElemDev& E, int a, int b) {
 //   J_B[lam sig] = sum_{mu nu in A} P_mu,nu (mu nu|lam sig)      (slots 10..19)
 // Jobs in their FP32 (mixed-precision) iterations read the FP32 copy of the
 // integrals and contract in FP32; F_AB = H_AB - 1/2 K is assembled in FP64.
-__global__ void __launch_bounds__(256) k_fock_pairs(
+// WPB wavefronts (pairs) per workgroup, picked by the autotuner; the per-pair
+// work does not depend on it.
+template <int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_fock_pairs(
     int mode, const PairDev* __restrict__ pairs, int npairs, const AtomDev* __restrict__ atoms,
     const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs, const JobState* __restrict__ state,
     const double* __restrict__ ints, const float* __restrict__ ints32, double* __restrict__ arena,
     double* __restrict__ part) {
-    __shared__ double sI[4][100], sPab[4][16], sWa[4][10], sWb[4][10];
-    __shared__ float fI[4][100], fPab[4][16], fWa[4][10], fWb[4][10];
+    __shared__ double sI[WPB][100], sPab[WPB][16], sWa[WPB][10], sWb[WPB][10];
+    __shared__ float fI[WPB][100], fPab[WPB][16], fWa[WPB][10], fWb[WPB][10];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int pidx = blockIdx.x * 4 + wave;
+    const int pidx = blockIdx.x * WPB + wave;
     bool act = pidx < npairs;
     PairDev pr;
     JobDev jb;
@@ -256,9 +272,17 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
 void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
                  const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
                  const float* ints32, double* arena, double* part) {
-    if (npairs > 0)
-        hipLaunchKernelGGL(k_fock_pairs, dim3((npairs + 3) / 4), dim3(256), 0, st, mode, pairs, npairs, atoms,
-                           elems, jobs, state, ints, ints32, arena, part);
+    if (npairs > 0) {
+        const int w = get_tuning().fock_wpb;
+#define NDDO_FOCK_PAIRS(W)                                                                                   \
+    hipLaunchKernelGGL(k_fock_pairs<W>, dim3((npairs + (W)-1) / (W)), dim3(64 * (W)), 0, st, mode, pairs, npairs, \
+                       atoms, elems, jobs, state, ints, ints32, arena, part)
+        if (w == 1) NDDO_FOCK_PAIRS(1);
+        else if (w == 2) NDDO_FOCK_PAIRS(2);
+        else if (w == 8) NDDO_FOCK_PAIRS(8);
+        else NDDO_FOCK_PAIRS(4);
+#undef NDDO_FOCK_PAIRS
+    }
     hipLaunchKernelGGL(k_fock_atoms, dim3((natoms + 3) / 4), dim3(256), 0, st, mode, atoms, natoms, elems, jobs,
                        state, arena, part);
 }

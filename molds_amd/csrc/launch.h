@@ -7,6 +7,15 @@
 
 namespace nddo {
 
+// Launch shapes picked by the engine's autotuner (engine.cpp), per HIP device;
+// none of them changes the arithmetic of a molecule.
+struct Tuning {
+    int fock_wpb = 4;                      // atom pairs (wavefronts) per workgroup of the pair Fock kernel
+    int bt_rows[4] = {64, 64, 64, 64};     // back-transformation rows per workgroup, n <= 128 / 256 / 384 / 512
+};
+Tuning get_tuning();               // of the current device
+void set_tuning(const Tuning& t);  // for the current device
+
 int upload_overlap_table(const double* table /* 32*25 */);
 
 void launch_setup(hipStream_t st, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,

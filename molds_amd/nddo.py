@@ -87,6 +87,8 @@ def load_library(path=LIBPATH):
     L.nddo_dbg_density.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.nddo_dbg_eigh_profile.argtypes = [vp, _dp, _dp]
     L.nddo_dbg_eigh_dc_phases.argtypes = [vp, _dp]
+    L.nddo_engine_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.nddo_dbg_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.nddo_dbg_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     _lib = L
     return L
@@ -148,6 +150,7 @@ class Engine:
 
     DEVICE_WIDE_EIGEN = 1   # NDDO_ENGINE_DEVICE_WIDE_EIGEN
     NO_GRAPHS = 2           # NDDO_ENGINE_NO_GRAPHS
+    NO_AUTOTUNE = 4         # NDDO_ENGINE_NO_AUTOTUNE
 
     def __init__(self, device=0, hbm_budget_bytes=0, max_batch_jobs=0, flags=0):
         L = load_library()
@@ -320,6 +323,18 @@ class Engine:
         hz = ctypes.c_double()
         self.L.nddo_dbg_eigh_profile(self.h, _p(t), ctypes.byref(hz))
         return dict(zip(("tridiag", "divide_conquer", "unused", "backtransform"), (t / hz.value * 1e3).tolist()))
+
+    def tuning(self):
+        w = ctypes.c_int()
+        rows = (ctypes.c_int * 4)()
+        self.L.nddo_engine_tuning(self.h, ctypes.byref(w), rows)
+        return {"fock_pairs_per_workgroup": w.value, "backtransform_rows": dict(zip((128, 256, 384, 512), rows))}
+
+    def set_tuning(self, fock_wpb, bt_rows):
+        rows = (ctypes.c_int * 4)(*bt_rows)
+        rc = self.L.nddo_dbg_set_tuning(self.h, fock_wpb, rows)
+        if rc != 0:
+            raise NddoError(rc, last_error())
 
     def graph_stats(self):
         c, r = ctypes.c_longlong(), ctypes.c_longlong()
