@@ -219,6 +219,9 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=2.0, help="seconds of arrivals per served leg")
     ap.add_argument("--served-max", type=int, default=400, help="at most this many requests per served leg")
     ap.add_argument("--no-large", action="store_true", help="skip the (H2O)_256 single-molecule leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (reference for the HBM byte "
+                         "counters FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--latency-rate", type=float, default=500.0, help="served rate whose p50/p99 is the headline")
     ap.add_argument("--plain-seconds", type=float, default=8.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -226,6 +229,15 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": a.numel() * 8, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the batched and served "
               f"workloads shard by molecule across GPUs, one engine per GPU)", file=sys.stderr)

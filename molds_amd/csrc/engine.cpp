@@ -106,14 +106,7 @@ static void overlap_table(double* tab) {
                     }
 }
 
-size_t jacobi_lds_bytes(int n, bool a_in_lds) {
-    if (heig_supported(n)) return heig_lds_bytes(n, a_in_lds);
-    if (n <= 512) return jacobi_rows_lds_bytes(n, a_in_lds);
-    const int nh = (n + (n & 1)) / 2;
-    size_t b = (size_t)3 * nh * 8 + (size_t)(3 * nh + (nh & 1)) * 4;
-    if (a_in_lds) b += (size_t)n * n * 8;
-    return b;
-}
+size_t eig_lds_bytes(int n, bool a_in_lds) { return heig_supported(n) ? heig_lds_bytes(n, a_in_lds) : 0; }
 
 // ------------------------------------------------------------------ batch
 Batch::~Batch() { release(); }
@@ -295,7 +288,7 @@ int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao) {
         int n = jobs[j].nao;
         // the Jacobi kernels also hold up to 8 KB of static LDS (reduction buffers)
         jobs[j].xl = 0;
-        if (n <= 512 && lds_max > 9216 && jacobi_lds_bytes(n, true) <= lds_max - 9216) {
+        if (n <= 512 && lds_max > 9216 && eig_lds_bytes(n, true) <= lds_max - 9216) {
             list_lds.push_back(j);
             maxn_lds = std::max(maxn_lds, n);
         } else if (heig_supported(n) && n < xl_min_nao) {
@@ -432,10 +425,8 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
     launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
     launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
-    launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri,
-                 jacobi_lds_bytes(maxn_lds, true), true);
-    launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri,
-                 jacobi_lds_bytes(maxn_glb, false), false);
+    launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri, true);
+    launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri, false);
     launch_xl_eigen_all(e, st);
     launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
     launch_xl_dots(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
@@ -515,22 +506,17 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
             HIPCHK(hipEventRecord(e->ev[1], st));
             launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
-            const bool hl = heig_supported(maxn_lds), hg = heig_supported(maxn_glb);
-            if (hl) launch_heig_part(0, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
-                                     d_perm, d_tri, true);
+            launch_heig_part(0, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
+                             d_tri, true);
             HIPCHK(hipEventRecord(e->ev[2], st));
-            if (hg) launch_heig_part(0, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps,
-                                     d_perm, d_tri, false);
-            if (!hl) launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
-                                  d_tri, jacobi_lds_bytes(maxn_lds, true), true);
-            if (!hg) launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
-                                  d_tri, jacobi_lds_bytes(maxn_glb, false), false);
+            launch_heig_part(0, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
+                             d_tri, false);
             HIPCHK(hipEventRecord(e->ev[3], st));
             for (int part = 1; part < 3; ++part) {
-                if (hl) launch_heig_part(part, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena,
-                                         d_eps, d_perm, d_tri, true);
-                if (hg) launch_heig_part(part, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena,
-                                         d_eps, d_perm, d_tri, false);
+                launch_heig_part(part, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
+                                 d_perm, d_tri, true);
+                launch_heig_part(part, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps,
+                                 d_perm, d_tri, false);
                 HIPCHK(hipEventRecord(e->ev[3 + part], st));
             }
             launch_xl_eigen_all(e, st);
@@ -761,7 +747,7 @@ int nddo_engine_info(nddo_engine* eh, size_t* lds_bytes, int* num_cus, size_t* h
     if (hbm_budget) *hbm_budget = e->hbm_budget;
     if (lds_tridiag_max_n) {
         int n = 0;
-        while (n < 512 && e->lds_max > 9216 && jacobi_lds_bytes(n + 1, true) <= e->lds_max - 9216) ++n;
+        while (n < 512 && e->lds_max > 9216 && eig_lds_bytes(n + 1, true) <= e->lds_max - 9216) ++n;
         *lds_tridiag_max_n = n;
     }
     return NDDO_OK;
@@ -1105,9 +1091,9 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
     const long long nn = (long long)n * n;
     HIPCHK(hipMemcpy(b.d_arena + (long long)AR_F * nn, F, nn * sizeof(double), hipMemcpyHostToDevice));
     launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_lds, (int)b.list_lds.size(), b.maxn_lds, b.d_arena, b.d_eps,
-                 b.d_perm, b.d_tri, jacobi_lds_bytes(b.maxn_lds, true), true);
+                 b.d_perm, b.d_tri, true);
     launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_glb, (int)b.list_glb.size(), b.maxn_glb, b.d_arena, b.d_eps,
-                 b.d_perm, b.d_tri, jacobi_lds_bytes(b.maxn_glb, false), false);
+                 b.d_perm, b.d_tri, false);
     b.launch_xl_eigen_all(e, e->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));

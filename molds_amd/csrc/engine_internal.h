@@ -21,7 +21,7 @@ constexpr int kNumElem = 5;   // H, C, N, O, F
 void set_error(const std::string& s);
 int elem_slot(int Z);
 int validate_job(const nddo_job_desc& d, int* nao, int* nocc);
-size_t jacobi_lds_bytes(int n, bool a_in_lds);
+size_t eig_lds_bytes(int n, bool a_in_lds);   // LDS of the one-workgroup eigensolver
 // Device bytes a job keeps resident (admission control; MolDS's MallocerFreer heap
 // cap, src/base/MallocerFreer.cpp:62-73, plays the same role on the host).
 size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter);

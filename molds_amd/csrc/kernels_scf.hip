@@ -287,380 +287,14 @@ void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, con
                        state, arena, part);
 }
 
-// ------------------------------------------------------------------ Jacobi
-// Cyclic two-sided Jacobi with the round-robin (circle) ordering: every round
-// rotates n/2 disjoint index pairs at once; a 2x2 block of rows {p_i,q_i} and
-// columns {p_j,q_j} transforms as J_i^T B J_j, so each thread updates whole
-// 2x2 blocks of the upper block triangle and mirrors them.  Eigenvector rows
-// VT (= C, rows are MOs as MolDS stores them) accumulate VT <- J^T VT.
-// Rotation parameters: Golub & Van Loan, Matrix Computations, Alg. 8.5.1.
-// A sweep ends the solve when it rotates nothing (all |a_pq| <= thr).
-template <bool ALDS>
-__global__ void __launch_bounds__(1024) k_jacobi(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
-                                                 const int* __restrict__ joblist, double* __restrict__ arena,
-                                                 double* __restrict__ eps, int* __restrict__ perm) {
-    extern __shared__ double dyn[];
-    __shared__ double red[1024];
-    const int job = joblist[blockIdx.x];
-    const JobDev jb = jobs[job];
-    if (state[job].phase != PH_ACTIVE) return;
-    const int n = jb.nao, npad = n + (n & 1), nh = npad / 2, m = npad - 1;
-    const int T = blockDim.x, tid = threadIdx.x;
-    double* sc = dyn;
-    double* ss = sc + nh;
-    double* stt = ss + nh;
-    int* sp = (int*)(stt + nh);
-    int* sq = sp + nh;
-    int* srot = sq + nh;
-    double* A = ALDS ? (double*)(srot + nh + (nh & 1)) : mat(arena, jb, AR_W);
-    const double* F = mat(arena, jb, AR_F);
-    double* VT = mat(arena, jb, AR_C);
-    const long long nn = (long long)n * n;
-
-    // cold start: VT = I.  Warm start: F already holds VT_prev F VT_prev^T
-    // (k_warm_gemm) and VT_prev is the initial accumulator, so the rotations
-    // found here compose with it into the eigenvectors of F.
-    const bool warm = state[job].warm != 0;
-    double fro = 0.0;
-    for (long long e = tid; e < nn; e += T) {
-        double v = F[e];
-        A[e] = v;
-        fro += v * v;
-        if (!warm) {
-            int i = (int)(e / n), j = (int)(e - (long long)i * n);
-            VT[e] = i == j ? 1.0 : 0.0;
-        }
-    }
-    red[tid] = fro;
-    __syncthreads();
-    for (int s = T / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
-    }
-    const double thr = 1e-15 * sqrt(red[0]);
-    __syncthreads();
-
-    int sweep = 0;
-    const int maxsweep = 60;
-    for (; sweep < maxsweep; ++sweep) {
-        int rotated = 0;
-        for (int r = 0; r < m; ++r) {
-            int flag = 0;
-            for (int k = tid; k < nh; k += T) {
-                int p, q;
-                if (k == 0) { p = r; q = m; }
-                else { p = (r + k) % m; q = (r - k + m) % m; }
-                if (p > q) { int t = p; p = q; q = t; }
-                double c = 1.0, s = 0.0, t = 0.0;
-                int rot = 0;
-                if (q < n) {
-                    double apq = A[(long long)p * n + q];
-                    if (fabs(apq) > thr) {
-                        double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
-                        double tau = (aqq - app) / (2.0 * apq);
-                        t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        c = 1.0 / sqrt(1.0 + t * t);
-                        s = t * c;
-                        rot = 1;
-                    }
-                }
-                sp[k] = p; sq[k] = q; sc[k] = c; ss[k] = s; stt[k] = t; srot[k] = rot;
-                flag |= rot;
-            }
-            if (!__syncthreads_or(flag)) continue;
-            rotated = 1;
-            const long long nb = (long long)nh * (nh + 1) / 2;
-            for (long long b = tid; b < nb; b += T) {
-                // b enumerates (hi >= lo): b = hi*(hi+1)/2 + lo
-                long long hi = (long long)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
-                while (hi * (hi + 1) / 2 > b) --hi;
-                while ((hi + 1) * (hi + 2) / 2 <= b) ++hi;
-                const int ki = (int)(b - hi * (hi + 1) / 2), kj = (int)hi;
-                const int ri = srot[ki], rj = srot[kj];
-                if (!ri && !rj) continue;
-                const int p1 = sp[ki], q1 = sq[ki], p2 = sp[kj], q2 = sq[kj];
-                if (ki == kj) {
-                    const double t = stt[ki];
-                    const double apq = A[(long long)p1 * n + q1];
-                    A[(long long)p1 * n + p1] -= t * apq;
-                    A[(long long)q1 * n + q1] += t * apq;
-                    A[(long long)p1 * n + q1] = 0.0;
-                    A[(long long)q1 * n + p1] = 0.0;
-                    continue;
-                }
-                const bool vq1 = q1 < n, vq2 = q2 < n;
-                const double c1 = sc[ki], s1 = ss[ki], c2 = sc[kj], s2 = ss[kj];
-                const double a = A[(long long)p1 * n + p2];
-                const double bq = vq2 ? A[(long long)p1 * n + q2] : 0.0;
-                const double cq = vq1 ? A[(long long)q1 * n + p2] : 0.0;
-                const double d = (vq1 && vq2) ? A[(long long)q1 * n + q2] : 0.0;
-                const double a1 = c1 * a - s1 * cq, b1 = c1 * bq - s1 * d;
-                const double c1r = s1 * a + c1 * cq, d1 = s1 * bq + c1 * d;
-                const double a2 = c2 * a1 - s2 * b1, b2 = s2 * a1 + c2 * b1;
-                const double c2r = c2 * c1r - s2 * d1, d2 = s2 * c1r + c2 * d1;
-                A[(long long)p1 * n + p2] = a2;
-                A[(long long)p2 * n + p1] = a2;
-                if (vq2) { A[(long long)p1 * n + q2] = b2; A[(long long)q2 * n + p1] = b2; }
-                if (vq1) { A[(long long)q1 * n + p2] = c2r; A[(long long)p2 * n + q1] = c2r; }
-                if (vq1 && vq2) { A[(long long)q1 * n + q2] = d2; A[(long long)q2 * n + q1] = d2; }
-            }
-            for (long long e = tid; e < (long long)nh * n; e += T) {
-                const int k = (int)(e / n), i = (int)(e - (long long)k * n);
-                if (!srot[k]) continue;
-                const int p = sp[k], q = sq[k];
-                const double c = sc[k], s = ss[k];
-                const double vp = VT[(long long)p * n + i], vq = VT[(long long)q * n + i];
-                VT[(long long)p * n + i] = c * vp - s * vq;
-                VT[(long long)q * n + i] = s * vp + c * vq;
-            }
-            __syncthreads();
-        }
-        if (!rotated) break;
-    }
-    // eigenvalues ascending (ties by index) and the row permutation
-    for (int p = tid; p < n; p += T) {
-        const double d = A[(long long)p * n + p];
-        int rank = 0;
-        for (int q = 0; q < n; ++q) {
-            const double e = A[(long long)q * n + q];
-            rank += (e < d) || (e == d && q < p);
-        }
-        eps[jb.vec0 + rank] = d;
-        perm[jb.ivec0 + rank] = p;
-    }
-    if (tid == 0) {
-        state[job].sweeps = sweep < maxsweep ? sweep + 1 : -1;
-        state[job].warm = 1;
-    }
-}
-
-// Row-pair streaming Jacobi.  Same rotation sequence as k_jacobi (round-robin
-// pairs, Golub-Van Loan parameters), but each wavefront owns one pair (p, q)
-// per round and rewrites rows p and q completely: element (i, j), with i in
-// pair P and j in pair Q, becomes sum over i' in P, j' in Q of
-// M_P[i][i'] M_Q[j][j'] A[i'][j'], where M = [[c, -s], [s, c]].  Rows are read
-// and written with coalesced accesses and no mirroring; the four products are
-// rounded separately and summed as [self-self + other-other] +
-// [self-other + other-self], which is the same expression for (i, j) and
-// (j, i), so A stays exactly symmetric.
-// (Ain and Aout alias in the LDS case, so they carry no __restrict__.)
-// In global memory the matrix ping-pongs between two arena buffers (AR_W and
-// AR_F, which is free once copied), so a round reads one and writes the other
-// and no wavefront ever waits on its own stores.  In LDS the update is in
-// place: a wavefront's LDS reads of rows p, q precede its writes in program
-// order, and the LDS unit serves a wavefront's instructions in order.
-template <int NPL>
-__device__ inline void jacobi_pair_rows(const double* Ain, double* Aout, int n, int lane,
-                                        int p, int q, const double* ca, const double* cb, const int* partner,
-                                        int rot, double t, double app, double aqq, double apq) {
-    const bool vq = q < n;
-    const double cap = ca[p], cbp = cb[p], caq = ca[q], cbq = cb[q];
-    double np_[NPL], nq_[NPL];
-#pragma unroll
-    for (int u = 0; u < NPL; ++u) {
-        const int j = lane + 64 * u;
-        np_[u] = 0.0;
-        nq_[u] = 0.0;
-        if (j < n) {
-            const int oj = partner[j];
-            const bool voj = oj < n;
-            const double cj = ca[j], dj = cb[j];
-            const double a_pj = Ain[(long long)p * n + j];
-            const double a_po = voj ? Ain[(long long)p * n + oj] : 0.0;
-            const double a_qj = vq ? Ain[(long long)q * n + j] : 0.0;
-            const double a_qo = (vq && voj) ? Ain[(long long)q * n + oj] : 0.0;
-            np_[u] = (__dmul_rn(cap * cj, a_pj) + __dmul_rn(cbp * dj, a_qo)) +
-                     (__dmul_rn(cap * dj, a_po) + __dmul_rn(cbp * cj, a_qj));
-            nq_[u] = (__dmul_rn(caq * cj, a_qj) + __dmul_rn(cbq * dj, a_po)) +
-                     (__dmul_rn(caq * dj, a_qo) + __dmul_rn(cbq * cj, a_pj));
-            if (rot) {
-                if (j == p) { np_[u] = app - t * apq; nq_[u] = 0.0; }
-                if (j == q) { np_[u] = 0.0; nq_[u] = aqq + t * apq; }
-            }
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < NPL; ++u) {
-        const int j = lane + 64 * u;
-        if (j < n) {
-            Aout[(long long)p * n + j] = np_[u];
-            if (vq) Aout[(long long)q * n + j] = nq_[u];
-        }
-    }
-}
-
-template <bool ALDS, int NPL>
-__global__ void __launch_bounds__(1024) k_jacobi_rows(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
-                                                      const int* __restrict__ joblist, double* __restrict__ arena,
-                                                      double* __restrict__ eps, int* __restrict__ perm) {
-    extern __shared__ double dyn[];
-    __shared__ double red[32];
-    const int job = joblist[blockIdx.x];
-    const JobDev jb = jobs[job];
-    if (state[job].phase != PH_ACTIVE) return;
-    const int n = jb.nao, npad = n + (n & 1), nh = npad / 2, m = npad - 1;
-    const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
-    double* ca = dyn;            // [npad] self coefficient of index i
-    double* cb = ca + npad;      // [npad] partner coefficient of index i
-    double* kt = cb + npad;      // [nh] per pair: t, a_pp, a_qq, a_pq
-    double* kapp = kt + nh;
-    double* kaqq = kapp + nh;
-    double* kapq = kaqq + nh;
-    int* partner = (int*)(kapq + nh);   // [npad]
-    int* kp = partner + npad;           // [nh]
-    int* kq = kp + nh;
-    int* krot = kq + nh;
-    const int nint = npad + 3 * nh;
-    double* F = mat(arena, jb, AR_F);
-    double* Abuf[2];
-    Abuf[0] = ALDS ? (double*)(krot + nh + (nint & 1)) : mat(arena, jb, AR_W);
-    Abuf[1] = ALDS ? Abuf[0] : F;
-    double* VT = mat(arena, jb, AR_C);
-    const long long nn = (long long)n * n;
-    const bool warm = state[job].warm != 0;
-
-    double fro = 0.0;
-    for (long long e = tid; e < nn; e += T) {
-        const double v = F[e];
-        Abuf[0][e] = v;
-        fro += v * v;
-        if (!warm) {
-            const int i = (int)(e / n), j = (int)(e - (long long)i * n);
-            VT[e] = i == j ? 1.0 : 0.0;
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) fro += __shfl_xor(fro, off);
-    if (lane == 0) red[wave] = fro;
-    __syncthreads();
-    if (tid == 0) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += red[w];
-        red[31] = s;
-    }
-    __syncthreads();
-    const double thr = 1e-15 * sqrt(red[31]);
-
-    int cur = 0;
-    int sweep = 0;
-    const int maxsweep = 60;
-    for (; sweep < maxsweep; ++sweep) {
-        int rotated = 0;
-        for (int r = 0; r < m; ++r) {
-            const double* A = Abuf[cur];
-            int flag = 0;
-            for (int k = tid; k < nh; k += T) {
-                int p, q;
-                if (k == 0) { p = r; q = m; }
-                else { p = (r + k) % m; q = (r - k + m) % m; }
-                if (p > q) { int t = p; p = q; q = t; }
-                double c = 1.0, sn = 0.0, t = 0.0, app = 0.0, aqq = 0.0, apq = 0.0;
-                int rot = 0;
-                if (q < n) {
-                    apq = A[(long long)p * n + q];
-                    app = A[(long long)p * n + p];
-                    aqq = A[(long long)q * n + q];
-                    if (fabs(apq) > thr) {
-                        const double tau = (aqq - app) / (2.0 * apq);
-                        t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        c = 1.0 / sqrt(1.0 + t * t);
-                        sn = t * c;
-                        rot = 1;
-                    }
-                }
-                kp[k] = p; kq[k] = q; krot[k] = rot;
-                kt[k] = t; kapp[k] = app; kaqq[k] = aqq; kapq[k] = apq;
-                partner[p] = q; partner[q] = p;
-                ca[p] = c; cb[p] = -sn;
-                ca[q] = c; cb[q] = sn;
-                flag |= rot;
-            }
-            if (!__syncthreads_or(flag)) continue;
-            rotated = 1;
-            double* Aout = Abuf[cur ^ 1];
-            for (int k = wave; k < nh; k += nw) {
-                const int p = kp[k], q = kq[k], rot = krot[k];
-                jacobi_pair_rows<NPL>(A, Aout, n, lane, p, q, ca, cb, partner, rot, kt[k], kapp[k], kaqq[k],
-                                      kapq[k]);
-                if (rot) {
-                    const double c = ca[p], sn = cb[q];
-                    for (int j = lane; j < n; j += 64) {
-                        const double vp = VT[(long long)p * n + j], vv = VT[(long long)q * n + j];
-                        VT[(long long)p * n + j] = c * vp - sn * vv;
-                        VT[(long long)q * n + j] = sn * vp + c * vv;
-                    }
-                }
-            }
-            cur ^= ALDS ? 0 : 1;
-            __syncthreads();
-        }
-        if (!rotated) break;
-    }
-    const double* A = Abuf[cur];
-    for (int p = tid; p < n; p += T) {
-        const double d = A[(long long)p * n + p];
-        int rank = 0;
-        for (int q = 0; q < n; ++q) {
-            const double e = A[(long long)q * n + q];
-            rank += (e < d) || (e == d && q < p);
-        }
-        eps[jb.vec0 + rank] = d;
-        perm[jb.ivec0 + rank] = p;
-    }
-    if (tid == 0) {
-        state[job].sweeps = sweep < maxsweep ? sweep + 1 : -1;
-        state[job].warm = 1;
-    }
-}
-
-template <bool ALDS, int NPL>
-static void launch_rows(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
-                        int threads, double* arena, double* eps, int* perm, size_t lds_bytes) {
-    hipFuncSetAttribute((const void*)k_jacobi_rows<ALDS, NPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds_bytes);
-    hipLaunchKernelGGL((k_jacobi_rows<ALDS, NPL>), dim3(nlist), dim3(threads), lds_bytes, st, jobs, state, joblist,
-                       arena, eps, perm);
-}
-
-size_t jacobi_rows_lds_bytes(int n, bool a_in_lds) {
-    const int npad = n + (n & 1), nh = npad / 2, nint = npad + 3 * nh;
-    size_t b = (size_t)(2 * npad + 4 * nh) * 8 + (size_t)(nint + (nint & 1)) * 4;
-    if (a_in_lds) b += (size_t)n * n * 8;
-    return b;
-}
-
+// ------------------------------------------------------------------ eigensolver dispatch
+// The one-workgroup Householder + divide-and-conquer solver (kernels_eig.hip)
+// for the LDS and device-memory classes; molecules beyond it are solved by the
+// device-wide block Jacobi (kernels_xl.hip), launched by the engine.
 void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
-                  double* arena, double* eps, int* perm, double* tri, size_t lds_bytes, bool a_in_lds) {
-    if (nlist <= 0) return;
-    if (heig_supported(max_n)) {
-        launch_heig_class(st, jobs, state, joblist, nlist, max_n, arena, eps, perm, tri, a_in_lds);
-        return;
-    }
-    launch_jacobi(st, jobs, state, joblist, nlist, max_n, arena, eps, perm, lds_bytes, a_in_lds);
-}
-
-// Cyclic Jacobi (kept for matrices beyond the one-workgroup Householder path).
-void launch_jacobi(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
-                   double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds) {
-    if (nlist <= 0) return;
-    const int threads = max_n <= 32 ? 256 : (max_n <= 64 ? 512 : 1024);
-    if (max_n <= 512) {
-        const size_t b = jacobi_rows_lds_bytes(max_n, a_in_lds);
-        if (a_in_lds) {
-            if (max_n <= 64) launch_rows<true, 1>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
-            else if (max_n <= 128) launch_rows<true, 2>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
-            else launch_rows<true, 4>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
-        } else {
-            if (max_n <= 128) launch_rows<false, 2>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
-            else if (max_n <= 256) launch_rows<false, 4>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
-            else if (max_n <= 384) launch_rows<false, 6>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
-            else launch_rows<false, 8>(st, jobs, state, joblist, nlist, threads, arena, eps, perm, b);
-        }
-        return;
-    }
-    // beyond 512 AOs: the 2x2-block kernel (global memory)
-    hipFuncSetAttribute((const void*)k_jacobi<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-    hipLaunchKernelGGL(k_jacobi<false>, dim3(nlist), dim3(1024), lds_bytes, st, jobs, state, joblist, arena, eps,
-                       perm);
+                  double* arena, double* eps, int* perm, double* tri, bool a_in_lds) {
+    if (nlist <= 0 || !heig_supported(max_n)) return;
+    launch_heig_class(st, jobs, state, joblist, nlist, max_n, arena, eps, perm, tri, a_in_lds);
 }
 
 // ------------------------------------------------------------------ warm start

@@ -38,11 +38,10 @@ void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, con
 void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, const JobState* state,
                  double* arena);
 
-// Eigen-decomposition of F for PH_ACTIVE jobs in joblist (one workgroup per job).
-// lds_ok: jobs whose n*n matrix fits the LDS budget take the LDS path.
-size_t jacobi_rows_lds_bytes(int n, bool a_in_lds);
-// Householder + divide-and-conquer eigensolver (kernels_eig.hip), one workgroup
-// per job, n <= 512; tri is the eigensolver workspace (EIG_WS_PER_AO, EIG_WS_PER_JOB).
+// Eigen-decomposition of F for PH_ACTIVE jobs in joblist, one workgroup per job:
+// Householder + divide and conquer (kernels_eig.hip), n <= 1200; tri is the
+// eigensolver workspace (EIG_WS_PER_AO, EIG_WS_PER_JOB); a_in_lds: the matrices
+// fit the LDS budget (heig_lds_bytes).
 size_t heig_lds_bytes(int n, bool a_in_lds);
 bool heig_supported(int n);
 void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
@@ -50,10 +49,8 @@ void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, cons
 // One part of it: 0 tridiagonalization, 1 divide and conquer, 2 back-transformation.
 void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
                       int max_n, double* arena, double* eps, int* perm, double* tri, bool a_in_lds);
-void launch_jacobi(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
-                   double* arena, double* eps, int* perm, size_t lds_bytes, bool a_in_lds);
 void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
-                  int max_n, double* arena, double* eps, int* perm, double* tri, size_t lds_bytes, bool a_in_lds);
+                  int max_n, double* arena, double* eps, int* perm, double* tri, bool a_in_lds);
 
 void launch_density(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs,
                     const JobState* state, double* arena, const int* perm, double* rmspart);

@@ -32,6 +32,17 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Neumaier's compensated summation: the energies are differences of sums of
+ * order 1e5 Hartree over up to millions of terms, which plain summation
+ * resolves only to ~1e-12 relative. */
+static void nsum_add(double* s, double* c, double x)
+{
+    const double t = *s + x;
+    if (fabs(*s) >= fabs(x)) *c += (*s - t) + x;
+    else *c += (x - t) + *s;
+    *s = t;
+}
+
 #define ORC_EV2AU      0.03674903
 #define ORC_KCAL2AU    0.00159360175
 static const double ORC_ANG2AU = 1.0 / 0.5291772;
@@ -525,7 +536,7 @@ void* orc_create(int theory, int natoms, const int* Z, const double* xyz_bohr, i
         }
 
     /* core repulsion, Cndo2.cpp:406-416 -> Am1.cpp:145-170 -> Mndo.cpp:223-240, 319-327 */
-    double e = 0.0;
+    double e = 0.0, ec = 0.0;
     for (int a = 0; a < natoms; a++)
         for (int b = a + 1; b < natoms; b++) {
             const orc_elem *A = &m->el[a], *B = &m->el[b];
@@ -546,9 +557,9 @@ void* orc_create(int theory, int natoms, const int* Z, const double* xyz_bohr, i
                 }
                 term += A->core * B->core * t * ORC_ANG2AU / r;
             }
-            e += term;
+            nsum_add(&e, &ec, term);
         }
-    m->ecore = e;
+    m->ecore = e + ec;
     return m;
 }
 
@@ -664,11 +675,11 @@ double orc_elec_energy(void* h, const double* P)
     double* Z = (double*)calloc((size_t)n * n, sizeof(double));
     orc_fock(h, P, 0, F);
     orc_fock(h, Z, 0, H);
-    double e = 0.0;
+    double e = 0.0, ec = 0.0;
     for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) e += P[(size_t)j * n + i] * (F[(size_t)i * n + j] + H[(size_t)i * n + j]);
+        for (int j = 0; j < n; j++) nsum_add(&e, &ec, P[(size_t)j * n + i] * (F[(size_t)i * n + j] + H[(size_t)i * n + j]));
     free(F); free(H); free(Z);
-    return 0.5 * e;
+    return 0.5 * (e + ec);
 }
 
 /* MNDO heat of formation minus E_total: sum_A (dHf_A - E_isol_A) (Mndo.cpp:395-404). */

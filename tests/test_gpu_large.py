@@ -123,3 +123,57 @@ def test_large_alone_equals_batch(engine, engine_dw, which):
         assert alone["status"] == 0
         assert abs(alone["energy"] - r["energy"]) * EV <= 1e-9
         assert alone["iterations"] == r["iterations"]
+
+
+def _large_vectors():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "large_vectors.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def test_water256_matches_oracle_vectors(engine):
+    """(H2O)_256 AM1, 1536 AOs (the single-molecule FP64-vs-mixed workload)
+    against the oracle's converged SCF (tests/golden/make_large_vectors.py):
+    FP64 within 1e-6 eV total energy, the same iteration count and RMS
+    trajectory, orbital energies within 1e-6 eV, density diagonal within 1e-8;
+    mixed precision within 1e-5 eV."""
+    from molds_amd import workloads as W
+    print("(H2O)_256 vs oracle vectors", flush=True)
+    ref = _large_vectors()["water256_am1"]
+    Z, x = W.water_cluster(256, seed=1, jitter=0.0)
+    xyz = W.to_bohr(x)
+    r = engine.scf("am1", Z, xyz, precision="fp64", **SCF)
+    rm = engine.scf("am1", Z, xyz, precision="mixed", want_C=False, want_P=False, **SCF)
+    assert r["status"] == 0 and rm["status"] == 0
+    assert abs(r["energy"] - ref["energy"]) * EV <= 1e-6
+    assert abs(r["e_core"] - ref["e_core"]) * EV <= 1e-9
+    assert r["iterations"] == ref["iterations"]
+    traj = np.array([[t[1], t[2]] for t in r["traj"]])
+    assert np.allclose(traj, np.array(ref["traj"]), rtol=1e-6, atol=1e-10)
+    assert np.max(np.abs(r["eps"] - np.array(ref["eps"]))) * EV <= 1e-6
+    P = r["P"]
+    assert np.max(np.abs(np.diag(P) - np.array(ref["diag_P"]))) <= 1e-8
+    assert abs(np.trace(P) - ref["trace_P"]) <= 1e-8
+    assert abs(np.linalg.norm(P) - ref["frob_P"]) <= 1e-8
+    assert abs(rm["energy"] - ref["energy"]) * EV <= 1e-5
+
+
+def test_water1000_first_iterations_match_oracle(engine):
+    """(H2O)_1000 PM3, 6000 AOs (device-wide eigensolver, stored integrals):
+    the first four iterations' density RMS and DIIS error against the oracle's
+    (this geometry does not converge under MolDS's DIIS / damping, in the
+    oracle either), plus tr P = n_elec and P^2 = 2P on the last density."""
+    from molds_amd import workloads as W
+    print("(H2O)_1000 first iterations vs oracle vectors", flush=True)
+    ref = _large_vectors()["water1000_pm3_first4"]
+    Z, x = W.water_cluster(1000, seed=4, jitter=0.0)
+    r = engine.scf("pm3", Z, W.to_bohr(x), precision="fp64", want_C=False, **dict(SCF, max_iter=4))
+    assert r["status"] == -1 and r["iterations"] == 4
+    traj = np.array([[t[1], t[2]] for t in r["traj"]])
+    assert np.allclose(traj, np.array(ref["traj"]), rtol=1e-9, atol=1e-12)
+    P = r["P"]
+    assert abs(np.trace(P) - 8000.0) <= 1e-7
+    Ps = P[:400]
+    assert np.max(np.abs(Ps @ P - 2.0 * Ps)) <= 1e-9
