@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
     const bool scf = phase == PH_ACTIVE;   // else the final-density Fock of the energy
     if (!((mode != 1 && scf) || (mode != 0 && phase == PH_ENERGY))) return;
     const JobDev jb = jobs[at.job];
-    const ElemDev E = elems[at.elem];
+    const ElemDev& E = elems[at.elem];
     const int n = jb.nao, N = jb.natoms, A = at.local, na = E.nao, np = na == 4 ? 10 : 1;
     const bool guess = scf && iter == 0 && jb.requires_guess;
     double* F = mat(arena, jb, AR_F);
@@ -238,22 +238,27 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
         return;
     }
     double acc[10];
+#pragma unroll
     for (int i = 0; i < 10; ++i) acc[i] = 0.0;
     for (int Bl = lane; Bl < N; Bl += 64) {
         if (Bl == A) continue;
         const double* p = A < Bl ? part + (jb.pair0 + tri_slot(A, Bl, N)) * 20
                                  : part + (jb.pair0 + tri_slot(Bl, A, N)) * 20 + 10;
-        for (int i = 0; i < np; ++i) acc[i] += p[i];
+#pragma unroll
+        for (int i = 0; i < 10; ++i)
+            if (i < np) acc[i] += p[i];
     }
-    for (int i = 0; i < np; ++i)
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
         for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off);
     if (lane < 16) {
         int mu = lane >> 2, nu = lane & 3;
         if (mu < na && nu < na) {
             int pi = pk_index(mu, nu);
             double j = 0.0;
-            for (int i = 0; i < np; ++i)
-                if (i == pi) j = acc[i];
+#pragma unroll
+            for (int i = 0; i < 10; ++i)
+                if (i == pi && i < np) j = acc[i];
             const long long base = (long long)at.ao * n + at.ao;
             double g;
             if (mu == nu) {
@@ -554,6 +559,7 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
     __shared__ double sh_rms;
     __shared__ double red[NDDO_MAX_DIIS + 1][256];
     __shared__ double coef[NDDO_MAX_DIIS + 1];
+    __shared__ double sa[(NDDO_MAX_DIIS + 1) * (NDDO_MAX_DIIS + 1)], srhs[NDDO_MAX_DIIS + 1];
     __shared__ int diis_on, damp_on;
     if (tid == 0) {
         double s = 0.0;
@@ -597,13 +603,23 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
             const int rs = (st->ring_start + 1) % k, slot_new = (rs + k - 1) % k;
             double* RP = mat(arena, jb, AR_RING);
             double* RE = RP + (long long)k * nn;
+            // per-slot accumulators and ring offsets with compile-time indices, so
+            // they stay in registers (a runtime-indexed array would live in scratch)
             double part[NDDO_MAX_DIIS];
-            for (int i = 0; i < k; ++i) part[i] = 0.0;
+            long long roff[NDDO_MAX_DIIS];
+#pragma unroll
+            for (int i = 0; i < NDDO_MAX_DIIS; ++i) {
+                part[i] = 0.0;
+                const int sl = i < k ? (rs + i) % k : 0;
+                roff[i] = sl == slot_new ? -1 : (long long)sl * nn;
+            }
             double amax = 0.0;
             if (xc) {   // chunk partials of k_xl_dots, fixed order per thread
                 for (int c = tid; c < nchunk; c += T) {
                     const double* rec = xlpart + ((long long)(jb.xl - 1) * nchunk + c) * XL_REC;
-                    for (int i = 0; i < k; ++i) part[i] += rec[i];
+#pragma unroll
+                    for (int i = 0; i < NDDO_MAX_DIIS; ++i)
+                        if (i < k) part[i] += rec[i];
                     amax = fmax(amax, rec[k]);
                 }
             }
@@ -612,13 +628,17 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
                 RP[(long long)slot_new * nn + e] = pv;
                 RE[(long long)slot_new * nn + e] = ev;
                 amax = fmax(amax, fabs(ev));
-                for (int i = 0; i < k; ++i) {
-                    const int sl = (rs + i) % k;
-                    const double x = sl == slot_new ? ev : RE[(long long)sl * nn + e];
-                    part[i] += x * ev;
+#pragma unroll
+                for (int i = 0; i < NDDO_MAX_DIIS; ++i) {
+                    if (i < k) {
+                        const double x = roff[i] < 0 ? ev : RE[roff[i] + e];
+                        part[i] += x * ev;
+                    }
                 }
             }
-            for (int i = 0; i < k; ++i) red[i][tid] = part[i];
+#pragma unroll
+            for (int i = 0; i < NDDO_MAX_DIIS; ++i)
+                if (i < k) red[i][tid] = part[i];
             red[k][tid] = amax;
             __syncthreads();
             for (int s = T / 2; s > 0; s >>= 1) {
@@ -644,13 +664,12 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
                 st->diis_error = err;
                 int on = 0;
                 if (k <= iter && jb.diis_end < err && err < jb.diis_start) {
-                    double a[(NDDO_MAX_DIIS + 1) * (NDDO_MAX_DIIS + 1)], rhs[NDDO_MAX_DIIS + 1];
                     for (int i = 0; i <= k; ++i) {
-                        rhs[i] = i == k ? -1.0 : 0.0;
-                        for (int j = 0; j <= k; ++j) a[i * (k + 1) + j] = B[i * ld + j];
+                        srhs[i] = i == k ? -1.0 : 0.0;
+                        for (int j = 0; j <= k; ++j) sa[i * (k + 1) + j] = B[i * ld + j];
                     }
-                    on = small_solve(k + 1, a, rhs) ? 1 : 0;
-                    for (int i = 0; i <= k; ++i) coef[i] = rhs[i];
+                    on = small_solve(k + 1, sa, srhs) ? 1 : 0;
+                    for (int i = 0; i <= k; ++i) coef[i] = srhs[i];
                 }
                 diis_on = on;
                 st->diis_on = on;

@@ -70,11 +70,18 @@ __device__ double multipole_pair(const ElemDev& A, const ElemDev& B, int ta, int
     int ka = mp_kind(ta), kb = mp_kind(tb);
     double a = A.rho[ka] + B.rho[kb];
     double a2 = a * a;
+    // four charge slots each, unused ones zero: the loops then have fixed trip
+    // counts (registers, not scratch) and the zero terms add exact zeros
     Charge ca[4], cb[4];
-    int na = charges(ta, A.D[ka], ca), nb = charges(tb, B.D[kb], cb);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ca[i] = cb[i] = Charge{0.0, 0.0, 0.0, 0.0};
+    const int na = charges(ta, A.D[ka], ca), nb = charges(tb, B.D[kb], cb);
     double v = 0.0;
-    for (int i = 0; i < na; ++i)
-        for (int j = 0; j < nb; ++j) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (i >= na || j >= nb) continue;
             double dx = cb[j].x - ca[i].x, dy = cb[j].y - ca[i].y, dz = r + cb[j].z - ca[i].z;
             v += ca[i].q * cb[j].q / sqrt(dz * dz + dx * dx + dy * dy + a2);
         }
@@ -191,7 +198,9 @@ __global__ void __launch_bounds__(256) k_pair_setup(
     const bool live = pidx < npairs;
     PairDev pr = live ? pairs[pidx] : pairs[0];
     const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
-    const ElemDev A = elems[aA.elem], B = elems[aB.elem];
+    // parameters read in place: a private copy indexed at run time would live in scratch
+    const ElemDev& A = elems[aA.elem];
+    const ElemDev& B = elems[aB.elem];
     const int nap = A.nao == 4 ? 10 : 1, nbp = B.nao == 4 ? 10 : 1;
     const double dx = aB.x - aA.x, dy = aB.y - aA.y, dz = aB.z - aA.z;
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -201,11 +210,7 @@ __global__ void __launch_bounds__(256) k_pair_setup(
 
     // local-frame integrals and the packed rotation T[i][k]
     for (int e = lane; e < nap * nbp; e += 64) L[e] = local_integral(A, B, e / nbp, e % nbp, r);
-    if (lane < 16) {
-        double R[4][4];
-        rotation(dx, dy, dz, R);
-        sR[wave][lane] = R[lane >> 2][lane & 3];
-    }
+    if (lane == 0) rotation(dx, dy, dz, reinterpret_cast<double(*)[4]>(sR[wave]));
     __syncthreads();
     for (int e = lane; e < 100; e += 64) {
         int i = e / 10, k = e % 10;
