@@ -71,12 +71,14 @@ def batched_leg(engine, jobs, steps, warmup):
         engine.scf_batch(jobs)
     engine.profile(True)
     engine.stage_times(reset=True)
+    engine.host_times(reset=True)
     t0 = time.perf_counter()
     res = None
     for _ in range(steps):
         res = engine.scf_batch(jobs)
     wall = time.perf_counter() - t0
     st = engine.stage_times(reset=True)
+    log(f"host phases: {engine.host_times(reset=True)}")
     engine.profile(False)
     return res, wall, st
 

@@ -194,6 +194,10 @@ int nddo_dbg_eigh_dc_phases(nddo_engine* e, double* ticks7);
 int nddo_engine_tuning(nddo_engine* e, int* fock_wpb, int* bt_rows4);
 /* Test hook: force the launch shapes. */
 int nddo_dbg_set_tuning(nddo_engine* e, int fock_wpb, const int* bt_rows4);
+/* Test hook: seconds nddo_scf_batch spent so far in its host phases: build and
+ * classify, device allocation and upload, the SCF loop, result copies, and
+ * returning the batch's device memory. */
+int nddo_dbg_host_times(nddo_engine* e, double* t5, int reset);
 int nddo_dbg_graph_stats(nddo_engine* e, long long* captured, long long* replays);
 
 #ifdef __cplusplus

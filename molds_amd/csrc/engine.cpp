@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "nddo.h"
@@ -119,7 +120,7 @@ void Batch::release() {
     for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
                      (void**)&d_ints, (void**)&d_ints32, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
                      (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
-                     (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_list_xl, (void**)&d_xl_off,
+                     (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_plist, (void**)&d_list_xl, (void**)&d_xl_off,
                      (void**)&d_xlws, (void**)&d_xlpart, (void**)&d_xlcoef, (void**)&d_bar, (void**)&d_sdbg}) {
         if (*p) {
             if (alloc_stream) hipFreeAsync(*p, alloc_stream);
@@ -195,66 +196,64 @@ size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter) {
     return b;
 }
 
+// Two passes: the per-job sizes and offsets (sequential, cheap), then the
+// atom, pair, tile and pair-class records of the jobs in parallel threads.
 int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* elem_index_base) {
     (void)elem_index_base;
+    (void)nd;
     atoms.clear(); pairs.clear(); jobs.clear(); tiles.clear(); list_lds.clear(); list_glb.clear(); list_xl.clear();
-    xl_off.clear();
+    xl_off.clear(); plist.clear();
     src = which;
     nslots = 0; nints = 0; arena_size = 0; nvec = 0; ntraj = 0;
     maxn_lds = 0; maxn_glb = 0; maxn_xl = 0; max_iter_all = 0;
-    for (int jj = 0; jj < (int)which.size(); ++jj) {
+    const int nj = (int)which.size();
+    struct Off { long long atom0, pair0, ints0, tile0, cls[3]; };
+    std::vector<Off> off(nj);
+    long long natoms_all = 0, ntiles_all = 0, ncls[3] = {0, 0, 0};
+    jobs.resize(nj);
+    for (int jj = 0; jj < nj; ++jj) {
         const nddo_job_desc& d = descs[which[jj]];
-        (void)nd;
-        JobDev jb;
+        JobDev& jb = jobs[jj];
         std::memset(&jb, 0, sizeof(jb));
-        jb.natoms = d.natoms;
-        jb.theory = d.theory;
-        jb.atom0 = (int)atoms.size();
+        long long nh = 0, nx = 0;
         int n = 0, ne = 0;
         for (int a = 0; a < d.natoms; ++a) {
-            AtomDev at;
-            at.x = d.xyz_bohr[3 * a];
-            at.y = d.xyz_bohr[3 * a + 1];
-            at.z = d.xyz_bohr[3 * a + 2];
-            at.elem = d.theory * kNumElem + elem_slot(d.Z[a]);
-            at.job = jj;
-            at.ao = n;
-            at.local = a;
-            ElemDev e;
-            elem_params(d.Z[a], d.theory, &e);
-            n += e.nao;
-            ne += (int)e.core;
-            atoms.push_back(at);
+            const int z = d.Z[a];
+            const bool h = z == 1;
+            nh += h;
+            nx += !h;
+            n += h ? 1 : 4;
+            ne += h ? 1 : z - 2;   // valence electrons: H 1, C 4, N 5, O 6, F 7
         }
+        jb.natoms = d.natoms;
+        jb.theory = d.theory;
+        jb.atom0 = (int)natoms_all;
         jb.nao = n;
         jb.nocc = ne / 2;
         jb.npairs = d.natoms * (d.natoms - 1) / 2;
         jb.pair0 = nslots;
-        for (int a = 0; a < d.natoms; ++a)
-            for (int b = a + 1; b < d.natoms; ++b) {
-                PairDev pr;
-                pr.a = jb.atom0 + a;
-                pr.b = jb.atom0 + b;
-                pr.job = jj;
-                pr.slot = (int)(nslots + (long long)a * d.natoms - (long long)a * (a + 1) / 2 + (b - a - 1));
-                pr.ioff = nints;
-                int na = atoms[pr.a].elem % kNumElem == 0 ? 1 : 10;
-                int nb = atoms[pr.b].elem % kNumElem == 0 ? 1 : 10;
-                nints += (long long)na * nb;
-                pairs.push_back(pr);
-            }
+        off[jj].atom0 = natoms_all;
+        off[jj].pair0 = nslots;
+        off[jj].ints0 = nints;
+        off[jj].tile0 = ntiles_all;
+        const long long c3[3] = {nh * (nh - 1) / 2, nh * nx, nx * (nx - 1) / 2};
+        for (int c = 0; c < 3; ++c) {
+            off[jj].cls[c] = ncls[c];
+            ncls[c] += c3[c];
+        }
+        natoms_all += d.natoms;
         nslots += jb.npairs;
+        nints += c3[0] + 10 * c3[1] + 100 * c3[2];
         jb.diis_k = d.opts.diis_num_error_vect;
         jb.arena = arena_size;
         arena_size += (long long)(ARENA_FIXED + 2 * jb.diis_k) * n * n;
         jb.vec0 = nvec;
         jb.ivec0 = nvec;
         nvec += n;
-        jb.tile0 = (int)tiles.size();
+        jb.tile0 = (int)ntiles_all;
         jb.tiles_per_dim = (n + 63) / 64;
-        for (int i0 = 0; i0 < n; i0 += 64)
-            for (int j0 = 0; j0 < n; j0 += 64) tiles.push_back({jj, i0, j0, 0});
-        jb.ntiles = (int)tiles.size() - jb.tile0;
+        jb.ntiles = jb.tiles_per_dim * jb.tiles_per_dim;
+        ntiles_all += jb.ntiles;
         jb.max_iter = d.opts.max_iter;
         jb.requires_guess = d.requires_guess ? 1 : 0;
         jb.precision = d.precision;
@@ -266,7 +265,64 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
         jb.traj0 = ntraj;
         ntraj += 4LL * d.opts.max_iter;
         max_iter_all = std::max(max_iter_all, d.opts.max_iter);
-        jobs.push_back(jb);
+    }
+    atoms.resize(natoms_all);
+    pairs.resize(nslots);
+    tiles.resize(ntiles_all);
+    plist.resize(nslots);
+    n_light = (int)(ncls[0] + ncls[1]);
+    n_heavy = (int)ncls[2];
+    const long long cbase[3] = {0, ncls[0], ncls[0] + ncls[1]};
+    auto fill = [&](int j0, int j1) {
+        for (int jj = j0; jj < j1; ++jj) {
+            const nddo_job_desc& d = descs[which[jj]];
+            const JobDev& jb = jobs[jj];
+            const Off& o = off[jj];
+            int ao = 0;
+            for (int a = 0; a < d.natoms; ++a) {
+                AtomDev& at = atoms[o.atom0 + a];
+                at.x = d.xyz_bohr[3 * a];
+                at.y = d.xyz_bohr[3 * a + 1];
+                at.z = d.xyz_bohr[3 * a + 2];
+                at.elem = d.theory * kNumElem + elem_slot(d.Z[a]);
+                at.job = jj;
+                at.ao = ao;
+                at.local = a;
+                ao += d.Z[a] == 1 ? 1 : 4;
+            }
+            long long io = o.ints0, slot = o.pair0, cpos[3] = {cbase[0] + o.cls[0], cbase[1] + o.cls[1],
+                                                               cbase[2] + o.cls[2]};
+            for (int a = 0; a < d.natoms; ++a) {
+                const bool ha = d.Z[a] == 1;
+                for (int b = a + 1; b < d.natoms; ++b, ++slot) {
+                    const bool hb = d.Z[b] == 1;
+                    PairDev& pr = pairs[slot];
+                    pr.a = (int)(o.atom0 + a);
+                    pr.b = (int)(o.atom0 + b);
+                    pr.job = jj;
+                    pr.slot = (int)slot;
+                    pr.ioff = io;
+                    io += (long long)(ha ? 1 : 10) * (hb ? 1 : 10);
+                    const int c = (ha && hb) ? 0 : ((ha || hb) ? 1 : 2);
+                    plist[cpos[c]++] = (int)slot;
+                }
+            }
+            long long t = o.tile0;
+            for (int i0 = 0; i0 < jb.nao; i0 += 64)
+                for (int j0 = 0; j0 < jb.nao; j0 += 64) tiles[t++] = {jj, i0, j0, 0};
+        }
+    };
+    const int nthreads = nj < 64 ? 1 : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (nthreads <= 1) {
+        fill(0, nj);
+    } else {
+        std::vector<std::thread> th;
+        const int per = (nj + nthreads - 1) / nthreads;
+        for (int t = 0; t < nthreads; ++t) {
+            const int j0 = t * per, j1 = std::min(nj, j0 + per);
+            if (j0 < j1) th.emplace_back(fill, j0, j1);
+        }
+        for (auto& x : th) x.join();
     }
     return NDDO_OK;
 }
@@ -322,6 +378,7 @@ int Batch::allocate(bool want_sdbg) {
     const size_t nj = std::max<size_t>(jobs.size(), 1);
     ACHK(dalloc((void**)&d_atoms, std::max<size_t>(atoms.size(), 1) * sizeof(AtomDev)));
     ACHK(dalloc((void**)&d_pairs, std::max<size_t>(pairs.size(), 1) * sizeof(PairDev)));
+    ACHK(dalloc((void**)&d_plist, std::max<size_t>(plist.size(), 1) * sizeof(int)));
     ACHK(dalloc((void**)&d_jobs, nj * sizeof(JobDev)));
     ACHK(dalloc((void**)&d_state, nj * sizeof(JobState)));
     ACHK(dalloc((void**)&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(TileDev)));
@@ -358,7 +415,8 @@ int Batch::allocate(bool want_sdbg) {
 // exactly one of the three eigensolver lists.
 size_t Batch::upload_bytes() const {
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return up(atoms.size() * sizeof(AtomDev)) + up(pairs.size() * sizeof(PairDev)) + up(jobs.size() * sizeof(JobDev)) +
+    return up(atoms.size() * sizeof(AtomDev)) + up(pairs.size() * sizeof(PairDev)) + up(plist.size() * sizeof(int)) +
+           up(jobs.size() * sizeof(JobDev)) +
            up(tiles.size() * sizeof(TileDev)) + 3 * up(jobs.size() * sizeof(int)) +
            up(jobs.size() * sizeof(long long)) + up(jobs.size() * sizeof(JobState)) + 256;
 }
@@ -379,6 +437,7 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs, char* stage) {
     };
     HIPCHK(h2d(d_atoms, atoms.data(), atoms.size() * sizeof(AtomDev)));
     HIPCHK(h2d(d_pairs, pairs.data(), pairs.size() * sizeof(PairDev)));
+    HIPCHK(h2d(d_plist, plist.data(), plist.size() * sizeof(int)));
     HIPCHK(h2d(d_jobs, jobs.data(), jobs.size() * sizeof(JobDev)));
     HIPCHK(h2d(d_tiles, tiles.data(), tiles.size() * sizeof(TileDev)));
     HIPCHK(h2d(d_list_lds, list_lds.data(), list_lds.size() * sizeof(int)));
@@ -420,7 +479,7 @@ void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
 // then evaluated (Cndo2.cpp:1324-1416) before the eigensolve of the others.
 void Batch::launch_iteration(Engine* e, hipStream_t st) {
     const int nxl = (int)list_xl.size();
-    launch_fock(st, 2, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
+    launch_fock(st, 2, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
                 d_arena, d_part);
     launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
@@ -499,7 +558,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             // the same launches as launch_iteration, with an event between stages
             // and between the parts of the eigensolver
             HIPCHK(hipEventRecord(e->ev[0], st));
-            launch_fock(st, 2, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
+            launch_fock(st, 2, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
                         d_ints, d_ints32, d_arena, d_part);
             HIPCHK(hipEventRecord(e->ev[9], st));
             launch_xl_energy(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
@@ -586,34 +645,43 @@ void order_and_phase(int n, const double* VT, const int* perm, double* C) {
 }
 
 // Copies job jj's results into res (status already set by the caller on failure).
-int Batch::fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* res, double hof_offset) {
-    const JobDev& jb = jobs[jj];
-    JobState s;
-    HIPCHK(hipMemcpyAsync(&s, d_state + jj, sizeof(JobState), hipMemcpyDeviceToHost, st));
+// All jobs' results with bulk copies of the scalar state, eigenvalues and
+// orbital order; per-job copies only for the requested arrays (C, P, trajectory).
+int Batch::fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* const* res, const double* hof_offsets) {
+    const size_t nj = jobs.size();
+    std::vector<JobState> s(nj);
+    std::vector<double> eps(std::max<long long>(nvec, 1));
+    std::vector<int> perm(std::max<long long>(nvec, 1));
+    HIPCHK(hipMemcpyAsync(s.data(), d_state, nj * sizeof(JobState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(eps.data(), d_eps, nvec * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(perm.data(), d_perm, nvec * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const int n = jb.nao;
-    fill_scalars(jb, s, d.theory, hof_offset, res);
-    std::vector<double> eps(n);
-    std::vector<int> perm(n);
-    HIPCHK(hipMemcpyAsync(eps.data(), d_eps + jb.vec0, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(perm.data(), d_perm + jb.ivec0, n * sizeof(int), hipMemcpyDeviceToHost, st));
-    const double* base = d_arena + jb.arena;
-    const long long nn = (long long)n * n;
-    std::vector<double> VT, P;
-    if (res->C) {
-        VT.resize(nn);
-        HIPCHK(hipMemcpyAsync(VT.data(), base + (long long)AR_C * nn, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<std::vector<double>> VT(nj);
+    for (size_t jj = 0; jj < nj; ++jj) {
+        const JobDev& jb = jobs[jj];
+        const nddo_job_desc& d = descs[src[jj]];
+        nddo_result* r = res[jj];
+        fill_scalars(jb, s[jj], d.theory, hof_offsets[jj], r);
+        const int n = jb.nao;
+        if (r->eps) std::memcpy(r->eps, eps.data() + jb.vec0, n * sizeof(double));
+        const double* base = d_arena + jb.arena;
+        const long long nn = (long long)n * n;
+        if (r->C) {
+            VT[jj].resize(nn);
+            HIPCHK(hipMemcpyAsync(VT[jj].data(), base + (long long)AR_C * nn, nn * sizeof(double), hipMemcpyDeviceToHost,
+                                  st));
+        }
+        if (r->P) {
+            const int pk = AR_P0 + ((s[jj].iter + 1) & 1);
+            HIPCHK(hipMemcpyAsync(r->P, base + (long long)pk * nn, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        if (r->trajectory)
+            HIPCHK(hipMemcpyAsync(r->trajectory, d_traj + jb.traj0, 4LL * jb.max_iter * sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
     }
-    if (res->P) {
-        const int pk = AR_P0 + ((s.iter + 1) & 1);
-        HIPCHK(hipMemcpyAsync(res->P, base + (long long)pk * nn, nn * sizeof(double), hipMemcpyDeviceToHost, st));
-    }
-    if (res->trajectory)
-        HIPCHK(hipMemcpyAsync(res->trajectory, d_traj + jb.traj0, 4LL * jb.max_iter * sizeof(double),
-                              hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    if (res->eps) std::memcpy(res->eps, eps.data(), n * sizeof(double));
-    if (res->C) order_and_phase(n, VT.data(), perm.data(), res->C);
+    for (size_t jj = 0; jj < nj; ++jj)
+        if (res[jj]->C) order_and_phase(jobs[jj].nao, VT[jj].data(), perm.data() + jobs[jj].ivec0, res[jj]->C);
     return NDDO_OK;
 }
 
@@ -718,24 +786,43 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
         if (s == NDDO_OK) ok.push_back(i);
     }
     if (ok.empty()) return NDDO_OK;
-    Batch b;
-    b.build(descs, njobs, ok, nullptr);
-    b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
-    int rc = b.allocate(false);
-    if (rc) return rc;
-    rc = b.upload(e->stream, descs);
-    if (rc) return rc;
-    rc = b.run_scf(e, e->stream);
-    if (rc) return rc;
-    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    using Clock = std::chrono::steady_clock;
+    auto lap = [](Clock::time_point& t) {
+        const auto now = Clock::now();
+        const double dt = std::chrono::duration<double>(now - t).count();
+        t = now;
+        return dt;
+    };
+    Clock::time_point tp = Clock::now();
+    int rc;
+    {
+        Batch b;
+        b.build(descs, njobs, ok, nullptr);
+        b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+        e->host_s[0] += lap(tp);
+        rc = b.allocate(false);
+        if (rc) return rc;
+        rc = b.upload(e->stream, descs);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(e->stream));
+        e->host_s[1] += lap(tp);
+        rc = b.run_scf(e, e->stream);
+        if (rc) return rc;
+        e->host_s[2] += lap(tp);
+        const double wall = std::chrono::duration<double>(Clock::now() - t0).count();
+    std::vector<nddo_result*> rp(ok.size());
+    std::vector<double> hof(ok.size(), 0.0);
     for (int jj = 0; jj < (int)ok.size(); ++jj) {
         const nddo_job_desc& d = descs[ok[jj]];
-        double hof = 0.0;
-        for (int a = 0; a < d.natoms; ++a) hof += mndo_atom_hof_offset(d.Z[a]);
-        rc = b.fetch(e->stream, jj, d, &res[ok[jj]], hof);
-        if (rc) return rc;
-        res[ok[jj]].latency_s = wall;
+        for (int a = 0; a < d.natoms; ++a) hof[jj] += mndo_atom_hof_offset(d.Z[a]);
+        rp[jj] = &res[ok[jj]];
     }
+        rc = b.fetch_all(e->stream, descs, rp.data(), hof.data());
+        if (rc) return rc;
+        for (int jj = 0; jj < (int)ok.size(); ++jj) res[ok[jj]].latency_s = wall;
+        e->host_s[3] += lap(tp);
+    }
+    e->host_s[4] += lap(tp);   // the batch's device memory returned
     return NDDO_OK;
 }
 
@@ -875,7 +962,7 @@ int nddo_dbg_fock(nddo_engine* eh, int theory, int natoms, const int* Z, const d
     }
     if (P) HIPCHK(hipMemcpy(b.d_arena + (long long)AR_P0 * nn, P, nn * sizeof(double), hipMemcpyHostToDevice));
     b.launch_setup_stage(e, e->stream);
-    launch_fock(e->stream, 0, b.d_pairs, (int)b.pairs.size(), b.d_atoms, (int)b.atoms.size(), e->d_elems, b.d_jobs,
+    launch_fock(e->stream, 0, b.d_pairs, b.d_plist, b.n_light, b.n_heavy, b.d_atoms, (int)b.atoms.size(), e->d_elems, b.d_jobs,
                 b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -1003,7 +1090,7 @@ static int autotune(Engine* e) {
             float ms = 0.f;
             rc = time_ms(e, [&] {
                 for (int r = 0; r < 4; ++r)
-                    launch_fock(e->stream, 0, b.d_pairs, (int)b.pairs.size(), b.d_atoms, (int)b.atoms.size(),
+                    launch_fock(e->stream, 0, b.d_pairs, b.d_plist, b.n_light, b.n_heavy, b.d_atoms, (int)b.atoms.size(),
                                 e->d_elems, b.d_jobs, b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
             }, 3, &ms);
             if (rc != NDDO_OK) return rc;
@@ -1131,6 +1218,16 @@ int nddo_dbg_eigh_profile(nddo_engine* eh, double* ticks4, double* tick_hz) {
         int khz = 0;
         hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device);
         *tick_hz = 1e3 * khz;
+    }
+    return NDDO_OK;
+}
+
+int nddo_dbg_host_times(nddo_engine* eh, double* t5, int reset) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !t5) return NDDO_BAD_ARG;
+    for (int i = 0; i < 5; ++i) {
+        t5[i] = e->host_s[i];
+        if (reset) e->host_s[i] = 0.0;
     }
     return NDDO_OK;
 }

@@ -40,6 +40,9 @@ struct Batch {
     std::vector<JobDev> jobs;
     std::vector<TileDev> tiles;
     std::vector<int> list_lds, list_glb, list_xl, src;
+    // pair indices: the light (H-H, heavy-H) pairs first, then the heavy-heavy ones
+    std::vector<int> plist;
+    int n_light = 0, n_heavy = 0;
     std::vector<long long> xl_off;   // per list_xl entry: offset in d_xlws
     int maxn_lds = 0, maxn_glb = 0, maxn_xl = 0, max_iter_all = 0;
     int xl_grid = 1, xl_chunks = 1;
@@ -64,6 +67,7 @@ struct Batch {
     int* d_remaining = nullptr;
     int* d_list_lds = nullptr;
     int* d_list_glb = nullptr;
+    int* d_plist = nullptr;
     // large molecules (kernels_xl.hip)
     int* d_list_xl = nullptr;
     long long* d_xl_off = nullptr;
@@ -94,7 +98,7 @@ struct Batch {
     int iterations_launched = 0;
     bool graph_off = false;
     int run_scf(Engine* e, hipStream_t st);
-    int fetch(hipStream_t st, int jj, const nddo_job_desc& d, nddo_result* res, double hof_offset);
+    int fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* const* res, const double* hof_offsets);
 };
 
 struct ServedJob;
@@ -122,6 +126,8 @@ struct Engine {
     hipEvent_t xl_chain = nullptr;
     double last_eig_prof[4] = {};
     double last_dc_phase[9] = {};
+    // nddo_scf_batch host phases (s): build + classify, allocate + upload, SCF, results, release
+    double host_s[5] = {};
 
     // serving state (server.cpp)
     std::mutex smu;

@@ -66,10 +66,10 @@ __device__ inline double h1c(const ElemDev& E, int a, int b) {
 // work does not depend on it.
 template <int WPB>
 __global__ void __launch_bounds__(64 * WPB) k_fock_pairs(
-    int mode, const PairDev* __restrict__ pairs, int npairs, const AtomDev* __restrict__ atoms,
-    const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs, const JobState* __restrict__ state,
-    const double* __restrict__ ints, const float* __restrict__ ints32, double* __restrict__ arena,
-    double* __restrict__ part) {
+    int mode, const PairDev* __restrict__ pairs, const int* __restrict__ plist, int npairs,
+    const AtomDev* __restrict__ atoms, const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs,
+    const JobState* __restrict__ state, const double* __restrict__ ints, const float* __restrict__ ints32,
+    double* __restrict__ arena, double* __restrict__ part) {
     __shared__ double sI[WPB][100], sPab[WPB][16], sWa[WPB][10], sWb[WPB][10];
     __shared__ float fI[WPB][100], fPab[WPB][16], fWa[WPB][10], fWb[WPB][10];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(64 * WPB) k_fock_pairs(
     JobDev jb;
     int phase = -1, iter = 0;
     if (act) {
-        pr = pairs[pidx];
+        pr = pairs[plist[pidx]];
         jb = jobs[pr.job];
         phase = state[pr.job].phase;
         iter = state[pr.job].iter;
@@ -208,6 +208,147 @@ __global__ void __launch_bounds__(64 * WPB) k_fock_pairs(
     }
 }
 
+// H-H and heavy-H pairs (1 and 10 packed integrals, about 80% of the pairs of
+// an organic or water molecule), one lane per pair: the same arithmetic, in
+// the same order, as k_fock_pairs, whose wavefront per pair these pairs would
+// leave mostly idle.
+template <typename R>
+__device__ inline R wpack(const double* P, int n, int ao, int q) {   // packed one-centre density, off-diagonal doubled
+    const int mu = q == 0 ? 0 : (q < 4 ? 0 : (q < 7 ? 1 : (q < 9 ? 2 : 3)));
+    const int nu = q < 4 ? q : (q < 7 ? q - 3 : (q < 9 ? q - 5 : 3));
+    return (R)((mu == nu ? 1.0 : 2.0) * P[(long long)(ao + mu) * n + ao + nu]);
+}
+
+// The three light cases spelled out, so that every array index is a constant
+// (registers, no scratch).  Sums start from zero and run in the order of
+// k_fock_pairs.
+template <typename R>
+__device__ inline void fock_light_pair(int nA, int nB, const R* I, const double* P, int n, int aoA, int aoB,
+                                       const double* H, double* F, double* pp, bool guess) {
+    if (nA == 1 && nB == 1) {   // H-H: one integral
+        const long long ij = (long long)aoA * n + aoB;
+        double v = H[ij];
+        if (!guess) {
+            const R wa = wpack<R>(P, n, aoA, 0), wb = wpack<R>(P, n, aoB, 0);
+            R s = (R)0;
+            s += I[0] * wb;
+            pp[0] = (double)s;
+            s = (R)0;
+            s += I[0] * wa;
+            pp[10] = (double)s;
+            R k = (R)0;
+            k += (R)P[ij] * I[0];
+            v -= 0.5 * (double)k;
+        }
+        F[ij] = v;
+        F[(long long)aoB * n + aoA] = v;
+        return;
+    }
+    if (nA == 4) {   // heavy A, hydrogen B: I[q], q packed on A
+        if (!guess) {
+            const R wb = wpack<R>(P, n, aoB, 0);
+            R wa[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) wa[q] = wpack<R>(P, n, aoA, q);
+#pragma unroll
+            for (int q = 0; q < 10; ++q) {
+                R s = (R)0;
+                s += I[q] * wb;
+                pp[q] = (double)s;
+            }
+            R s = (R)0;
+#pragma unroll
+            for (int i = 0; i < 10; ++i) s += I[i] * wa[i];
+            pp[10] = (double)s;
+        }
+        R pab[4];
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) pab[nu] = (R)P[(long long)(aoA + nu) * n + aoB];
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) {
+            const long long ij = (long long)(aoA + mu) * n + aoB;
+            double v = H[ij];
+            if (!guess) {
+                R k = (R)0;
+#pragma unroll
+                for (int nu = 0; nu < 4; ++nu) k += pab[nu] * I[pk_index(mu, nu)];
+                v -= 0.5 * (double)k;
+            }
+            F[ij] = v;
+            F[(long long)aoB * n + aoA + mu] = v;
+        }
+        return;
+    }
+    // hydrogen A, heavy B: I[j], j packed on B
+    if (!guess) {
+        const R wa = wpack<R>(P, n, aoA, 0);
+        R wb[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j) wb[j] = wpack<R>(P, n, aoB, j);
+        R s = (R)0;
+#pragma unroll
+        for (int j = 0; j < 10; ++j) s += I[j] * wb[j];
+        pp[0] = (double)s;
+#pragma unroll
+        for (int j = 0; j < 10; ++j) {
+            R t = (R)0;
+            t += I[j] * wa;
+            pp[10 + j] = (double)t;
+        }
+    }
+    R pab[4];
+#pragma unroll
+    for (int sg = 0; sg < 4; ++sg) pab[sg] = (R)P[(long long)aoA * n + aoB + sg];
+#pragma unroll
+    for (int la = 0; la < 4; ++la) {
+        const long long ij = (long long)aoA * n + aoB + la;
+        double v = H[ij];
+        if (!guess) {
+            R k = (R)0;
+#pragma unroll
+            for (int sg = 0; sg < 4; ++sg) k += pab[sg] * I[pk_index(la, sg)];
+            v -= 0.5 * (double)k;
+        }
+        F[ij] = v;
+        F[(long long)(aoB + la) * n + aoA] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fock_light(
+    int mode, const PairDev* __restrict__ pairs, const int* __restrict__ plist, int nlight,
+    const AtomDev* __restrict__ atoms, const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs,
+    const JobState* __restrict__ state, const double* __restrict__ ints, const float* __restrict__ ints32,
+    double* __restrict__ arena, double* __restrict__ part) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nlight) return;
+    const PairDev pr = pairs[plist[idx]];
+    const JobState& st = state[pr.job];
+    const int phase = st.phase, iter = st.iter;
+    if (!((mode != 1 && phase == PH_ACTIVE) || (mode != 0 && phase == PH_ENERGY))) return;
+    const bool scf = phase == PH_ACTIVE;   // else the final-density Fock of the energy
+    const JobDev& jb = jobs[pr.job];
+    const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
+    const int nA = elems[aA.elem].nao, nB = elems[aB.elem].nao;
+    const bool guess = scf && iter == 0 && jb.requires_guess;
+    const bool f32 = scf && ints32 != nullptr && st.fp32 != 0;
+    const double* P = mat(arena, jb, scf ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1)));
+    double* F = mat(arena, jb, AR_F);
+    const double* H = mat(arena, jb, AR_H);
+    double* pp = part + (long long)pr.slot * 20;
+    const int ni = (nA == 4 ? 10 : 1) * (nB == 4 ? 10 : 1);
+    if (f32 && !guess) {
+        float I[10];
+#pragma unroll
+        for (int e = 0; e < 10; ++e) I[e] = e < ni ? ints32[pr.ioff + e] : 0.0f;
+        fock_light_pair<float>(nA, nB, I, P, jb.nao, aA.ao, aB.ao, H, F, pp, false);
+    } else {
+        double I[10];
+#pragma unroll
+        for (int e = 0; e < 10; ++e) I[e] = (e < ni && !guess) ? ints[pr.ioff + e] : 0.0;
+        fock_light_pair<double>(nA, nB, I, P, jb.nao, aA.ao, aB.ao, H, F, pp, guess);
+    }
+}
+
 // Per atom: F_AA = H_AA + one-centre G(P_AA) + sum_B J partials (fixed-order
 // reduction over B); on the guess iteration F_AA = diag(U) (Mndo.cpp:422-488, 509-514).
 __global__ void __launch_bounds__(256) k_fock_atoms(
@@ -274,13 +415,18 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
     }
 }
 
-void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
-                 const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
-                 const float* ints32, double* arena, double* part) {
-    if (npairs > 0) {
+void launch_fock(hipStream_t st, int mode, const PairDev* pairs, const int* plist, int nlight, int nheavy,
+                 const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state,
+                 const double* ints, const float* ints32, double* arena, double* part) {
+    if (nlight > 0)
+        hipLaunchKernelGGL(k_fock_light, dim3((nlight + 255) / 256), dim3(256), 0, st, mode, pairs, plist, nlight,
+                           atoms, elems, jobs, state, ints, ints32, arena, part);
+    if (nheavy > 0) {
+        const int npairs = nheavy;
+        const int* hl = plist + nlight;
         const int w = get_tuning().fock_wpb;
-#define NDDO_FOCK_PAIRS(W)                                                                                   \
-    hipLaunchKernelGGL(k_fock_pairs<W>, dim3((npairs + (W)-1) / (W)), dim3(64 * (W)), 0, st, mode, pairs, npairs, \
+#define NDDO_FOCK_PAIRS(W)                                                                                       \
+    hipLaunchKernelGGL(k_fock_pairs<W>, dim3((npairs + (W)-1) / (W)), dim3(64 * (W)), 0, st, mode, pairs, hl, npairs, \
                        atoms, elems, jobs, state, ints, ints32, arena, part)
         if (w == 1) NDDO_FOCK_PAIRS(1);
         else if (w == 2) NDDO_FOCK_PAIRS(2);

@@ -30,9 +30,11 @@ void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long lon
 // of the batch is mixed-precision);
 // mode 1: non-guess FP64 Fock of the final density for PH_ENERGY jobs;
 // mode 2: both in one launch (what the SCF loop uses).
-void launch_fock(hipStream_t st, int mode, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
-                 const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
-                 const float* ints32, double* arena, double* part);
+// plist: the pair indices, first the nlight H-H and heavy-H pairs (one lane
+// each), then the nheavy heavy-heavy pairs (one wavefront each).
+void launch_fock(hipStream_t st, int mode, const PairDev* pairs, const int* plist, int nlight, int nheavy,
+                 const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state,
+                 const double* ints, const float* ints32, double* arena, double* part);
 
 // Warm start: F <- VT F VT^T for PH_ACTIVE jobs whose VT holds a previous solve.
 void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, const JobState* state,

@@ -89,6 +89,7 @@ def load_library(path=LIBPATH):
     L.nddo_dbg_eigh_dc_phases.argtypes = [vp, _dp]
     L.nddo_engine_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.nddo_dbg_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.nddo_dbg_host_times.argtypes = [vp, _dp, ctypes.c_int]
     L.nddo_dbg_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     _lib = L
     return L
@@ -335,6 +336,11 @@ class Engine:
         rc = self.L.nddo_dbg_set_tuning(self.h, fock_wpb, rows)
         if rc != 0:
             raise NddoError(rc, last_error())
+
+    def host_times(self, reset=False):
+        t = np.zeros(5)
+        self.L.nddo_dbg_host_times(self.h, _p(t), 1 if reset else 0)
+        return dict(zip(("build_s", "upload_s", "scf_s", "results_s", "release_s"), t.tolist()))
 
     def graph_stats(self):
         c, r = ctypes.c_longlong(), ctypes.c_longlong()
