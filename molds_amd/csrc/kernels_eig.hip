@@ -1,10 +1,13 @@
 // FP64 symmetric eigensolver of the SCF loop (replaces LAPACKE_dsyevd,
 // src/wrappers/Lapack.cpp:102-161) as three batched kernels, one workgroup per
-// molecule each, for matrices up to 512 AOs:
+// molecule each, for matrices up to 1200 AOs:
 //   1. k_tridiag        Householder tridiagonalization T = Q^T F Q (Golub & Van
-//                       Loan, Alg. 8.3.1); each step's rank-2 update is fused
-//                       with the next step's matrix-vector product, so the
-//                       trailing block is read and written once per step.
+//                       Loan, Alg. 8.3.1) with the matrix in LDS (n <= 137);
+//                       each step's rank-2 update is fused with the next step's
+//                       matrix-vector product, so the trailing block is read
+//                       and written once per step.
+//      k_tridiag_sym    the same for matrices in device memory, on the upper
+//                       triangle only.
 //   2. k_dc             divide and conquer on T (Cuppen, Numer. Math. 36, 177
 //                       (1981)), bottom-up from 1x1 leaves, all merges of a
 //                       level at once: deflation as in Dongarra & Sorensen
@@ -71,15 +74,13 @@ constexpr int kHeigMaxN = 1200;
 // ------------------------------------------------------------------ 1. tridiagonalize
 // Reflector k acts on indices k+1..n-1: H_k = I - tau_k v v^T with v[0] = 1,
 // kept in row k of W (columns k+1..n-1); d, e, tau go to the job's workspace.
-template <bool ALDS>
 __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
                                                   const int* __restrict__ joblist, double* __restrict__ arena,
                                                   double* __restrict__ tri) {
     extern __shared__ double dyn[];
     __shared__ double red[16], red2[16], sh_alpha;
-    // segment partials of A v (S * m2 <= max(blockDim.x, m2)); the LDS variant
-    // serves at most 137 AOs and keeps its static LDS small
-    __shared__ double part[ALDS ? 1024 : 2048];
+    // segment partials of A v (S * m2 <= max(blockDim.x, m2), m2 <= 136)
+    __shared__ double part[1024];
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
@@ -91,7 +92,7 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
     double* p = vnext + n;
     double* pnext = p + n;
     double* W = amat(arena, jb, AR_W);
-    double* A = ALDS ? pnext + n : W;
+    double* A = pnext + n;
     const double* F = amat(arena, jb, AR_F);
     double* d = eig_ws(tri, jb, job);
     double* e = d + n;
@@ -239,9 +240,286 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
         tmp = p; p = pnext; pnext = tmp;
         tk = tn;
     }
-    if (ALDS) {   // keep the reflectors (upper rows) for the back-transformation
+    // keep the reflectors (upper rows) for the back-transformation
+    __syncthreads();
+    for (long long x = tid; x < nn; x += T) W[x] = A[x];
+    if (tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
+}
+
+// Upper-triangle variant for matrices in device memory (n > 137): the same
+// reflectors and fused rank-2 update / matrix-vector product as k_tridiag, but
+// only the elements r <= c of the trailing block are read and written, which
+// halves the HBM traffic of a step. A stored element a_rc (r < c) adds a_rc v_r
+// to (A v)_c and a_rc v_c to (A v)_r. Work items are bands of BR rows by 64
+// columns; a lane owns one column, so its column sum stays in a register and
+// the row sums of each 8-row slice are reduced across the wave by a butterfly.
+// Per-item partial sums go to colpart[band][t] and rowpart[tile][t] (t = index
+// in the trailing block), in LDS when PG is false and in the job's AR_U matrix
+// (unused until divide and conquer) when it is true, and are added in a fixed
+// order, so the result does not depend on timing.
+constexpr int kSymMaxLdsN = 512;   // largest n whose partial sums fit in LDS
+
+__device__ inline int sym_col_cap(int n) { return ((n + 31) >> 5) * n; }
+
+size_t tridiag_sym_lds_bytes(int n, bool pg) {
+    size_t d = 4 * (size_t)n;
+    if (!pg) d += (size_t)(((n + 31) >> 5) + ((n + 63) >> 6)) * n;
+    return d * 8;
+}
+
+// Cross-lane exchanges on doubles without LDS: v_permlane32_swap trades the
+// upper 32 lanes of its first operand for the lower 32 lanes of its second,
+// v_permlane16_swap the odd 16-lane rows of the first for the even rows of the
+// second; DPP row_ror:8 is lane ^ 8 within a row, row_half_mirror pairs lanes
+// i and 7 - i of each 8, and the two quad permutations are lane ^ 2 and ^ 1.
+typedef unsigned int u32;
+__device__ inline void swap32_d(double& a, double& b) {
+    const unsigned long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane32_swap((u32)x, (u32)y, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((u32)(x >> 32), (u32)(y >> 32), false, false);
+    a = __longlong_as_double(((unsigned long long)hi[0] << 32) | lo[0]);
+    b = __longlong_as_double(((unsigned long long)hi[1] << 32) | lo[1]);
+}
+__device__ inline void swap16_d(double& a, double& b) {
+    const unsigned long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane16_swap((u32)x, (u32)y, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((u32)(x >> 32), (u32)(y >> 32), false, false);
+    a = __longlong_as_double(((unsigned long long)hi[0] << 32) | lo[0]);
+    b = __longlong_as_double(((unsigned long long)hi[1] << 32) | lo[1]);
+}
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+    const unsigned long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u32)x, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u32)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((unsigned long long)(u32)hi << 32) | (u32)lo);
+}
+constexpr int kDppRor8 = 0x128, kDppHalfMirror = 0x141, kDppXor2 = 0x4E, kDppXor1 = 0xB1;
+
+// Sum over aligned groups of G lanes (G = 1, 2, 4, ..., 64, wave-uniform);
+// every lane of a group gets the group's sum.
+__device__ inline double group_sum(double v, int G) {
+    if (G >= 64) { double a = v, b = v; swap32_d(a, b); v = a + b; }
+    if (G >= 32) { double a = v, b = v; swap16_d(a, b); v = a + b; }
+    if (G >= 16) v += dpp_d<kDppRor8>(v);
+    if (G >= 8) v += dpp_d<kDppHalfMirror>(v);
+    if (G >= 4) v += dpp_d<kDppXor2>(v);
+    if (G >= 2) v += dpp_d<kDppXor1>(v);
+    return v;
+}
+
+// x0..x7 hold one value per row of an 8-row slice; afterwards every lane holds
+// the sum over the 64 lanes of row (lane >> 3) & 7.
+__device__ inline double rows8_reduce(double x0, double x1, double x2, double x3, double x4, double x5, double x6,
+                                      double x7, int lane) {
+    // lanes 0-31 keep rows 0-3 and lanes 32-63 rows 4-7
+    swap32_d(x0, x4); swap32_d(x1, x5); swap32_d(x2, x6); swap32_d(x3, x7);
+    double y0 = x0 + x4, y1 = x1 + x5, y2 = x2 + x6, y3 = x3 + x7;
+    // even 16-lane rows keep y0, y1 and odd rows y2, y3
+    swap16_d(y0, y2); swap16_d(y1, y3);
+    const double z0 = y0 + y2, z1 = y1 + y3;
+    // lanes with bit 3 clear keep z0, the others z1
+    const bool h3 = lane & 8;
+    double s = (h3 ? z1 : z0) + dpp_d<kDppRor8>(h3 ? z0 : z1);
+    s += dpp_d<kDppHalfMirror>(s);
+    s += dpp_d<kDppXor2>(s);
+    s += dpp_d<kDppXor1>(s);
+    return s;
+}
+
+template <bool PG>
+__global__ void __launch_bounds__(1024) k_tridiag_sym(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                                      const int* __restrict__ joblist, double* __restrict__ arena,
+                                                      double* __restrict__ tri) {
+    extern __shared__ double dyn[];
+    __shared__ double red[16], red2[16], sh_alpha;
+    const int job = joblist[blockIdx.x];
+    const JobDev jb = jobs[job];
+    if (state[job].phase != PH_ACTIVE) return;
+    const unsigned long long t0 = wall_clock64();
+    const int n = jb.nao;
+    // the wave index as a scalar: everything derived from it (work items, rows)
+    // stays in scalar registers and scalar branches
+    const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, nw = T >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    double* vcur = dyn;
+    double* vnext = vcur + n;
+    double* p = vnext + n;
+    double* pnext = p + n;
+    double* A = amat(arena, jb, AR_W);
+    double* colpart = PG ? amat(arena, jb, AR_U) : pnext + n;
+    const int ccap = sym_col_cap(n);
+    const double* F = amat(arena, jb, AR_F);
+    double* d = eig_ws(tri, jb, job);
+    double* e = d + n;
+    double* tau = e + n;
+    const long long nn = (long long)n * n;
+    for (long long x = tid; x < nn; x += T) A[x] = F[x];
+    for (int i = tid; i < n; i += T) { e[i] = 0.0; tau[i] = 0.0; }
+    __syncthreads();
+    if (n < 3) {
+        if (tid == 0) {
+            d[0] = A[0];
+            if (n == 2) { e[0] = A[1]; d[1] = A[3]; }
+            state[job].prof[0] = (double)(wall_clock64() - t0);
+        }
+        return;
+    }
+    double tk;
+    {   // reflector of row 0, then p = tau_0 A[1:, 1:] v_0 from the full (symmetric) rows
+        double ss = 0.0;
+        for (int t = 2 + tid; t < n; t += T) { const double x = A[t]; ss += x * x; }
+        ss = group_sum(ss, 64);
+        if (lane == 0) red2[wave] = ss;
         __syncthreads();
-        for (long long x = tid; x < nn; x += T) W[x] = A[x];
+        ss = 0.0;
+        for (int w = 0; w < nw; ++w) ss += red2[w];
+        const double alpha = A[1];
+        double beta = alpha, scale = 0.0;
+        tk = 0.0;
+        if (ss > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            tk = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        for (int t = 1 + tid; t < n - 1; t += T) vcur[t] = ss > 0.0 ? A[1 + t] * scale : 0.0;
+        if (tid == 0) { vcur[0] = 1.0; d[0] = A[0]; e[0] = beta; tau[0] = tk; }
+        __syncthreads();
+        for (int t = 1 + tid; t < n - 1; t += T) A[1 + t] = vcur[t];
+        if (tid == 0) A[1] = 1.0;
+        double pv = 0.0;
+        for (int i = 1 + wave; i < n; i += nw) {
+            double s = 0.0;
+            for (int j = 1 + lane; j < n; j += 64) s += A[(long long)i * n + j] * vcur[j - 1];
+            s = group_sum(s, 64);
+            if (lane == 0) { p[i - 1] = tk * s; pv += tk * s * vcur[i - 1]; }
+        }
+        if (lane == 0) red[wave] = pv;
+        __syncthreads();
+    }
+    for (int k = 0; k <= n - 3; ++k) {
+        const int m = n - 1 - k;   // length of v_k
+        double pv = 0.0;
+        for (int w = 0; w < nw; ++w) pv += red[w];
+        const double K = -0.5 * tk * pv;   // w = p + K v
+        const double w0 = p[0] + K * vcur[0];
+        if (k == n - 3) {   // last trailing 2x2 block (rows/cols n-2, n-1)
+            if (tid == 0) {
+                const double w1 = p[1] + K * vcur[1];
+                d[n - 2] = A[(long long)(n - 2) * n + n - 2] - 2.0 * vcur[0] * w0;
+                e[n - 2] = A[(long long)(n - 2) * n + n - 1] - (vcur[0] * w1 + w0 * vcur[1]);
+                d[n - 1] = A[(long long)(n - 1) * n + n - 1] - 2.0 * vcur[1] * w1;
+            }
+            break;
+        }
+        // row k+1 of the updated block (upper part); its tail defines the next reflector
+        const long long r1 = (long long)(k + 1) * n + k + 1;
+        double ss = 0.0;
+        for (int t = tid; t < m; t += T) {
+            const double a = A[r1 + t] - (vcur[0] * (p[t] + K * vcur[t]) + w0 * vcur[t]);
+            A[r1 + t] = a;
+            if (t >= 2) ss += a * a;
+            else if (t == 1) sh_alpha = a;
+            else d[k + 1] = a;
+        }
+        ss = group_sum(ss, 64);
+        if (lane == 0) red2[wave] = ss;
+        __syncthreads();   // 1
+        ss = 0.0;
+        for (int w = 0; w < nw; ++w) ss += red2[w];
+        const double alpha = sh_alpha;
+        double beta = alpha, tn = 0.0, scale = 0.0;
+        if (ss > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            tn = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        const int m2 = m - 1;   // length of v_{k+1}
+        for (int t = 1 + tid; t < m2; t += T) {
+            const double v = ss > 0.0 ? A[r1 + 1 + t] * scale : 0.0;
+            vnext[t] = v;
+            A[r1 + 1 + t] = v;
+        }
+        if (tid == 0) { vnext[0] = 1.0; A[r1 + 1] = 1.0; e[k + 1] = beta; tau[k + 1] = tn; }
+        __syncthreads();   // 2
+        // upper triangle of rows/columns base .. n-1: rank-2 update of step k and
+        // the partial sums of A v_{k+1}
+        const int base = k + 2;
+        const int nct = (m2 + 63) >> 6;
+        // smallest band height whose column partials fit in ccap doubles
+        const int BR = ((m2 + 7) >> 3) * m2 <= ccap ? 8 : (((m2 + 15) >> 4) * m2 <= ccap ? 16 : 32);
+        const int per_tile = 64 / BR, nb = (m2 + BR - 1) / BR;
+        double* rowpart = colpart + ccap;
+        int nitems = 0;
+        for (int ct = 0; ct < nct; ++ct) nitems += min(per_tile * (ct + 1), nb);
+        for (int it = wave; it < nitems; it += nw) {
+            int ct = 0, b = it;
+            while (b >= min(per_tile * (ct + 1), nb)) { b -= min(per_tile * (ct + 1), nb); ++ct; }
+            const int c = base + 64 * ct + lane;
+            const bool cv = c < n;
+            const unsigned int cbyte = (unsigned int)(cv ? c : 0) * 8u;
+            const int tc = c - k - 1;
+            const double vi = cv ? vcur[tc] : 0.0;
+            const double wi = cv ? p[tc] + K * vi : 0.0;
+            const double vnc = cv ? vnext[tc - 1] : 0.0;
+            double colacc = 0.0;
+            const int r0 = base + b * BR;
+            for (int sc = 0; sc < BR; sc += 8) {
+                const int rs = r0 + sc;
+                if (rs >= n) break;
+                // row pointers are wave-uniform; the lane adds a 32-bit byte offset
+                char* Ar = (char*)(A + (long long)rs * n);
+                double a[8], x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int r = rs + u;
+                    a[u] = (cv && r <= c) ? *(const double*)(Ar + (long long)u * n * 8 + cbyte) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int r = rs + u;
+                    const bool ok = cv && r <= c;
+                    const int tj = min(r, n - 1) - k - 1;
+                    const double vj = r < n ? vcur[tj] : 0.0;
+                    const double wj = r < n ? p[tj] + K * vj : 0.0;
+                    const double vnr = r < n ? vnext[tj - 1] : 0.0;
+                    const double an = a[u] - (vj * wi + wj * vi);
+                    if (ok) *(double*)(Ar + (long long)u * n * 8 + cbyte) = an;
+                    colacc += ok ? an * vnr : 0.0;
+                    x[u] = (ok && r < c) ? an * vnc : 0.0;
+                }
+                const double s = rows8_reduce(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], lane);
+                const int rr = rs + (lane >> 3);
+                if ((lane & 7) == 0 && rr < n) rowpart[ct * m2 + (rr - base)] = s;
+            }
+            if (cv) colpart[b * m2 + (c - base)] = colacc;
+        }
+        __syncthreads();   // 3
+        // (A v_{k+1})_t = sum of column partials of bands 0 .. t/BR and row
+        // partials of column tiles t/64 .. nct-1, S lanes per t
+        int S = 1;
+        while (S < 64 && 2 * S * m2 <= T) S *= 2;
+        pv = 0.0;
+        for (int t0s = 0; t0s < m2; t0s += T / S) {
+            const int t = t0s + tid / S, s = tid & (S - 1);
+            double q = 0.0;
+            if (t < m2) {
+                for (int bb = s; bb <= t / BR; bb += S) q += colpart[bb * m2 + t];
+                for (int cc = (t >> 6) + s; cc < nct; cc += S) q += rowpart[cc * m2 + t];
+            }
+            q = group_sum(q, S);
+            if (s == 0 && t < m2) {
+                const double pn = tn * q;
+                pnext[t] = pn;
+                pv += pn * vnext[t];
+            }
+        }
+        pv = group_sum(pv, 64);
+        if (lane == 0) red[wave] = pv;
+        __syncthreads();   // 4
+        double* tmp = vcur; vcur = vnext; vnext = tmp;
+        tmp = p; p = pnext; pnext = tmp;
+        tk = tn;
     }
     if (tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
 }
@@ -919,7 +1197,9 @@ static void launch_bt(hipStream_t st, const JobDev* jobs, JobState* state, const
 
 bool heig_supported(int n) { return n <= kHeigMaxN; }
 
-size_t heig_lds_bytes(int n, bool a_in_lds) { return tridiag_lds_bytes(n, a_in_lds); }
+size_t heig_lds_bytes(int n, bool a_in_lds) {
+    return a_in_lds ? tridiag_lds_bytes(n, true) : tridiag_sym_lds_bytes(n, n > kSymMaxLdsN);
+}
 
 void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
                       int max_n, double* arena, double* eps, int* perm, double* tri, bool a_in_lds) {
@@ -928,11 +1208,16 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
         const int threads = max_n <= 32 ? 256 : (max_n <= 96 ? 512 : 1024);
         const size_t lt = tridiag_lds_bytes(max_n, a_in_lds);
         if (a_in_lds) {
-            hipFuncSetAttribute((const void*)k_tridiag<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
-            hipLaunchKernelGGL(k_tridiag<true>, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
+            hipFuncSetAttribute((const void*)k_tridiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
+            hipLaunchKernelGGL(k_tridiag, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
+        } else if (max_n <= kSymMaxLdsN) {
+            const size_t ls = tridiag_sym_lds_bytes(max_n, false);
+            hipFuncSetAttribute((const void*)k_tridiag_sym<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
+            hipLaunchKernelGGL(k_tridiag_sym<false>, dim3(nlist), dim3(1024), ls, st, jobs, state, joblist, arena, tri);
         } else {
-            hipFuncSetAttribute((const void*)k_tridiag<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
-            hipLaunchKernelGGL(k_tridiag<false>, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
+            const size_t ls = tridiag_sym_lds_bytes(max_n, true);
+            hipFuncSetAttribute((const void*)k_tridiag_sym<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
+            hipLaunchKernelGGL(k_tridiag_sym<true>, dim3(nlist), dim3(1024), ls, st, jobs, state, joblist, arena, tri);
         }
     } else if (part == 1) {
         const int tdc = max_n <= 64 ? 256 : (max_n <= 128 ? 512 : 1024);

@@ -103,7 +103,7 @@ def test_fock_matches_oracle(engine, theory, tag):
 # sizes cover 1x1/2x2 corner cases, ragged divide-and-conquer trees (carried
 # blocks at several levels), the LDS/global tridiagonalization boundary near
 # 135 and the largest supported one-workgroup size
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 17, 33, 62, 96, 130, 136, 201, 257, 384, 512])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 17, 33, 62, 96, 130, 136, 140, 201, 257, 384, 512])
 def test_eigh(engine, n):
     print(f"eigh n={n}", flush=True)
     rng = np.random.default_rng(n)
