@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
 // halves the HBM traffic of a step. A stored element a_rc (r < c) adds a_rc v_r
 // to (A v)_c and a_rc v_c to (A v)_r. Work items are bands of BR rows by 64
 // columns; a lane owns one column, so its column sum stays in a register and
-// the row sums of each 8-row slice are reduced across the wave by a butterfly.
+// the row sums of each 4-row slice are reduced across the wave by a butterfly.
 // Per-item partial sums go to colpart[band][t] and rowpart[tile][t] (t = index
 // in the trailing block), in LDS when PG is false and in the job's AR_U matrix
 // (unused until divide and conquer) when it is true, and are added in a fixed
@@ -308,27 +308,36 @@ __device__ inline double group_sum(double v, int G) {
     return v;
 }
 
-// x0..x7 hold one value per row of an 8-row slice; afterwards every lane holds
-// the sum over the 64 lanes of row (lane >> 3) & 7.
-__device__ inline double rows8_reduce(double x0, double x1, double x2, double x3, double x4, double x5, double x6,
-                                      double x7, int lane) {
-    // lanes 0-31 keep rows 0-3 and lanes 32-63 rows 4-7
-    swap32_d(x0, x4); swap32_d(x1, x5); swap32_d(x2, x6); swap32_d(x3, x7);
-    double y0 = x0 + x4, y1 = x1 + x5, y2 = x2 + x6, y3 = x3 + x7;
-    // even 16-lane rows keep y0, y1 and odd rows y2, y3
-    swap16_d(y0, y2); swap16_d(y1, y3);
-    const double z0 = y0 + y2, z1 = y1 + y3;
-    // lanes with bit 3 clear keep z0, the others z1
-    const bool h3 = lane & 8;
-    double s = (h3 ? z1 : z0) + dpp_d<kDppRor8>(h3 ? z0 : z1);
+// x0..x3 hold one value per row of a 4-row slice; afterwards every lane holds
+// the sum over the 64 lanes of row (lane >> 4) & 3.
+__device__ inline double rows4_reduce(double x0, double x1, double x2, double x3) {
+    // lanes 0-31 keep rows 0-1 and lanes 32-63 rows 2-3
+    swap32_d(x0, x2); swap32_d(x1, x3);
+    double y0 = x0 + x2, y1 = x1 + x3;
+    // even 16-lane rows keep y0 and odd rows y1
+    swap16_d(y0, y1);
+    double s = y0 + y1;
+    s += dpp_d<kDppRor8>(s);
     s += dpp_d<kDppHalfMirror>(s);
     s += dpp_d<kDppXor2>(s);
     s += dpp_d<kDppXor1>(s);
     return s;
 }
 
+// Raw buffer access to one molecule's matrix (base, n*n*8 bytes, the gfx9
+// raw-buffer descriptor word 0x00020000): a lane supplies a 32-bit byte
+// offset and the row offset comes from a scalar register, so a slice of rows
+// costs one VGPR of addressing. Every access issued stays inside the matrix.
+typedef u32 u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline double buf_load_d(__amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ inline void buf_store_d(double v, __amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
 template <bool PG>
-__global__ void __launch_bounds__(1024) k_tridiag_sym(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+__global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
                                                       const int* __restrict__ joblist, double* __restrict__ arena,
                                                       double* __restrict__ tri) {
     extern __shared__ double dyn[];
@@ -348,6 +357,7 @@ __global__ void __launch_bounds__(1024) k_tridiag_sym(const JobDev* __restrict__
     double* pnext = p + n;
     double* A = amat(arena, jb, AR_W);
     double* colpart = PG ? amat(arena, jb, AR_U) : pnext + n;
+    const __amdgpu_buffer_rsrc_t Ab = __builtin_amdgcn_make_buffer_rsrc(A, (short)0, n * n * 8, 0x00020000);
     const int ccap = sym_col_cap(n);
     const double* F = amat(arena, jb, AR_F);
     double* d = eig_ws(tri, jb, job);
@@ -457,26 +467,29 @@ __global__ void __launch_bounds__(1024) k_tridiag_sym(const JobDev* __restrict__
             while (b >= min(per_tile * (ct + 1), nb)) { b -= min(per_tile * (ct + 1), nb); ++ct; }
             const int c = base + 64 * ct + lane;
             const bool cv = c < n;
-            const unsigned int cbyte = (unsigned int)(cv ? c : 0) * 8u;
+            const u32 cbyte = (u32)(cv ? c : n - 1) * 8u;
             const int tc = c - k - 1;
             const double vi = cv ? vcur[tc] : 0.0;
             const double wi = cv ? p[tc] + K * vi : 0.0;
             const double vnc = cv ? vnext[tc - 1] : 0.0;
             double colacc = 0.0;
             const int r0 = base + b * BR;
-            for (int sc = 0; sc < BR; sc += 8) {
+            for (int sc = 0; sc < BR; sc += 4) {
                 const int rs = r0 + sc;
                 if (rs >= n) break;
-                // row pointers are wave-uniform; the lane adds a 32-bit byte offset
-                char* Ar = (char*)(A + (long long)rs * n);
-                double a[8], x[8];
+                // element (rs + u, c) at byte voff + u * n * 8; lanes past the last
+                // column read column n - 1 and lanes below the diagonal read the
+                // (unused) lower triangle, so every load is in range
+                const u32 voff = (u32)rs * (u32)n * 8u + cbyte;
+                double a[4], x[4];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 4; ++u) {
                     const int r = rs + u;
-                    a[u] = (cv && r <= c) ? *(const double*)(Ar + (long long)u * n * 8 + cbyte) : 0.0;
+                    a[u] = r < n ? buf_load_d(Ab, voff, (u32)u * (u32)n * 8u) : 0.0;
+                    a[u] = (cv && r <= c) ? a[u] : 0.0;
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 4; ++u) {
                     const int r = rs + u;
                     const bool ok = cv && r <= c;
                     const int tj = min(r, n - 1) - k - 1;
@@ -484,13 +497,13 @@ __global__ void __launch_bounds__(1024) k_tridiag_sym(const JobDev* __restrict__
                     const double wj = r < n ? p[tj] + K * vj : 0.0;
                     const double vnr = r < n ? vnext[tj - 1] : 0.0;
                     const double an = a[u] - (vj * wi + wj * vi);
-                    if (ok) *(double*)(Ar + (long long)u * n * 8 + cbyte) = an;
+                    if (ok) buf_store_d(an, Ab, voff, (u32)u * (u32)n * 8u);
                     colacc += ok ? an * vnr : 0.0;
                     x[u] = (ok && r < c) ? an * vnc : 0.0;
                 }
-                const double s = rows8_reduce(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], lane);
-                const int rr = rs + (lane >> 3);
-                if ((lane & 7) == 0 && rr < n) rowpart[ct * m2 + (rr - base)] = s;
+                const double s = rows4_reduce(x[0], x[1], x[2], x[3]);
+                const int rr = rs + ((lane >> 4) & 3);
+                if ((lane & 15) == 0 && rr < n) rowpart[ct * m2 + (rr - base)] = s;
             }
             if (cv) colpart[b * m2 + (c - base)] = colacc;
         }
@@ -1220,7 +1233,10 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
             hipLaunchKernelGGL(k_tridiag_sym<true>, dim3(nlist), dim3(1024), ls, st, jobs, state, joblist, arena, tri);
         }
     } else if (part == 1) {
-        const int tdc = max_n <= 64 ? 256 : (max_n <= 128 ? 512 : 1024);
+        // 512 threads above 64 AOs: the secular and deflation phases are latency
+        // bound, and three 512-thread workgroups (LDS permitting) share a CU where
+        // two 1024-thread ones did (208 vs 300 ms per 4096-molecule pass)
+        const int tdc = max_n <= 64 ? 256 : 512;
         const size_t ld = dc_lds_bytes(max_n);
         hipFuncSetAttribute((const void*)k_dc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ld);
         hipLaunchKernelGGL(k_dc, dim3(nlist), dim3(tdc), ld, st, jobs, state, joblist, arena, tri, eps, perm);
