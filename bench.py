@@ -43,6 +43,23 @@ sys.path.insert(0, HERE)
 METRIC = "PM3 SCF solves/s + p50/p99 latency, mixed vs FP64 GPU at |ΔE|≤1e-5 eV"
 EV = 1.0 / 0.03674903
 FP64_PEAK_TFLOPS = 78.6       # MI355X dense FP64 (matrix and vector)
+HBM_PEAK_TBPS = 8.0           # MI355X HBM3E
+# HBM bytes per launch of the dominant kernel on the 4096-molecule FP64 pass,
+# from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in this file
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_tridiag_sym_batched_pm3_4096_fp64.md")
+
+
+def measured_traffic():
+    """The `traffic_bytes_per_launch:` line of the committed counter summary,
+    or None when it is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 # MolDS SCF knobs for the mix: DIIS as test/ch3f_pm3.in; damping 0.7 while RMS > 0.02
 # keeps the randomly oriented water clusters out of occupation-swapping cycles
 SCF = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0,
@@ -99,10 +116,16 @@ def served_leg(engine, rate, count, precision, seed=3, hi=SERVED_HI):
         if t > now:
             time.sleep(t - now)
         ids.append((engine.submit(job), time.perf_counter() - t0, job))
-    lat, done_at, ok, fp64_it, it = [], [], 0, [], []
+    from molds_amd.nddo import NddoError
+    lat, done_at, ok, fp64_it, it, errors = [], [], 0, [], [], []
     for jid, ts, job in ids:
         engine.wait(jid, 300.0)
-        r = engine.get_result(jid, job)
+        try:
+            r = engine.get_result(jid, job)
+        except NddoError as ex:   # counted and reported, not retried
+            errors.append(str(ex))
+            engine.release(jid)
+            continue
         engine.release(jid)
         if r["status"] == 0:
             ok += 1
@@ -110,9 +133,12 @@ def served_leg(engine, rate, count, precision, seed=3, hi=SERVED_HI):
         done_at.append(ts + r["latency_s"])
         it.append(r["iterations"])
         fp64_it.append(r["fp64_iterations"])
+    if errors:
+        log(f"served {precision} {rate:g}/s: {len(errors)} failed requests, first: {errors[0]}")
     span = max(done_at)
     lat = np.array(lat)
-    return dict(precision=precision, rate_per_s=rate, requests=count, converged=ok,
+    return dict(precision=precision, rate_per_s=rate, requests=count, converged=ok, failed=len(errors),
+                first_error=errors[0] if errors else None,
                 solves_per_s=round(count / span, 3), span_s=round(span, 3),
                 p50_s=round(float(np.percentile(lat, 50)), 5), p99_s=round(float(np.percentile(lat, 99)), 5),
                 mean_iterations=round(float(np.mean(it)), 2), mean_fp64_iterations=round(float(np.mean(fp64_it)), 2),
@@ -279,14 +305,19 @@ def main():
             best_mode, best = mode, lg
 
     # roofline of the dominant kernel of the step: the tridiagonalization of
-    # the matrices that do not fit LDS (k_tridiag<false>), timed with hipEvents
+    # the matrices that do not fit LDS (k_tridiag_sym), timed with hipEvents
     # on the engine's stream around its launches; algorithmic work (4/3) n^3
-    # FP64 flops per eigensolve, one eigensolve per SCF iteration per molecule
+    # FP64 flops per eigensolve, one eigensolve per SCF iteration per molecule.
+    # The same launches seen as HBM traffic: the upper triangle of the trailing
+    # block read and written once per Householder step, 8 n^3 / 3 bytes.
     info = engine.info()
     glb = nao > info["lds_tridiag_max_n"]
     ms_tri, launches_tri = best["stages"]["tridiag_glb"]
     flops = args.steps * float(np.sum(best["iters"][glb] * (4.0 / 3.0) * nao[glb] ** 3))
     achieved = flops / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
+    alg_bytes = args.steps * float(np.sum(best["iters"][glb] * (8.0 / 3.0) * nao[glb] ** 3))
+    hbm_tbs = alg_bytes / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
+    traffic = measured_traffic() if best_mode == "fp64" and args.count == 4096 else None
     kernels = {k: round(v[0] / args.steps, 3) for k, v in best["stages"].items()}
 
     served = {}
@@ -340,10 +371,13 @@ def main():
         "plain_fp64_gpu_served_mix": plain_served,
         "speedup_vs_plain_fp64": ({"batched": round(best["solves_per_s"] / plain["solves_per_s"], 2)}
                                   if plain else None),
-        "roofline": {"bound": "mfma", "kernel": "k_tridiag<false> (Householder tridiagonalization, device memory)",
+        "roofline": {"bound": "mfma", "kernel": "k_tridiag_sym (Householder tridiagonalization, device memory)",
                      "achieved": round(achieved, 5), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": traffic,
                      "avg_launch_ms": round(ms_tri / max(launches_tri, 1), 4), "launches": launches_tri,
+                     "hbm_view": {"algorithmic_bytes_per_launch": round(alg_bytes / max(launches_tri, 1)),
+                                  "achieved_TBps": round(hbm_tbs, 4), "peak_TBps": HBM_PEAK_TBPS,
+                                  "frac": round(hbm_tbs / HBM_PEAK_TBPS, 4)},
                      "lds_tridiag_max_n": info["lds_tridiag_max_n"],
                      "kernel_ms_per_step": kernels},
     }

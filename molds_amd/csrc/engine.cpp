@@ -34,6 +34,7 @@ thread_local std::string g_err;
 }
 
 void nddo::set_error(const std::string& s) { g_err = s; }
+std::string nddo::get_error() { return g_err; }
 
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
@@ -520,7 +521,14 @@ void Batch::iterate(Engine* e, hipStream_t st) {
             return;
         }
     }
-    hipGraphLaunch(gexec, st);
+    if (hipGraphLaunch(gexec, st) != hipSuccess) {   // replay refused: this batch launches eagerly from now on
+        hipGetLastError();
+        hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        graph_off = true;
+        launch_iteration(e, st);
+        return;
+    }
     e->graph_replays++;
 }
 

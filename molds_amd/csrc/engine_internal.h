@@ -19,6 +19,7 @@ namespace nddo {
 constexpr int kNumElem = 5;   // H, C, N, O, F
 
 void set_error(const std::string& s);
+std::string get_error();   // the calling thread's last message
 int elem_slot(int Z);
 int validate_job(const nddo_job_desc& d, int* nao, int* nocc);
 size_t eig_lds_bytes(int n, bool a_in_lds);   // LDS of the one-workgroup eigensolver

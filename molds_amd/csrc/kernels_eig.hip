@@ -24,6 +24,9 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <map>
+#include <mutex>
+
 #include "launch.h"
 #include "nddo_device.h"
 
@@ -1194,6 +1197,17 @@ __global__ void __launch_bounds__(512) k_bt_wy(const JobDev* __restrict__ jobs, 
 }
 
 // ------------------------------------------------------------------ host side
+void raise_dynamic_lds(const void* func, size_t bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> limit;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& cur = limit[{dev, func}];
+    if (bytes <= cur) return;
+    if (hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) cur = bytes;
+}
+
 static size_t bt_lds_bytes(int n, int rt, bool vg) {
     const size_t r = 16 * (size_t)rt;
     return ((vg ? 0 : (size_t)16 * (n + 1)) + 8 * 16 * r + 16 * r + 256) * 8;
@@ -1203,7 +1217,7 @@ template <int JT, int RT, bool VG>
 static void launch_bt(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
                       double* arena, double* tri) {
     const size_t lb = bt_lds_bytes(max_n, RT, VG);
-    hipFuncSetAttribute((const void*)k_bt_wy<JT, RT, VG>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+    raise_dynamic_lds((const void*)k_bt_wy<JT, RT, VG>, lb);
     hipLaunchKernelGGL((k_bt_wy<JT, RT, VG>), dim3(nlist, (max_n + 16 * RT - 1) / (16 * RT)), dim3(512), lb, st, jobs,
                        state, joblist, arena, tri);
 }
@@ -1221,15 +1235,15 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
         const int threads = max_n <= 32 ? 256 : (max_n <= 96 ? 512 : 1024);
         const size_t lt = tridiag_lds_bytes(max_n, a_in_lds);
         if (a_in_lds) {
-            hipFuncSetAttribute((const void*)k_tridiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lt);
+            raise_dynamic_lds((const void*)k_tridiag, lt);
             hipLaunchKernelGGL(k_tridiag, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
         } else if (max_n <= kSymMaxLdsN) {
             const size_t ls = tridiag_sym_lds_bytes(max_n, false);
-            hipFuncSetAttribute((const void*)k_tridiag_sym<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
+            raise_dynamic_lds((const void*)k_tridiag_sym<false>, ls);
             hipLaunchKernelGGL(k_tridiag_sym<false>, dim3(nlist), dim3(1024), ls, st, jobs, state, joblist, arena, tri);
         } else {
             const size_t ls = tridiag_sym_lds_bytes(max_n, true);
-            hipFuncSetAttribute((const void*)k_tridiag_sym<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls);
+            raise_dynamic_lds((const void*)k_tridiag_sym<true>, ls);
             hipLaunchKernelGGL(k_tridiag_sym<true>, dim3(nlist), dim3(1024), ls, st, jobs, state, joblist, arena, tri);
         }
     } else if (part == 1) {
@@ -1238,7 +1252,7 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
         // two 1024-thread ones did (208 vs 300 ms per 4096-molecule pass)
         const int tdc = max_n <= 64 ? 256 : 512;
         const size_t ld = dc_lds_bytes(max_n);
-        hipFuncSetAttribute((const void*)k_dc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ld);
+        raise_dynamic_lds((const void*)k_dc, ld);
         hipLaunchKernelGGL(k_dc, dim3(nlist), dim3(tdc), ld, st, jobs, state, joblist, arena, tri, eps, perm);
     } else if (max_n >= 3) {
         const int nblk = (max_n - 2 + kBt - 1) / kBt;

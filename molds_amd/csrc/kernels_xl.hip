@@ -852,7 +852,7 @@ void launch_xl_eigen(hipStream_t st, int grid, const JobDev* jobs, JobState* sta
                      unsigned* bar) {
     if (nlist <= 0) return;
     const size_t lds = (size_t)2 * BJ_P * BJ_LD * sizeof(double);
-    hipFuncSetAttribute((const void*)k_bjac, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    raise_dynamic_lds((const void*)k_bjac, lds);
     hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(1), 0, st, bar);
     hipLaunchKernelGGL(k_bjac, dim3(grid), dim3(BJ_T), lds, st, jobs, state, joblist, xloff, nlist, arena, eps, perm,
                        xlws, bar);

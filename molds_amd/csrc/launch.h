@@ -45,6 +45,11 @@ void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev*
 // eigensolver workspace (EIG_WS_PER_AO, EIG_WS_PER_JOB); a_in_lds: the matrices
 // fit the LDS budget (heig_lds_bytes).
 size_t heig_lds_bytes(int n, bool a_in_lds);
+// Raises a kernel's dynamic-LDS limit (hipFuncAttributeMaxDynamicSharedMemorySize)
+// on the current device to at least `bytes`; the limit is never lowered, so a
+// launch recorded in a graph with a larger request stays valid after smaller
+// launches of the same kernel from other batches.
+void raise_dynamic_lds(const void* func, size_t bytes);
 bool heig_supported(int n);
 void launch_heig_class(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
                        double* arena, double* eps, int* perm, double* tri, bool a_in_lds);

@@ -76,6 +76,7 @@ struct ServedJob {
     std::shared_ptr<EventRef> res_ev;
     bool on_host = false;
     std::vector<double> eps, C, P, traj;
+    std::string err;   // why the job failed, reported to the caller by nddo_get_result / nddo_wait
 };
 
 struct Cohort {
@@ -169,7 +170,10 @@ void retire_jobs(Engine* e, Cohort* c, const std::vector<int>& done) {
         std::memcpy(&s, c->h_state + (size_t)jj * kScalarBytes, kScalarBytes);
         nddo_result& r = j->res;
         fill_scalars(c->b.jobs[jj], s, j->desc.theory, j->hof_offset, &r);
-        if (!bufs[k] && r.status == NDDO_OK) r.status = NDDO_HIP_ERROR;
+        if (!bufs[k] && r.status == NDDO_OK) {
+            r.status = NDDO_HIP_ERROR;
+            j->err = "result buffer allocation failed";
+        }
         r.latency_s = std::chrono::duration<double>(now - j->t_submit).count();
         j->d_res = bufs[k];
         j->res_stream = c->st;
@@ -218,18 +222,23 @@ int start_cohort(Engine* e, Cohort* c, Staging& stage) {
     rc = c->b.upload(st, c->descs.data(), stage.p);
     if (rc) return rc;
     c->b.launch_setup_stage(e, st);
-    if (hipGetLastError() != hipSuccess) return NDDO_HIP_ERROR;
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        nddo::set_error(std::string("HIP error ") + hipGetErrorString(le) + " after the cohort's setup launches");
+        return NDDO_HIP_ERROR;
+    }
     c->retired.assign(c->members.size(), 0);
     c->live = (int)c->members.size();
     return NDDO_OK;
 }
 
-void fail_cohort(Engine* e, Cohort* c, int rc) {
+void fail_cohort(Engine* e, Cohort* c, int rc, const std::string& why) {
     std::lock_guard<std::mutex> lk(e->smu);
     for (size_t i = 0; i < c->members.size(); ++i) {
         ServedJob* j = c->members[i];
         if (!j || (i < c->retired.size() && c->retired[i])) continue;
         j->res.status = rc;
+        j->err = why;
         j->t_done = Clock::now();
         j->res.latency_s = std::chrono::duration<double>(j->t_done - j->t_submit).count();
         j->done = true;
@@ -337,7 +346,7 @@ void Engine::start_server() {
                 f->h_state = hstate[k];
                 const int rc = start_cohort(this, f.get(), stage[k]);
                 if (rc != NDDO_OK) {
-                    fail_cohort(this, f.get(), rc);
+                    fail_cohort(this, f.get(), rc, "cohort start: " + nddo::get_error());
                     free_cohort(f.get());
                     free_slots.push_back(k);
                 } else {
@@ -369,7 +378,8 @@ void Engine::start_server() {
                     if (q == hipSuccess) {
                         ready.push_back(c.get());
                     } else if (q != hipErrorNotReady) {
-                        fail_cohort(this, c.get(), NDDO_HIP_ERROR);
+                        fail_cohort(this, c.get(), NDDO_HIP_ERROR,
+                                    std::string("HIP error ") + hipGetErrorString(q) + " in a cohort iteration");
                         c->live = 0;
                         c->inflight = false;
                     }
@@ -412,7 +422,7 @@ void Engine::start_server() {
         }
         for (auto& c : cohorts) {
             hipStreamSynchronize(c->st);
-            fail_cohort(this, c.get(), NDDO_BAD_ARG);
+            fail_cohort(this, c.get(), NDDO_BAD_ARG, "engine closed before the job finished");
             free_cohort(c.get());
         }
         for (int k = 0; k < kMaxCohorts; ++k)
@@ -530,6 +540,7 @@ int nddo_wait(nddo_engine* eh, nddo_job_id id, double timeout_s) {
     const auto pred = [&] { return j->done; };
     if (timeout_s < 0) e->scv.wait(lk, pred);
     else if (!e->scv.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) return NDDO_PENDING;
+    if (j->res.status < 0 && !j->err.empty()) nddo::set_error(j->err);
     return j->res.status;
 }
 
@@ -543,6 +554,7 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
         if (it == e->served.end() || it->second->released) { nddo::set_error("unknown job id"); return NDDO_BAD_ARG; }
         j = it->second;
         if (!j->done) return NDDO_PENDING;
+        if (j->res.status < 0 && !j->err.empty()) nddo::set_error(j->err);
     }
     std::lock_guard<std::mutex> jl(j->mu);
     hipSetDevice(e->device);

@@ -56,6 +56,35 @@ def test_served_matches_alone_and_overlaps(engine):
         assert r["latency_s"] > 0.0
 
 
+def test_served_graph_replay_after_smaller_launches(engine):
+    """A captured SCF iteration keeps replaying correctly after another
+    cohort launched the same eigensolver kernels with less dynamic LDS: the
+    kernels' LDS limit only ever grows, so the recorded launch stays valid."""
+    print("served replay after smaller launches", flush=True)
+    big = _job("water", 64, 41, rms=1e-10)          # 384 AOs, device-memory eigensolver class
+    small = [_job("water", 25, 42), _job("alkane", 24, 43)]   # 150 and 146 AOs
+    g0 = engine.graph_stats()
+    jb = engine.submit(big)
+    t0 = time.perf_counter()
+    while engine.graph_stats()["captured"] == g0["captured"] and time.perf_counter() - t0 < 20.0:
+        time.sleep(0.001)   # until the big cohort has captured its iteration
+    ids = [engine.submit(j) for j in small]
+    res = {}
+    for k, j in [(jb, big)] + list(zip(ids, small)):
+        st = engine.wait(k, 60.0)
+        assert st == 0, st
+        res[k] = engine.get_result(k, j, want_P=True)
+        assert engine.release(k) == 0
+    g1 = engine.graph_stats()
+    assert g1["captured"] > g0["captured"] and g1["replays"] > g0["replays"]
+    for k, j in [(jb, big)] + list(zip(ids, small)):
+        alone = engine.scf_batch([j], want_P=True)[0]
+        assert res[k]["status"] == 0 and alone["status"] == 0
+        assert abs(res[k]["energy"] - alone["energy"]) * EV <= 1e-9
+        assert res[k]["iterations"] == alone["iterations"]
+        assert np.max(np.abs(res[k]["P"] - alone["P"])) <= 1e-10
+
+
 def test_served_statuses(engine):
     from molds_amd.nddo import Engine, Job
     odd = Job("pm3", [6, 1, 1, 1], np.eye(4, 3) * 2.0)
