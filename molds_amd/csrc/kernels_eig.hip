@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
             while (b >= min(per_tile * (ct + 1), nb)) { b -= min(per_tile * (ct + 1), nb); ++ct; }
             const int c = base + 64 * ct + lane;
             const bool cv = c < n;
-            const u32 cbyte = (u32)(cv ? c : n - 1) * 8u;
+            const u32 cbyte = (u32)(cv ? c : 0) * 8u;
             const int tc = c - k - 1;
             const double vi = cv ? vcur[tc] : 0.0;
             const double wi = cv ? p[tc] + K * vi : 0.0;
@@ -480,16 +480,14 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
             for (int sc = 0; sc < BR; sc += 4) {
                 const int rs = r0 + sc;
                 if (rs >= n) break;
-                // element (rs + u, c) at byte voff + u * n * 8; lanes past the last
-                // column read column n - 1 and lanes below the diagonal read the
-                // (unused) lower triangle, so every load is in range
+                // element (rs + u, c) at byte voff + u * n * 8; only lanes on or
+                // above the diagonal load (the lower triangle is never touched)
                 const u32 voff = (u32)rs * (u32)n * 8u + cbyte;
                 double a[4], x[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int r = rs + u;
-                    a[u] = r < n ? buf_load_d(Ab, voff, (u32)u * (u32)n * 8u) : 0.0;
-                    a[u] = (cv && r <= c) ? a[u] : 0.0;
+                    a[u] = (cv && r <= c) ? buf_load_d(Ab, voff, (u32)u * (u32)n * 8u) : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
