@@ -470,8 +470,8 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs, char* stage) {
 }
 
 void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
-    launch_setup(st, d_pairs, (int)pairs.size(), d_atoms, (int)atoms.size(), e->d_elems, d_jobs, (int)jobs.size(),
-                 d_ints, d_arena, d_part, d_ecore, d_state, d_sdbg);
+    launch_setup(st, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs,
+                 (int)jobs.size(), d_ints, d_arena, d_part, d_ecore, d_state, d_sdbg);
     if (d_ints32) launch_ints_f32(st, d_ints, d_ints32, nints);
 }
 

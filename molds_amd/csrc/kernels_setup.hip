@@ -1,5 +1,6 @@
 // Per-geometry setup kernels: two-centre two-electron integrals, overlap,
-// core Hamiltonian and core repulsion, one wavefront per atom pair.
+// core Hamiltonian and core repulsion: one lane per pair with an H atom, one
+// wavefront per pair of heavy atoms.
 //
 // Reference path (MolDS): Mndo::CalcTwoElecsTwoAtomCores (src/mndo/Mndo.cpp:3703-3820)
 // -> CalcDiatomicTwoElecsTwoCores (:3948-4025) -> GetNddoRepulsionIntegral
@@ -65,7 +66,7 @@ __device__ int charges(int t, double D, Charge* c) {
     return 0;
 }
 
-__device__ double multipole_pair(const ElemDev& A, const ElemDev& B, int ta, int tb, double r) {
+__device__ __forceinline__ double multipole_pair(const ElemDev& A, const ElemDev& B, int ta, int tb, double r) {
     if (parity_x(ta) != parity_x(tb) || parity_y(ta) != parity_y(tb)) return 0.0;
     int ka = mp_kind(ta), kb = mp_kind(tb);
     double a = A.rho[ka] + B.rho[kb];
@@ -88,7 +89,7 @@ __device__ double multipole_pair(const ElemDev& A, const ElemDev& B, int ta, int
     return v;
 }
 
-__device__ double local_integral_sum(const ElemDev& A, const ElemDev& B, int i, int j, double r) {
+__device__ __forceinline__ double local_integral_sum(const ElemDev& A, const ElemDev& B, int i, int j, double r) {
     double v = 0.0;
     for (int u = 0; u < 2; ++u) {
         int ta = c_dist[i][u];
@@ -103,7 +104,7 @@ __device__ double local_integral_sum(const ElemDev& A, const ElemDev& B, int i, 
 }
 
 // Diatomic-frame (mu nu|lam sig) for packed indices i (on A), j (on B).
-__device__ double local_integral(const ElemDev& A, const ElemDev& B, int i, int j, double r) {
+__device__ __forceinline__ double local_integral(const ElemDev& A, const ElemDev& B, int i, int j, double r) {
     if (i == 6 || j == 6) {
         if (i == 6 && j == 6)   // (px py|px py) = 1/2[(px px|px px) - (px px|py py)]
             return 0.5 * (local_integral_sum(A, B, 9, 9, r) - local_integral_sum(A, B, 9, 4, r));
@@ -158,7 +159,7 @@ __device__ double aux_B(int k, double q) {
     return (k % 2 == 0) ? 2.0 / (1.0 + k) : 0.0;
 }
 
-__device__ double overlap_local(const ElemDev& A, const ElemDev& B, int la, int lb, int m, double r) {
+__device__ __forceinline__ double overlap_local(const ElemDev& A, const ElemDev& B, int la, int lb, int m, double r) {
     double za = A.zeta[la], zb = B.zeta[lb];
     int na = A.nshell, nb = B.nshell;
     int cid = ((((na - 1) * 2 + la) * 2 + (nb - 1)) * 2 + lb) * 2 + m;
@@ -181,13 +182,36 @@ __device__ double overlap_local(const ElemDev& A, const ElemDev& B, int la, int 
     return pre * s;
 }
 
-// One wavefront per pair.  Writes:
+// Pair core repulsion (Mndo.cpp:223-240, 319-327; Am1.cpp:145-170) from the
+// rotated (ss|ss) integral gss.
+__device__ __forceinline__ double pair_core_repulsion(const ElemDev& A, const ElemDev& B, double r, double gss, int theory) {
+    const double ang = 1.0 / 0.5291772;
+    double f;
+    if (A.Z == 1 && (B.Z == 7 || B.Z == 8))
+        f = 1.0 + (r / ang) * exp(-B.alpha * r) + exp(-A.alpha * r);
+    else if (B.Z == 1 && (A.Z == 7 || A.Z == 8))
+        f = 1.0 + (r / ang) * exp(-A.alpha * r) + exp(-B.alpha * r);
+    else
+        f = 1.0 + exp(-A.alpha * r) + exp(-B.alpha * r);
+    double e = A.core * B.core * gss * f;
+    if (theory != 0) {
+        double t = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            t += A.K[k] * exp(-A.L[k] * (r - A.M[k]) * (r - A.M[k]));
+            t += B.K[k] * exp(-B.L[k] * (r - B.M[k]) * (r - B.M[k]));
+        }
+        e += A.core * B.core * t * ang / r;
+    }
+    return e;
+}
+
+// One wavefront per pair (the X-X pairs, plist[nlight ..]).  Writes:
 //   ints[ioff + i*nbp + j]  rotated (mu nu|lam sig), packed mu<=nu on A, lam<=sig on B
 //   H (arena AR_H) off-diagonal block  beta_bar * S  and its transpose
 //   part[slot*20 + 0..9]   -Z_B (mu nu|s s) for A,  part[slot*20+10..19] -Z_A (s s|lam sig) for B
 //   ecore[slot]            pair core repulsion
 __global__ void __launch_bounds__(256) k_pair_setup(
-    const PairDev* __restrict__ pairs, int npairs, const AtomDev* __restrict__ atoms,
+    const PairDev* __restrict__ pairs, const int* __restrict__ plist, int npairs, const AtomDev* __restrict__ atoms,
     const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs,
     double* __restrict__ ints, double* __restrict__ arena, double* __restrict__ part,
     double* __restrict__ ecore, double* __restrict__ sdbg) {
@@ -196,7 +220,7 @@ __global__ void __launch_bounds__(256) k_pair_setup(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int pidx = blockIdx.x * 4 + wave;
     const bool live = pidx < npairs;
-    PairDev pr = live ? pairs[pidx] : pairs[0];
+    PairDev pr = pairs[plist[live ? pidx : 0]];
     const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
     // parameters read in place: a private copy indexed at run time would live in scratch
     const ElemDev& A = elems[aA.elem];
@@ -282,27 +306,119 @@ __global__ void __launch_bounds__(256) k_pair_setup(
     // electron-core attraction partials and the core repulsion
     if (lane < nap) part[(long long)pr.slot * 20 + lane] = -B.core * L[lane * nbp + 0];
     if (lane >= 16 && lane < 16 + nbp) part[(long long)pr.slot * 20 + 10 + (lane - 16)] = -A.core * L[lane - 16];
-    if (lane == 63) {
-        const double ang = 1.0 / 0.5291772;
-        double gss = L[0];
-        double f;
-        if (A.Z == 1 && (B.Z == 7 || B.Z == 8))
-            f = 1.0 + (r / ang) * exp(-B.alpha * r) + exp(-A.alpha * r);
-        else if (B.Z == 1 && (A.Z == 7 || A.Z == 8))
-            f = 1.0 + (r / ang) * exp(-A.alpha * r) + exp(-B.alpha * r);
-        else
-            f = 1.0 + exp(-A.alpha * r) + exp(-B.alpha * r);
-        double e = A.core * B.core * gss * f;
-        if (jb.theory != 0) {
-            double t = 0.0;
-            for (int k = 0; k < 4; ++k) {
-                t += A.K[k] * exp(-A.L[k] * (r - A.M[k]) * (r - A.M[k]));
-                t += B.K[k] * exp(-B.L[k] * (r - B.M[k]) * (r - B.M[k]));
-            }
-            e += A.core * B.core * t * ang / r;
+    if (lane == 63) ecore[pr.slot] = pair_core_repulsion(A, B, r, L[0], jb.theory);
+}
+
+// One lane per light pair (H-H, heavy-H, H-heavy: plist[0 .. nlight)), with
+// the operations of k_pair_setup in the same order (this file is compiled
+// without contraction), so every value is bitwise the same.  Terms that are
+// products with an exact zero of the rotation (the H rows and columns) are
+// left out: adding them changes at most the sign of a zero.
+__device__ inline double sdot_ri(const double R[4][4], const double* Sl, int i) {
+    // (R S_loc R^T)[i][s] for an H on B: sum_k R[i][k] S_loc[k][s]
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s += R[i][k] * Sl[k];
+    return s;
+}
+
+__global__ void __launch_bounds__(256) k_setup_light(
+    const PairDev* __restrict__ pairs, const int* __restrict__ plist, int nlight, const AtomDev* __restrict__ atoms,
+    const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs, double* __restrict__ ints,
+    double* __restrict__ arena, double* __restrict__ part, double* __restrict__ ecore, double* __restrict__ sdbg) {
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= nlight) return;
+    const PairDev pr = pairs[plist[id]];
+    const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
+    const ElemDev& A = elems[aA.elem];
+    const ElemDev& B = elems[aB.elem];
+    const double dx = aB.x - aA.x, dy = aB.y - aA.y, dz = aB.z - aA.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const JobDev jb = jobs[pr.job];
+    const int n = jb.nao;
+    double* H = arena + jb.arena + (long long)AR_H * n * n;
+    double* pp = part + (long long)pr.slot * 20;
+    auto put_h = [&](int i, int j, double sv) {
+        const double bb = 0.5 * (A.beta[i == ORB_S ? 0 : 1] + B.beta[j == ORB_S ? 0 : 1]);
+        const int mu = aA.ao + i, nu = aB.ao + j;
+        H[(long long)mu * n + nu] = bb * sv;
+        H[(long long)nu * n + mu] = bb * sv;
+        if (sdbg) {
+            sdbg[jb.arena + (long long)mu * n + nu] = sv;
+            sdbg[jb.arena + (long long)nu * n + mu] = sv;
         }
-        ecore[pr.slot] = e;
+    };
+    if (A.nao == 1 && B.nao == 1) {   // H-H: one integral, the rotation-invariant (s|s) overlap
+        const double g = local_integral(A, B, 0, 0, r);
+        ints[pr.ioff] = g;
+        put_h(0, 0, overlap_local(A, B, 0, 0, 0, r));
+        pp[0] = -B.core * g;
+        pp[10] = -A.core * g;
+        ecore[pr.slot] = pair_core_repulsion(A, B, r, g, jb.theory);
+        return;
     }
+    double R[4][4];
+    rotation(dx, dy, dz, R);
+    // the lane's rotation, element-major in LDS, for the runtime-indexed reads below
+    __shared__ double sR[16 * 256];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) sR[(a * 4 + b) * 256 + threadIdx.x] = R[a][b];
+    const double* Rl = sR + threadIdx.x;
+    double I[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) I[i] = 0.0;
+    if (A.nao == 4) {   // heavy on A, H on B: I[i] = sum_k T_A[i][k] L[k] (10 x 1)
+        for (int k = 0; k < 10; ++k) {
+            const double Lk = local_integral(A, B, k, 0, r);
+            const int al = c_pk[k][0], be = c_pk[k][1];
+#pragma unroll
+            for (int i = 0; i < 10; ++i) {
+                const int mu = c_pk[i][0], nu = c_pk[i][1];
+                double t = Rl[(mu * 4 + al) * 256] * Rl[(nu * 4 + be) * 256];
+                if (al != be) t += Rl[(mu * 4 + be) * 256] * Rl[(nu * 4 + al) * 256];
+                I[i] += t * Lk;
+            }
+        }
+        const double Sl[4] = {overlap_local(A, B, 0, 0, 0, r), 0.0, overlap_local(A, B, 1, 0, 0, r), 0.0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) put_h(i, 0, sdot_ri(R, Sl, i));
+#pragma unroll
+        for (int e = 0; e < 10; ++e) {
+            ints[pr.ioff + e] = I[e];
+            pp[e] = -B.core * I[e];
+        }
+        pp[10] = -A.core * I[0];
+    } else {            // H on A, heavy on B: I[j] = sum_l L[l] T_B[j][l] (1 x 10)
+        for (int l = 0; l < 10; ++l) {
+            const double Ll = local_integral(A, B, 0, l, r);
+            const int al = c_pk[l][0], be = c_pk[l][1];
+#pragma unroll
+            for (int jj = 0; jj < 10; ++jj) {
+                const int mu = c_pk[jj][0], nu = c_pk[jj][1];
+                double t = Rl[(mu * 4 + al) * 256] * Rl[(nu * 4 + be) * 256];
+                if (al != be) t += Rl[(mu * 4 + be) * 256] * Rl[(nu * 4 + al) * 256];
+                I[jj] += Ll * t;
+            }
+        }
+        // (R S_loc R^T)[s][j] = sum_l S_loc[s][l] R[j][l]
+        const double Sl[4] = {overlap_local(A, B, 0, 0, 0, r), 0.0, overlap_local(A, B, 0, 1, 0, r), 0.0};
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            double sv = 0.0;
+#pragma unroll
+            for (int l = 0; l < 4; ++l) sv += Sl[l] * R[jj][l];
+            put_h(0, jj, sv);
+        }
+#pragma unroll
+        for (int e = 0; e < 10; ++e) {
+            ints[pr.ioff + e] = I[e];
+            pp[10 + e] = -A.core * I[e];
+        }
+        pp[0] = -B.core * I[0];
+    }
+    ecore[pr.slot] = pair_core_repulsion(A, B, r, I[0], jb.theory);
 }
 
 __device__ inline long long tri_slot(int a, int b, int N) {   // a < b, job-local
@@ -374,12 +490,15 @@ __global__ void __launch_bounds__(256) k_ecore_reduce(const JobDev* __restrict__
     }
 }
 
-void launch_setup(hipStream_t st, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
-                  const ElemDev* elems, const JobDev* jobs, int njobs, double* ints, double* arena,
-                  double* part, double* ecore, JobState* state, double* sdbg) {
-    if (npairs > 0)
-        hipLaunchKernelGGL(k_pair_setup, dim3((npairs + 3) / 4), dim3(256), 0, st, pairs, npairs, atoms,
+void launch_setup(hipStream_t st, const PairDev* pairs, const int* plist, int nlight, int nheavy,
+                  const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, int njobs, double* ints,
+                  double* arena, double* part, double* ecore, JobState* state, double* sdbg) {
+    if (nlight > 0)
+        hipLaunchKernelGGL(k_setup_light, dim3((nlight + 255) / 256), dim3(256), 0, st, pairs, plist, nlight, atoms,
                            elems, jobs, ints, arena, part, ecore, sdbg);
+    if (nheavy > 0)
+        hipLaunchKernelGGL(k_pair_setup, dim3((nheavy + 3) / 4), dim3(256), 0, st, pairs, plist + nlight, nheavy,
+                           atoms, elems, jobs, ints, arena, part, ecore, sdbg);
     hipLaunchKernelGGL(k_atom_hcore, dim3((natoms + 3) / 4), dim3(256), 0, st, atoms, natoms, elems, jobs,
                        part, arena);
     hipLaunchKernelGGL(k_ecore_reduce, dim3(njobs), dim3(256), 0, st, jobs, ecore, state);

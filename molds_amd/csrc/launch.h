@@ -18,9 +18,9 @@ void set_tuning(const Tuning& t);  // for the current device
 
 int upload_overlap_table(const double* table /* 32*25 */);
 
-void launch_setup(hipStream_t st, const PairDev* pairs, int npairs, const AtomDev* atoms, int natoms,
-                  const ElemDev* elems, const JobDev* jobs, int njobs, double* ints, double* arena,
-                  double* part, double* ecore, JobState* state, double* sdbg);
+void launch_setup(hipStream_t st, const PairDev* pairs, const int* plist, int nlight, int nheavy,
+                  const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, int njobs, double* ints,
+                  double* arena, double* part, double* ecore, JobState* state, double* sdbg);
 
 // FP32 copy of the packed integrals for mixed-precision jobs.
 void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long long nints);
