@@ -556,17 +556,43 @@ struct SecSums {
     double psi, dpsi, phi, dphi, asum;
 };
 
+// sum_{t0 <= t < t1} z_t^2 / ((dl_t - dO) - tau) and its derivative sum, with
+// sum |terms|, as four interleaved partial sums so that four terms' reciprocal
+// chains are in flight at once.
+__device__ inline void dc_sums_range(const double* dl, const double* zl, int t0, int t1, double dO, double tau,
+                                     double& sum, double& dsum, double& asum) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0}, ds[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
+    int t = t0;
+    for (; t + 3 < t1; t += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double z = zl[t + u];
+            const double inv = rcp_nr((dl[t + u] - dO) - tau);
+            const double term = z * z * inv;
+            s[u] += term;
+            ds[u] += term * inv;
+            as[u] += fabs(term);
+        }
+    }
+    for (; t < t1; ++t) {
+        const double z = zl[t];
+        const double inv = rcp_nr((dl[t] - dO) - tau);
+        const double term = z * z * inv;
+        s[0] += term;
+        ds[0] += term * inv;
+        as[0] += fabs(term);
+    }
+    sum = (s[0] + s[1]) + (s[2] + s[3]);
+    dsum = (ds[0] + ds[1]) + (ds[2] + ds[3]);
+    asum = (as[0] + as[1]) + (as[2] + as[3]);
+}
+
 __device__ inline SecSums dc_sums(const double* dl, const double* zl, int kk, double rho, int j, double dO,
                                   double tau) {
-    double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, asum = 0.0;
-    for (int t = 0; t < kk; ++t) {
-        const double inv = rcp_nr((dl[t] - dO) - tau);
-        const double term = zl[t] * zl[t] * inv, dterm = term * inv;
-        if (t <= j) { psi += term; dpsi += dterm; }
-        else { phi += term; dphi += dterm; }
-        asum += fabs(term);
-    }
-    return SecSums{rho * psi, rho * dpsi, rho * phi, rho * dphi, asum};
+    double psi, dpsi, phi, dphi, a1, a2;
+    dc_sums_range(dl, zl, 0, j + 1, dO, tau, psi, dpsi, a1);
+    dc_sums_range(dl, zl, j + 1, kk, dO, tau, phi, dphi, a2);
+    return SecSums{rho * psi, rho * dpsi, rho * phi, rho * dphi, a1 + a2};
 }
 
 // Root j (0-based) of the secular equation 1 + rho sum_t z_t^2 / (dl_t - x) = 0
@@ -576,31 +602,34 @@ __device__ inline SecSums dc_sums(const double* dl, const double* zl, int kk, do
 // the iteration.
 __device__ void dc_secular(const double* dl, const double* zl, int kk, double rho, int j, int* orig,
                            double* taup) {
-    int o;
-    double lo, hi, tau;
-    SecSums S;
-    if (j < kk - 1) {
-        const double mid = 0.5 * (dl[j + 1] - dl[j]);
-        S = dc_sums(dl, zl, kk, rho, j, dl[j], mid);
-        if (1.0 + S.psi + S.phi >= 0.0) { o = j; lo = 0.0; hi = mid; tau = mid; }
-        else { o = j + 1; lo = -mid; hi = 0.0; tau = -mid; }
-    } else {
+    int o = kk - 1;
+    double lo = 0.0, hi = 0.0, tau = 0.0, dO = dl[kk - 1];
+    // the first evaluation point: the middle of (dl_j, dl_j+1) seen from dl_j,
+    // or for the last root half of rho |z|^2 above the last pole
+    const bool inner = j < kk - 1;
+    const double mid = inner ? 0.5 * (dl[j + 1] - dl[j]) : 0.0;
+    double evO = inner ? dl[j] : dl[kk - 1], evT = mid;
+    if (!inner) {
         double zz = 0.0;
         for (int t = 0; t < kk; ++t) zz += zl[t] * zl[t];
-        o = kk - 1; lo = 0.0; hi = rho * zz;
+        hi = rho * zz;
         tau = 0.5 * hi;
-        S = dc_sums(dl, zl, kk, rho, j, dl[o], tau);
+        evT = tau;
     }
-    const double dO = dl[o];
     for (int it = 0; it < 100; ++it) {
-        if (it > 0) S = dc_sums(dl, zl, kk, rho, j, dO, tau);
+        const SecSums S = dc_sums(dl, zl, kk, rho, j, evO, evT);   // the one evaluation site
+        if (it == 0 && inner) {   // the first value picks the nearer pole as origin
+            if (1.0 + S.psi + S.phi >= 0.0) { o = j; lo = 0.0; hi = mid; tau = mid; }
+            else { o = j + 1; lo = -mid; hi = 0.0; tau = -mid; }
+            dO = dl[o];
+        }
         const double f = 1.0 + S.psi + S.phi;
         if (fabs(f) <= 8.0 * kEps * (1.0 + rho * S.asum)) break;
         if (f < 0.0) lo = tau;
         else hi = tau;
         double eta = 0.0;
         bool have = false;
-        if (j < kk - 1) {
+        if (inner) {
             // psi ~ a1 + b1/(p_j - x), phi ~ a2 + b2/(p_j+1 - x), matched in value and slope at tau
             const double u1 = (dl[j] - dO) - tau, u2 = (dl[j + 1] - dO) - tau;
             const double b1 = u1 * u1 * S.dpsi, b2 = u2 * u2 * S.dphi;
@@ -632,6 +661,8 @@ __device__ void dc_secular(const double* dl, const double* zl, int kk, double rh
             break;
         }
         tau = nt;
+        evO = dO;
+        evT = tau;
     }
     *orig = o;
     *taup = tau;
@@ -775,8 +806,12 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             } else if (lane == 0) {
                 int pj = -1, nr = 0;
                 double zp = 0.0, dp = 0.0;
+                // the next entry's (z, d) is loaded one iteration ahead: entries at
+                // or after jj are not written before their own iteration
+                double zn = zs[a], dn = ds[a];
                 for (int jj = a; jj < b; ++jj) {
-                    double zj = zs[jj], dj = ds[jj];
+                    double zj = zn, dj = dn;
+                    if (jj + 1 < b) { zn = zs[jj + 1]; dn = ds[jj + 1]; }
                     if (rho * fabs(zj) <= tol) { flag[jj] = 1; continue; }
                     flag[jj] = 0;
                     if (pj >= 0) {
