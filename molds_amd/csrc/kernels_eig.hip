@@ -339,55 +339,60 @@ __device__ inline void buf_store_d(double v, __amdgpu_buffer_rsrc_t r, u32 voff,
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
 }
 
-template <bool PG>
-__global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
-                                                      const int* __restrict__ joblist, double* __restrict__ arena,
-                                                      double* __restrict__ tri) {
-    extern __shared__ double dyn[];
-    __shared__ double red[16], red2[16], sh_alpha;
-    const int job = joblist[blockIdx.x];
-    const JobDev jb = jobs[job];
-    if (state[job].phase != PH_ACTIVE) return;
-    const unsigned long long t0 = wall_clock64();
-    const int n = jb.nao;
-    // the wave index as a scalar: everything derived from it (work items, rows)
-    // stays in scalar registers and scalar branches
+// Storage of the trailing matrix: FP64, or FP32 for mixed-precision jobs in
+// their FP32 iterations (half the HBM bytes; every product and sum in FP64).
+template <typename TS>
+__device__ inline double sym_load(__amdgpu_buffer_rsrc_t r, u32 voff, u32 soff);
+template <>
+__device__ inline double sym_load<double>(__amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+    return buf_load_d(r, voff, soff);
+}
+template <>
+__device__ inline double sym_load<float>(__amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+    return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <typename TS>
+__device__ inline void sym_store(double v, __amdgpu_buffer_rsrc_t r, u32 voff, u32 soff);
+template <>
+__device__ inline void sym_store<double>(double v, __amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+    buf_store_d(v, r, voff, soff);
+}
+template <>
+__device__ inline void sym_store<float>(double v, __amdgpu_buffer_rsrc_t r, u32 voff, u32 soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(u32, (float)v), r, voff, soff, 0);
+}
+
+// The reduction of one job. As: the trailing matrix (upper triangle) in storage
+// TS; W: the reflector rows (row k holds v_k in columns k+1..n-1).
+template <bool PG, typename TS>
+__device__ void tridiag_sym_body(TS* __restrict__ As, double* __restrict__ W, const double* __restrict__ F,
+                                 double* colpart, double* dyn, int n, int job, double* d, double* e,
+                                 double* tau, double* red, double* red2) {
     const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, nw = T >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr u32 ES = sizeof(TS);
     double* vcur = dyn;
     double* vnext = vcur + n;
     double* p = vnext + n;
     double* pnext = p + n;
-    double* A = amat(arena, jb, AR_W);
-    double* colpart = PG ? amat(arena, jb, AR_U) : pnext + n;
-    const __amdgpu_buffer_rsrc_t Ab = __builtin_amdgcn_make_buffer_rsrc(A, (short)0, n * n * 8, 0x00020000);
+    // the pivot row between its update and its reflector (free between the
+    // reduction of one step and the band pass of the next)
+    double* xrow = colpart;
+    const __amdgpu_buffer_rsrc_t Ab = __builtin_amdgcn_make_buffer_rsrc(As, (short)0, n * n * (int)ES, 0x00020000);
     const int ccap = sym_col_cap(n);
-    const double* F = amat(arena, jb, AR_F);
-    double* d = eig_ws(tri, jb, job);
-    double* e = d + n;
-    double* tau = e + n;
-    const long long nn = (long long)n * n;
-    for (long long x = tid; x < nn; x += T) A[x] = F[x];
-    for (int i = tid; i < n; i += T) { e[i] = 0.0; tau[i] = 0.0; }
+    for (int r = wave; r < n; r += nw)
+        for (int c = r + lane; c < n; c += 64) As[(long long)r * n + c] = (TS)F[(long long)r * n + c];
     __syncthreads();
-    if (n < 3) {
-        if (tid == 0) {
-            d[0] = A[0];
-            if (n == 2) { e[0] = A[1]; d[1] = A[3]; }
-            state[job].prof[0] = (double)(wall_clock64() - t0);
-        }
-        return;
-    }
     double tk;
-    {   // reflector of row 0, then p = tau_0 A[1:, 1:] v_0 from the full (symmetric) rows
+    {   // reflector of row 0, then p = tau_0 F[1:, 1:] v_0 from the full (symmetric) rows of F
         double ss = 0.0;
-        for (int t = 2 + tid; t < n; t += T) { const double x = A[t]; ss += x * x; }
+        for (int t = 2 + tid; t < n; t += T) { const double x = F[t]; ss += x * x; }
         ss = group_sum(ss, 64);
         if (lane == 0) red2[wave] = ss;
         __syncthreads();
         ss = 0.0;
         for (int w = 0; w < nw; ++w) ss += red2[w];
-        const double alpha = A[1];
+        const double alpha = F[1];
         double beta = alpha, scale = 0.0;
         tk = 0.0;
         if (ss > 0.0) {
@@ -395,15 +400,15 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
             tk = (beta - alpha) / beta;
             scale = 1.0 / (alpha - beta);
         }
-        for (int t = 1 + tid; t < n - 1; t += T) vcur[t] = ss > 0.0 ? A[1 + t] * scale : 0.0;
-        if (tid == 0) { vcur[0] = 1.0; d[0] = A[0]; e[0] = beta; tau[0] = tk; }
+        for (int t = 1 + tid; t < n - 1; t += T) vcur[t] = ss > 0.0 ? F[1 + t] * scale : 0.0;
+        if (tid == 0) { vcur[0] = 1.0; d[0] = F[0]; e[0] = beta; tau[0] = tk; }
         __syncthreads();
-        for (int t = 1 + tid; t < n - 1; t += T) A[1 + t] = vcur[t];
-        if (tid == 0) A[1] = 1.0;
+        for (int t = 1 + tid; t < n - 1; t += T) W[1 + t] = vcur[t];
+        if (tid == 0) W[1] = 1.0;
         double pv = 0.0;
         for (int i = 1 + wave; i < n; i += nw) {
             double s = 0.0;
-            for (int j = 1 + lane; j < n; j += 64) s += A[(long long)i * n + j] * vcur[j - 1];
+            for (int j = 1 + lane; j < n; j += 64) s += F[(long long)i * n + j] * vcur[j - 1];
             s = group_sum(s, 64);
             if (lane == 0) { p[i - 1] = tk * s; pv += tk * s * vcur[i - 1]; }
         }
@@ -419,28 +424,26 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
         if (k == n - 3) {   // last trailing 2x2 block (rows/cols n-2, n-1)
             if (tid == 0) {
                 const double w1 = p[1] + K * vcur[1];
-                d[n - 2] = A[(long long)(n - 2) * n + n - 2] - 2.0 * vcur[0] * w0;
-                e[n - 2] = A[(long long)(n - 2) * n + n - 1] - (vcur[0] * w1 + w0 * vcur[1]);
-                d[n - 1] = A[(long long)(n - 1) * n + n - 1] - 2.0 * vcur[1] * w1;
+                d[n - 2] = (double)As[(long long)(n - 2) * n + n - 2] - 2.0 * vcur[0] * w0;
+                e[n - 2] = (double)As[(long long)(n - 2) * n + n - 1] - (vcur[0] * w1 + w0 * vcur[1]);
+                d[n - 1] = (double)As[(long long)(n - 1) * n + n - 1] - 2.0 * vcur[1] * w1;
             }
             break;
         }
-        // row k+1 of the updated block (upper part); its tail defines the next reflector
+        // row k+1 of the updated block (upper part) into xrow; its tail defines the next reflector
         const long long r1 = (long long)(k + 1) * n + k + 1;
         double ss = 0.0;
         for (int t = tid; t < m; t += T) {
-            const double a = A[r1 + t] - (vcur[0] * (p[t] + K * vcur[t]) + w0 * vcur[t]);
-            A[r1 + t] = a;
+            const double a = (double)As[r1 + t] - (vcur[0] * (p[t] + K * vcur[t]) + w0 * vcur[t]);
+            xrow[t] = a;
             if (t >= 2) ss += a * a;
-            else if (t == 1) sh_alpha = a;
-            else d[k + 1] = a;
         }
         ss = group_sum(ss, 64);
         if (lane == 0) red2[wave] = ss;
         __syncthreads();   // 1
         ss = 0.0;
         for (int w = 0; w < nw; ++w) ss += red2[w];
-        const double alpha = sh_alpha;
+        const double alpha = xrow[1];
         double beta = alpha, tn = 0.0, scale = 0.0;
         if (ss > 0.0) {
             beta = -copysign(sqrt(alpha * alpha + ss), alpha);
@@ -449,11 +452,11 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
         }
         const int m2 = m - 1;   // length of v_{k+1}
         for (int t = 1 + tid; t < m2; t += T) {
-            const double v = ss > 0.0 ? A[r1 + 1 + t] * scale : 0.0;
+            const double v = ss > 0.0 ? xrow[1 + t] * scale : 0.0;
             vnext[t] = v;
-            A[r1 + 1 + t] = v;
+            W[r1 + 1 + t] = v;
         }
-        if (tid == 0) { vnext[0] = 1.0; A[r1 + 1] = 1.0; e[k + 1] = beta; tau[k + 1] = tn; }
+        if (tid == 0) { vnext[0] = 1.0; W[r1 + 1] = 1.0; d[k + 1] = xrow[0]; e[k + 1] = beta; tau[k + 1] = tn; }
         __syncthreads();   // 2
         // upper triangle of rows/columns base .. n-1: rank-2 update of step k and
         // the partial sums of A v_{k+1}
@@ -470,24 +473,48 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
             while (b >= min(per_tile * (ct + 1), nb)) { b -= min(per_tile * (ct + 1), nb); ++ct; }
             const int c = base + 64 * ct + lane;
             const bool cv = c < n;
-            const u32 cbyte = (u32)(cv ? c : 0) * 8u;
+            const u32 cbyte = (u32)(cv ? c : 0) * ES;
             const int tc = c - k - 1;
             const double vi = cv ? vcur[tc] : 0.0;
             const double wi = cv ? p[tc] + K * vi : 0.0;
             const double vnc = cv ? vnext[tc - 1] : 0.0;
             double colacc = 0.0;
-            const int r0 = base + b * BR;
-            for (int sc = 0; sc < BR; sc += 4) {
-                const int rs = r0 + sc;
-                if (rs >= n) break;
-                // element (rs + u, c) at byte voff + u * n * 8; only lanes on or
-                // above the diagonal load (the lower triangle is never touched)
-                const u32 voff = (u32)rs * (u32)n * 8u + cbyte;
-                double a[4], x[4];
+            const int r0 = base + b * BR, rend = min(r0 + BR, n);
+            // element (r, c) at byte r * n * ES + cbyte; only lanes on or above
+            // the diagonal load (the lower triangle is never touched). With FP32
+            // storage the next 4-row slice is loaded while the current one is
+            // processed (four more registers); FP64 storage has no room for it.
+            constexpr bool PF = sizeof(TS) == 4;
+            TS nx[4];
+            if constexpr (PF) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int r = rs + u;
-                    a[u] = (cv && r <= c) ? buf_load_d(Ab, voff, (u32)u * (u32)n * 8u) : 0.0;
+                    const int r = r0 + u;
+                    nx[u] = (cv && r <= c && r < rend) ? (TS)sym_load<TS>(Ab, (u32)r0 * (u32)n * ES + cbyte,
+                                                                          (u32)u * (u32)n * ES)
+                                                        : (TS)0;
+                }
+            }
+            for (int rs = r0; rs < rend; rs += 4) {
+                const u32 voff = (u32)rs * (u32)n * ES + cbyte;
+                double a[4], x[4];
+                if constexpr (PF) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) a[u] = (double)nx[u];
+                    const int rn = rs + 4;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int r = rn + u;
+                        nx[u] = (cv && r <= c && r < rend) ? (TS)sym_load<TS>(Ab, (u32)rn * (u32)n * ES + cbyte,
+                                                                              (u32)u * (u32)n * ES)
+                                                            : (TS)0;
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int r = rs + u;
+                        a[u] = (cv && r <= c) ? sym_load<TS>(Ab, voff, (u32)u * (u32)n * ES) : 0.0;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -498,13 +525,13 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
                     const double wj = r < n ? p[tj] + K * vj : 0.0;
                     const double vnr = r < n ? vnext[tj - 1] : 0.0;
                     const double an = a[u] - (vj * wi + wj * vi);
-                    if (ok) buf_store_d(an, Ab, voff, (u32)u * (u32)n * 8u);
+                    if (ok) sym_store<TS>(an, Ab, voff, (u32)u * (u32)n * ES);
                     colacc += ok ? an * vnr : 0.0;
                     x[u] = (ok && r < c) ? an * vnc : 0.0;
                 }
-                const double s = rows4_reduce(x[0], x[1], x[2], x[3]);
+                const double s4 = rows4_reduce(x[0], x[1], x[2], x[3]);
                 const int rr = rs + ((lane >> 4) & 3);
-                if ((lane & 15) == 0 && rr < n) rowpart[ct * m2 + (rr - base)] = s;
+                if ((lane & 15) == 0 && rr < n) rowpart[ct * m2 + (rr - base)] = s4;
             }
             if (cv) colpart[b * m2 + (c - base)] = colacc;
         }
@@ -515,14 +542,14 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
         while (S < 64 && 2 * S * m2 <= T) S *= 2;
         pv = 0.0;
         for (int t0s = 0; t0s < m2; t0s += T / S) {
-            const int t = t0s + tid / S, s = tid & (S - 1);
+            const int t = t0s + tid / S, sl = tid & (S - 1);
             double q = 0.0;
             if (t < m2) {
-                for (int bb = s; bb <= t / BR; bb += S) q += colpart[bb * m2 + t];
-                for (int cc = (t >> 6) + s; cc < nct; cc += S) q += rowpart[cc * m2 + t];
+                for (int bb = sl; bb <= t / BR; bb += S) q += colpart[bb * m2 + t];
+                for (int cc = (t >> 6) + sl; cc < nct; cc += S) q += rowpart[cc * m2 + t];
             }
             q = group_sum(q, S);
-            if (s == 0 && t < m2) {
+            if (sl == 0 && t < m2) {
                 const double pn = tn * q;
                 pnext[t] = pn;
                 pv += pn * vnext[t];
@@ -535,6 +562,42 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
         tmp = p; p = pnext; pnext = tmp;
         tk = tn;
     }
+}
+
+template <bool PG>
+__global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                                      const int* __restrict__ joblist, double* __restrict__ arena,
+                                                      double* __restrict__ tri) {
+    extern __shared__ double dyn[];
+    __shared__ double red[16], red2[16];
+    const int job = joblist[blockIdx.x];
+    const JobDev jb = jobs[job];
+    if (state[job].phase != PH_ACTIVE) return;
+    const unsigned long long t0 = wall_clock64();
+    const int n = jb.nao, T = blockDim.x, tid = threadIdx.x;
+    double* W = amat(arena, jb, AR_W);
+    const double* F = amat(arena, jb, AR_F);
+    double* d = eig_ws(tri, jb, job);
+    double* e = d + n;
+    double* tau = e + n;
+    for (int i = tid; i < n; i += T) { e[i] = 0.0; tau[i] = 0.0; }
+    if (n < 3) {
+        __syncthreads();
+        if (tid == 0) {
+            d[0] = F[0];
+            if (n == 2) { e[0] = F[1]; d[1] = F[3]; }
+            state[job].prof[0] = (double)(wall_clock64() - t0);
+        }
+        return;
+    }
+    double* colpart = PG ? amat(arena, jb, AR_U) : dyn + 4 * n;
+    // a mixed-precision job in its FP32 iterations keeps the trailing matrix in
+    // FP32 in its AR_U matrix (free until divide and conquer)
+    if (!PG && state[job].fp32)
+        tridiag_sym_body<false, float>(reinterpret_cast<float*>(amat(arena, jb, AR_U)), W, F, colpart, dyn, n, job,
+                                       d, e, tau, red, red2);
+    else
+        tridiag_sym_body<PG, double>(W, W, F, colpart, dyn, n, job, d, e, tau, red, red2);
     if (tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
 }
 
