@@ -44,16 +44,17 @@ METRIC = "PM3 SCF solves/s + p50/p99 latency, mixed vs FP64 GPU at |ΔE|≤1e-5 
 EV = 1.0 / 0.03674903
 FP64_PEAK_TFLOPS = 78.6       # MI355X dense FP64 (matrix and vector)
 HBM_PEAK_TBPS = 8.0           # MI355X HBM3E
-# HBM bytes per launch of the dominant kernel on the 4096-molecule FP64 pass,
-# from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in this file
-TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_tridiag_sym_batched_pm3_4096_fp64.md")
+# HBM bytes per launch of the dominant kernel on the 4096-molecule pass of a
+# precision mode, from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised
+# in these files
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_tridiag_sym_batched_pm3_4096_{mode}.md")
 
 
-def measured_traffic():
-    """The `traffic_bytes_per_launch:` line of the committed counter summary,
-    or None when it is absent."""
+def measured_traffic(mode):
+    """The `traffic_bytes_per_launch:` line of the committed counter summary
+    for `mode`, or None when it is absent."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(TRAFFIC_PROFILE.format(mode=mode)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return int(float(line.split(":", 1)[1]))
@@ -285,7 +286,8 @@ def main():
         res, wall, st = batched_leg(engine, jobs, args.steps, args.warmup)
         ok = np.array([r["status"] == 0 for r in res])
         iters = np.array([r["iterations"] for r in res], dtype=float)
-        legs[mode] = dict(res=res, wall=wall, stages=st, ok=ok, iters=iters,
+        it64 = np.array([r["fp64_iterations"] for r in res], dtype=float)
+        legs[mode] = dict(res=res, wall=wall, stages=st, ok=ok, iters=iters, it64=it64,
                           solves_per_s=len(jobs) * args.steps / wall, ms_per_step=1e3 * wall / args.steps)
         log(f"{mode}: {legs[mode]['solves_per_s']:.1f} solves/s, {legs[mode]['ms_per_step']:.1f} ms/step, "
             f"converged {ok.sum()}/{len(ok)}, mean iters {iters.mean():.1f}, stages {st}")
@@ -309,15 +311,17 @@ def main():
     # on the engine's stream around its launches; algorithmic work (4/3) n^3
     # FP64 flops per eigensolve, one eigensolve per SCF iteration per molecule.
     # The same launches seen as HBM traffic: the upper triangle of the trailing
-    # block read and written once per Householder step, 8 n^3 / 3 bytes.
+    # block read and written once per Householder step, 8 n^3 / 3 bytes with
+    # FP64 storage, 4 n^3 / 3 in the FP32 iterations of a mixed-precision job.
     info = engine.info()
     glb = nao > info["lds_tridiag_max_n"]
     ms_tri, launches_tri = best["stages"]["tridiag_glb"]
     flops = args.steps * float(np.sum(best["iters"][glb] * (4.0 / 3.0) * nao[glb] ** 3))
     achieved = flops / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
-    alg_bytes = args.steps * float(np.sum(best["iters"][glb] * (8.0 / 3.0) * nao[glb] ** 3))
+    it64 = best["it64"][glb]
+    alg_bytes = args.steps * float(np.sum((it64 * 8.0 + (best["iters"][glb] - it64) * 4.0) / 3.0 * nao[glb] ** 3))
     hbm_tbs = alg_bytes / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
-    traffic = measured_traffic() if best_mode == "fp64" and args.count == 4096 else None
+    traffic = measured_traffic(best_mode) if args.count == 4096 else None
     kernels = {k: round(v[0] / args.steps, 3) for k, v in best["stages"].items()}
 
     served = {}

@@ -246,6 +246,7 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
         nslots += jb.npairs;
         nints += c3[0] + 10 * c3[1] + 100 * c3[2];
         jb.diis_k = d.opts.diis_num_error_vect;
+        jb.want_c = 1;
         jb.arena = arena_size;
         arena_size += (long long)(ARENA_FIXED + 2 * jb.diis_k) * n * n;
         jb.vec0 = nvec;
@@ -806,6 +807,8 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
     {
         Batch b;
         b.build(descs, njobs, ok, nullptr);
+        // jobs whose caller takes no C back-transform only the occupied eigenvectors
+        for (size_t jj = 0; jj < ok.size(); ++jj) b.jobs[jj].want_c = res[ok[jj]].C != nullptr ? 1 : 0;
         b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
         e->host_s[0] += lap(tp);
         rc = b.allocate(false);
@@ -987,6 +990,7 @@ static int matrix_batch(Engine* e, Batch& b, int n, int nocc, int K) {
         std::memset(&jb, 0, sizeof(jb));
         jb.nao = n;
         jb.nocc = nocc;
+        jb.want_c = 1;   // the hooks read every eigenvector
         jb.max_iter = 1;
         jb.rms_thresh = 0.0;
         jb.arena = (long long)k * ARENA_FIXED * n * n;

@@ -1169,7 +1169,9 @@ __global__ void __launch_bounds__(512) k_bt_wy(const JobDev* __restrict__ jobs, 
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
     const int n = jb.nao, nref = n - 2, r0 = blockIdx.y * R;
-    if (nref <= 0 || r0 >= n) return;
+    // rows (eigenvectors) wanted: all, or the occupied ones when the caller takes no C
+    const int nrow = jb.want_c ? n : jb.nocc;
+    if (nref <= 0 || r0 >= nrow) return;
     const unsigned long long t0 = wall_clock64();
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
     const int ld = n + 1;

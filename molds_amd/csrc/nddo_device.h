@@ -78,6 +78,8 @@ struct JobDev {
     // SCF options (MolDS Parameters SCF knobs)
     int max_iter, diis_k, requires_guess, precision;
     int xl;               // 1 + position in the batch's large-molecule list (0: not large)
+    int want_c;           // 1: back-transform every eigenvector; 0: only the nocc occupied ones
+                          // (all the density needs; the caller asked for no C)
     double rms_thresh, damp_thresh, damp_weight, diis_start, diis_end;
     long long traj0;      // offset (doubles) into the trajectory buffer: max_iter * 4
 };
