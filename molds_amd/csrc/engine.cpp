@@ -118,7 +118,7 @@ void Batch::release() {
     gexec = nullptr;
     iterations_launched = 0;
     graph_off = false;
-    for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
+    for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_arows, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
                      (void**)&d_ints, (void**)&d_ints32, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
                      (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
                      (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_plist, (void**)&d_list_xl, (void**)&d_xl_off,
@@ -198,12 +198,13 @@ size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter) {
 }
 
 // Two passes: the per-job sizes and offsets (sequential, cheap), then the
-// atom, pair, tile and pair-class records of the jobs in parallel threads.
+// atom, pair-row and tile records of the jobs in parallel threads (the pairs
+// themselves are written on the device).
 int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* elem_index_base) {
     (void)elem_index_base;
     (void)nd;
-    atoms.clear(); pairs.clear(); jobs.clear(); tiles.clear(); list_lds.clear(); list_glb.clear(); list_xl.clear();
-    xl_off.clear(); plist.clear();
+    atoms.clear(); arows.clear(); npairs_all = 0; jobs.clear(); tiles.clear(); list_lds.clear(); list_glb.clear();
+    list_xl.clear(); xl_off.clear();
     src = which;
     nslots = 0; nints = 0; arena_size = 0; nvec = 0; ntraj = 0;
     maxn_lds = 0; maxn_glb = 0; maxn_xl = 0; max_iter_all = 0;
@@ -269,9 +270,9 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
         max_iter_all = std::max(max_iter_all, d.opts.max_iter);
     }
     atoms.resize(natoms_all);
-    pairs.resize(nslots);
+    arows.resize(natoms_all);
+    npairs_all = nslots;
     tiles.resize(ntiles_all);
-    plist.resize(nslots);
     n_light = (int)(ncls[0] + ncls[1]);
     n_heavy = (int)ncls[2];
     const long long cbase[3] = {0, ncls[0], ncls[0] + ncls[1]};
@@ -292,22 +293,28 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
                 at.local = a;
                 ao += d.Z[a] == 1 ? 1 : 4;
             }
-            long long io = o.ints0, slot = o.pair0, cpos[3] = {cbase[0] + o.cls[0], cbase[1] + o.cls[1],
-                                                               cbase[2] + o.cls[2]};
+            // where each pair row starts; the pairs (a, b) themselves are written on
+            // the device (k_build_pairs), in the same order: slots row by row,
+            // integrals packed 1 / 10 / 100 per H-H / heavy-H / heavy-heavy pair,
+            // and each class of the pair list in (a, b) order
+            int nh = 0;
+            for (int a = 0; a < d.natoms; ++a) nh += d.Z[a] == 1;
+            const int nx = d.natoms - nh;
+            long long io = o.ints0, cpos[3] = {cbase[0] + o.cls[0], cbase[1] + o.cls[1], cbase[2] + o.cls[2]};
+            int h_upto = 0, x_upto = 0;
+            const long long N = d.natoms;
             for (int a = 0; a < d.natoms; ++a) {
                 const bool ha = d.Z[a] == 1;
-                for (int b = a + 1; b < d.natoms; ++b, ++slot) {
-                    const bool hb = d.Z[b] == 1;
-                    PairDev& pr = pairs[slot];
-                    pr.a = (int)(o.atom0 + a);
-                    pr.b = (int)(o.atom0 + b);
-                    pr.job = jj;
-                    pr.slot = (int)slot;
-                    pr.ioff = io;
-                    io += (long long)(ha ? 1 : 10) * (hb ? 1 : 10);
-                    const int c = (ha && hb) ? 0 : ((ha || hb) ? 1 : 2);
-                    plist[cpos[c]++] = (int)slot;
-                }
+                h_upto += ha;
+                x_upto += !ha;
+                const long long h_after = nh - h_upto, x_after = nx - x_upto;
+                AtomRowDev& rw = arows[o.atom0 + a];
+                rw.slot0 = o.pair0 + a * N - (long long)a * (a + 1) / 2;
+                rw.ioff0 = io;
+                for (int c = 0; c < 3; ++c) rw.cpos[c] = cpos[c];
+                io += (ha ? 1 : 10) * (h_after + 10 * x_after);
+                if (ha) { cpos[0] += h_after; cpos[1] += x_after; }
+                else { cpos[1] += h_after; cpos[2] += x_after; }
             }
             long long t = o.tile0;
             for (int i0 = 0; i0 < jb.nao; i0 += 64)
@@ -379,8 +386,9 @@ int Batch::allocate(bool want_sdbg) {
     } while (0)
     const size_t nj = std::max<size_t>(jobs.size(), 1);
     ACHK(dalloc((void**)&d_atoms, std::max<size_t>(atoms.size(), 1) * sizeof(AtomDev)));
-    ACHK(dalloc((void**)&d_pairs, std::max<size_t>(pairs.size(), 1) * sizeof(PairDev)));
-    ACHK(dalloc((void**)&d_plist, std::max<size_t>(plist.size(), 1) * sizeof(int)));
+    ACHK(dalloc((void**)&d_pairs, std::max<long long>(npairs_all, 1) * sizeof(PairDev)));
+    ACHK(dalloc((void**)&d_plist, std::max<long long>(npairs_all, 1) * sizeof(int)));
+    ACHK(dalloc((void**)&d_arows, std::max<size_t>(arows.size(), 1) * sizeof(AtomRowDev)));
     ACHK(dalloc((void**)&d_jobs, nj * sizeof(JobDev)));
     ACHK(dalloc((void**)&d_state, nj * sizeof(JobState)));
     ACHK(dalloc((void**)&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(TileDev)));
@@ -417,7 +425,7 @@ int Batch::allocate(bool want_sdbg) {
 // exactly one of the three eigensolver lists.
 size_t Batch::upload_bytes() const {
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return up(atoms.size() * sizeof(AtomDev)) + up(pairs.size() * sizeof(PairDev)) + up(plist.size() * sizeof(int)) +
+    return up(atoms.size() * sizeof(AtomDev)) + up(arows.size() * sizeof(AtomRowDev)) +
            up(jobs.size() * sizeof(JobDev)) +
            up(tiles.size() * sizeof(TileDev)) + 3 * up(jobs.size() * sizeof(int)) +
            up(jobs.size() * sizeof(long long)) + up(jobs.size() * sizeof(JobState)) + 256;
@@ -438,8 +446,7 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs, char* stage) {
         return err;
     };
     HIPCHK(h2d(d_atoms, atoms.data(), atoms.size() * sizeof(AtomDev)));
-    HIPCHK(h2d(d_pairs, pairs.data(), pairs.size() * sizeof(PairDev)));
-    HIPCHK(h2d(d_plist, plist.data(), plist.size() * sizeof(int)));
+    HIPCHK(h2d(d_arows, arows.data(), arows.size() * sizeof(AtomRowDev)));
     HIPCHK(h2d(d_jobs, jobs.data(), jobs.size() * sizeof(JobDev)));
     HIPCHK(h2d(d_tiles, tiles.data(), tiles.size() * sizeof(TileDev)));
     HIPCHK(h2d(d_list_lds, list_lds.data(), list_lds.size() * sizeof(int)));
@@ -471,6 +478,7 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs, char* stage) {
 }
 
 void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
+    launch_build_pairs(st, d_atoms, d_arows, (int)atoms.size(), d_jobs, e->d_elems, d_pairs, d_plist);
     launch_setup(st, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs,
                  (int)jobs.size(), d_ints, d_arena, d_part, d_ecore, d_state, d_sdbg);
     if (d_ints32) launch_ints_f32(st, d_ints, d_ints32, nints);
@@ -934,7 +942,10 @@ int nddo_dbg_integrals(nddo_engine* eh, int theory, int natoms, const int* Z, co
         const int N = natoms;
         std::memset(g_dense, 0, sizeof(double) * (size_t)N * N * 256);
         static const int pk[10][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 1}, {1, 2}, {1, 3}, {2, 2}, {2, 3}, {3, 3}};
-        for (const PairDev& pr : b.pairs) {
+        std::vector<PairDev> pairs(std::max<long long>(b.npairs_all, 1));   // as k_build_pairs wrote them
+        HIPCHK(hipMemcpy(pairs.data(), b.d_pairs, b.npairs_all * sizeof(PairDev), hipMemcpyDeviceToHost));
+        pairs.resize(b.npairs_all);
+        for (const PairDev& pr : pairs) {
             const int a = pr.a, bb = pr.b;
             const int nap = b.atoms[a].elem % kNumElem == 0 ? 1 : 10;
             const int nbp = b.atoms[bb].elem % kNumElem == 0 ? 1 : 10;
@@ -984,7 +995,7 @@ int nddo_dbg_fock(nddo_engine* eh, int theory, int natoms, const int* Z, const d
 // A pseudo job of n AOs and no atoms, used by the eigensolver and density hooks.
 // K jobs of one n x n matrix each (the stage-level test hooks and the autotuner).
 static int matrix_batch(Engine* e, Batch& b, int n, int nocc, int K) {
-    b.atoms.clear(); b.pairs.clear(); b.jobs.clear(); b.tiles.clear();
+    b.atoms.clear(); b.arows.clear(); b.npairs_all = 0; b.jobs.clear(); b.tiles.clear();
     for (int k = 0; k < K; ++k) {
         JobDev jb;
         std::memset(&jb, 0, sizeof(jb));

@@ -37,12 +37,12 @@ void order_and_phase(int n, const double* VT, const int* perm, double* C);
 // A set of jobs sharing one launch per SCF stage.
 struct Batch {
     std::vector<AtomDev> atoms;
-    std::vector<PairDev> pairs;
+    std::vector<AtomRowDev> arows;   // per atom: the pair row it starts (pairs are built on the device)
+    long long npairs_all = 0;
     std::vector<JobDev> jobs;
     std::vector<TileDev> tiles;
     std::vector<int> list_lds, list_glb, list_xl, src;
-    // pair indices: the light (H-H, heavy-H) pairs first, then the heavy-heavy ones
-    std::vector<int> plist;
+    // d_plist: pair indices, the light (H-H, heavy-H) pairs first, then the heavy-heavy ones
     int n_light = 0, n_heavy = 0;
     std::vector<long long> xl_off;   // per list_xl entry: offset in d_xlws
     int maxn_lds = 0, maxn_glb = 0, maxn_xl = 0, max_iter_all = 0;
@@ -52,6 +52,7 @@ struct Batch {
 
     AtomDev* d_atoms = nullptr;
     PairDev* d_pairs = nullptr;
+    AtomRowDev* d_arows = nullptr;
     JobDev* d_jobs = nullptr;
     JobState* d_state = nullptr;
     TileDev* d_tiles = nullptr;

@@ -490,6 +490,36 @@ __global__ void __launch_bounds__(256) k_ecore_reduce(const JobDev* __restrict__
     }
 }
 
+// One thread per atom a: the pairs (a, b), b > a, of its job, at the slots,
+// integral offsets and class-list positions its row starts at.
+__global__ void __launch_bounds__(256) k_build_pairs(const AtomDev* __restrict__ atoms,
+                                                     const AtomRowDev* __restrict__ rows, int natoms,
+                                                     const JobDev* __restrict__ jobs,
+                                                     const ElemDev* __restrict__ elems, PairDev* __restrict__ pairs,
+                                                     int* __restrict__ plist) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= natoms) return;
+    const AtomDev at = atoms[g];
+    const int nat = jobs[at.job].natoms, a = at.local, atom0 = g - a;
+    const AtomRowDev rw = rows[g];
+    long long slot = rw.slot0, io = rw.ioff0, c0 = rw.cpos[0], c1 = rw.cpos[1], c2 = rw.cpos[2];
+    const bool ha = elems[at.elem].nao == 1;
+    for (int b = a + 1; b < nat; ++b, ++slot) {
+        const bool hb = elems[atoms[atom0 + b].elem].nao == 1;
+        pairs[slot] = PairDev{g, atom0 + b, at.job, (int)slot, io};
+        io += (ha ? 1 : 10) * (hb ? 1 : 10);
+        const long long pos = (ha && hb) ? c0++ : ((ha || hb) ? c1++ : c2++);
+        plist[pos] = (int)slot;
+    }
+}
+
+void launch_build_pairs(hipStream_t st, const AtomDev* atoms, const AtomRowDev* rows, int natoms, const JobDev* jobs,
+                        const ElemDev* elems, PairDev* pairs, int* plist) {
+    if (natoms > 0)
+        hipLaunchKernelGGL(k_build_pairs, dim3((natoms + 255) / 256), dim3(256), 0, st, atoms, rows, natoms, jobs,
+                           elems, pairs, plist);
+}
+
 void launch_setup(hipStream_t st, const PairDev* pairs, const int* plist, int nlight, int nheavy,
                   const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, int njobs, double* ints,
                   double* arena, double* part, double* ecore, JobState* state, double* sdbg) {

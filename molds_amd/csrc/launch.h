@@ -18,6 +18,9 @@ void set_tuning(const Tuning& t);  // for the current device
 
 int upload_overlap_table(const double* table /* 32*25 */);
 
+// The batch's pair list and its class-sorted index from the per-atom row starts.
+void launch_build_pairs(hipStream_t st, const AtomDev* atoms, const AtomRowDev* rows, int natoms, const JobDev* jobs,
+                        const ElemDev* elems, PairDev* pairs, int* plist);
 void launch_setup(hipStream_t st, const PairDev* pairs, const int* plist, int nlight, int nheavy,
                   const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, int njobs, double* ints,
                   double* arena, double* part, double* ecore, JobState* state, double* sdbg);

@@ -44,6 +44,14 @@ struct PairDev {
     long long ioff;    // offset of the packed integrals (doubles)
 };
 
+// Row a of a job's pair triangle (pairs (a, b), b > a, in order): where its
+// entries go, from which k_build_pairs writes the pair list on the device.
+struct AtomRowDev {
+    long long slot0;    // global slot of pair (a, a+1)
+    long long ioff0;    // integral offset (doubles) of that pair
+    long long cpos[3];  // its position in the class-sorted pair list: H-H, heavy-H, heavy-heavy
+};
+
 // Arena layout per job, in units of n*n doubles.
 enum ArenaKind {
     AR_F = 0,      // Fock matrix
