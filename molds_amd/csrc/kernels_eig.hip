@@ -41,6 +41,7 @@ __device__ inline double* amat(double* arena, const JobDev& jb, int kind) {
 }
 
 constexpr double kEps = 1.1102230246251565e-16;        // 2^-53, unit roundoff
+constexpr double kEps32 = 5.9604644775390625e-08;      // 2^-24, FP32 unit roundoff
 
 // Per-job eigensolver workspace (EIG_WS_PER_AO doubles per AO plus
 // EIG_WS_PER_JOB per job): d[n], e[n], tau[n], then the compact-WY T blocks
@@ -663,8 +664,10 @@ __device__ inline SecSums dc_sums(const double* dl, const double* zl, int kk, do
 // that dl_t - x = (dl_t - dl_o) - tau keeps its relative accuracy.  The first
 // evaluation, at the middle of (dl_j, dl_j+1), both picks the pole and starts
 // the iteration.
+// ue: the unit roundoff the root is resolved to (FP64, or FP32 for a job in
+// its FP32 iterations, whose tridiagonal is only that accurate).
 __device__ void dc_secular(const double* dl, const double* zl, int kk, double rho, int j, int* orig,
-                           double* taup) {
+                           double* taup, double ue) {
     int o = kk - 1;
     double lo = 0.0, hi = 0.0, tau = 0.0, dO = dl[kk - 1];
     // the first evaluation point: the middle of (dl_j, dl_j+1) seen from dl_j,
@@ -687,7 +690,7 @@ __device__ void dc_secular(const double* dl, const double* zl, int kk, double rh
             dO = dl[o];
         }
         const double f = 1.0 + S.psi + S.phi;
-        if (fabs(f) <= 8.0 * kEps * (1.0 + rho * S.asum)) break;
+        if (fabs(f) <= 8.0 * ue * (1.0 + rho * S.asum)) break;
         if (f < 0.0) lo = tau;
         else hi = tau;
         double eta = 0.0;
@@ -717,9 +720,9 @@ __device__ void dc_secular(const double* dl, const double* zl, int kk, double rh
             const double c = f - b1 / u1;
             if (c != 0.0) { eta = u1 + b1 / c; have = true; }
         }
-        if (have && fabs(eta) <= 2.0 * kEps * fabs(tau)) { tau += eta; break; }
+        if (have && fabs(eta) <= 2.0 * ue * fabs(tau)) { tau += eta; break; }
         const double nt = (have && lo < tau + eta && tau + eta < hi) ? tau + eta : 0.5 * (lo + hi);
-        if (fabs(nt - tau) <= 2.0 * kEps * fabs(nt) || hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi))) {
+        if (fabs(nt - tau) <= 2.0 * ue * fabs(nt) || hi - lo <= 2.0 * ue * fmax(fabs(lo), fabs(hi))) {
             tau = nt;
             break;
         }
@@ -749,6 +752,9 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
+    // deflation and secular roots to FP32 accuracy for a mixed-precision job in
+    // its FP32 iterations (its tridiagonal comes from FP32-stored data)
+    const double ue = state[job].fp32 ? kEps32 : kEps;
     const unsigned long long t0 = wall_clock64();
     const int n = jb.nao;
     const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = T >> 6;
@@ -862,7 +868,7 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
                 dmax = fmax(dmax, __shfl_xor(dmax, off));
                 zmax = fmax(zmax, __shfl_xor(zmax, off));
             }
-            const double tol = 8.0 * kEps * fmax(dmax, zmax);
+            const double tol = 8.0 * ue * fmax(dmax, zmax);
             if (rho * zmax <= tol) {
                 for (int jj = a + lane; jj < b; jj += 64) flag[jj] = 1;
                 if (lane == 0) nrot[q] = 0;
@@ -947,7 +953,7 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             if (j < kk) {
                 int o;
                 double tau;
-                dc_secular(dl + a, zl + a, kk, rhoq[q], j, &o, &tau);
+                dc_secular(dl + a, zl + a, kk, rhoq[q], j, &o, &tau, ue);
                 rorig[i] = a + o;
                 rtau[i] = tau;
                 vals[i] = dl[a + o] + tau;
