@@ -746,6 +746,7 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     e->xl_min_nao = (e->opts.flags & NDDO_ENGINE_DEVICE_WIDE_EIGEN) ? 513 : 1201;
     e->use_graphs = !(e->opts.flags & NDDO_ENGINE_NO_GRAPHS);
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&e->api_stream, hipStreamNonBlocking));
     std::vector<ElemDev> el(3 * kNumElem);
     for (int t = 0; t < 3; ++t)
         for (int i = 0; i < kNumElem; ++i) elem_params(kElemZ[i], t, &el[t * kNumElem + i]);
@@ -783,6 +784,7 @@ int nddo_engine_destroy(nddo_engine* eh) {
         if (e->ev[i]) hipEventDestroy(e->ev[i]);
     if (e->xl_chain) hipEventDestroy(e->xl_chain);
     if (e->stream) hipStreamDestroy(e->stream);
+    if (e->api_stream) hipStreamDestroy(e->api_stream);
     delete e;
     return NDDO_OK;
 }

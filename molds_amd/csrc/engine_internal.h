@@ -109,6 +109,10 @@ struct Engine {
     int device = 0;
     nddo_engine_opts opts{0, 0, 0};
     hipStream_t stream = nullptr;
+    // the API threads' own stream for served-result copies and frees: work
+    // issued from a caller's thread never goes to a cohort stream, which the
+    // scheduler thread may be capturing into a graph at that moment
+    hipStream_t api_stream = nullptr;
     ElemDev* d_elems = nullptr;
     size_t lds_max = 65536;
     int num_cus = 0;

@@ -72,7 +72,6 @@ struct ServedJob {
     // arrays of a retired job: on the device until the first nddo_get_result
     std::mutex mu;
     double* d_res = nullptr;   // eps[n] | VT[n*n] | P[n*n] | traj[4*max_iter] | perm[n] (int)
-    hipStream_t res_stream = nullptr;
     std::shared_ptr<EventRef> res_ev;
     bool on_host = false;
     std::vector<double> eps, C, P, traj;
@@ -151,6 +150,7 @@ void retire_jobs(Engine* e, Cohort* c, const std::vector<int>& done) {
         }
     }
     if (ev && ev->ev) hipEventRecord(ev->ev, c->st);
+    else if (ev) hipStreamSynchronize(c->st);   // no event: the copies are complete before the jobs show done
     std::lock_guard<std::mutex> lk(e->smu);
     const auto now = Clock::now();
     for (size_t k = 0; k < done.size(); ++k) {
@@ -176,7 +176,6 @@ void retire_jobs(Engine* e, Cohort* c, const std::vector<int>& done) {
         }
         r.latency_s = std::chrono::duration<double>(now - j->t_submit).count();
         j->d_res = bufs[k];
-        j->res_stream = c->st;
         j->res_ev = ev;
         j->t_done = now;
         j->done = true;
@@ -256,21 +255,24 @@ void free_cohort(Cohort* c) { c->b.release(); }
 
 // Copies a retired job's arrays to the host (once), applying MolDS's row order
 // and sign rule to C.
-int materialize(ServedJob* j) {
+// Runs on the caller's thread: it waits on the retire event and copies and
+// frees on the engine's API stream, never on the cohort stream.
+int materialize(Engine* e, ServedJob* j) {
     if (j->on_host || !j->d_res) return NDDO_OK;
     const int n = j->res.nao, mi = j->desc.opts.max_iter;
     const size_t nn = (size_t)n * n;
-    if (j->res_ev && j->res_ev->ev) hipEventSynchronize(j->res_ev->ev);
-    else hipStreamSynchronize(j->res_stream);
+    if (j->res_ev && j->res_ev->ev && hipEventSynchronize(j->res_ev->ev) != hipSuccess) return NDDO_HIP_ERROR;
     std::vector<char> h(result_bytes(n, mi));
-    if (hipMemcpy(h.data(), j->d_res, h.size(), hipMemcpyDeviceToHost) != hipSuccess) return NDDO_HIP_ERROR;
+    if (hipMemcpyAsync(h.data(), j->d_res, h.size(), hipMemcpyDeviceToHost, e->api_stream) != hipSuccess ||
+        hipStreamSynchronize(e->api_stream) != hipSuccess)
+        return NDDO_HIP_ERROR;
     const double* hd = (const double*)h.data();
     j->eps.assign(hd, hd + n);
     j->C.resize(nn);
     order_and_phase(n, hd + n, (const int*)(hd + n + 2 * nn + 4 * (size_t)mi), j->C.data());
     j->P.assign(hd + n + nn, hd + n + 2 * nn);
     j->traj.assign(hd + n + 2 * nn, hd + n + 2 * nn + 4 * (size_t)mi);
-    hipFreeAsync(j->d_res, j->res_stream);
+    hipFreeAsync(j->d_res, e->api_stream);
     j->d_res = nullptr;
     j->res_ev.reset();
     j->on_host = true;
@@ -558,7 +560,7 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
     }
     std::lock_guard<std::mutex> jl(j->mu);
     hipSetDevice(e->device);
-    if (nddo::materialize(j) != NDDO_OK) { nddo::set_error("result copy failed"); return NDDO_HIP_ERROR; }
+    if (nddo::materialize(e, j) != NDDO_OK) { nddo::set_error("result copy failed"); return NDDO_HIP_ERROR; }
     double* eps = res->eps;
     double* C = res->C;
     double* P = res->P;
@@ -602,9 +604,10 @@ int nddo_release(nddo_engine* eh, nddo_job_id id) {
     }
     {
         std::lock_guard<std::mutex> jl(j->mu);
-        if (j->d_res) {
+        if (j->d_res) {   // on the API stream, after the retire copies (never on the cohort stream)
             hipSetDevice(e->device);
-            hipFreeAsync(j->d_res, j->res_stream);   // ordered after the result copies
+            if (j->res_ev && j->res_ev->ev) hipStreamWaitEvent(e->api_stream, j->res_ev->ev, 0);
+            hipFreeAsync(j->d_res, e->api_stream);
         }
     }
     delete j;
