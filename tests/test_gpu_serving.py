@@ -85,6 +85,36 @@ def test_served_graph_replay_after_smaller_launches(engine):
         assert np.max(np.abs(res[k]["P"] - alone["P"])) <= 1e-10
 
 
+def test_served_results_taken_while_cohorts_capture(engine):
+    """Results fetched and released by the caller while later cohorts start
+    and capture their iteration graphs: every request succeeds, and the
+    results equal the alone solves (the caller's copies and frees never touch
+    a cohort stream)."""
+    print("served results under capture", flush=True)
+    waves = [[_job("water", 3 + (7 * w + k) % 20, 60 + 10 * w + k) if k % 2 else
+              _job("alkane", 2 + (5 * w + k) % 18, 60 + 10 * w + k) for k in range(6)] for w in range(5)]
+    pending, res = [], {}
+    for wave in waves:
+        ids = [(engine.submit(j), j) for j in wave]
+        # take the previous wave's results while this wave's cohorts start
+        for jid, j in pending:
+            assert engine.wait(jid, 60.0) == 0
+            res[jid] = (engine.get_result(jid, j), j)
+            assert engine.release(jid) == 0
+        pending = ids
+    for jid, j in pending:
+        assert engine.wait(jid, 60.0) == 0
+        res[jid] = (engine.get_result(jid, j), j)
+        assert engine.release(jid) == 0
+    assert len(res) == 30
+    for jid in list(res)[::5]:
+        r, j = res[jid]
+        alone = engine.scf_batch([j])[0]
+        assert r["status"] == 0 and alone["status"] == 0
+        assert abs(r["energy"] - alone["energy"]) * EV <= 1e-9
+        assert r["iterations"] == alone["iterations"]
+
+
 def test_served_statuses(engine):
     from molds_amd.nddo import Engine, Job
     odd = Job("pm3", [6, 1, 1, 1], np.eye(4, 3) * 2.0)
