@@ -873,14 +873,17 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
                 for (int jj = a + lane; jj < b; jj += 64) flag[jj] = 1;
                 if (lane == 0) nrot[q] = 0;
             } else if (lane == 0) {
-                int pj = -1, nr = 0;
+                int pj = -1, nr = 0, ps = 0;
                 double zp = 0.0, dp = 0.0;
                 // the next entry's (z, d) is loaded one iteration ahead: entries at
                 // or after jj are not written before their own iteration
                 double zn = zs[a], dn = ds[a];
+                int cn = col[a];
                 for (int jj = a; jj < b; ++jj) {
                     double zj = zn, dj = dn;
-                    if (jj + 1 < b) { zn = zs[jj + 1]; dn = ds[jj + 1]; }
+                    // halves of QO the row spans (1 left, 2 right, 3 both after a rotation)
+                    int js = cn < m ? 1 : 2;
+                    if (jj + 1 < b) { zn = zs[jj + 1]; dn = ds[jj + 1]; cn = col[jj + 1]; }
                     if (rho * fabs(zj) <= tol) { flag[jj] = 1; continue; }
                     flag[jj] = 0;
                     if (pj >= 0) {
@@ -901,9 +904,13 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
                             ds[jj] = dj;
                             zj = tz;
                             flag[pj] = 1;
+                            js |= ps;
+                            // kept, with support in both halves (2: kept, mixed)
+                            if (js == 3) flag[jj] = 2;
                         }
                     }
                     pj = jj;
+                    ps = js;
                     zp = zj;
                     dp = dj;
                 }
@@ -915,24 +922,69 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
             int k = 0;
             for (int c0 = a; c0 < b; c0 += 64) {
                 const int jj = c0 + lane;
-                const bool keep = jj < b && !flag[jj];
+                const int fj = jj < b ? flag[jj] : 1;
+                const bool keep = fj != 1;
                 const unsigned long long mk = __ballot(keep);
                 const int pre = __popcll(mk & ((1ull << lane) - 1ull));
-                if (keep) { dl[a + k + pre] = ds[jj]; zl[a + k + pre] = zs[jj]; cl[a + k + pre] = col[jj]; }
+                if (keep) {
+                    dl[a + k + pre] = ds[jj];
+                    zl[a + k + pre] = zs[jj];
+                    cl[a + k + pre] = col[jj];
+                    rorig[a + k + pre] = fj;  // mixed mark until the secular roots
+                }
                 k += __popcll(mk);
             }
             int t2 = k;
             for (int c0 = a; c0 < b; c0 += 64) {
                 const int jj = c0 + lane;
-                const bool dfl = jj < b && flag[jj];
+                const bool dfl = jj < b && flag[jj] == 1;
                 const unsigned long long mk = __ballot(dfl);
                 const int pre = __popcll(mk & ((1ull << lane) - 1ull));
                 if (dfl) { dl[a + t2 + pre] = ds[jj]; zl[a + t2 + pre] = 0.0; cl[a + t2 + pre] = col[jj]; }
                 t2 += __popcll(mk);
             }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // the non-deflated rows by the halves of QO = diag(Q1, Q2) they span:
+            // flag[a ..) lists the rows t that meet the left half, then those that
+            // meet the right half (a row rotated with one from the other half is in
+            // both); the merge products of a column tile within one half run over
+            // that half's list only (the other rows meet zeros there). Only at the
+            // levels whose products run on MFMA, l >= 5: there nm <= n / 32, so
+            // kkq[n/2 + q] and kkq[3n/4 + q] are clear of the counts kkq[0 .. nm).
+            // Lists longer than the merge's flag slots leave the full range (-1).
+            int kl = -1, ke = 0;
+            if (l >= 5) {
+                int nl = 0, nr = 0;
+                for (int c0 = 0; c0 < k; c0 += 64) {
+                    const int t = c0 + lane;
+                    const int sd = t < k ? (rorig[a + t] == 2 ? 3 : (cl[a + t] < m ? 1 : 2)) : 0;
+                    nl += __popcll(__ballot(sd & 1));
+                    nr += __popcll(__ballot(sd & 2));
+                }
+                if (nl + nr <= b - a) {
+                    kl = 0;
+                    ke = nl;
+                    for (int c0 = 0; c0 < k; c0 += 64) {
+                        const int t = c0 + lane;
+                        const int sd = t < k ? (rorig[a + t] == 2 ? 3 : (cl[a + t] < m ? 1 : 2)) : 0;
+                        const unsigned long long ml = __ballot(sd & 1), mr = __ballot(sd & 2);
+                        const unsigned long long below = (1ull << lane) - 1ull;
+                        if (sd & 1) flag[a + kl + __popcll(ml & below)] = t;
+                        if (sd & 2) flag[a + ke + __popcll(mr & below)] = t;
+                        kl += __popcll(ml);
+                        ke += __popcll(mr);
+                    }
+                    // kl ends at nl, ke at nl + nr
+                }
+            }
             if (lane == 0) {
                 kkq[q] = k;
                 rhoq[q] = rho;
+                if (l >= 5) {
+                    kkq[(n >> 1) + q] = kl;
+                    kkq[(n >> 1) + (n >> 2) + q] = ke;
+                }
             }
         }
         __syncthreads();
@@ -1040,17 +1092,22 @@ __global__ void __launch_bounds__(1024) k_dc(const JobDev* __restrict__ jobs, Jo
                 const int q = ti / (tpm * tpm), rem = ti - q * tpm * tpm;
                 const int jt = rem / tpm, rt = rem - jt * tpm;
                 const int kk = kkq[q];
-                const int a = q << l, b = min(a + s, n);
+                const int a = q << l, b = min(a + s, n), m = min(a + (s >> 1), n);
                 const int j0 = jt * 32, r0 = a + rt * 32;
                 if (kk <= 0 || j0 >= kk || r0 >= b) continue;
+                // rows that meet this tile's columns: one half's list, or all kk
+                const int kl = kkq[(n >> 1) + q], ke = kkq[(n >> 1) + (n >> 2) + q];
+                const bool lh = kl >= 0 && r0 + 32 <= m, rh = kl >= 0 && r0 >= m;
+                const int i0 = rh ? kl : 0, i1 = lh ? kl : (rh ? ke : kk);
                 dbl4 acc[2][2];
 #pragma unroll
                 for (int x = 0; x < 2; ++x)
 #pragma unroll
                     for (int y = 0; y < 2; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
-                for (int k0 = 0; k0 < kk; k0 += 4) {
-                    const int t = k0 + lk;
-                    const bool tv = t < kk;
+                for (int k0 = i0; k0 < i1; k0 += 4) {
+                    const int ii = k0 + lk;
+                    const bool tv = ii < i1;
+                    const int t = (lh || rh) ? (tv ? flag[a + ii] : 0) : ii;
                     const long long urow = (long long)(a + t) * n + a;
                     const long long qrow = tv ? (long long)cl[a + t] * n : 0;
                     double av[2], bv[2];
