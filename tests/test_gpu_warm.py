@@ -1,0 +1,100 @@
+"""Warm-start geometry updates: DoSCF(requiresGuess=false) after a geometry
+step, the way MolDS's MD and optimiser loops call it (md/MD.cpp:87-108,
+optimization/Optimizer.cpp:243): iteration 0 builds the Fock matrix from the
+previous density instead of the guess Fock (Cndo2.cpp:585-729 with
+requiresGuess = false; Mndo.cpp:434-435).
+
+The oracle is started from the same P0, so the bar is the FP64 one:
+1e-6 eV, density RMS 1e-8 and the same iteration count; mixed precision is held
+to 1e-5 eV of the FP64 warm solve.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EV = 1.0 / 0.03674903          # Hartree -> eV with MolDS's constant
+OPTS = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0,
+            damping_weight=0.0, max_iter=200)
+ORACLE_OPTS = dict(rms_density=1e-8, diis_k=5, diis_start=0.1, diis_end=0.0, damping_weight=0.0, max_iter=200)
+
+
+def _mol(tag):
+    from molds_amd import workloads as W
+    if tag == "decane":
+        Z, x = W.decane()
+    elif tag == "water8":
+        Z, x = W.water_cluster(8, seed=5, jitter=0.02)
+    else:
+        Z, x = W.alkane(2, seed=22)
+    return np.asarray(Z), W.to_bohr(x)
+
+
+def _step(xyz, seed, size=0.01):
+    """A small geometry update (bohr), like one MD step."""
+    rng = np.random.default_rng(seed)
+    return xyz + rng.normal(0.0, size, xyz.shape)
+
+
+def _oracle(theory, Z, xyz, **kw):
+    from oracle import scf as O
+    return O.scf(theory, Z, xyz, **kw, **ORACLE_OPTS)
+
+
+def _check(r, o):
+    assert r["status"] == 0
+    assert abs(r["energy"] - o["energy"]) * EV <= 1e-6
+    assert np.sqrt(np.mean((r["P"] - o["P"]) ** 2)) <= 1e-8
+    assert r["iterations"] == o["iterations"]
+
+
+@pytest.mark.parametrize("theory,tag", [("pm3", "decane"), ("am1", "water8"), ("mndo", "ethane")])
+def test_warm_step_matches_oracle(engine, theory, tag):
+    Z, x0 = _mol(tag)
+    r0 = engine.scf(theory, Z, x0, **OPTS)
+    assert r0["status"] == 0
+    x1 = _step(x0, seed=31)
+    warm = engine.scf(theory, Z, x1, requires_guess=False, P0=r0["P"], **OPTS)
+    _check(warm, _oracle(theory, Z, x1, requires_guess=False, P0=r0["P"]))
+    cold = engine.scf(theory, Z, x1, **OPTS)
+    assert cold["status"] == 0
+    assert abs(warm["energy"] - cold["energy"]) * EV <= 1e-6
+    assert warm["iterations"] < cold["iterations"]
+
+
+def test_no_guess_from_zero_density(engine):
+    """requiresGuess = false with no previous density: iteration 0 is the Fock
+    matrix of P = 0 (the core Hamiltonian), not the guess Fock."""
+    Z, x = _mol("ethane")
+    r = engine.scf("pm3", Z, x, requires_guess=False, **OPTS)
+    _check(r, _oracle("pm3", Z, x, requires_guess=False, P0=None))
+
+
+def test_warm_trajectory(engine):
+    """Three geometry updates in a row, each started from the last density."""
+    Z, x = _mol("decane")
+    r = engine.scf("pm3", Z, x, **OPTS)
+    for step in range(3):
+        x = _step(x, seed=40 + step)
+        P0 = r["P"]
+        r = engine.scf("pm3", Z, x, requires_guess=False, P0=P0, **OPTS)
+        _check(r, _oracle("pm3", Z, x, requires_guess=False, P0=P0))
+
+
+def test_warm_mixed_and_batched(engine):
+    """Warm jobs in one batch, FP64 and mixed, each equal to its solve alone."""
+    from molds_amd.nddo import Job
+    mols = [_mol("decane"), _mol("water8"), _mol("ethane")]
+    theories = ["pm3", "am1", "pm3"]
+    starts = [engine.scf(t, Z, x, **OPTS) for t, (Z, x) in zip(theories, mols)]
+    moved = [(Z, _step(x, seed=50 + i)) for i, (Z, x) in enumerate(mols)]
+    jobs = [Job(t, Z, x, "fp64", False, s["P"], **OPTS) for t, (Z, x), s in zip(theories, moved, starts)]
+    jobs_mx = [Job(t, Z, x, "mixed", False, s["P"], **OPTS) for t, (Z, x), s in zip(theories, moved, starts)]
+    out = engine.scf_batch(jobs + jobs_mx)
+    for i, j in enumerate(jobs):
+        alone = engine.scf_batch([j])[0]
+        b, m = out[i], out[len(jobs) + i]
+        assert alone["status"] == 0 and b["status"] == 0 and m["status"] == 0
+        assert abs(alone["energy"] - b["energy"]) * EV <= 1e-9
+        assert alone["iterations"] == b["iterations"]
+        assert abs(m["energy"] - b["energy"]) * EV <= 1e-5
