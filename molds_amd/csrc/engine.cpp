@@ -122,7 +122,7 @@ void Batch::release() {
                      (void**)&d_ints, (void**)&d_ints32, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
                      (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
                      (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_plist, (void**)&d_list_xl, (void**)&d_xl_off,
-                     (void**)&d_xlws, (void**)&d_xlpart, (void**)&d_xlcoef, (void**)&d_bar, (void**)&d_sdbg}) {
+                     (void**)&d_xlws, (void**)&d_xlpart, (void**)&d_xlcoef, (void**)&d_tpart, (void**)&d_bar, (void**)&d_sdbg}) {
         if (*p) {
             if (alloc_stream) hipFreeAsync(*p, alloc_stream);
             else hipFree(*p);
@@ -193,7 +193,9 @@ size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter) {
     const size_t npairs = (size_t)natoms * (natoms - 1) / 2;
     size_t b = (ARENA_FIXED + 2 * (size_t)diis_k) * n2 * 8;   // arena
     b += npairs * (100 * 8 + 20 * 8 + 8 + sizeof(PairDev));   // integrals (upper bound), partials, ecore
-    b += (size_t)nao * 12 + (size_t)max_iter * 32 + sizeof(JobDev) + sizeof(JobState);
+    const size_t tpd = ((size_t)nao + 63) / 64;
+    b += tpd * tpd * (sizeof(TileDev) + (1 + XL_REC) * 8);           // density tiles and their partials
+    b += (size_t)nao * 12 + (size_t)max_iter * 32 + sizeof(JobDev) + sizeof(JobState) + XL_COEF * 8;
     return b;
 }
 
@@ -404,6 +406,8 @@ int Batch::allocate(bool want_sdbg) {
     ACHK(dalloc((void**)&d_tri, (EIG_WS_PER_AO * std::max<long long>(nvec, 1) + EIG_WS_PER_JOB * (long long)nj) *
                                  sizeof(double)));
     ACHK(dalloc((void**)&d_rmspart, std::max<size_t>(tiles.size(), 1) * sizeof(double)));
+    ACHK(dalloc((void**)&d_tpart, std::max<size_t>(tiles.size(), 1) * XL_REC * sizeof(double)));
+    ACHK(dalloc((void**)&d_xlcoef, std::max<size_t>(nj, 1) * XL_COEF * sizeof(double)));
     ACHK(dalloc((void**)&d_traj, std::max<long long>(ntraj, 1) * sizeof(double)));
     ACHK(dalloc((void**)&d_remaining, sizeof(int)));
     ACHK(dalloc((void**)&d_list_lds, std::max<size_t>(list_lds.size(), 1) * sizeof(int)));
@@ -413,7 +417,6 @@ int Batch::allocate(bool want_sdbg) {
         ACHK(dalloc((void**)&d_xl_off, list_xl.size() * sizeof(long long)));
         ACHK(dalloc((void**)&d_xlws, xlws_size * sizeof(double)));
         ACHK(dalloc((void**)&d_xlpart, list_xl.size() * (size_t)xl_chunks * XL_REC * sizeof(double)));
-        ACHK(dalloc((void**)&d_xlcoef, nj * XL_COEF * sizeof(double)));
         ACHK(dalloc((void**)&d_bar, 2 * sizeof(unsigned)));
     }
     if (want_sdbg) ACHK(dalloc((void**)&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double)));
@@ -497,10 +500,11 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
     launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri, true);
     launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri, false);
     launch_xl_eigen_all(e, st);
-    launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
+    launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart, d_tpart);
     launch_xl_dots(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
-    launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining, d_xlpart, d_xlcoef,
+    launch_step(st, d_jobs, (int)jobs.size(), d_state, d_rmspart, d_tpart, d_traj, d_remaining, d_xlpart, d_xlcoef,
                 xl_chunks);
+    launch_apply(st, d_tiles, (int)tiles.size(), d_jobs, d_arena, d_xlcoef);
     launch_xl_apply(st, d_jobs, d_list_xl, nxl, d_arena, d_xlcoef, xl_chunks);
 }
 
@@ -597,11 +601,12 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             }
             launch_xl_eigen_all(e, st);
             HIPCHK(hipEventRecord(e->ev[10], st));
-            launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart);
+            launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart, d_tpart);
             HIPCHK(hipEventRecord(e->ev[6], st));
             launch_xl_dots(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
-            launch_step(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_rmspart, d_traj, d_remaining, d_xlpart,
+            launch_step(st, d_jobs, (int)jobs.size(), d_state, d_rmspart, d_tpart, d_traj, d_remaining, d_xlpart,
                         d_xlcoef, xl_chunks);
+            launch_apply(st, d_tiles, (int)tiles.size(), d_jobs, d_arena, d_xlcoef);
             launch_xl_apply(st, d_jobs, d_list_xl, (int)list_xl.size(), d_arena, d_xlcoef, xl_chunks);
             HIPCHK(hipEventRecord(e->ev[7], st));
         }
@@ -1011,6 +1016,7 @@ static int matrix_batch(Engine* e, Batch& b, int n, int nocc, int K) {
         jb.ivec0 = (long long)k * n;
         jb.traj0 = 4LL * k;
         jb.tile0 = (int)b.tiles.size();
+        jb.tiles_per_dim = (n + 63) / 64;
         for (int i0 = 0; i0 < n; i0 += 64)
             for (int j0 = 0; j0 < n; j0 += 64) b.tiles.push_back({k, i0, j0, 0});
         jb.ntiles = (int)b.tiles.size() - jb.tile0;
@@ -1285,7 +1291,8 @@ int nddo_dbg_density(nddo_engine* eh, int n, int nocc, const double* C, double* 
     for (int i = 0; i < n; ++i) perm[i] = i;
     HIPCHK(hipMemcpy(b.d_perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.d_arena + (long long)AR_C * nn, C, nn * sizeof(double), hipMemcpyHostToDevice));
-    launch_density(e->stream, b.d_tiles, (int)b.tiles.size(), b.d_jobs, b.d_state, b.d_arena, b.d_perm, b.d_rmspart);
+    launch_density(e->stream, b.d_tiles, (int)b.tiles.size(), b.d_jobs, b.d_state, b.d_arena, b.d_perm, b.d_rmspart,
+                   b.d_tpart);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(P, b.d_arena + (long long)AR_P1 * nn, nn * sizeof(double), hipMemcpyDeviceToHost));

@@ -65,6 +65,7 @@ struct Batch {
     int* d_perm = nullptr;
     double* d_tri = nullptr;     // eigensolver workspace (EIG_WS_PER_AO / EIG_WS_PER_JOB)
     double* d_rmspart = nullptr;
+    double* d_tpart = nullptr;   // per density tile: DIIS dot products and max |e| (XL_REC doubles)
     double* d_traj = nullptr;
     int* d_remaining = nullptr;
     int* d_list_lds = nullptr;
@@ -75,7 +76,7 @@ struct Batch {
     long long* d_xl_off = nullptr;
     double* d_xlws = nullptr;
     double* d_xlpart = nullptr;
-    double* d_xlcoef = nullptr;
+    double* d_xlcoef = nullptr;  // per job: DIIS coefficients and what to apply (XL_COEF doubles)
     unsigned* d_bar = nullptr;
     double* d_sdbg = nullptr;
     hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream

@@ -521,20 +521,31 @@ void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev*
 
 // ------------------------------------------------------------------ density
 // P = 2 C_occ^T C_occ (rows of C are MOs; occupied = the nocc lowest after the
-// eigenvalue sort), one 64x64 tile per workgroup; each tile also emits its
-// partial sum of (P - P_old)^2 for the RMS test.
+// eigenvalue sort). One workgroup per upper 64x64 tile (I <= J) computes the
+// tile and writes it and its mirror (J, I), so P is exactly symmetric, as
+// dsyrk's upper triangle mirrored to the lower (Blas.cpp:716-740). Each tile
+// also emits
+//  - its partial of sum (P - P_old)^2 over both tiles (the RMS test);
+//  - for a DIIS job (not a large one, not the guess iteration), the new ring
+//    slot (P and e = P - P_old) on the upper tile and the tile's share of
+//    <e_i, e_new> for every ring slot and of max |e_new| (Cndo2.cpp:955-988):
+//    e is symmetric, so an off-diagonal tile counts twice for its mirror and
+//    the ring holds the upper tiles only.
 
 typedef float flt4 __attribute__((ext_vector_type(4)));
+
+constexpr int TLD = 65;   // LDS row stride of a staged 64x64 tile (transposed reads without bank conflicts)
 
 // Mixed precision: every operand split as x = hi + lo (hi = fp32(x),
 // lo = fp32(x - hi)) and P ~ 2 (A_hi B_hi + A_hi B_lo + A_lo B_hi) on
 // v_mfma_f32_16x16x4f32 with FP32 accumulation; each wavefront owns a 32x32
-// quadrant of the tile (2 x 2 MFMA blocks).
+// quadrant of the tile (2 x 2 MFMA blocks). The tile is staged in T.
 __device__ void density_tile_split_f32(const TileDev& tl, const JobDev& jb, const double* VT, const int* perm,
-                                       const double* Pin, double* Pout, double* red, double* rmspart) {
+                                       double (*T)[TLD]) {
     const int n = jb.nao, nocc = jb.nocc;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
-    const int ib = tl.i0 + 32 * (wave >> 1), jbb = tl.j0 + 32 * (wave & 1);
+    const int qi = 32 * (wave >> 1), qj = 32 * (wave & 1);
+    const int ib = tl.i0 + qi, jbb = tl.j0 + qj;
     flt4 acc[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -564,96 +575,231 @@ __device__ void density_tile_split_f32(const TileDev& tl, const JobDev& jb, cons
             }
     }
     // D[i][j] of a 16x16 FP32 block sits in lane j + 16 (i / 4), element i % 4
-    double loc = 0.0;
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = ib + 16 * x + 4 * lk + v, j = jbb + 16 * y + lr;
-                if (i < n && j < n) {
-                    const double pv = 2.0 * (double)acc[x][y][v];
-                    const long long ij = (long long)i * n + j;
-                    const double dd = pv - Pin[ij];
-                    Pout[ij] = pv;
-                    loc += dd * dd;
-                }
-            }
-    red[tid] = loc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
-    }
-    if (tid == 0) rmspart[blockIdx.x] = red[0];
+            for (int v = 0; v < 4; ++v) T[qi + 16 * x + 4 * lk + v][qj + 16 * y + lr] = 2.0 * (double)acc[x][y][v];
 }
-__global__ void __launch_bounds__(256) k_density(const TileDev* __restrict__ tiles, const JobDev* __restrict__ jobs,
-                                                 const JobState* __restrict__ state, double* __restrict__ arena,
-                                                 const int* __restrict__ perm, double* __restrict__ rmspart) {
-    const TileDev tl = tiles[blockIdx.x];
-    if (state[tl.job].phase != PH_ACTIVE) return;
-    const JobDev jb = jobs[tl.job];
-    const int iter = state[tl.job].iter;
-    const int n = jb.nao, nocc = jb.nocc;
-    const double* VT = mat(arena, jb, AR_C);
+
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ inline double wave_max_d(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Writes the staged tile T (upper tile, value (r, c) = P(i0 + r, j0 + c); a
+// diagonal tile takes its upper half for both halves) and its mirror; the RMS
+// and DIIS partials of the tile go to rmspart[tile] and tpart[tile * XL_REC ..]
+// (k dot products in logical ring order, then max |e_new|).
+__device__ void density_store(const TileDev& tl, const JobDev& jb, const JobState& st, const double (*T)[TLD],
+                              double* __restrict__ arena, double (*red)[NDDO_MAX_DIIS + 3],
+                              double* __restrict__ rmspart, double* __restrict__ tpart) {
+    const int n = jb.nao, k = jb.diis_k, iter = st.iter;
+    const int tid = threadIdx.x, c = tid & 63, lane = tid & 63, wave = tid >> 6;
+    const long long nn = (long long)n * n;
     const double* Pin = mat(arena, jb, AR_P0 + (iter & 1));
     double* Pout = mat(arena, jb, AR_P0 + ((iter + 1) & 1));
-    __shared__ double sA[16][68], sB[16][68];
-    __shared__ double red[256];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    if (state[tl.job].fp32) {
-        density_tile_split_f32(tl, jb, VT, perm, Pin, Pout, red, rmspart);
-        return;
-    }
-    double acc[4][4];
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    for (int o0 = 0; o0 < nocc; o0 += 16) {
-        for (int e = tid; e < 16 * 64; e += 256) {
-            const int kk = e >> 6, ii = e & 63, o = o0 + kk;
-            const int row = o < nocc ? perm[jb.ivec0 + o] : -1;
-            sA[kk][ii] = (row >= 0 && tl.i0 + ii < n) ? VT[(long long)row * n + tl.i0 + ii] : 0.0;
-            sB[kk][ii] = (row >= 0 && tl.j0 + ii < n) ? VT[(long long)row * n + tl.j0 + ii] : 0.0;
-        }
-        __syncthreads();
+    const bool diag = tl.i0 == tl.j0;
+    const bool ring = k > 0 && !jb.xl && !(iter == 0 && jb.requires_guess);
+    const int rs = ring ? (st.ring_start + 1) % k : 0, slot_new = ring ? (rs + k - 1) % k : 0;
+    double* RP = mat(arena, jb, AR_RING);
+    double* RE = RP + (long long)k * nn;
+    // the older slots (logical 0 .. k-2) with compile-time indices, so the
+    // accumulators stay in registers
+    double part[NDDO_MAX_DIIS - 1];
+    long long roff[NDDO_MAX_DIIS - 1];
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            double av[4], bv[4];
-            for (int a = 0; a < 4; ++a) av[a] = sA[kk][ty * 4 + a];
-            for (int b = 0; b < 4; ++b) bv[b] = sB[kk][tx * 4 + b];
-            for (int a = 0; a < 4; ++a)
-                for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
-        }
-        __syncthreads();
+    for (int q = 0; q < NDDO_MAX_DIIS - 1; ++q) {
+        part[q] = 0.0;
+        roff[q] = (ring && q < k - 1) ? (long long)((rs + q) % k) * nn : 0;
     }
-    double loc = 0.0;
-    for (int a = 0; a < 4; ++a) {
-        const int i = tl.i0 + ty * 4 + a;
-        if (i >= n) continue;
-        for (int b = 0; b < 4; ++b) {
-            const int j = tl.j0 + tx * 4 + b;
-            if (j >= n) continue;
-            const double v = 2.0 * acc[a][b];
+    double loc = 0.0, self = 0.0, amax = 0.0;
+    for (int r = wave; r < 64; r += 4) {
+        const int i = tl.i0 + r, j = tl.j0 + c;
+        if (i >= n || j >= n) continue;
+        const double v = (diag && c < r) ? T[c][r] : T[r][c];
+        const long long ij = (long long)i * n + j;
+        const double d = v - Pin[ij];
+        Pout[ij] = v;
+        loc += d * d;
+        if (ring) {
+            RP[(long long)slot_new * nn + ij] = v;
+            RE[(long long)slot_new * nn + ij] = d;
+            amax = fmax(amax, fabs(d));
+            self += d * d;
+#pragma unroll
+            for (int q = 0; q < NDDO_MAX_DIIS - 1; ++q)
+                if (q < k - 1) part[q] += RE[roff[q] + ij] * d;
+        }
+    }
+    if (!diag) {
+        for (int r = wave; r < 64; r += 4) {
+            const int i = tl.j0 + r, j = tl.i0 + c;
+            if (i >= n || j >= n) continue;
+            const double v = T[c][r];
             const long long ij = (long long)i * n + j;
             const double d = v - Pin[ij];
             Pout[ij] = v;
             loc += d * d;
         }
     }
-    red[tid] = loc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
+    // fixed-order reductions: butterfly within each wavefront, then the four
+    // wavefronts in order
+    loc = wave_sum_d(loc);
+    if (ring) {
+        self = wave_sum_d(self);
+        amax = wave_max_d(amax);
+#pragma unroll
+        for (int q = 0; q < NDDO_MAX_DIIS - 1; ++q)
+            if (q < k - 1) part[q] = wave_sum_d(part[q]);
     }
-    if (tid == 0) rmspart[blockIdx.x] = red[0];
+    if (lane == 0) {
+        red[wave][0] = loc;
+        if (ring) {
+#pragma unroll
+            for (int q = 0; q < NDDO_MAX_DIIS - 1; ++q)
+                if (q < k - 1) red[wave][1 + q] = part[q];
+            red[wave][k] = self;
+            red[wave][k + 1] = amax;
+        }
+    }
+    __syncthreads();
+    if (tid <= (ring ? k + 1 : 0)) {
+        const int q = tid;
+        double s = red[0][q];
+        for (int w = 1; w < 4; ++w) s = q == k + 1 ? fmax(s, red[w][q]) : s + red[w][q];
+        const double wt = diag ? 1.0 : 2.0;
+        if (q == 0) rmspart[blockIdx.x] = s;
+        else if (q <= k) tpart[(long long)blockIdx.x * XL_REC + q - 1] = wt * s;
+        else tpart[(long long)blockIdx.x * XL_REC + k] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_density(const TileDev* __restrict__ tiles, const JobDev* __restrict__ jobs,
+                                                 const JobState* __restrict__ state, double* __restrict__ arena,
+                                                 const int* __restrict__ perm, double* __restrict__ rmspart,
+                                                 double* __restrict__ tpart) {
+    const TileDev tl = tiles[blockIdx.x];
+    if (tl.i0 > tl.j0) return;   // lower tiles: written by their upper mirror
+    const JobState& st = state[tl.job];
+    if (st.phase != PH_ACTIVE) return;
+    const JobDev jb = jobs[tl.job];
+    const int n = jb.nao, nocc = jb.nocc;
+    const double* VT = mat(arena, jb, AR_C);
+    // the staged tile; the FP64 path's operand panels alias its start
+    __shared__ double smem[64 * TLD];
+    __shared__ double red[4][NDDO_MAX_DIIS + 3];
+    double (*T)[TLD] = reinterpret_cast<double (*)[TLD]>(smem);
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    if (st.fp32) {
+        density_tile_split_f32(tl, jb, VT, perm, T);
+    } else {
+        double (*sA)[68] = reinterpret_cast<double (*)[68]>(smem);
+        double (*sB)[68] = sA + 16;
+        double acc[4][4];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+        for (int o0 = 0; o0 < nocc; o0 += 16) {
+            for (int e = tid; e < 16 * 64; e += 256) {
+                const int kk = e >> 6, ii = e & 63, o = o0 + kk;
+                const int row = o < nocc ? perm[jb.ivec0 + o] : -1;
+                sA[kk][ii] = (row >= 0 && tl.i0 + ii < n) ? VT[(long long)row * n + tl.i0 + ii] : 0.0;
+                sB[kk][ii] = (row >= 0 && tl.j0 + ii < n) ? VT[(long long)row * n + tl.j0 + ii] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) {
+                double av[4], bv[4];
+                for (int a = 0; a < 4; ++a) av[a] = sA[kk][ty * 4 + a];
+                for (int b = 0; b < 4; ++b) bv[b] = sB[kk][tx * 4 + b];
+                for (int a = 0; a < 4; ++a)
+                    for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+            }
+            __syncthreads();
+        }
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) T[ty * 4 + a][tx * 4 + b] = 2.0 * acc[a][b];
+    }
+    __syncthreads();
+    density_store(tl, jb, st, T, arena, red, rmspart, tpart);
 }
 
 void launch_density(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, const JobState* state,
-                    double* arena, const int* perm, double* rmspart) {
+                    double* arena, const int* perm, double* rmspart, double* tpart) {
     if (ntiles > 0)
-        hipLaunchKernelGGL(k_density, dim3(ntiles), dim3(256), 0, st, tiles, jobs, state, arena, perm, rmspart);
+        hipLaunchKernelGGL(k_density, dim3(ntiles), dim3(256), 0, st, tiles, jobs, state, arena, perm, rmspart, tpart);
+}
+
+// DIIS extrapolation and damping of the ordinary (not large) jobs after k_step
+// (Cndo2.cpp:1016-1061), one workgroup per upper tile, written with its mirror
+// like the density; k_step leaves the coefficients, the ring start and what
+// to apply in coef[job * XL_COEF ..]. Large jobs: k_xl_apply.
+__global__ void __launch_bounds__(256) k_apply(const TileDev* __restrict__ tiles, const JobDev* __restrict__ jobs,
+                                               double* __restrict__ arena, const double* __restrict__ coef) {
+    const TileDev tl = tiles[blockIdx.x];
+    if (tl.i0 > tl.j0) return;
+    const double* cf = coef + (long long)tl.job * XL_COEF;
+    const int flags = (int)cf[XL_COEF_FLAGS];
+    if (!flags) return;
+    const JobDev jb = jobs[tl.job];
+    if (jb.xl) return;
+    const int n = jb.nao, k = jb.diis_k, iter = (int)cf[XL_COEF_ITER], rs = (int)cf[XL_COEF_RS];
+    const long long nn = (long long)n * n;
+    const double* Pin = mat(arena, jb, AR_P0 + (iter & 1));
+    double* Pout = mat(arena, jb, AR_P0 + ((iter + 1) & 1));
+    const double* RP = mat(arena, jb, AR_RING);
+    const double w = jb.damp_weight;
+    double cq[NDDO_MAX_DIIS];
+    long long so[NDDO_MAX_DIIS];
+#pragma unroll
+    for (int q = 0; q < NDDO_MAX_DIIS; ++q) {
+        cq[q] = q < k ? cf[q] : 0.0;
+        so[q] = q < k ? (long long)((rs + q) % k) * nn : 0;
+    }
+    __shared__ double T[64][TLD];
+    const int tid = threadIdx.x, c = tid & 63, wave = tid >> 6;
+    const bool diag = tl.i0 == tl.j0;
+    for (int r = wave; r < 64; r += 4) {
+        const int i = tl.i0 + r, j = tl.j0 + c;
+        if (i >= n || j >= n) continue;
+        const long long ij = (long long)i * n + j;
+        double v;
+        if (flags & 1) {
+            v = 0.0;
+#pragma unroll
+            for (int q = 0; q < NDDO_MAX_DIIS; ++q)
+                if (q < k) v += cq[q] * RP[so[q] + ij];
+        } else {
+            v = Pout[ij];
+        }
+        if (flags & 2) v = v * (1.0 - w) + w * Pin[ij];
+        T[r][c] = v;
+    }
+    __syncthreads();
+    for (int r = wave; r < 64; r += 4) {
+        const int i = tl.i0 + r, j = tl.j0 + c;
+        if (i >= n || j >= n) continue;
+        Pout[(long long)i * n + j] = (diag && c < r) ? T[c][r] : T[r][c];
+    }
+    if (!diag)
+        for (int r = wave; r < 64; r += 4) {
+            const int i = tl.j0 + r, j = tl.i0 + c;
+            if (i >= n || j >= n) continue;
+            Pout[(long long)i * n + j] = T[c][r];
+        }
+}
+
+void launch_apply(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, double* arena,
+                  const double* coef) {
+    if (ntiles > 0) hipLaunchKernelGGL(k_apply, dim3(ntiles), dim3(256), 0, st, tiles, jobs, arena, coef);
 }
 
 // ------------------------------------------------------------------ step
@@ -687,7 +833,7 @@ __device__ bool small_solve(int m, double* a /* m*m row-major, destroyed */, dou
 }
 
 __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
-                                              double* __restrict__ arena, const double* __restrict__ rmspart,
+                                              const double* __restrict__ rmspart, const double* __restrict__ tpart,
                                               double* __restrict__ traj, int* __restrict__ remaining,
                                               const double* __restrict__ xlpart, double* __restrict__ xlcoef,
                                               int nchunk) {
@@ -695,21 +841,25 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
     JobState* st = &state[job];
     const JobDev jb = jobs[job];
     const int tid = threadIdx.x, T = blockDim.x;
-    // large molecules: the element-wise work runs in k_xl_dots / k_xl_apply;
-    // nothing to apply unless the DIIS / damping decisions below say so
-    double* xc = jb.xl ? xlcoef + (long long)job * XL_COEF : nullptr;
-    if (xc && tid == 0) xc[XL_COEF_FLAGS] = 0.0;
+    // the element-wise work runs elsewhere: the ring update and the dot products
+    // in k_density's tiles (k_xl_dots' chunks for large molecules), the
+    // extrapolation and damping in k_apply (k_xl_apply) from the coefficients
+    // and flags left here; nothing to apply unless the decisions below say so
+    double* xc = xlcoef + (long long)job * XL_COEF;
+    if (tid == 0) xc[XL_COEF_FLAGS] = 0.0;
     if (st->phase != PH_ACTIVE) return;
-    const int n = jb.nao, iter = st->iter, k = jb.diis_k;
+    const int n = jb.nao, iter = st->iter, k = jb.diis_k, tpd = jb.tiles_per_dim;
     const long long nn = (long long)n * n;
     __shared__ double sh_rms;
     __shared__ double red[NDDO_MAX_DIIS + 1][256];
     __shared__ double coef[NDDO_MAX_DIIS + 1];
     __shared__ double sa[(NDDO_MAX_DIIS + 1) * (NDDO_MAX_DIIS + 1)], srhs[NDDO_MAX_DIIS + 1];
-    __shared__ int diis_on, damp_on;
+    __shared__ int diis_on;
     if (tid == 0) {
+        // the upper tiles (I <= J) carry the RMS partials of their mirrors too
         double s = 0.0;
-        for (int t = 0; t < jb.ntiles; ++t) s += rmspart[jb.tile0 + t];
+        for (int t = 0; t < jb.ntiles; ++t)
+            if (t / tpd <= t % tpd) s += rmspart[jb.tile0 + t];
         sh_rms = sqrt(s / (double)nn);
         double* tr = traj + jb.traj0 + 4LL * iter;
         tr[0] = sh_rms;
@@ -742,25 +892,16 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
         st->switch_iter = iter + 1;
     }
     const bool guess = iter == 0 && jb.requires_guess;
-    const double* Pin = mat(arena, jb, AR_P0 + (iter & 1));
-    double* Pout = mat(arena, jb, AR_P0 + ((iter + 1) & 1));
     if (!guess) {
         if (k > 0) {
-            const int rs = (st->ring_start + 1) % k, slot_new = (rs + k - 1) % k;
-            double* RP = mat(arena, jb, AR_RING);
-            double* RE = RP + (long long)k * nn;
-            // per-slot accumulators and ring offsets with compile-time indices, so
-            // they stay in registers (a runtime-indexed array would live in scratch)
+            const int rs = (st->ring_start + 1) % k;
+            // per-slot accumulators with compile-time indices, so they stay in
+            // registers; partial records in a fixed order per thread
             double part[NDDO_MAX_DIIS];
-            long long roff[NDDO_MAX_DIIS];
 #pragma unroll
-            for (int i = 0; i < NDDO_MAX_DIIS; ++i) {
-                part[i] = 0.0;
-                const int sl = i < k ? (rs + i) % k : 0;
-                roff[i] = sl == slot_new ? -1 : (long long)sl * nn;
-            }
+            for (int i = 0; i < NDDO_MAX_DIIS; ++i) part[i] = 0.0;
             double amax = 0.0;
-            if (xc) {   // chunk partials of k_xl_dots, fixed order per thread
+            if (jb.xl) {
                 for (int c = tid; c < nchunk; c += T) {
                     const double* rec = xlpart + ((long long)(jb.xl - 1) * nchunk + c) * XL_REC;
 #pragma unroll
@@ -768,18 +909,14 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
                         if (i < k) part[i] += rec[i];
                     amax = fmax(amax, rec[k]);
                 }
-            }
-            for (long long e = xc ? nn : tid; e < nn; e += T) {
-                const double pv = Pout[e], ev = pv - Pin[e];
-                RP[(long long)slot_new * nn + e] = pv;
-                RE[(long long)slot_new * nn + e] = ev;
-                amax = fmax(amax, fabs(ev));
+            } else {
+                for (int t = tid; t < jb.ntiles; t += T) {
+                    if (t / tpd > t % tpd) continue;
+                    const double* rec = tpart + (long long)(jb.tile0 + t) * XL_REC;
 #pragma unroll
-                for (int i = 0; i < NDDO_MAX_DIIS; ++i) {
-                    if (i < k) {
-                        const double x = roff[i] < 0 ? ev : RE[roff[i] + e];
-                        part[i] += x * ev;
-                    }
+                    for (int i = 0; i < NDDO_MAX_DIIS; ++i)
+                        if (i < k) part[i] += rec[i];
+                    amax = fmax(amax, rec[k]);
                 }
             }
 #pragma unroll
@@ -822,27 +959,18 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
                 st->ring_start = rs;
             }
             __syncthreads();
-            if (xc && tid == 0) {
+            if (tid == 0) {
                 for (int i = 0; i < k; ++i) xc[i] = coef[i];
                 xc[XL_COEF_FLAGS] = diis_on ? 1.0 : 0.0;
                 xc[XL_COEF_ITER] = (double)iter;
                 xc[XL_COEF_RS] = (double)rs;
             }
-            if (diis_on && !xc) {
-                for (long long e = tid; e < nn; e += T) {
-                    double s = 0.0;
-                    for (int i = 0; i < k; ++i) s += coef[i] * RP[(long long)((rs + i) % k) * nn + e];
-                    Pout[e] = s;
-                }
-            }
         }
         const double w = jb.damp_weight;
         const int don = (0.0 < w && jb.damp_thresh < rms) ? 1 : 0;
-        if (don && !xc)
-            for (long long e = tid; e < nn; e += T) Pout[e] = Pout[e] * (1.0 - w) + w * Pin[e];
         if (tid == 0) {
             st->damp_on = don;
-            if (xc && don) {
+            if (don) {
                 xc[XL_COEF_FLAGS] += 2.0;
                 xc[XL_COEF_ITER] = (double)iter;
                 if (k <= 0) xc[XL_COEF_RS] = 0.0;
@@ -860,10 +988,9 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
     }
 }
 
-void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
-                 const double* rmspart, double* traj, int* remaining, const double* xlpart, double* xlcoef,
-                 int nchunk) {
-    hipLaunchKernelGGL(k_step, dim3(njobs), dim3(256), 0, st, jobs, state, arena, rmspart, traj, remaining, xlpart,
+void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, const double* rmspart,
+                 const double* tpart, double* traj, int* remaining, const double* xlpart, double* xlcoef, int nchunk) {
+    hipLaunchKernelGGL(k_step, dim3(njobs), dim3(256), 0, st, jobs, state, rmspart, tpart, traj, remaining, xlpart,
                        xlcoef, nchunk);
 }
 

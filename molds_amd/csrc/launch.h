@@ -62,14 +62,21 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
 void launch_eigen(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist,
                   int max_n, double* arena, double* eps, int* perm, double* tri, bool a_in_lds);
 
+// Density of the upper tiles, mirrored; rmspart: one RMS partial per tile;
+// tpart: XL_REC doubles per tile (DIIS dot products and max |e|).
 void launch_density(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs,
-                    const JobState* state, double* arena, const int* perm, double* rmspart);
+                    const JobState* state, double* arena, const int* perm, double* rmspart, double* tpart);
 
-// xlpart / xlcoef / nchunk: chunk partials and DIIS coefficients of the large
-// molecules (JobDev::xl > 0); unused by the others.
-void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
-                 const double* rmspart, double* traj, int* remaining, const double* xlpart, double* xlcoef,
+// RMS test and DIIS / damping decisions per job from the tile partials
+// (xlpart / nchunk: the chunk partials of the large molecules, JobDev::xl > 0);
+// leaves XL_COEF doubles per job in xlcoef for launch_apply / launch_xl_apply.
+void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, const double* rmspart,
+                 const double* tpart, double* traj, int* remaining, const double* xlpart, double* xlcoef,
                  int nchunk);
+
+// DIIS extrapolation and damping of the jobs that are not large, per upper tile.
+void launch_apply(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, double* arena,
+                  const double* coef);
 
 void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
                    int* remaining, const double* xlpart, int nchunk);
