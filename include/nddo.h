@@ -147,7 +147,10 @@ int nddo_release(nddo_engine* e, nddo_job_id id);
  * and the parts of the eigen stage: 6 tridiagonalization of the matrices kept
  * in LDS, 7 tridiagonalization of the matrices in device memory, 8 divide and
  * conquer, 9 back-transformation (these four up to 512 AOs), 10 the
- * device-wide block-Jacobi solver of the larger molecules. */
+ * device-wide block-Jacobi solver of the larger molecules.  In a batch
+ * (nddo_scf_batch) with matrices of both kinds, the two classes' eigensolves
+ * run concurrently on two streams: stage 6 then times the whole eigensolve of
+ * the LDS class on its stream, and stages 8 and 9 the device-memory class. */
 enum { NDDO_STAGE_SETUP = 0, NDDO_STAGE_FOCK, NDDO_STAGE_EIGEN, NDDO_STAGE_DENSITY, NDDO_STAGE_STEP,
        NDDO_STAGE_ENERGY, NDDO_STAGE_TRIDIAG_LDS, NDDO_STAGE_TRIDIAG_GLB, NDDO_STAGE_DC,
        NDDO_STAGE_BACKTRANSFORM, NDDO_STAGE_BLOCK_JACOBI, NDDO_STAGE_COUNT };

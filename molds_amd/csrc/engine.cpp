@@ -496,10 +496,19 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
                 d_arena, d_part);
     launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
-    launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
-    launch_eigen(st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri, true);
+    // the warm-start rotation is for the block-Jacobi (large) jobs only
+    if (nxl) launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
+    const bool fork = fork_stream && !list_lds.empty() && !list_glb.empty();
+    if (fork) {
+        hipEventRecord(fork_ev, st);
+        hipStreamWaitEvent(fork_stream, fork_ev, 0);
+    }
+    launch_eigen(fork ? fork_stream : st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
+                 d_perm, d_tri, true);
+    if (fork) hipEventRecord(join_ev, fork_stream);
     launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri, false);
     launch_xl_eigen_all(e, st);
+    if (fork) hipStreamWaitEvent(st, join_ev, 0);
     launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart, d_tpart);
     launch_xl_dots(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_step(st, d_jobs, (int)jobs.size(), d_state, d_rmspart, d_tpart, d_traj, d_remaining, d_xlpart, d_xlcoef,
@@ -572,6 +581,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
         e->stage_launches[NDDO_STAGE_SETUP] += 1;
     }
     int* h_rem = e->h_remaining;
+    const bool fork = fork_stream && !list_lds.empty() && !list_glb.empty();
     for (int it = 0; it <= max_iter_all + 1; ++it) {
         if (!prof) {
             iterate(e, st);
@@ -585,21 +595,36 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             launch_xl_energy(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
             launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
             HIPCHK(hipEventRecord(e->ev[1], st));
-            launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
-            launch_heig_part(0, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm,
-                             d_tri, true);
+            if (!list_xl.empty()) launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
+            if (fork) {
+                // the LDS class on the second stream, timed as one stage (all three parts)
+                HIPCHK(hipEventRecord(fork_ev, st));
+                HIPCHK(hipStreamWaitEvent(fork_stream, fork_ev, 0));
+                HIPCHK(hipEventRecord(e->ev[11], fork_stream));
+                for (int part = 0; part < 3; ++part)
+                    launch_heig_part(part, fork_stream, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds,
+                                     d_arena, d_eps, d_perm, d_tri, true);
+                HIPCHK(hipEventRecord(e->ev[12], fork_stream));
+                HIPCHK(hipEventRecord(join_ev, fork_stream));
+            } else {
+                launch_heig_part(0, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
+                                 d_perm, d_tri, true);
+            }
             HIPCHK(hipEventRecord(e->ev[2], st));
             launch_heig_part(0, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
                              d_tri, false);
             HIPCHK(hipEventRecord(e->ev[3], st));
             for (int part = 1; part < 3; ++part) {
-                launch_heig_part(part, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
-                                 d_perm, d_tri, true);
+                if (!fork)
+                    launch_heig_part(part, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena,
+                                     d_eps, d_perm, d_tri, true);
                 launch_heig_part(part, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps,
                                  d_perm, d_tri, false);
                 HIPCHK(hipEventRecord(e->ev[3 + part], st));
             }
             launch_xl_eigen_all(e, st);
+            HIPCHK(hipEventRecord(e->ev[13], st));
+            if (fork) HIPCHK(hipStreamWaitEvent(st, join_ev, 0));
             HIPCHK(hipEventRecord(e->ev[10], st));
             launch_density(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena, d_perm, d_rmspart, d_tpart);
             HIPCHK(hipEventRecord(e->ev[6], st));
@@ -615,11 +640,14 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
         HIPCHK(hipStreamSynchronize(st));
         if (prof) {
             // (stage, first event, last event)
-            const int map[10][3] = {{NDDO_STAGE_FOCK, 0, 9},         {NDDO_STAGE_EIGEN, 1, 10},
-                                    {NDDO_STAGE_TRIDIAG_LDS, 1, 2},  {NDDO_STAGE_TRIDIAG_GLB, 2, 3},
-                                    {NDDO_STAGE_DC, 3, 4},           {NDDO_STAGE_BACKTRANSFORM, 4, 5},
-                                    {NDDO_STAGE_BLOCK_JACOBI, 5, 10}, {NDDO_STAGE_DENSITY, 10, 6},
-                                    {NDDO_STAGE_STEP, 6, 7},         {NDDO_STAGE_ENERGY, 9, 1}};
+            // forked: the LDS-class stage is its whole eigensolve on the second
+            // stream, and D&C / back-transform time the device-memory class
+            const int l0 = fork ? 11 : 1, l1 = fork ? 12 : 2;
+            const int map[10][3] = {{NDDO_STAGE_FOCK, 0, 9},          {NDDO_STAGE_EIGEN, 1, 10},
+                                    {NDDO_STAGE_TRIDIAG_LDS, l0, l1}, {NDDO_STAGE_TRIDIAG_GLB, 2, 3},
+                                    {NDDO_STAGE_DC, 3, 4},            {NDDO_STAGE_BACKTRANSFORM, 4, 5},
+                                    {NDDO_STAGE_BLOCK_JACOBI, 5, 13}, {NDDO_STAGE_DENSITY, 10, 6},
+                                    {NDDO_STAGE_STEP, 6, 7},          {NDDO_STAGE_ENERGY, 9, 1}};
             for (int k = 0; k < 10; ++k) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, e->ev[map[k][1]], e->ev[map[k][2]]));
@@ -752,6 +780,9 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     e->use_graphs = !(e->opts.flags & NDDO_ENGINE_NO_GRAPHS);
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&e->api_stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&e->eig_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&e->eig_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->eig_join, hipEventDisableTiming));
     std::vector<ElemDev> el(3 * kNumElem);
     for (int t = 0; t < 3; ++t)
         for (int i = 0; i < kNumElem; ++i) elem_params(kElemZ[i], t, &el[t * kNumElem + i]);
@@ -790,6 +821,9 @@ int nddo_engine_destroy(nddo_engine* eh) {
     if (e->xl_chain) hipEventDestroy(e->xl_chain);
     if (e->stream) hipStreamDestroy(e->stream);
     if (e->api_stream) hipStreamDestroy(e->api_stream);
+    if (e->eig_fork) hipEventDestroy(e->eig_fork);
+    if (e->eig_join) hipEventDestroy(e->eig_join);
+    if (e->eig_stream) hipStreamDestroy(e->eig_stream);
     delete e;
     return NDDO_OK;
 }
@@ -825,6 +859,9 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
         // jobs whose caller takes no C back-transform only the occupied eigenvectors
         for (size_t jj = 0; jj < ok.size(); ++jj) b.jobs[jj].want_c = res[ok[jj]].C != nullptr ? 1 : 0;
         b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+        b.fork_stream = e->eig_stream;
+        b.fork_ev = e->eig_fork;
+        b.join_ev = e->eig_join;
         e->host_s[0] += lap(tp);
         rc = b.allocate(false);
         if (rc) return rc;

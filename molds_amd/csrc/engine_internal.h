@@ -80,6 +80,10 @@ struct Batch {
     unsigned* d_bar = nullptr;
     double* d_sdbg = nullptr;
     hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream
+    // set: the two one-workgroup eigensolver classes run concurrently, the LDS
+    // class on this stream (forked from and joined to the iteration's stream)
+    hipStream_t fork_stream = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 
     ~Batch();
     int dalloc(void** p, size_t bytes);
@@ -114,6 +118,10 @@ struct Engine {
     // issued from a caller's thread never goes to a cohort stream, which the
     // scheduler thread may be capturing into a graph at that moment
     hipStream_t api_stream = nullptr;
+    // batch runs: the LDS-class eigensolves run on eig_stream beside the
+    // device-memory class on the batch stream (fork and join events)
+    hipStream_t eig_stream = nullptr;
+    hipEvent_t eig_fork = nullptr, eig_join = nullptr;
     ElemDev* d_elems = nullptr;
     size_t lds_max = 65536;
     int num_cus = 0;
