@@ -493,7 +493,7 @@ void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
 void Batch::launch_iteration(Engine* e, hipStream_t st) {
     const int nxl = (int)list_xl.size();
     launch_fock(st, 2, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
-                d_arena, d_part);
+                d_arena, d_part, fork_stream, fork_ev2, join_ev2);
     launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
     // the warm-start rotation is for the block-Jacobi (large) jobs only
@@ -590,7 +590,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             // and between the parts of the eigensolver
             HIPCHK(hipEventRecord(e->ev[0], st));
             launch_fock(st, 2, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
-                        d_ints, d_ints32, d_arena, d_part);
+                        d_ints, d_ints32, d_arena, d_part, fork_stream, fork_ev2, join_ev2);
             HIPCHK(hipEventRecord(e->ev[9], st));
             launch_xl_energy(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
             launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
@@ -783,6 +783,8 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     HIPCHK(hipStreamCreateWithFlags(&e->eig_stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&e->eig_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&e->eig_join, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->fock_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->fock_join, hipEventDisableTiming));
     std::vector<ElemDev> el(3 * kNumElem);
     for (int t = 0; t < 3; ++t)
         for (int i = 0; i < kNumElem; ++i) elem_params(kElemZ[i], t, &el[t * kNumElem + i]);
@@ -823,6 +825,8 @@ int nddo_engine_destroy(nddo_engine* eh) {
     if (e->api_stream) hipStreamDestroy(e->api_stream);
     if (e->eig_fork) hipEventDestroy(e->eig_fork);
     if (e->eig_join) hipEventDestroy(e->eig_join);
+    if (e->fock_fork) hipEventDestroy(e->fock_fork);
+    if (e->fock_join) hipEventDestroy(e->fock_join);
     if (e->eig_stream) hipStreamDestroy(e->eig_stream);
     delete e;
     return NDDO_OK;
@@ -862,6 +866,8 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
         b.fork_stream = e->eig_stream;
         b.fork_ev = e->eig_fork;
         b.join_ev = e->eig_join;
+        b.fork_ev2 = e->fock_fork;
+        b.join_ev2 = e->fock_join;
         e->host_s[0] += lap(tp);
         rc = b.allocate(false);
         if (rc) return rc;

@@ -81,9 +81,10 @@ struct Batch {
     double* d_sdbg = nullptr;
     hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream
     // set: the two one-workgroup eigensolver classes run concurrently, the LDS
-    // class on this stream (forked from and joined to the iteration's stream)
+    // class on this stream (forked from and joined to the iteration's stream),
+    // and so do the light-pair and heavy-pair Fock kernels (fork_ev2, join_ev2)
     hipStream_t fork_stream = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, fork_ev2 = nullptr, join_ev2 = nullptr;
 
     ~Batch();
     int dalloc(void** p, size_t bytes);
@@ -118,10 +119,11 @@ struct Engine {
     // issued from a caller's thread never goes to a cohort stream, which the
     // scheduler thread may be capturing into a graph at that moment
     hipStream_t api_stream = nullptr;
-    // batch runs: the LDS-class eigensolves run on eig_stream beside the
-    // device-memory class on the batch stream (fork and join events)
+    // batch runs: the LDS-class eigensolves and the light-pair Fock kernel run
+    // on eig_stream beside the rest of the iteration on the batch stream (fork
+    // and join events of each)
     hipStream_t eig_stream = nullptr;
-    hipEvent_t eig_fork = nullptr, eig_join = nullptr;
+    hipEvent_t eig_fork = nullptr, eig_join = nullptr, fock_fork = nullptr, fock_join = nullptr;
     ElemDev* d_elems = nullptr;
     size_t lds_max = 65536;
     int num_cus = 0;

@@ -417,10 +417,20 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
 
 void launch_fock(hipStream_t st, int mode, const PairDev* pairs, const int* plist, int nlight, int nheavy,
                  const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state,
-                 const double* ints, const float* ints32, double* arena, double* part) {
+                 const double* ints, const float* ints32, double* arena, double* part, hipStream_t st2,
+                 hipEvent_t fork_ev, hipEvent_t join_ev) {
+    // the light-pair and heavy-pair kernels write disjoint F blocks and pair
+    // slots: with a second stream they run concurrently, joined before the
+    // per-atom reduction
+    const bool fork = st2 && nlight > 0 && nheavy > 0;
+    if (fork) {
+        hipEventRecord(fork_ev, st);
+        hipStreamWaitEvent(st2, fork_ev, 0);
+    }
     if (nlight > 0)
-        hipLaunchKernelGGL(k_fock_light, dim3((nlight + 255) / 256), dim3(256), 0, st, mode, pairs, plist, nlight,
-                           atoms, elems, jobs, state, ints, ints32, arena, part);
+        hipLaunchKernelGGL(k_fock_light, dim3((nlight + 255) / 256), dim3(256), 0, fork ? st2 : st, mode, pairs, plist,
+                           nlight, atoms, elems, jobs, state, ints, ints32, arena, part);
+    if (fork) hipEventRecord(join_ev, st2);
     if (nheavy > 0) {
         const int npairs = nheavy;
         const int* hl = plist + nlight;
@@ -434,6 +444,7 @@ void launch_fock(hipStream_t st, int mode, const PairDev* pairs, const int* plis
         else NDDO_FOCK_PAIRS(4);
 #undef NDDO_FOCK_PAIRS
     }
+    if (fork) hipStreamWaitEvent(st, join_ev, 0);
     hipLaunchKernelGGL(k_fock_atoms, dim3((natoms + 3) / 4), dim3(256), 0, st, mode, atoms, natoms, elems, jobs,
                        state, arena, part);
 }

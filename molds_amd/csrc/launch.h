@@ -35,9 +35,12 @@ void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long lon
 // mode 2: both in one launch (what the SCF loop uses).
 // plist: the pair indices, first the nlight H-H and heavy-H pairs (one lane
 // each), then the nheavy heavy-heavy pairs (one wavefront each).
+// st2 (optional): the light pairs run on it, forked from st by fork_ev and
+// joined by join_ev before the per-atom reduction on st.
 void launch_fock(hipStream_t st, int mode, const PairDev* pairs, const int* plist, int nlight, int nheavy,
                  const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state,
-                 const double* ints, const float* ints32, double* arena, double* part);
+                 const double* ints, const float* ints32, double* arena, double* part, hipStream_t st2 = nullptr,
+                 hipEvent_t fork_ev = nullptr, hipEvent_t join_ev = nullptr);
 
 // Warm start: F <- VT F VT^T for PH_ACTIVE jobs whose VT holds a previous solve.
 void launch_warm(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, const JobState* state,
