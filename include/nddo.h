@@ -97,6 +97,8 @@ typedef struct {
     double* C;               /* [nao*nao] MO rows */
     double* P;               /* [nao*nao] density */
     double* trajectory;      /* [max_iter*4]: rms, diis_error, diis_on, damping_on per iteration */
+    double* mulliken;        /* [natoms] Mulliken charges of the final density: core charge minus the
+                                atom's sum of P_mu,mu (Cndo2.cpp:733, 1187-1199, 1733-1747) */
 } nddo_result;
 
 typedef struct nddo_engine nddo_engine;

@@ -697,6 +697,19 @@ void order_and_phase(int n, const double* VT, const int* perm, double* C) {
 // Copies job jj's results into res (status already set by the caller on failure).
 // All jobs' results with bulk copies of the scalar state, eigenvalues and
 // orbital order; per-job copies only for the requested arrays (C, P, trajectory).
+// Mulliken charges of a final density: the atom's core charge (valence
+// electrons: H 1, C 4, N 5, O 6, F 7) minus its sum of P_mu,mu
+// (Cndo2.cpp:733, 1187-1199, 1733-1747); pdiag[k * stride] = P_kk.
+void mulliken_charges(int natoms, const int* Z, const double* pdiag, long long stride, double* q) {
+    long long mu = 0;
+    for (int a = 0; a < natoms; ++a) {
+        const bool h = Z[a] == 1;
+        double pop = 0.0;
+        for (int k = 0; k < (h ? 1 : 4); ++k, ++mu) pop += pdiag[mu * stride];
+        q[a] = (h ? 1.0 : (double)(Z[a] - 2)) - pop;
+    }
+}
+
 int Batch::fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* const* res, const double* hof_offsets) {
     const size_t nj = jobs.size();
     std::vector<JobState> s(nj);
@@ -706,7 +719,7 @@ int Batch::fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* co
     HIPCHK(hipMemcpyAsync(eps.data(), d_eps, nvec * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(perm.data(), d_perm, nvec * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    std::vector<std::vector<double>> VT(nj);
+    std::vector<std::vector<double>> VT(nj), PD(nj);
     for (size_t jj = 0; jj < nj; ++jj) {
         const JobDev& jb = jobs[jj];
         const nddo_job_desc& d = descs[src[jj]];
@@ -721,17 +734,28 @@ int Batch::fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* co
             HIPCHK(hipMemcpyAsync(VT[jj].data(), base + (long long)AR_C * nn, nn * sizeof(double), hipMemcpyDeviceToHost,
                                   st));
         }
+        const int pk = AR_P0 + ((s[jj].iter + 1) & 1);
         if (r->P) {
-            const int pk = AR_P0 + ((s[jj].iter + 1) & 1);
             HIPCHK(hipMemcpyAsync(r->P, base + (long long)pk * nn, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+        } else if (r->mulliken) {   // only the diagonal of P
+            PD[jj].resize(n);
+            HIPCHK(hipMemcpy2DAsync(PD[jj].data(), sizeof(double), base + (long long)pk * nn, (n + 1) * sizeof(double),
+                                    sizeof(double), n, hipMemcpyDeviceToHost, st));
         }
         if (r->trajectory)
             HIPCHK(hipMemcpyAsync(r->trajectory, d_traj + jb.traj0, 4LL * jb.max_iter * sizeof(double),
                                   hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
-    for (size_t jj = 0; jj < nj; ++jj)
-        if (res[jj]->C) order_and_phase(jobs[jj].nao, VT[jj].data(), perm.data() + jobs[jj].ivec0, res[jj]->C);
+    for (size_t jj = 0; jj < nj; ++jj) {
+        nddo_result* r = res[jj];
+        if (r->C) order_and_phase(jobs[jj].nao, VT[jj].data(), perm.data() + jobs[jj].ivec0, r->C);
+        if (r->mulliken) {
+            const nddo_job_desc& d = descs[src[jj]];
+            if (r->P) mulliken_charges(d.natoms, d.Z, r->P, jobs[jj].nao + 1, r->mulliken);
+            else mulliken_charges(d.natoms, d.Z, PD[jj].data(), 1, r->mulliken);
+        }
+    }
     return NDDO_OK;
 }
 

@@ -111,6 +111,9 @@ struct Batch {
 
 struct ServedJob;
 
+// Mulliken charges from the diagonal of a final density, pdiag[k * stride] = P_kk.
+void mulliken_charges(int natoms, const int* Z, const double* pdiag, long long stride, double* q);
+
 struct Engine {
     int device = 0;
     nddo_engine_opts opts{0, 0, 0};

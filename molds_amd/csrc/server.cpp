@@ -565,17 +565,21 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
     double* C = res->C;
     double* P = res->P;
     double* tr = res->trajectory;
+    double* q = res->mulliken;
     *res = j->res;
     res->eps = eps;
     res->C = C;
     res->P = P;
     res->trajectory = tr;
+    res->mulliken = q;
     const size_t n = (size_t)std::max(j->res.nao, 0);
     if (j->on_host) {
         if (eps) std::memcpy(eps, j->eps.data(), n * sizeof(double));
         if (C) std::memcpy(C, j->C.data(), n * n * sizeof(double));
         if (P) std::memcpy(P, j->P.data(), n * n * sizeof(double));
         if (tr) std::memcpy(tr, j->traj.data(), j->traj.size() * sizeof(double));
+        if (q && j->P.size() == n * n && n > 0)
+            nddo::mulliken_charges((int)j->Z.size(), j->Z.data(), j->P.data(), (long long)n + 1, q);
     }
     return j->res.status;
 }

@@ -46,7 +46,7 @@ class Result(ctypes.Structure):
                 ("energy_elec", ctypes.c_double), ("energy_core", ctypes.c_double),
                 ("heat_of_formation", ctypes.c_double), ("final_rms", ctypes.c_double),
                 ("latency_s", ctypes.c_double), ("eps", _dp), ("C", _dp), ("P", _dp),
-                ("trajectory", _dp)]
+                ("trajectory", _dp), ("mulliken", _dp)]
 
 
 class EngineOpts(ctypes.Structure):
@@ -181,7 +181,7 @@ class Engine:
         self.close()
 
     # --------------------------------------------------------------- SCF
-    def _result_buffers(self, job, want_C, want_P, want_traj):
+    def _result_buffers(self, job, want_C, want_P, want_traj, want_mulliken=False):
         n = self._nao(job)
         bufs = dict(eps=np.zeros(n))
         if want_C:
@@ -190,11 +190,14 @@ class Engine:
             bufs["P"] = np.zeros((n, n))
         if want_traj:
             bufs["traj"] = np.zeros((job.max_iter, 4))
+        if want_mulliken:
+            bufs["mulliken"] = np.zeros(len(job.Z))
         r = Result()
         r.eps = _p(bufs["eps"])
         r.C = _p(bufs.get("C"))
         r.P = _p(bufs.get("P"))
         r.trajectory = _p(bufs.get("traj"))
+        r.mulliken = _p(bufs.get("mulliken"))
         return r, bufs
 
     @staticmethod
@@ -206,7 +209,7 @@ class Engine:
         out = ScfResult(status=r.status, status_name=STATUS.get(r.status, str(r.status)), nao=r.nao, nocc=r.nocc,
                         iterations=r.iterations, fp64_iterations=r.fp64_iterations, energy=r.energy_total, e_el=r.energy_elec, e_core=r.energy_core,
                         hof=r.heat_of_formation, rms=r.final_rms, latency_s=r.latency_s, eps=bufs["eps"],
-                        C=bufs.get("C"), P=bufs.get("P"))
+                        C=bufs.get("C"), P=bufs.get("P"), mulliken=bufs.get("mulliken"))
         if "traj" in bufs:
             it = max(r.iterations, 0)
             t = bufs["traj"][:it]
@@ -214,17 +217,17 @@ class Engine:
         return out
 
     def scf(self, theory, Z, xyz_bohr, precision="fp64", requires_guess=True, P0=None, want_C=True, want_P=True,
-            want_traj=True, **opts):
+            want_traj=True, want_mulliken=True, **opts):
         return self.scf_batch([Job(theory, Z, xyz_bohr, precision, requires_guess, P0, **opts)],
-                              want_C=want_C, want_P=want_P, want_traj=want_traj)[0]
+                              want_C=want_C, want_P=want_P, want_traj=want_traj, want_mulliken=want_mulliken)[0]
 
-    def scf_batch(self, jobs, want_C=False, want_P=False, want_traj=False):
+    def scf_batch(self, jobs, want_C=False, want_P=False, want_traj=False, want_mulliken=False):
         """All jobs share one launch per SCF stage.  Returns per-job results."""
         descs = (JobDesc * len(jobs))(*[j.desc for j in jobs])
         results = (Result * len(jobs))()
         keep = []
         for i, j in enumerate(jobs):
-            r, bufs = self._result_buffers(j, want_C, want_P, want_traj)
+            r, bufs = self._result_buffers(j, want_C, want_P, want_traj, want_mulliken)
             results[i] = r
             keep.append(bufs)
         rc = self.L.nddo_scf_batch(self.h, len(jobs), descs, results)
@@ -270,8 +273,8 @@ class Engine:
     def wait(self, jid, timeout_s=60.0):
         return self.L.nddo_wait(self.h, jid, timeout_s)
 
-    def get_result(self, jid, job, want_C=False, want_P=False, want_traj=False):
-        r, bufs = self._result_buffers(job, want_C, want_P, want_traj)
+    def get_result(self, jid, job, want_C=False, want_P=False, want_traj=False, want_mulliken=False):
+        r, bufs = self._result_buffers(job, want_C, want_P, want_traj, want_mulliken)
         rc = self.L.nddo_get_result(self.h, jid, ctypes.byref(r))
         if rc < 0 and rc not in (-1, -5):
             raise NddoError(rc, last_error())

@@ -152,6 +152,21 @@ def dsyevd_rows(F):
     return eps, C
 
 
+def mulliken(Z, P):
+    """Mulliken charges of a density: core charge minus the atom's sum of P_mu,mu
+    (Cndo2::CalcAtomicElectronPopulation, Cndo2.cpp:1733-1747; printed as
+    core charge - population, Cndo2.cpp:1187-1199). Core charges are the valence
+    electron counts (H 1, C 4, N 5, O 6, F 7); AOs per atom: H 1, heavy 4."""
+    q = np.zeros(len(Z))
+    d = np.diag(P)
+    mu = 0
+    for a, z in enumerate(Z):
+        na = 1 if z == 1 else 4
+        q[a] = (1.0 if z == 1 else float(z - 2)) - d[mu:mu + na].sum()
+        mu += na
+    return q
+
+
 def scf(theory, Z, xyz_bohr, requires_guess=True, P0=None, log=None, **opts):
     """Restatement of Cndo2::DoSCF for MNDO/AM1/PM3 closed-shell RHF.
 

@@ -53,6 +53,19 @@ def test_oracle_reproduces_molds_goldens(case):
         assert r["iterations"] == len(e["iters"])
 
 
+@pytest.mark.parametrize("case", [c for c in _cases() if c["expect"]["mulliken"]], ids=lambda c: c["name"])
+def test_oracle_mulliken_matches_goldens(case):
+    """The printed "Mulliken charge(SCF)" lines (core charge minus the atom's
+    diagonal density, Cndo2.cpp:1187-1199, 1733-1747) to their 7 digits."""
+    r = O.scf(case["theory"], case["Z"], case_xyz_bohr(case), **oracle_opts(case))
+    q = O.mulliken(case["Z"], r["P"])
+    g = np.array(case["expect"]["mulliken"])
+    assert q.shape == g.shape
+    assert np.all(np.abs(q - g) <= REL * np.abs(g) + 1e-9), (q, g)
+    # a neutral molecule: the charges sum to zero (tr P = number of valence electrons)
+    assert abs(q.sum()) < 1e-12
+
+
 # ------------------------------------------------------------ multipoles
 # Each NDDO multipole is a point-charge set (Dewar & Thiel 1977); the
 # interaction of two multipoles is the Klopman-Ohno sum over charge pairs.
