@@ -98,3 +98,31 @@ def test_warm_mixed_and_batched(engine):
         assert abs(alone["energy"] - b["energy"]) * EV <= 1e-9
         assert alone["iterations"] == b["iterations"]
         assert abs(m["energy"] - b["energy"]) * EV <= 1e-5
+
+
+@pytest.mark.parametrize("k,damp", [(0, 0.0), (1, 0.0), (3, 0.7), (16, 0.0), (16, 0.7)])
+def test_diis_ring_sizes_match_oracle(engine, k, damp):
+    """Every DIIS ring size the engine accepts (the ring update and dot
+    products in the density tiles, extrapolation and damping per tile), with
+    and without damping (Cndo2.cpp:920-1061), against the oracle."""
+    from oracle import scf as O
+    Z, x = _mol("decane")
+    opts = dict(OPTS, diis_num_error_vect=k, damping_weight=damp, damping_thresh=0.02)
+    r = engine.scf("pm3", Z, x, **opts)
+    o = O.scf("pm3", Z, x, rms_density=1e-8, diis_k=k, diis_start=0.1, diis_end=0.0, damping_weight=damp,
+              damping_thresh=0.02, max_iter=200)
+    assert r["status"] == 0
+    assert abs(r["energy"] - o["energy"]) * EV <= 1e-6
+    assert np.sqrt(np.mean((r["P"] - o["P"]) ** 2)) <= 1e-8
+    # the trajectory while the RMS is large pins the ring order, the bordered
+    # solve, the DIIS window and the damping; near 1e-8 a long ring's nearly
+    # singular B amplifies rounding, so where the oracle's last RMS above the
+    # threshold is within a factor 2 of it (k = 16 with damping: 1.25e-8) the
+    # iteration count may move by one
+    for (it, rms, de, don, dmp), (_, orms, ode, odon, odmp) in zip(r["traj"], o["traj"]):
+        if orms < 1e-5:
+            break
+        assert abs(rms - orms) <= 1e-6 * orms and abs(de - ode) <= 1e-6 * ode + 1e-300, it
+        assert don == odon and dmp == odmp, it
+    near = len(o["traj"]) > 1 and o["traj"][-2][1] < 2e-8
+    assert abs(r["iterations"] - o["iterations"]) <= (1 if near else 0)
