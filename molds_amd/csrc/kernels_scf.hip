@@ -814,9 +814,11 @@ void launch_apply(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev
 }
 
 // ------------------------------------------------------------------ step
-// RMS test, density-DIIS and damping for one job per workgroup
-// (Cndo2.cpp:663-695, 920-1061).  The DIIS ring keeps the reference's logical
-// order (oldest first) over physical slots starting at ring_start.
+// RMS test, density-DIIS decisions and damping decisions, one job per
+// workgroup (Cndo2.cpp:663-695, 920-1061), from the tile (or chunk) partials;
+// k_apply / k_xl_apply then update the density. The DIIS ring keeps the
+// reference's logical order (oldest first) over physical slots starting at
+// ring_start.
 __device__ bool small_solve(int m, double* a /* m*m row-major, destroyed */, double* b) {
     for (int c = 0; c < m; ++c) {
         int piv = c;

@@ -104,9 +104,11 @@ struct JobState {
 
 struct TileDev { int job, i0, j0, pad; };
 
-// Large molecules (kernels_xl.hip): the chunked DIIS / energy reductions keep
-// NDDO_MAX_DIIS + 1 doubles per (job, chunk), and k_step leaves the DIIS
-// coefficients and what to apply for k_xl_apply in XL_COEF doubles per job.
+// DIIS partial records: NDDO_MAX_DIIS + 1 doubles (the error dot products in
+// logical ring order, then max |e|) per density tile (k_density) or, for large
+// molecules, per chunk (k_xl_dots, with the energy partials of k_xl_energy);
+// k_step leaves the DIIS coefficients and what to apply for k_apply /
+// k_xl_apply in XL_COEF doubles per job.
 #define XL_REC (NDDO_MAX_DIIS + 1)
 #define XL_COEF (NDDO_MAX_DIIS + 4)
 #define XL_COEF_FLAGS (NDDO_MAX_DIIS + 1)   // bit 0: DIIS extrapolation, bit 1: damping
