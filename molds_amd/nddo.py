@@ -123,6 +123,8 @@ class Job:
 
     def __init__(self, theory, Z, xyz_bohr, precision="fp64", requires_guess=True, P0=None, **opts):
         self.Z = np.ascontiguousarray(Z, dtype=np.int32)
+        nh = int(np.count_nonzero(self.Z == 1))
+        self.nao = nh + 4 * (len(self.Z) - nh)   # AOs: H 1, heavy atoms 4
         self.xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
         self.P0 = None if P0 is None else np.ascontiguousarray(P0, dtype=np.float64)
         self.opts = dict(opts)
@@ -181,28 +183,32 @@ class Engine:
         self.close()
 
     # --------------------------------------------------------------- SCF
-    def _result_buffers(self, job, want_C, want_P, want_traj, want_mulliken=False):
-        n = self._nao(job)
-        bufs = dict(eps=np.zeros(n))
+    def _result_buffers(self, job, want_C, want_P, want_traj, want_mulliken=False, r=None):
+        """Caller buffers for one result; fills r (a Result, possibly an element
+        of a results array) and leaves the pointers of unwanted arrays NULL."""
+        n = job.nao
+        if r is None:
+            r = Result()
+        e = np.zeros(n)
+        bufs = {"eps": e}
+        r.eps = e.ctypes.data_as(_dp)
         if want_C:
             bufs["C"] = np.zeros((n, n))
+            r.C = bufs["C"].ctypes.data_as(_dp)
         if want_P:
             bufs["P"] = np.zeros((n, n))
+            r.P = bufs["P"].ctypes.data_as(_dp)
         if want_traj:
             bufs["traj"] = np.zeros((job.max_iter, 4))
+            r.trajectory = bufs["traj"].ctypes.data_as(_dp)
         if want_mulliken:
             bufs["mulliken"] = np.zeros(len(job.Z))
-        r = Result()
-        r.eps = _p(bufs["eps"])
-        r.C = _p(bufs.get("C"))
-        r.P = _p(bufs.get("P"))
-        r.trajectory = _p(bufs.get("traj"))
-        r.mulliken = _p(bufs.get("mulliken"))
+            r.mulliken = bufs["mulliken"].ctypes.data_as(_dp)
         return r, bufs
 
     @staticmethod
     def _nao(job):
-        return int(sum(1 if z == 1 else 4 for z in job.Z))
+        return job.nao
 
     @staticmethod
     def _pack(r, bufs, job):
@@ -225,11 +231,19 @@ class Engine:
         """All jobs share one launch per SCF stage.  Returns per-job results."""
         descs = (JobDesc * len(jobs))(*[j.desc for j in jobs])
         results = (Result * len(jobs))()
-        keep = []
-        for i, j in enumerate(jobs):
-            r, bufs = self._result_buffers(j, want_C, want_P, want_traj, want_mulliken)
-            results[i] = r
-            keep.append(bufs)
+        if want_C or want_P or want_traj or want_mulliken:
+            keep = [self._result_buffers(j, want_C, want_P, want_traj, want_mulliken, results[i])[1]
+                    for i, j in enumerate(jobs)]
+        else:
+            # only the MO energies: one array for all jobs, its addresses written
+            # into the results' eps fields in one vectorised store
+            naos = np.fromiter((j.nao for j in jobs), dtype=np.int64, count=len(jobs))
+            starts = np.concatenate(([0], np.cumsum(naos)[:-1]))
+            eps_all = np.zeros(int(naos.sum()) if len(jobs) else 0)
+            raw = np.frombuffer(results, dtype=np.uint8).reshape(len(jobs), ctypes.sizeof(Result))
+            addr = (np.uint64(eps_all.ctypes.data) + 8 * starts.astype(np.uint64)).astype(np.uint64)
+            raw[:, Result.eps.offset:Result.eps.offset + 8] = addr.view(np.uint8).reshape(len(jobs), 8)
+            keep = [{"eps": eps_all[a:a + n], "_all": eps_all} for a, n in zip(starts.tolist(), naos.tolist())]
         rc = self.L.nddo_scf_batch(self.h, len(jobs), descs, results)
         if rc != 0:
             raise NddoError(rc, last_error())
