@@ -7,7 +7,8 @@
 //   line 2: natoms
 //   then natoms lines "Z x y z" (bohr)
 // Prints "energy <E> core <Ec> iterations <it> nao <n> eps0 <e0> epsN <eN>" and
-// "warm_iterations <k> warm_energy <E>", exits 0 when the exception checks pass.
+// "warm_iterations <k> warm_energy <E>", "forces <3 N values>" (GetForce(0),
+// Hartree/bohr), exits 0 when the exception checks pass.
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -74,6 +75,13 @@ int main(int argc, char** argv) {
     const double e0 = es.GetElectronicEnergy(0);
     es.DoSCF(false);   // from the converged density: few iterations, same energy
     std::printf("warm_iterations %d warm_energy %.15e\n", es.GetNumberIterationsSCF(), es.GetElectronicEnergy(0));
+    {   // forces of the last SCF, one line of 3 N values
+        double const* const* F = es.GetForce(0);
+        std::printf("forces");
+        for (int a = 0; a < natoms; ++a)
+            for (int c = 0; c < 3; ++c) std::printf(" %.17g", F[a][c]);
+        std::printf("\n");
+    }
     if (std::fabs(es.GetElectronicEnergy(0) - e0) > 1e-9) { std::printf("FAIL warm energy\n"); ++fails; }
 
     nddo::Molecule odd = mol;

@@ -1,12 +1,14 @@
 """The C++ ElectronicStructure-shaped adapter (include/nddo_es.hpp) run the way
-MolDS runs Cndo2 -- SetMolecule, DoSCF, getters, DoSCF(false), exceptions -- on
-MolDS golden cases; energies within the goldens' 7 printed digits."""
+MolDS runs Cndo2 -- SetMolecule, DoSCF, getters, DoSCF(false), GetForce(0),
+exceptions -- on MolDS golden cases; energies within the goldens' 7 printed
+digits, forces against the oracle's central differences."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
-from conftest import ROOT, case_xyz_bohr, golden_opts, load_goldens
+from conftest import ROOT, case_xyz_bohr, golden_opts, load_goldens, oracle_opts
 
 BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_es_adapter")
 THEORY = {"mndo": 0, "am1": 1, "pm3": 2}
@@ -41,6 +43,17 @@ def test_cpp_adapter_on_goldens(tmp_path, name):
     assert abs(float(vals["energy"]) - e["energy"]) <= 5.01e-7 * abs(e["energy"])
     assert abs(float(vals["core"]) - e["ecore"]) <= 5.01e-7 * abs(e["ecore"])
     assert int(warm["warm_iterations"]) <= 3
+    # GetForce(0): the same central differences on the oracle's SCF energies
+    from oracle import scf as O
+    from molds_amd.forces import stencil_forces
+    line = [l for l in r.stdout.split("\n") if l.startswith("forces ")][0]
+    F = np.array([float(v) for v in line.split()[1:]]).reshape(-1, 3)
+    oo = dict(oracle_opts(c), rms_density=1e-10, max_iter=200)
+    x0 = case_xyz_bohr(c).reshape(-1, 3)
+    P = O.scf(c["theory"], c["Z"], x0, **oo)["P"]
+    Fo = stencil_forces(lambda gs: [O.scf(c["theory"], c["Z"], g, requires_guess=False, P0=P, **oo)["energy"]
+                                    for g in gs], x0, 1e-3)
+    assert np.max(np.abs(F - Fo)) <= 1e-8
     assert r.stdout.strip().endswith("OK")
 
 
