@@ -526,10 +526,14 @@ void Batch::iterate(Engine* e, hipStream_t st) {
     }
     if (!gexec) {
         hipGraph_t g = nullptr;
-        bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
-        if (ok) {
-            launch_iteration(e, st);
-            ok = hipStreamEndCapture(st, &g) == hipSuccess && g;
+        bool ok;
+        {
+            std::lock_guard<std::mutex> cap(e->capture_mu);
+            ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                launch_iteration(e, st);
+                ok = hipStreamEndCapture(st, &g) == hipSuccess && g;
+            }
         }
         if (ok) ok = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) == hipSuccess;
         if (g) hipGraphDestroy(g);

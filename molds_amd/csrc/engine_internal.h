@@ -136,6 +136,10 @@ struct Engine {
     size_t hbm_budget = 0;
     int* h_remaining = nullptr;
     std::mutex run_mu;   // one synchronous batch at a time
+    // held while a stream captures an iteration graph; HIP calls made from a
+    // caller's thread (served-result copies and frees) take it, so none of them
+    // runs while another thread's capture is open
+    std::mutex capture_mu;
     bool profile = false;
     double stage_ms[NDDO_STAGE_COUNT] = {};
     long long stage_launches[NDDO_STAGE_COUNT] = {};

@@ -261,18 +261,40 @@ int materialize(Engine* e, ServedJob* j) {
     if (j->on_host || !j->d_res) return NDDO_OK;
     const int n = j->res.nao, mi = j->desc.opts.max_iter;
     const size_t nn = (size_t)n * n;
-    if (j->res_ev && j->res_ev->ev && hipEventSynchronize(j->res_ev->ev) != hipSuccess) return NDDO_HIP_ERROR;
+    // wait for the cohort's result copy by polling its event, each query (and
+    // the copy below) outside any capture window of the scheduler thread
+    if (j->res_ev && j->res_ev->ev) {
+        for (;;) {
+            hipError_t q;
+            {
+                std::lock_guard<std::mutex> cap(e->capture_mu);
+                q = hipEventQuery(j->res_ev->ev);
+            }
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady) {
+                nddo::set_error(std::string("result event: ") + hipGetErrorString(q));
+                return NDDO_HIP_ERROR;
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+    }
     std::vector<char> h(result_bytes(n, mi));
-    if (hipMemcpyAsync(h.data(), j->d_res, h.size(), hipMemcpyDeviceToHost, e->api_stream) != hipSuccess ||
-        hipStreamSynchronize(e->api_stream) != hipSuccess)
-        return NDDO_HIP_ERROR;
+    {
+        std::lock_guard<std::mutex> cap(e->capture_mu);
+        hipError_t err = hipMemcpyAsync(h.data(), j->d_res, h.size(), hipMemcpyDeviceToHost, e->api_stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(e->api_stream);
+        if (err == hipSuccess) err = hipFreeAsync(j->d_res, e->api_stream);
+        if (err != hipSuccess) {
+            nddo::set_error(std::string("result copy: ") + hipGetErrorString(err));
+            return NDDO_HIP_ERROR;
+        }
+    }
     const double* hd = (const double*)h.data();
     j->eps.assign(hd, hd + n);
     j->C.resize(nn);
     order_and_phase(n, hd + n, (const int*)(hd + n + 2 * nn + 4 * (size_t)mi), j->C.data());
     j->P.assign(hd + n + nn, hd + n + 2 * nn);
     j->traj.assign(hd + n + 2 * nn, hd + n + 2 * nn + 4 * (size_t)mi);
-    hipFreeAsync(j->d_res, e->api_stream);
     j->d_res = nullptr;
     j->res_ev.reset();
     j->on_host = true;
@@ -560,7 +582,10 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
     }
     std::lock_guard<std::mutex> jl(j->mu);
     hipSetDevice(e->device);
-    if (nddo::materialize(e, j) != NDDO_OK) { nddo::set_error("result copy failed"); return NDDO_HIP_ERROR; }
+    if (nddo::materialize(e, j) != NDDO_OK) {
+        nddo::set_error("result copy failed: " + nddo::get_error());
+        return NDDO_HIP_ERROR;
+    }
     double* eps = res->eps;
     double* C = res->C;
     double* P = res->P;
@@ -610,6 +635,7 @@ int nddo_release(nddo_engine* eh, nddo_job_id id) {
         std::lock_guard<std::mutex> jl(j->mu);
         if (j->d_res) {   // on the API stream, after the retire copies (never on the cohort stream)
             hipSetDevice(e->device);
+            std::lock_guard<std::mutex> cap(e->capture_mu);
             if (j->res_ev && j->res_ev->ev) hipStreamWaitEvent(e->api_stream, j->res_ev->ev, 0);
             hipFreeAsync(j->d_res, e->api_stream);
         }
