@@ -105,7 +105,8 @@ typedef struct nddo_engine nddo_engine;
 typedef int64_t nddo_job_id;
 
 typedef struct {
-    size_t hbm_budget_bytes;  /* admission budget for resident job state; 0 = 90% of free HBM */
+    size_t hbm_budget_bytes;  /* device bytes for job state: the served jobs' resident total and the largest
+                                 synchronous batch (larger batches are split); 0 = 90% of free HBM */
     int max_batch_jobs;       /* served batch capacity; 0 = 4096 */
     int flags;                /* NDDO_ENGINE_* bits, 0 = defaults */
 } nddo_engine_opts;
@@ -130,8 +131,21 @@ int nddo_engine_destroy(nddo_engine* e);
 /* Synchronous single solve (what a DoSCF adapter calls).  Returns the job status. */
 int nddo_scf_once(nddo_engine* e, const nddo_job_desc* job, nddo_result* res);
 /* Synchronous batch: all jobs share one launch per SCF stage.  Returns NDDO_OK
- * when the batch ran (per-job status in res[i].status) or a negative error. */
+ * when the batch ran (per-job status in res[i].status) or a negative error.
+ * The batch's device buffers are carved out of one engine-owned workspace that
+ * grows to the largest batch seen and is reused by later calls.  A batch that
+ * does not fit the device's free memory runs as smaller batches one after the
+ * other; a job that does not fit on its own ends NDDO_ADMISSION_DENIED (the
+ * role of MolDS's heap cap, MallocerFreer.cpp:62-73). */
 int nddo_scf_batch(nddo_engine* e, int njobs, const nddo_job_desc* jobs, nddo_result* res);
+/* Returns the batch workspace and the stream-ordered pool's unused reserve to
+ * the device (no counterpart in MolDS). */
+int nddo_engine_trim(nddo_engine* e);
+/* Device memory as the engine sees it: free and total bytes (hipMemGetInfo),
+ * the batch workspace's size, and the bytes of served results that retired but
+ * were not yet fetched with nddo_get_result (no counterpart in MolDS). */
+int nddo_engine_memory(nddo_engine* e, size_t* free_bytes, size_t* total_bytes, size_t* workspace_bytes,
+                       size_t* served_result_bytes);
 
 /* Serving: continuous batching.  submit copies the job and returns at once;
  * jobs are admitted as device memory frees up and retired at convergence. */

@@ -33,6 +33,7 @@ namespace {
 thread_local std::string g_err;
 }
 
+
 void nddo::set_error(const std::string& s) { g_err = s; }
 std::string nddo::get_error() { return g_err; }
 
@@ -118,17 +119,17 @@ void Batch::release() {
     gexec = nullptr;
     iterations_launched = 0;
     graph_off = false;
-    for (void** p : {(void**)&d_atoms, (void**)&d_pairs, (void**)&d_arows, (void**)&d_jobs, (void**)&d_state, (void**)&d_tiles,
-                     (void**)&d_ints, (void**)&d_ints32, (void**)&d_arena, (void**)&d_part, (void**)&d_ecore, (void**)&d_eps,
-                     (void**)&d_perm, (void**)&d_tri, (void**)&d_rmspart, (void**)&d_traj, (void**)&d_remaining,
-                     (void**)&d_list_lds, (void**)&d_list_glb, (void**)&d_plist, (void**)&d_list_xl, (void**)&d_xl_off,
-                     (void**)&d_xlws, (void**)&d_xlpart, (void**)&d_xlcoef, (void**)&d_tpart, (void**)&d_bar, (void**)&d_sdbg}) {
-        if (*p) {
+    std::vector<std::pair<void**, size_t>> req;
+    plan(true, true, &req);
+    for (auto& r : req) {
+        void** p = r.first;
+        if (*p && !carved) {
             if (alloc_stream) hipFreeAsync(*p, alloc_stream);
             else hipFree(*p);
         }
         *p = nullptr;
     }
+    carved = false;
 }
 
 // Stream-ordered allocations (served cohorts) do not synchronize the device.
@@ -379,49 +380,82 @@ int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao) {
     return NDDO_OK;
 }
 
-int Batch::allocate(bool want_sdbg) {
-    release();
-#define ACHK(x)                      \
-    do {                             \
-        const int rc_ = (x);         \
-        if (rc_ != NDDO_OK) return rc_; \
-    } while (0)
+// Every device buffer of the batch with its size in bytes, in one fixed order.
+// all = true lists the optional buffers too (release() clears every pointer).
+void Batch::plan(bool want_sdbg, bool all, std::vector<std::pair<void**, size_t>>* req) {
+    req->clear();
+    auto add = [&](void* p, size_t bytes) { req->emplace_back((void**)p, bytes); };
     const size_t nj = std::max<size_t>(jobs.size(), 1);
-    ACHK(dalloc((void**)&d_atoms, std::max<size_t>(atoms.size(), 1) * sizeof(AtomDev)));
-    ACHK(dalloc((void**)&d_pairs, std::max<long long>(npairs_all, 1) * sizeof(PairDev)));
-    ACHK(dalloc((void**)&d_plist, std::max<long long>(npairs_all, 1) * sizeof(int)));
-    ACHK(dalloc((void**)&d_arows, std::max<size_t>(arows.size(), 1) * sizeof(AtomRowDev)));
-    ACHK(dalloc((void**)&d_jobs, nj * sizeof(JobDev)));
-    ACHK(dalloc((void**)&d_state, nj * sizeof(JobState)));
-    ACHK(dalloc((void**)&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(TileDev)));
-    ACHK(dalloc((void**)&d_ints, std::max<long long>(nints, 1) * sizeof(double)));
+    add(&d_atoms, std::max<size_t>(atoms.size(), 1) * sizeof(AtomDev));
+    add(&d_pairs, std::max<long long>(npairs_all, 1) * sizeof(PairDev));
+    add(&d_plist, std::max<long long>(npairs_all, 1) * sizeof(int));
+    add(&d_arows, std::max<size_t>(arows.size(), 1) * sizeof(AtomRowDev));
+    add(&d_jobs, nj * sizeof(JobDev));
+    add(&d_state, nj * sizeof(JobState));
+    add(&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(TileDev));
+    add(&d_ints, std::max<long long>(nints, 1) * sizeof(double));
     bool any_mixed = false;
     for (const JobDev& jb : jobs) any_mixed |= jb.precision == NDDO_MIXED;
-    if (any_mixed) ACHK(dalloc((void**)&d_ints32, std::max<long long>(nints, 1) * sizeof(float)));
-    ACHK(dalloc((void**)&d_arena, std::max<long long>(arena_size, 1) * sizeof(double)));
-    ACHK(dalloc((void**)&d_part, std::max<long long>(nslots, 1) * 20 * sizeof(double)));
-    ACHK(dalloc((void**)&d_ecore, std::max<long long>(nslots, 1) * sizeof(double)));
-    ACHK(dalloc((void**)&d_eps, std::max<long long>(nvec, 1) * sizeof(double)));
-    ACHK(dalloc((void**)&d_perm, std::max<long long>(nvec, 1) * sizeof(int)));
-    ACHK(dalloc((void**)&d_tri, (EIG_WS_PER_AO * std::max<long long>(nvec, 1) + EIG_WS_PER_JOB * (long long)nj) *
-                                 sizeof(double)));
-    ACHK(dalloc((void**)&d_rmspart, std::max<size_t>(tiles.size(), 1) * sizeof(double)));
-    ACHK(dalloc((void**)&d_tpart, std::max<size_t>(tiles.size(), 1) * XL_REC * sizeof(double)));
-    ACHK(dalloc((void**)&d_xlcoef, std::max<size_t>(nj, 1) * XL_COEF * sizeof(double)));
-    ACHK(dalloc((void**)&d_traj, std::max<long long>(ntraj, 1) * sizeof(double)));
-    ACHK(dalloc((void**)&d_remaining, sizeof(int)));
-    ACHK(dalloc((void**)&d_list_lds, std::max<size_t>(list_lds.size(), 1) * sizeof(int)));
-    ACHK(dalloc((void**)&d_list_glb, std::max<size_t>(list_glb.size(), 1) * sizeof(int)));
-    if (!list_xl.empty()) {
-        ACHK(dalloc((void**)&d_list_xl, list_xl.size() * sizeof(int)));
-        ACHK(dalloc((void**)&d_xl_off, list_xl.size() * sizeof(long long)));
-        ACHK(dalloc((void**)&d_xlws, xlws_size * sizeof(double)));
-        ACHK(dalloc((void**)&d_xlpart, list_xl.size() * (size_t)xl_chunks * XL_REC * sizeof(double)));
-        ACHK(dalloc((void**)&d_bar, 2 * sizeof(unsigned)));
+    if (any_mixed || all) add(&d_ints32, std::max<long long>(nints, 1) * sizeof(float));
+    add(&d_arena, std::max<long long>(arena_size, 1) * sizeof(double));
+    add(&d_part, std::max<long long>(nslots, 1) * 20 * sizeof(double));
+    add(&d_ecore, std::max<long long>(nslots, 1) * sizeof(double));
+    add(&d_eps, std::max<long long>(nvec, 1) * sizeof(double));
+    add(&d_perm, std::max<long long>(nvec, 1) * sizeof(int));
+    add(&d_tri, (EIG_WS_PER_AO * std::max<long long>(nvec, 1) + EIG_WS_PER_JOB * (long long)nj) * sizeof(double));
+    add(&d_rmspart, std::max<size_t>(tiles.size(), 1) * sizeof(double));
+    add(&d_tpart, std::max<size_t>(tiles.size(), 1) * XL_REC * sizeof(double));
+    add(&d_xlcoef, nj * XL_COEF * sizeof(double));
+    add(&d_traj, std::max<long long>(ntraj, 1) * sizeof(double));
+    add(&d_remaining, sizeof(int));
+    add(&d_list_lds, std::max<size_t>(list_lds.size(), 1) * sizeof(int));
+    add(&d_list_glb, std::max<size_t>(list_glb.size(), 1) * sizeof(int));
+    if (!list_xl.empty() || all) {
+        add(&d_list_xl, std::max<size_t>(list_xl.size(), 1) * sizeof(int));
+        add(&d_xl_off, std::max<size_t>(list_xl.size(), 1) * sizeof(long long));
+        add(&d_xlws, std::max<long long>(xlws_size, 1) * sizeof(double));
+        add(&d_xlpart, std::max<size_t>(list_xl.size(), 1) * (size_t)xl_chunks * XL_REC * sizeof(double));
+        add(&d_bar, 2 * sizeof(unsigned));
     }
-    if (want_sdbg) ACHK(dalloc((void**)&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double)));
+    if (want_sdbg || all) add(&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double));
+}
+
+size_t Batch::device_bytes(bool want_sdbg) {
+    std::vector<std::pair<void**, size_t>> req;
+    plan(want_sdbg, false, &req);
+    size_t b = 0;
+    for (auto& r : req) b += align_up(r.second);
+    return b;
+}
+
+// With `region` set the buffers are carved out of it in plan order (nothing is
+// allocated or freed); otherwise each comes from hipMalloc, or from the
+// stream-ordered pool when alloc_stream is set.
+int Batch::allocate(bool want_sdbg) {
+    release();
+    std::vector<std::pair<void**, size_t>> req;
+    plan(want_sdbg, false, &req);
+    if (region) {
+        size_t off = 0;
+        for (auto& r : req) {
+            const size_t b = align_up(r.second);
+            if (off + b > region_cap) {
+                for (auto& q : req) *q.first = nullptr;
+                set_error("the batch needs " + std::to_string(device_bytes(want_sdbg)) +
+                          " device bytes; its region holds " + std::to_string(region_cap));
+                return NDDO_ADMISSION_DENIED;
+            }
+            *r.first = region + off;
+            off += b;
+        }
+        carved = true;
+        return NDDO_OK;
+    }
+    for (auto& r : req) {
+        const int rc = dalloc(r.first, r.second);
+        if (rc != NDDO_OK) return rc;
+    }
     return NDDO_OK;
-#undef ACHK
 }
 
 // Staging bytes upload() needs; independent of classify(): every job sits in
@@ -763,6 +797,68 @@ int Batch::fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* co
     return NDDO_OK;
 }
 
+size_t device_available(int device) {
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return 0;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t reserved = 0, used = 0;
+        if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) == hipSuccess &&
+            hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess && reserved > used)
+            f += (size_t)(reserved - used);
+    }
+    return f;
+}
+
+void trim_pool(int device) {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) hipMemPoolTrimTo(pool, 0);
+}
+
+void free_workspace(Engine* e) {
+    if (e->d_ws) hipFree(e->d_ws);
+    e->d_ws = nullptr;
+    e->ws_cap = 0;
+}
+
+// Grow-only: a batch that fits the current workspace reuses it as it is; a
+// larger one replaces it with one of the new size plus 1/16 (so a slightly
+// larger batch next time does not reallocate), capped by what the device can
+// give.  1 GiB of the device's free memory is always left to the runtime.
+int ensure_workspace(Engine* e, size_t bytes) {
+    if (bytes <= e->ws_cap) return NDDO_OK;
+    if (bytes > e->hbm_budget) {
+        set_error("a batch needs " + std::to_string(bytes) + " device bytes; the engine's budget is " +
+                  std::to_string(e->hbm_budget));
+        return NDDO_ADMISSION_DENIED;
+    }
+    const size_t margin = (size_t)1 << 30;
+    free_workspace(e);
+    size_t avail = device_available(e->device);
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) == hipSuccess && avail > f) {   // the pool's unused reserve goes back first
+        trim_pool(e->device);
+        if (hipMemGetInfo(&f, &t) == hipSuccess) avail = f;
+    }
+    if (bytes + margin > avail) {
+        set_error("a batch needs " + std::to_string(bytes) + " device bytes; " + std::to_string(avail) +
+                  " are free");
+        return NDDO_ADMISSION_DENIED;
+    }
+    const size_t want = std::min(std::min(bytes + bytes / 16, avail - margin), std::max(bytes, e->hbm_budget));
+    void* p = nullptr;
+    const hipError_t err = hipMalloc(&p, want);
+    if (err != hipSuccess) {
+        hipGetLastError();
+        set_error(std::string("workspace allocation of ") + std::to_string(want) + " bytes: " + hipGetErrorString(err));
+        return NDDO_ADMISSION_DENIED;
+    }
+    e->d_ws = (char*)p;
+    e->ws_cap = want;
+    ++e->ws_grows;
+    return NDDO_OK;
+}
+
 }  // namespace nddo
 
 // ------------------------------------------------------------------ C ABI
@@ -844,6 +940,7 @@ int nddo_engine_destroy(nddo_engine* eh) {
     if (!e) return NDDO_BAD_ARG;
     e->stop_server();
     hipSetDevice(e->device);
+    free_workspace(e);
     if (e->d_elems) hipFree(e->d_elems);
     if (e->h_remaining) hipHostFree(e->h_remaining);
     for (int i = 0; i < 16; ++i)
@@ -857,6 +954,71 @@ int nddo_engine_destroy(nddo_engine* eh) {
     if (e->fock_join) hipEventDestroy(e->fock_join);
     if (e->eig_stream) hipStreamDestroy(e->eig_stream);
     delete e;
+    return NDDO_OK;
+}
+
+// Runs the jobs `which` (indices into descs) as one batch carved out of the
+// engine's workspace.  When they do not fit the device's free memory they run
+// as two halves, one after the other (a job's results do not depend on the
+// batch it shares); a single job that does not fit ends ADMISSION_DENIED.
+static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_result* res, const std::vector<int>& which,
+                     std::chrono::steady_clock::time_point t0) {
+    using Clock = std::chrono::steady_clock;
+    auto lap = [](Clock::time_point& t) {
+        const auto now = Clock::now();
+        const double dt = std::chrono::duration<double>(now - t).count();
+        t = now;
+        return dt;
+    };
+    Clock::time_point tp = Clock::now();
+    Batch b;
+    b.build(descs, njobs, which, nullptr);
+    // jobs whose caller takes no C back-transform only the occupied eigenvectors
+    for (size_t jj = 0; jj < which.size(); ++jj) b.jobs[jj].want_c = res[which[jj]].C != nullptr ? 1 : 0;
+    b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+    const size_t need = b.device_bytes(false);
+    int rc = ensure_workspace(e, need);
+    if (rc == NDDO_ADMISSION_DENIED) {
+        if (which.size() == 1) {
+            res[which[0]].status = NDDO_ADMISSION_DENIED;
+            return NDDO_OK;
+        }
+        const size_t h = which.size() / 2;
+        const std::vector<int> lo(which.begin(), which.begin() + h), hi(which.begin() + h, which.end());
+        rc = run_batch(e, descs, njobs, res, lo, t0);
+        if (rc) return rc;
+        return run_batch(e, descs, njobs, res, hi, t0);
+    }
+    if (rc) return rc;
+    b.region = e->d_ws;
+    b.region_cap = e->ws_cap;
+    b.fork_stream = e->eig_stream;
+    b.fork_ev = e->eig_fork;
+    b.join_ev = e->eig_join;
+    b.fork_ev2 = e->fock_fork;
+    b.join_ev2 = e->fock_join;
+    e->host_s[0] += lap(tp);
+    rc = b.allocate(false);
+    if (rc) return rc;
+    rc = b.upload(e->stream, descs);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->host_s[1] += lap(tp);
+    rc = b.run_scf(e, e->stream);
+    if (rc) return rc;
+    e->host_s[2] += lap(tp);
+    const double wall = std::chrono::duration<double>(Clock::now() - t0).count();
+    std::vector<nddo_result*> rp(which.size());
+    std::vector<double> hof(which.size(), 0.0);
+    for (size_t jj = 0; jj < which.size(); ++jj) {
+        const nddo_job_desc& d = descs[which[jj]];
+        for (int a = 0; a < d.natoms; ++a) hof[jj] += mndo_atom_hof_offset(d.Z[a]);
+        rp[jj] = &res[which[jj]];
+    }
+    rc = b.fetch_all(e->stream, descs, rp.data(), hof.data());
+    if (rc) return rc;
+    for (size_t jj = 0; jj < which.size(); ++jj) res[which[jj]].latency_s = wall;
+    e->host_s[3] += lap(tp);
     return NDDO_OK;
 }
 
@@ -876,50 +1038,37 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
         if (s == NDDO_OK) ok.push_back(i);
     }
     if (ok.empty()) return NDDO_OK;
-    using Clock = std::chrono::steady_clock;
-    auto lap = [](Clock::time_point& t) {
-        const auto now = Clock::now();
-        const double dt = std::chrono::duration<double>(now - t).count();
-        t = now;
-        return dt;
-    };
-    Clock::time_point tp = Clock::now();
-    int rc;
-    {
-        Batch b;
-        b.build(descs, njobs, ok, nullptr);
-        // jobs whose caller takes no C back-transform only the occupied eigenvectors
-        for (size_t jj = 0; jj < ok.size(); ++jj) b.jobs[jj].want_c = res[ok[jj]].C != nullptr ? 1 : 0;
-        b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
-        b.fork_stream = e->eig_stream;
-        b.fork_ev = e->eig_fork;
-        b.join_ev = e->eig_join;
-        b.fork_ev2 = e->fock_fork;
-        b.join_ev2 = e->fock_join;
-        e->host_s[0] += lap(tp);
-        rc = b.allocate(false);
-        if (rc) return rc;
-        rc = b.upload(e->stream, descs);
-        if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(e->stream));
-        e->host_s[1] += lap(tp);
-        rc = b.run_scf(e, e->stream);
-        if (rc) return rc;
-        e->host_s[2] += lap(tp);
-        const double wall = std::chrono::duration<double>(Clock::now() - t0).count();
-    std::vector<nddo_result*> rp(ok.size());
-    std::vector<double> hof(ok.size(), 0.0);
-    for (int jj = 0; jj < (int)ok.size(); ++jj) {
-        const nddo_job_desc& d = descs[ok[jj]];
-        for (int a = 0; a < d.natoms; ++a) hof[jj] += mndo_atom_hof_offset(d.Z[a]);
-        rp[jj] = &res[ok[jj]];
+    const int rc = run_batch(e, descs, njobs, res, ok, t0);
+    for (int i : ok)
+        if (res[i].status == NDDO_ADMISSION_DENIED)
+            set_error("a job's device footprint exceeds the free device memory");
+    return rc;
+}
+
+int nddo_engine_trim(nddo_engine* eh) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    free_workspace(e);
+    trim_pool(e->device);
+    return NDDO_OK;
+}
+
+int nddo_engine_memory(nddo_engine* eh, size_t* free_bytes, size_t* total_bytes, size_t* workspace_bytes,
+                       size_t* served_result_bytes) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return NDDO_BAD_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    if (workspace_bytes) {
+        std::lock_guard<std::mutex> lk(e->run_mu);
+        *workspace_bytes = e->ws_cap;
     }
-        rc = b.fetch_all(e->stream, descs, rp.data(), hof.data());
-        if (rc) return rc;
-        for (int jj = 0; jj < (int)ok.size(); ++jj) res[ok[jj]].latency_s = wall;
-        e->host_s[3] += lap(tp);
-    }
-    e->host_s[4] += lap(tp);   // the batch's device memory returned
+    if (served_result_bytes) *served_result_bytes = (size_t)std::max(0LL, e->result_resident.load());
     return NDDO_OK;
 }
 

@@ -18,6 +18,8 @@ namespace nddo {
 
 constexpr int kNumElem = 5;   // H, C, N, O, F
 
+inline size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
 void set_error(const std::string& s);
 std::string get_error();   // the calling thread's last message
 int elem_slot(int Z);
@@ -30,6 +32,16 @@ size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter);
 size_t result_bytes(int n, int max_iter);
 
 struct Engine;
+
+// Device bytes an allocation can get now: hipMemGetInfo's free bytes plus what
+// the default stream-ordered pool holds reserved but unused.
+size_t device_available(int device);
+// Returns the default pool's unused reserve to the device.
+void trim_pool(int device);
+// Makes the engine's batch workspace at least `bytes` (caller holds run_mu);
+// NDDO_ADMISSION_DENIED when the device cannot provide it.
+int ensure_workspace(Engine* e, size_t bytes);
+void free_workspace(Engine* e);
 
 void fill_scalars(const JobDev& jb, const JobState& s, int theory, double hof_offset, nddo_result* res);
 void order_and_phase(int n, const double* VT, const int* perm, double* C);
@@ -80,6 +92,11 @@ struct Batch {
     unsigned* d_bar = nullptr;
     double* d_sdbg = nullptr;
     hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream
+    // set: every device buffer is carved out of [region, region + region_cap)
+    // (the engine's batch workspace); nothing is allocated or freed per batch
+    char* region = nullptr;
+    size_t region_cap = 0;
+    bool carved = false;
     // set: the two one-workgroup eigensolver classes run concurrently, the LDS
     // class on this stream (forked from and joined to the iteration's stream),
     // and so do the light-pair and heavy-pair Fock kernels (fork_ev2, join_ev2)
@@ -89,6 +106,10 @@ struct Batch {
     ~Batch();
     int dalloc(void** p, size_t bytes);
     void release();
+    // Every device buffer allocate() makes and its size (all: the optional ones too).
+    void plan(bool want_sdbg, bool all, std::vector<std::pair<void**, size_t>>* req);
+    // Bytes allocate() takes, each buffer rounded up to 256 bytes.
+    size_t device_bytes(bool want_sdbg);
     int build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* unused);
     // xl_min_nao: smallest molecule of the device-wide eigensolver class
     int classify(size_t lds_max, int num_cus, int xl_min_nao);
@@ -152,6 +173,13 @@ struct Engine {
     double last_dc_phase[9] = {};
     // nddo_scf_batch host phases (s): build + classify, allocate + upload, SCF, results, release
     double host_s[5] = {};
+    // nddo_scf_batch's device buffers: one allocation, grown when a batch needs
+    // more and reused by every later batch (guarded by run_mu)
+    char* d_ws = nullptr;
+    size_t ws_cap = 0;
+    long long ws_grows = 0;
+    // device bytes of retired served results not yet copied to the host
+    std::atomic<long long> result_resident{0};
 
     // serving state (server.cpp)
     std::mutex smu;

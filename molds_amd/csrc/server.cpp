@@ -134,6 +134,7 @@ void retire_jobs(Engine* e, Cohort* c, const std::vector<int>& done) {
         const long long n = jb.nao, nn = n * n;
         double* d = nullptr;
         if (hipMallocAsync((void**)&d, result_bytes((int)n, jb.max_iter), c->st) != hipSuccess) continue;
+        e->result_resident += (long long)result_bytes((int)n, jb.max_iter);
         const double* base = c->b.d_arena + jb.arena;
         const int pk = AR_P0 + ((s.iter + 1) & 1);
         hipMemcpyAsync(d, c->b.d_eps + jb.vec0, n * sizeof(double), hipMemcpyDeviceToDevice, c->st);
@@ -160,7 +161,10 @@ void retire_jobs(Engine* e, Cohort* c, const std::vector<int>& done) {
         --c->live;
         if (!j) continue;
         if (j->released) {   // dropped by the caller while it ran
-            if (bufs[k]) hipFreeAsync(bufs[k], c->st);
+            if (bufs[k]) {
+                hipFreeAsync(bufs[k], c->st);
+                e->result_resident -= (long long)result_bytes(c->b.jobs[jj].nao, c->b.jobs[jj].max_iter);
+            }
             e->served.erase(j->id);
             delete j;
             c->members[jj] = nullptr;
@@ -296,6 +300,7 @@ int materialize(Engine* e, ServedJob* j) {
     j->P.assign(hd + n + nn, hd + n + 2 * nn);
     j->traj.assign(hd + n + 2 * nn, hd + n + 2 * nn + 4 * (size_t)mi);
     j->d_res = nullptr;
+    e->result_resident -= (long long)result_bytes(n, mi);
     j->res_ev.reset();
     j->on_host = true;
     return NDDO_OK;
@@ -307,7 +312,12 @@ void Engine::start_server() {
     stopping = false;
     server = std::thread([this]() {
         hipSetDevice(device);
-        // keep freed cohort memory in the stream-ordered pool instead of returning it
+        // keep freed cohort memory in the stream-ordered pool, where later
+        // cohorts reuse it: device memory that an instantiated iteration graph
+        // has used did not come back to the device when it was freed (measured
+        // with hipMemGetInfo), so memory released by the pool would be lost to
+        // the process.  The reserve goes back only on request (trim_pool: the
+        // synchronous batch path when it needs the room, nddo_engine_trim).
         hipMemPool_t pool;
         if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
             uint64_t keep = UINT64_MAX;
@@ -333,12 +343,23 @@ void Engine::start_server() {
             // 1. admission: queued requests are grouped by size (AO count) so that a
             //    cohort's iteration time is set by molecules of like size
             std::vector<std::unique_ptr<Cohort>> fresh;
+            bool starved = false;   // requests wait and nothing is resident to free memory
             {
                 std::unique_lock<std::mutex> lk(smu);
                 if (cohorts.empty()) scv.wait(lk, [&] { return stopping || !queue.empty(); });
                 if (stopping) break;
                 std::unique_ptr<Cohort> bucket[kSizeClasses];
                 size_t nb = 0, admitted = 0;
+                // the budget, the device memory free right now (a synchronous
+                // batch's workspace or other users may hold some) and the bytes
+                // of retired results not yet fetched, less 1 GiB for the runtime
+                const size_t results = (size_t)std::max(0LL, result_resident.load());
+                const size_t margin = (size_t)1 << 30;
+                const size_t avail = queue.empty() ? 0 : device_available(device);
+                const size_t room_dev = avail > margin ? avail - margin : 0;
+                const size_t used = resident + results;
+                const size_t room_budget = hbm_budget > used ? hbm_budget - used : 0;
+                const size_t room = std::min(room_dev, room_budget);
                 for (auto it = queue.begin(); it != queue.end();) {
                     ServedJob* j = *it;
                     const int k = size_class(j->res.nao, xl_min_nao);
@@ -349,7 +370,7 @@ void Engine::start_server() {
                     }
                     Cohort* c = bucket[k].get();
                     const int kcap = k == kSizeClasses - 1 ? std::min(cap, xl_max_jobs()) : cap;
-                    if ((int)c->members.size() >= kcap || resident + admitted + j->footprint > hbm_budget) {
+                    if ((int)c->members.size() >= kcap || admitted + j->footprint > room) {
                         ++it;
                         continue;
                     }
@@ -360,6 +381,16 @@ void Engine::start_server() {
                 }
                 for (auto& b : bucket)
                     if (b && !b->members.empty()) fresh.push_back(std::move(b));
+                starved = fresh.empty() && cohorts.empty() && !queue.empty();
+            }
+            if (starved) {
+                // nothing resident will free memory: return an idle synchronous
+                // batch workspace and the pool's reserve, then try again
+                std::unique_lock<std::mutex> rl(run_mu, std::try_to_lock);
+                if (rl.owns_lock() && d_ws) free_workspace(this);
+                if (rl.owns_lock()) trim_pool(device);
+                std::this_thread::sleep_for(std::chrono::milliseconds(1));
+                continue;
             }
             for (auto& f : fresh) {
                 const int k = free_slots.back();
@@ -456,6 +487,7 @@ void Engine::start_server() {
             for (auto& kv : served)
                 if (kv.second->d_res) {
                     hipFree(kv.second->d_res);
+                    result_resident -= (long long)result_bytes(kv.second->res.nao, kv.second->desc.opts.max_iter);
                     kv.second->d_res = nullptr;
                 }
         }
@@ -638,6 +670,8 @@ int nddo_release(nddo_engine* eh, nddo_job_id id) {
             std::lock_guard<std::mutex> cap(e->capture_mu);
             if (j->res_ev && j->res_ev->ev) hipStreamWaitEvent(e->api_stream, j->res_ev->ev, 0);
             hipFreeAsync(j->d_res, e->api_stream);
+            e->result_resident -= (long long)nddo::result_bytes(j->res.nao, j->desc.opts.max_iter);
+            j->d_res = nullptr;
         }
     }
     delete j;

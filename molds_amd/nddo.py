@@ -91,6 +91,9 @@ def load_library(path=LIBPATH):
     L.nddo_dbg_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.nddo_dbg_host_times.argtypes = [vp, _dp, ctypes.c_int]
     L.nddo_dbg_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+    L.nddo_engine_trim.argtypes = [vp]
+    _szp = ctypes.POINTER(ctypes.c_size_t)
+    L.nddo_engine_memory.argtypes = [vp, _szp, _szp, _szp, _szp]
     _lib = L
     return L
 
@@ -258,6 +261,19 @@ class Engine:
         it = ctypes.c_longlong()
         self.L.nddo_dbg_server_stats(self.h, _p(t), ctypes.byref(it), 1 if reset else 0)
         return dict(zip(("admit_setup_s", "launch_s", "wait_s", "retire_s"), t.tolist()), cohort_iterations=it.value)
+
+    def trim(self):
+        """Returns the batch workspace and the pool's unused reserve to the device."""
+        rc = self.L.nddo_engine_trim(self.h)
+        if rc != 0:
+            raise NddoError(rc, last_error())
+
+    def memory(self):
+        f, t, w, r = (ctypes.c_size_t() for _ in range(4))
+        rc = self.L.nddo_engine_memory(self.h, ctypes.byref(f), ctypes.byref(t), ctypes.byref(w), ctypes.byref(r))
+        if rc != 0:
+            raise NddoError(rc, last_error())
+        return dict(free=f.value, total=t.value, workspace=w.value, served_results=r.value)
 
     def info(self):
         lds, cus, budget, nmax = ctypes.c_size_t(), ctypes.c_int(), ctypes.c_size_t(), ctypes.c_int()
