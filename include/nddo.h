@@ -138,6 +138,18 @@ int nddo_scf_once(nddo_engine* e, const nddo_job_desc* job, nddo_result* res);
  * other; a job that does not fit on its own ends NDDO_ADMISSION_DENIED (the
  * role of MolDS's heap cap, MallocerFreer.cpp:62-73). */
 int nddo_scf_batch(nddo_engine* e, int njobs, const nddo_job_desc* jobs, nddo_result* res);
+/* nddo_scf_once without any device allocation: no counterpart in MolDS; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Every device buffer of
+ * the solve is carved out of a fixed file-scope scratch array of
+ * nddo_noalloc_scratch_bytes() (64 MiB) per GPU, which the library holds until
+ * the process ends; a job that needs more (about 500 AOs for water clusters, or
+ * any molecule of the device-wide eigensolver class) ends NDDO_ADMISSION_DENIED
+ * with a message naming both sizes.  The iterations launch eagerly (no graph
+ * instantiation).  Calls that overlap on one GPU share its array and race; the
+ * engine serialises the calls made through it. */
+int nddo_scf_once_noalloc(nddo_engine* e, const nddo_job_desc* job, nddo_result* res);
+size_t nddo_noalloc_scratch_bytes(void);
 /* Returns the batch workspace and the stream-ordered pool's unused reserve to
  * the device (no counterpart in MolDS). */
 int nddo_engine_trim(nddo_engine* e);

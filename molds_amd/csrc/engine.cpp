@@ -31,8 +31,15 @@ using namespace nddo;
 
 namespace {
 thread_local std::string g_err;
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
 }
 
+// Device scratch of nddo_scf_once_noalloc: every buffer of its one-molecule
+// batch is carved out of this array, so that entry point allocates nothing.
+// Each GPU the library runs on holds its own copy until the process ends; host
+// code uses only the address hipGetSymbolAddress gives for the current device
+// (Engine::d_scratch), never the array's name.
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
 
 void nddo::set_error(const std::string& s) { g_err = s; }
 std::string nddo::get_error() { return g_err; }
@@ -552,7 +559,7 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
 }
 
 void Batch::iterate(Engine* e, hipStream_t st) {
-    const bool eager = !e->use_graphs || !list_xl.empty() || graph_off || iterations_launched == 0;
+    const bool eager = !e->use_graphs || !list_xl.empty() || graph_off || eager_only || iterations_launched == 0;
     ++iterations_launched;
     if (eager) {
         launch_iteration(e, st);
@@ -923,6 +930,15 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     HIPCHK(hipHostMalloc((void**)&e->h_remaining, sizeof(int), hipHostMallocDefault));
     for (int i = 0; i < 16; ++i) HIPCHK(hipEventCreate(&e->ev[i]));
     HIPCHK(hipEventCreateWithFlags(&e->xl_chain, hipEventDisableTiming));
+    {
+        void* ptr = nullptr;
+        const hipError_t se = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (se != hipSuccess || !ptr) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se));
+            return NDDO_HIP_ERROR;
+        }
+        e->d_scratch = (char*)ptr;
+    }
     size_t freeb = 0, totb = 0;
     HIPCHK(hipMemGetInfo(&freeb, &totb));
     e->hbm_budget = e->opts.hbm_budget_bytes ? e->opts.hbm_budget_bytes : (size_t)(0.9 * freeb);
@@ -1044,6 +1060,56 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
             set_error("a job's device footprint exceeds the free device memory");
     return rc;
 }
+
+// nddo_scf_once on buffers carved out of the fixed scratch array: no device
+// allocation or free, no graph instantiation (the iterations launch eagerly).
+// Calls that overlap on one device share that device's array and race.
+int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result* res) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !job || !res) { set_error("bad arguments"); return NDDO_BAD_ARG; }
+    auto t0 = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    int n = 0, no = 0;
+    const int s = validate_job(*job, &n, &no);
+    res->status = s;
+    res->nao = n;
+    res->nocc = no;
+    if (s != NDDO_OK) return s;
+    Batch b;
+    const std::vector<int> which{0};
+    b.build(job, 1, which, nullptr);
+    b.jobs[0].want_c = res->C != nullptr ? 1 : 0;
+    b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+    const size_t need = b.device_bytes(false);
+    if (need > kNoAllocScratchBytes || !b.list_xl.empty()) {
+        set_error("nddo_scf_once_noalloc: the job needs " + std::to_string(need) +
+                  " device bytes; the fixed scratch holds " + std::to_string(kNoAllocScratchBytes));
+        res->status = NDDO_ADMISSION_DENIED;
+        return NDDO_ADMISSION_DENIED;
+    }
+    b.region = e->d_scratch;
+    b.region_cap = kNoAllocScratchBytes;
+    int rc = b.allocate(false);
+    if (rc) return rc;
+    b.eager_only = true;
+    rc = b.upload(e->stream, job);   // clears the arena and trajectory the last call left
+    if (rc) return rc;
+    rc = b.run_scf(e, e->stream);
+    if (rc) return rc;
+    double hof = 0.0;
+    for (int a = 0; a < job->natoms; ++a) hof += mndo_atom_hof_offset(job->Z[a]);
+    nddo_result* rp = res;
+    rc = b.fetch_all(e->stream, job, &rp, &hof);
+    if (rc) return rc;
+    res->latency_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (res->status == NDDO_NOT_CONVERGED)
+        set_error("Error in cndo::Cndo2::DoSCF: SCF did not met convergence criterion. maxIterationsSCF=" +
+                  std::to_string(job->opts.max_iter));
+    return res->status;
+}
+
+size_t nddo_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
 
 int nddo_engine_trim(nddo_engine* eh) {
     Engine* e = reinterpret_cast<Engine*>(eh);

@@ -93,7 +93,8 @@ struct Batch {
     double* d_sdbg = nullptr;
     hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream
     // set: every device buffer is carved out of [region, region + region_cap)
-    // (the engine's batch workspace); nothing is allocated or freed per batch
+    // (the engine's batch workspace, or the fixed scratch array of
+    // nddo_scf_once_noalloc); nothing is allocated or freed per batch
     char* region = nullptr;
     size_t region_cap = 0;
     bool carved = false;
@@ -126,6 +127,7 @@ struct Batch {
     hipGraphExec_t gexec = nullptr;
     int iterations_launched = 0;
     bool graph_off = false;
+    bool eager_only = false;   // never capture (nddo_scf_once_noalloc: instantiation may allocate)
     int run_scf(Engine* e, hipStream_t st);
     int fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* const* res, const double* hof_offsets);
 };
@@ -180,6 +182,9 @@ struct Engine {
     long long ws_grows = 0;
     // device bytes of retired served results not yet copied to the host
     std::atomic<long long> result_resident{0};
+    // device address of this device's copy of the file-scope scratch array of
+    // nddo_scf_once_noalloc (engine.cpp), from hipGetSymbolAddress at creation
+    char* d_scratch = nullptr;
 
     // serving state (server.cpp)
     std::mutex smu;

@@ -92,6 +92,8 @@ def load_library(path=LIBPATH):
     L.nddo_dbg_host_times.argtypes = [vp, _dp, ctypes.c_int]
     L.nddo_dbg_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.nddo_engine_trim.argtypes = [vp]
+    L.nddo_scf_once_noalloc.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(Result)]
+    L.nddo_noalloc_scratch_bytes.restype = ctypes.c_size_t
     _szp = ctypes.POINTER(ctypes.c_size_t)
     L.nddo_engine_memory.argtypes = [vp, _szp, _szp, _szp, _szp]
     _lib = L
@@ -251,6 +253,14 @@ class Engine:
         if rc != 0:
             raise NddoError(rc, last_error())
         return [self._pack(results[i], keep[i], jobs[i]) for i in range(len(jobs))]
+
+    def scf_once_noalloc(self, job, want_C=True, want_P=True, want_traj=True, want_mulliken=True):
+        """nddo_scf_once_noalloc: one solve on the library's fixed device scratch."""
+        r, bufs = self._result_buffers(job, want_C, want_P, want_traj, want_mulliken)
+        rc = self.L.nddo_scf_once_noalloc(self.h, ctypes.byref(job.desc), ctypes.byref(r))
+        if rc < 0 and rc not in (-1, -5):
+            raise NddoError(rc, last_error())
+        return self._pack(r, bufs, job)
 
     # --------------------------------------------------------------- profiling
     STAGES = ("setup", "fock", "eigen", "density", "step", "energy", "tridiag_lds", "tridiag_glb", "divide_conquer",
