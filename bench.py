@@ -43,6 +43,7 @@ sys.path.insert(0, HERE)
 METRIC = "PM3 SCF solves/s + p50/p99 latency, mixed vs FP64 GPU at |ΔE|≤1e-5 eV"
 EV = 1.0 / 0.03674903
 FP64_PEAK_TFLOPS = 78.6       # MI355X dense FP64 (matrix and vector)
+FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 MFMA
 HBM_PEAK_TBPS = 8.0           # MI355X HBM3E
 # HBM bytes per launch of the dominant kernel on the 4096-molecule pass of a
 # precision mode, from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised
@@ -85,20 +86,32 @@ def make_jobs(mols, precision, opts=SCF):
 
 
 def batched_leg(engine, jobs, steps, warmup):
+    """Warmup passes, then `steps` timed passes on the product path (hipGraph
+    replay of every iteration after a batch's first, no profiling events),
+    then one more pass with per-stage hipEvents (eager launches, the same
+    kernels) for the stage times and the roofline, outside the timed region."""
     for _ in range(warmup):
         engine.scf_batch(jobs)
-    engine.profile(True)
-    engine.stage_times(reset=True)
     engine.host_times(reset=True)
+    g0 = engine.graph_stats()
     t0 = time.perf_counter()
     res = None
     for _ in range(steps):
         res = engine.scf_batch(jobs)
     wall = time.perf_counter() - t0
+    g1 = engine.graph_stats()
+    log(f"host phases: {engine.host_times(reset=True)}; graph replays in the timed passes: "
+        f"{g1['replays'] - g0['replays']}")
+    engine.profile(True)
+    engine.stage_times(reset=True)
+    t1 = time.perf_counter()
+    prof_res = engine.scf_batch(jobs)
+    prof_wall = time.perf_counter() - t1
     st = engine.stage_times(reset=True)
-    log(f"host phases: {engine.host_times(reset=True)}")
     engine.profile(False)
-    return res, wall, st
+    same = all(a["iterations"] == b["iterations"] and a["energy"] == b["energy"] for a, b in zip(res, prof_res))
+    return res, wall, st, dict(graph_replays=g1["replays"] - g0["replays"], profiled_pass_s=round(prof_wall, 4),
+                               profiled_pass_bitwise_equal=bool(same))
 
 
 SERVED_LO, SERVED_HI = 5, 600   # request sizes (atoms), log-uniform
@@ -210,36 +223,134 @@ def plain_leg(engine, jobs, seconds):
                                                          f"each, FP64 ({dt:.1f} s)")
 
 
-def cpu_leg(mols, seconds):
-    """The oracle (C restatement + numpy/LAPACK SCF driver) on the same molecules,
-    in order, until `seconds` of CPU work: a bounded sample of the workload."""
+def _cpu_one(args):
+    """One oracle solve on one thread (process-pool worker)."""
+    Z, x = args
+    from threadpoolctl import threadpool_limits
     from oracle import scf as O
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
-    except Exception:
-        threads = 1
-    t0 = time.perf_counter()
-    done = conv = 0
-    for kind, size, Z, x in mols:
+    with threadpool_limits(1):
         try:
             O.scf("pm3", Z, np.asarray(x) * O.ANG2AU, **CPU_SCF)
-            conv += 1
+            return 1
         except O.OracleError:
-            pass
-        done += 1
-        if time.perf_counter() - t0 > seconds:
-            break
+            return 0
+
+
+def _host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(nproc=os.cpu_count(), model=model)
+
+
+def cpu_leg(mols, seconds, workers):
+    """The oracle (C restatement of MolDS's integrals / Fock / core repulsion +
+    numpy LAPACK dsyevd SCF driver) on the same molecules, in order, on the
+    host: (a) one thread, one molecule after another; (b) `workers` processes,
+    one molecule per process, one thread each.  Each runs until `seconds` of
+    wall time: a bounded sample of the workload.  Runs before the engine
+    touches the GPU, so the pool's processes start from a GPU-free parent."""
+    import multiprocessing as mp
+    from threadpoolctl import threadpool_limits
+    from oracle import scf as O
+    O.build() if not os.path.exists(os.path.join(HERE, "oracle", "_build", "liboracle.so")) else None
+    out = dict(host=_host_cpu(), kind="port")
+    t0 = time.perf_counter()
+    done = conv = 0
+    with threadpool_limits(1):
+        for kind, size, Z, x in mols:
+            try:
+                O.scf("pm3", Z, np.asarray(x) * O.ANG2AU, **CPU_SCF)
+                conv += 1
+            except O.OracleError:
+                pass
+            done += 1
+            if time.perf_counter() - t0 > seconds:
+                break
     dt = time.perf_counter() - t0
-    return dict(value=round(done / dt, 3), unit="solves/s", cores=threads, kind="port",
-                sample=f"first {done} molecules of the same PM3 mix ({conv} converged), solved one at a time "
-                       f"(oracle: C Fock/integrals + numpy LAPACK dsyevd, {dt:.1f} s)")
+    out["one_thread"] = dict(value=round(done / dt, 3), unit="solves/s", cores=1,
+                             sample=f"first {done} molecules of the mix ({conv} converged), one thread, {dt:.1f} s")
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    done = conv = 0
+    with ctx.Pool(workers) as pool:
+        for ok in pool.imap(_cpu_one, [(Z, x) for _, _, Z, x in mols], chunksize=1):
+            done += 1
+            conv += ok
+            if time.perf_counter() - t0 > seconds:
+                break
+        pool.terminate()
+    dt = time.perf_counter() - t0
+    out.update(value=round(done / dt, 3), unit="solves/s", cores=workers,
+               sample=f"first {done} molecules of the same PM3 mix ({conv} converged), one molecule per process on "
+                      f"{workers} processes of one thread each, {dt:.1f} s (oracle: C Fock/integrals + numpy LAPACK "
+                      f"dsyevd); one_thread: the same code on one core, molecules in order")
+    return out
+
+
+def unique_integrals(Z):
+    """I = 100 C(N_hv, 2) + 10 N_hv N_H + C(N_H, 2): the packed two-centre integrals."""
+    nh = int(sum(1 for z in Z if z == 1))
+    nx = len(Z) - nh
+    return 100 * nx * (nx - 1) // 2 + 10 * nx * nh + nh * (nh - 1) // 2
+
+
+def stage_rooflines(lg, nao, nocc, nints, lds_max_n, steps_prof=1):
+    """Each SCF stage of the profiled pass against the roofline that bounds it,
+    with the algorithmic work per molecule-iteration of the scope table:
+    Fock B = 8 I + 16 n^2 bytes (FP32 iterations of a mixed job: 4 I + 8 n^2;
+    one more FP64 build per solve for the energy), eigensolve (10/3) n^3 FP64
+    flops (tridiagonalization (4/3) n^3), density 2 n^2 n_occ flops."""
+    st = lg["stages"]
+    it = lg["iters"]
+    it64 = lg["it64"]
+    it32 = it - it64
+    glb = nao > lds_max_n
+    out = {}
+    ms = st["fock"][0] / steps_prof
+    b = float(np.sum(it32 * (4.0 * nints + 8.0 * nao ** 2) + (it64 + 1) * (8.0 * nints + 16.0 * nao ** 2)))
+    out["fock"] = dict(bound="hbm", ms=round(ms, 3), algorithmic_GB=round(b / 1e9, 3),
+                       achieved_GBps=round(b / (ms * 1e-3) / 1e9, 1), frac=round(b / (ms * 1e-3) / 8e12, 4))
+    ms = st["eigen"][0] / steps_prof
+    f = float(np.sum(it * (10.0 / 3.0) * nao ** 3))
+    out["eigensolve"] = dict(bound="fp64", ms=round(ms, 3), algorithmic_TFLOP=round(f / 1e12, 4),
+                             achieved_TFLOPs=round(f / (ms * 1e-3) / 1e12, 3),
+                             frac=round(f / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                             note="both eigensolver classes, which run concurrently on two streams")
+    ms = st["tridiag_glb"][0] / steps_prof
+    f = float(np.sum(it[glb] * (4.0 / 3.0) * nao[glb] ** 3))
+    out["tridiag_device_memory"] = dict(bound="fp64", ms=round(ms, 3), algorithmic_TFLOP=round(f / 1e12, 4),
+                                        achieved_TFLOPs=round(f / (ms * 1e-3) / 1e12, 3) if ms > 0 else None,
+                                        frac=round(f / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4) if ms > 0 else None,
+                                        molecules=int(glb.sum()))
+    ms = st["tridiag_lds"][0] / steps_prof
+    f = float(np.sum(it[~glb] * (10.0 / 3.0) * nao[~glb] ** 3))
+    out["eigensolve_lds_class"] = dict(bound="fp64", ms=round(ms, 3), algorithmic_TFLOP=round(f / 1e12, 4),
+                                       achieved_TFLOPs=round(f / (ms * 1e-3) / 1e12, 3) if ms > 0 else None,
+                                       frac=round(f / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4) if ms > 0 else None,
+                                       molecules=int((~glb).sum()),
+                                       note="its whole eigensolve on the second stream, beside the other class")
+    ms = st["density"][0] / steps_prof
+    f = float(np.sum(it * 2.0 * nao ** 2 * nocc))
+    f32 = float(np.sum(it32 * 2.0 * nao ** 2 * nocc))
+    peak = FP64_PEAK_TFLOPS if f32 == 0 else FP32_PEAK_TFLOPS
+    out["density"] = dict(bound="mfma", ms=round(ms, 3), algorithmic_TFLOP=round(f / 1e12, 4),
+                          achieved_TFLOPs=round(f / (ms * 1e-3) / 1e12, 3),
+                          frac=round(f / (ms * 1e-3) / 1e12 / peak, 4), peak_TFLOPs=peak,
+                          note="includes the DIIS ring write and error dot products of the same tiles")
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--modes", default="fp64,mixed")
@@ -253,7 +364,8 @@ def main():
                          "counters FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--latency-rate", type=float, default=500.0, help="served rate whose p50/p99 is the headline")
     ap.add_argument("--plain-seconds", type=float, default=8.0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-workers", type=int, default=16, help="host processes of the CPU baseline (at most nproc)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -273,25 +385,33 @@ def main():
         sys.exit(2)
 
     from molds_amd import workloads as W
-    from molds_amd.nddo import Engine
-    engine = Engine(0)
     t = time.perf_counter()
     mols = list(W.batch_mix(args.count))
     nao = np.array([nao_of(m[2]) for m in mols], dtype=float)
+    nocc = np.array([sum(1 if z == 1 else z - 2 for z in m[2]) // 2 for m in mols], dtype=float)
+    nints = np.array([unique_integrals(m[2]) for m in mols], dtype=float)
     log(f"built {len(mols)} molecules in {time.perf_counter() - t:.1f} s")
+    cpu = None
+    if not args.no_cpu:   # before the engine touches the GPU (the pool forks this process)
+        cpu = cpu_leg(mols, args.cpu_seconds, max(1, min(args.cpu_workers, os.cpu_count() or 1)))
+        log(f"cpu baseline: {cpu}")
 
+    from molds_amd.nddo import Engine
+    engine = Engine(0)
     legs = {}
     for mode in args.modes.split(","):
         jobs = make_jobs(mols, mode)
-        res, wall, st = batched_leg(engine, jobs, args.steps, args.warmup)
+        res, wall, st, extra = batched_leg(engine, jobs, args.steps, args.warmup)
         ok = np.array([r["status"] == 0 for r in res])
         iters = np.array([r["iterations"] for r in res], dtype=float)
         it64 = np.array([r["fp64_iterations"] for r in res], dtype=float)
-        legs[mode] = dict(res=res, wall=wall, stages=st, ok=ok, iters=iters, it64=it64,
+        legs[mode] = dict(res=res, wall=wall, stages=st, ok=ok, iters=iters, it64=it64, extra=extra,
                           solves_per_s=len(jobs) * args.steps / wall, ms_per_step=1e3 * wall / args.steps)
         log(f"{mode}: {legs[mode]['solves_per_s']:.1f} solves/s, {legs[mode]['ms_per_step']:.1f} ms/step, "
-            f"converged {ok.sum()}/{len(ok)}, mean iters {iters.mean():.1f}, stages {st}")
+            f"converged {ok.sum()}/{len(ok)}, mean iters {iters.mean():.1f}, {extra}, stages {st}")
         del jobs
+    mem = engine.memory()
+    engine.trim()   # the served legs and the (H2O)_256 leg get the batch workspace back
 
     # a mode counts only if its energies agree with the FP64 leg within 1e-5 eV;
     # without an FP64 leg (profiling runs of one mode) the first mode stands
@@ -314,22 +434,23 @@ def main():
             best_mode, best = mode, lg
 
     # roofline of the dominant kernel of the step: the tridiagonalization of
-    # the matrices that do not fit LDS (k_tridiag_sym), timed with hipEvents
-    # on the engine's stream around its launches; algorithmic work (4/3) n^3
-    # FP64 flops per eigensolve, one eigensolve per SCF iteration per molecule.
+    # the matrices that do not fit LDS (k_tridiag_sym), timed with hipEvents on
+    # the engine's stream around its launches in the profiled pass (the same
+    # kernels and inputs as the timed passes); algorithmic work (4/3) n^3 FP64
+    # flops per eigensolve, one eigensolve per SCF iteration per molecule.
     # The same launches seen as HBM traffic: the upper triangle of the trailing
     # block read and written once per Householder step, 8 n^3 / 3 bytes with
     # FP64 storage, 4 n^3 / 3 in the FP32 iterations of a mixed-precision job.
     info = engine.info()
     glb = nao > info["lds_tridiag_max_n"]
     ms_tri, launches_tri = best["stages"]["tridiag_glb"]
-    flops = args.steps * float(np.sum(best["iters"][glb] * (4.0 / 3.0) * nao[glb] ** 3))
+    flops = float(np.sum(best["iters"][glb] * (4.0 / 3.0) * nao[glb] ** 3))
     achieved = flops / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
     it64 = best["it64"][glb]
-    alg_bytes = args.steps * float(np.sum((it64 * 8.0 + (best["iters"][glb] - it64) * 4.0) / 3.0 * nao[glb] ** 3))
+    alg_bytes = float(np.sum((it64 * 8.0 + (best["iters"][glb] - it64) * 4.0) / 3.0 * nao[glb] ** 3))
     hbm_tbs = alg_bytes / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
     traffic = measured_traffic(best_mode) if args.count == 4096 else None
-    kernels = {k: round(v[0] / args.steps, 3) for k, v in best["stages"].items()}
+    stages = {m: stage_rooflines(lg, nao, nocc, nints, info["lds_tridiag_max_n"]) for m, lg in legs.items()}
 
     served = {}
     if not args.no_served:
@@ -340,6 +461,7 @@ def main():
                 leg = served_leg(engine, rate, count, prec)
                 served[prec].append(leg)
                 log(f"served {prec} {rate:g}/s: {leg}")
+        engine.trim()
     large = None
     if not args.no_large:
         large = large_leg(engine)
@@ -370,11 +492,17 @@ def main():
                                f"density RMS < 1e-8, one launch per SCF stage",
                    "molecules": args.count, "mode": best_mode},
         "fp64_solves_per_s": round(legs["fp64"]["solves_per_s"], 3) if "fp64" in legs else None,
-        "modes": {m: {"solves_per_s": round(lg["solves_per_s"], 3), "converged": int(lg["ok"].sum()),
+        "modes": {m: {"solves_per_s": round(lg["solves_per_s"], 3), "ms_per_step": round(lg["ms_per_step"], 3),
+                      "converged": int(lg["ok"].sum()),
                       "mean_iterations": round(float(lg["iters"].mean()), 2),
+                      "mean_fp64_iterations": round(float(lg["it64"].mean()), 2),
                       "max_abs_dE_vs_fp64_eV": max_de.get(m, 0.0),
-                      "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in lg["stages"].items()}}
+                      "timed_path": f"hipGraph replay, {lg['extra']['graph_replays']} replays in the timed passes",
+                      "profiled_pass": lg["extra"],
+                      "stage_ms_profiled_pass": {k: round(v[0], 3) for k, v in lg["stages"].items()},
+                      "stage_rooflines": stages[m]}
                   for m, lg in legs.items()},
+        "batch_workspace_GiB": round(mem["workspace"] / 2**30, 3),
         "latency_s": (headline_latency(served, args.latency_rate) if served else None),
         "served": served or None,
         "large_single": large,
@@ -389,16 +517,16 @@ def main():
                      "frac": round(hbm_tbs / HBM_PEAK_TBPS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": round(alg_bytes / max(launches_tri, 1)),
                      "avg_launch_ms": round(ms_tri / max(launches_tri, 1), 4), "launches": launches_tri,
+                     "timed_in": "the profiled pass after the timed passes (hipEvents on the kernel's stream)",
                      "fp64_view": {"achieved_TFLOPs": round(achieved, 5), "peak_TFLOPs": FP64_PEAK_TFLOPS,
                                    "frac": round(achieved / FP64_PEAK_TFLOPS, 6)},
-                     "lds_tridiag_max_n": info["lds_tridiag_max_n"],
-                     "kernel_ms_per_step": kernels},
+                     "lds_tridiag_max_n": info["lds_tridiag_max_n"]},
     }
     if plain_served and served:
         peak = max(leg["solves_per_s"] for leg in served["mixed"])
         out["speedup_vs_plain_fp64"]["served_mixed_peak_same_mix"] = round(peak / plain_served["solves_per_s"], 2)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_leg(mols, args.cpu_seconds)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     engine.close()
     print(json.dumps(out), flush=True)
 
