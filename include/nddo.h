@@ -207,6 +207,13 @@ int nddo_dbg_fock(nddo_engine* e, int theory, int natoms, const int* Z, const do
                   const double* P, int guess, double* F);
 int nddo_dbg_eigh(nddo_engine* e, int n, const double* F, double* eps, double* C, int* sweeps);
 int nddo_dbg_density(nddo_engine* e, int n, int nocc, const double* C, double* P);
+/* The engine's parameter table (host only, no device call): the values
+ * MolDS's Atom accessors return (Atom.cpp:263-1109; tables Hatom.cpp, Catom.cpp,
+ * Natom.cpp, Oatom.cpp, Fatom.cpp) for element Z and theory, in atomic units:
+ * core charge, U_ss, U_pp, zeta_s, zeta_p, beta_s, beta_p, alpha, Gss, Gpp, Gsp,
+ * Gpp2, Hsp, Hpp, D0..D2, rho0..rho2, K[4], L[4], M[4], and the MNDO atomic
+ * term dHf_atom - E_isol,atom (Mndo.cpp:395-404); 33 doubles. */
+int nddo_element_params(int Z, int theory, double* out);
 /* Serving scheduler's host wall time in seconds: admission + cohort setup,
  * launches, waiting for the device, retirement; and cohort iterations run. */
 int nddo_dbg_server_stats(nddo_engine* e, double* seconds4, long long* iterations, int reset);

@@ -1111,6 +1111,17 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
 
 size_t nddo_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
 
+int nddo_element_params(int Z, int theory, double* out) {
+    ElemDev e;
+    if (!out || elem_params(Z, theory, &e) != 0) { set_error("unsupported element or theory"); return NDDO_BAD_ARG; }
+    const double v[33] = {e.core, e.U[0], e.U[1], e.zeta[0], e.zeta[1], e.beta[0], e.beta[1], e.alpha,
+                          e.Gss, e.Gpp, e.Gsp, e.Gpp2, e.Hsp, e.Hpp, e.D[0], e.D[1], e.D[2], e.rho[0], e.rho[1],
+                          e.rho[2], e.K[0], e.K[1], e.K[2], e.K[3], e.L[0], e.L[1], e.L[2], e.L[3], e.M[0], e.M[1],
+                          e.M[2], e.M[3], mndo_atom_hof_offset(Z)};
+    std::memcpy(out, v, sizeof(v));
+    return NDDO_OK;
+}
+
 int nddo_engine_trim(nddo_engine* eh) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e) return NDDO_BAD_ARG;

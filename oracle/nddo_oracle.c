@@ -702,6 +702,21 @@ int orc_param(int Z, int theory, double* out /* D0 D1 D2 rho0 rho1 rho2 zs zp Gs
     return 0;
 }
 
+/* Every converted parameter of element Z (test hook): core, U[2], zeta[2],
+ * beta[2], alpha, Gss, Gpp, Gsp, Gpp2, Hsp, Hpp, D[3], rho[3], K[4], L[4],
+ * M[4], and the MNDO atomic term dHf - E_isol (Mndo.cpp:395-404): 33 doubles. */
+int orc_elem_dump(int Z, int theory, double* out)
+{
+    orc_elem e;
+    if (orc_elem_params(Z, theory, &e) != 0) return -1;
+    double v[33] = { e.core, e.U[0], e.U[1], e.zeta[0], e.zeta[1], e.beta[0], e.beta[1], e.alpha,
+                     e.Gss, e.Gpp, e.Gsp, e.Gpp2, e.Hsp, e.Hpp, e.D[0], e.D[1], e.D[2], e.rho[0], e.rho[1], e.rho[2],
+                     e.K[0], e.K[1], e.K[2], e.K[3], e.L[0], e.L[1], e.L[2], e.L[3], e.M[0], e.M[1], e.M[2], e.M[3],
+                     -e.eisol + e.dhf };
+    memcpy(out, v, sizeof(v));
+    return 0;
+}
+
 /* Multipole probe for tests (brute-force point-charge check). */
 double orc_multipole_probe(int Za, int Zb, int theory, int ma, int mb, double r)
 {

@@ -1,0 +1,88 @@
+"""The MNDO / AM1 / PM3 parameter tables against MolDS's own values.
+
+tests/golden/params_fixture.json holds the literals of MolDS's atom classes
+(src/base/atoms/{H,C,N,O,F}atom.cpp, read as text by make_param_fixture.py)
+with the unit each is converted from.  Both hand-typed tables -- the engine's
+(molds_amd/csrc/params.cpp, through nddo_element_params) and the oracle's
+(oracle/nddo_oracle.c, through orc_elem_dump) -- must reproduce every value
+after MolDS's conversions (Parameters.cpp:45-48), to the last bit or one
+rounding of the conversion.  Host-only: no GPU call.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+EV2AU = 0.03674903
+KCAL2AU = 0.00159360175
+ANG2AU = 1.0 / 0.5291772
+THEORIES = ("mndo", "am1", "pm3")
+LAYOUT = (["core", "CoreintegralS", "CoreintegralP", "OrbitalExponentS", "OrbitalExponentP", "BondingParameterS",
+           "BondingParameterP", "Alpha", "Gss", "Gpp", "Gsp", "Gpp2", "Hsp", "Hpp"]
+          + [f"DerivedParameterD[{k}]" for k in range(3)] + [f"DerivedParameterRho[{k}]" for k in range(3)]
+          + [f"Parameter{x}[{k}]" for x in "KLM" for k in range(4)] + ["hof_offset"])
+
+
+def _fixture():
+    with open(os.path.join(ROOT, "tests", "golden", "params_fixture.json")) as f:
+        return json.load(f)["elements"]
+
+
+def _convert(val, unit):
+    return {"eV": lambda v: v * EV2AU, "kcal": lambda v: v * KCAL2AU, "1/A": lambda v: v / ANG2AU,
+            "1/A2": lambda v: v / ANG2AU ** 2, "A": lambda v: v * ANG2AU,
+            "half/": lambda v: 0.5 / v if v != 0.0 else 0.0, "1": lambda v: v}[unit](val)
+
+
+def expected(el, theory):
+    t = el[theory]
+    out = {"core": el["core"]}
+    for name in LAYOUT[1:-1]:
+        if name == "Hpp":
+            continue
+        if name in t:
+            out[name] = _convert(*t[name])
+        else:   # MNDO has no Gaussian core terms
+            assert name.startswith("Parameter") and theory == "mndo", (theory, name)
+            out[name] = 0.0
+    out["Hpp"] = 0.5 * (out["Gpp"] - out["Gpp2"])   # Atom.cpp:1051-1070
+    m = el["mndo"]
+    out["hof_offset"] = -_convert(*m["ElecEnergyAtom"]) + _convert(*m["HeatsFormAtom"])
+    return np.array([out[k] for k in LAYOUT])
+
+
+def _tables():
+    from molds_amd.nddo import LIBPATH
+    from oracle import scf as O
+    eng = ctypes.CDLL(LIBPATH)
+    orc = O.lib()
+    return eng.nddo_element_params, orc.orc_elem_dump
+
+
+@pytest.mark.parametrize("Z", [1, 6, 7, 8, 9])
+@pytest.mark.parametrize("theory", THEORIES)
+def test_tables_match_molds_literals(Z, theory):
+    el = _fixture()[str(Z)]
+    want = expected(el, theory)
+    for fn in _tables():
+        got = np.zeros(33)
+        assert fn(Z, THEORIES.index(theory), got.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+        bad = [(LAYOUT[i], got[i], want[i]) for i in range(33)
+               if abs(got[i] - want[i]) > 2.3e-16 * abs(want[i])]
+        assert not bad, (fn.__name__, Z, theory, bad)
+
+
+def test_fixture_covers_the_scf_fields():
+    el = _fixture()
+    for Z in ("1", "6", "7", "8", "9"):
+        for th in THEORIES:
+            names = set(el[Z][th])
+            for f in ("CoreintegralS", "OrbitalExponentS", "BondingParameterS", "Alpha", "Gss",
+                      "DerivedParameterRho[0]"):
+                assert f in names, (Z, th, f)
+            if th != "mndo":
+                assert "ParameterK[0]" in names and "ParameterM[1]" in names
