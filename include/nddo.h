@@ -90,7 +90,10 @@ typedef struct {
     double energy_total;     /* elecSCFEnergy: electronic + core repulsion (Hartree) */
     double energy_elec;
     double energy_core;      /* coreRepulsionEnergy (Hartree) */
-    double heat_of_formation;/* MNDO only (Hartree); NaN for AM1 / PM3 (Am1.cpp:261-263) */
+    double heat_of_formation;/* Hartree: E_total - sum E_isol,atom + sum dHf_atom.  MNDO as
+                                Mndo::CalcHeatsFormation (Mndo.cpp:395-404); AM1 / PM3 (MolDS leaves it
+                                unset, Am1.cpp:261-263) with E_isol from the one-centre parameters
+                                (nddo_element_eisol) */
     double final_rms;
     double latency_s;        /* submit -> retire (served jobs), or wall time of the solve */
     double* eps;             /* [nao] MO energies (Hartree), ascending */
@@ -214,6 +217,10 @@ int nddo_dbg_density(nddo_engine* e, int n, int nocc, const double* C, double* P
  * Gpp2, Hsp, Hpp, D0..D2, rho0..rho2, K[4], L[4], M[4], and the MNDO atomic
  * term dHf_atom - E_isol,atom (Mndo.cpp:395-404); 33 doubles. */
 int nddo_element_params(int Z, int theory, double* out);
+/* Isolated-atom electronic energy (eV) of element Z in a theory, from its
+ * one-centre parameters (Dewar & Thiel 1977; MOPAC's EISOL): the term the
+ * AM1 / PM3 heat of formation subtracts.  No counterpart in MolDS. */
+int nddo_element_eisol(int Z, int theory, double* eisol_ev);
 /* Serving scheduler's host wall time in seconds: admission + cohort setup,
  * launches, waiting for the device, retirement; and cohort iterations run. */
 int nddo_dbg_server_stats(nddo_engine* e, double* seconds4, long long* iterations, int reset);

@@ -175,7 +175,8 @@ public:
         return energy_;
     }
     double GetCoreRepulsionEnergy() const { return e_core_; }
-    double GetHeatsFormation() const { return hof_; }   // MNDO only (Mndo.cpp:395-404); NaN otherwise
+    // Mndo.cpp:395-404 for MNDO; AM1 / PM3 with E_isol from the one-centre parameters
+    double GetHeatsFormation() const { return hof_; }
 
     // GetForce(elecState) (ElectronicStructure.h:36; Mndo::CalcForce, Mndo.cpp:2711-2986):
     // force[a][c] in Hartree/bohr of the ground state at the last DoSCF's

@@ -719,7 +719,8 @@ void fill_scalars(const JobDev& jb, const JobState& s, int theory, double hof_of
     res->energy_core = s.e_core;
     res->energy_elec = s.e_el;
     res->energy_total = s.e_el + s.e_core;
-    res->heat_of_formation = theory == NDDO_MNDO ? res->energy_total + hof_offset : NAN;
+    (void)theory;
+    res->heat_of_formation = res->energy_total + hof_offset;
     res->final_rms = s.rms;
     if (res->status != NDDO_OK) {
         res->energy_elec = NAN;
@@ -1028,7 +1029,7 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     std::vector<double> hof(which.size(), 0.0);
     for (size_t jj = 0; jj < which.size(); ++jj) {
         const nddo_job_desc& d = descs[which[jj]];
-        for (int a = 0; a < d.natoms; ++a) hof[jj] += mndo_atom_hof_offset(d.Z[a]);
+        for (int a = 0; a < d.natoms; ++a) hof[jj] += atom_hof_offset(d.Z[a], d.theory);
         rp[jj] = &res[which[jj]];
     }
     rc = b.fetch_all(e->stream, descs, rp.data(), hof.data());
@@ -1098,7 +1099,7 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
     rc = b.run_scf(e, e->stream);
     if (rc) return rc;
     double hof = 0.0;
-    for (int a = 0; a < job->natoms; ++a) hof += mndo_atom_hof_offset(job->Z[a]);
+    for (int a = 0; a < job->natoms; ++a) hof += atom_hof_offset(job->Z[a], job->theory);
     nddo_result* rp = res;
     rc = b.fetch_all(e->stream, job, &rp, &hof);
     if (rc) return rc;
@@ -1110,6 +1111,15 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
 }
 
 size_t nddo_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+int nddo_element_eisol(int Z, int theory, double* eisol_ev_out) {
+    if (!eisol_ev_out || !element_supported(Z) || theory < 0 || theory > 2) {
+        set_error("unsupported element or theory");
+        return NDDO_BAD_ARG;
+    }
+    *eisol_ev_out = eisol_ev(Z, theory);
+    return NDDO_OK;
+}
 
 int nddo_element_params(int Z, int theory, double* out) {
     ElemDev e;

@@ -137,6 +137,33 @@ double mndo_atom_hof_offset(int Z) {
     return -e->mndo_eisol_ev * kEV2AU + e->mndo_dhf_kcal * kKcal2AU;
 }
 
+// Electronic energy of the isolated atom in its ground configuration s^ns p^np
+// (Hund's rule), from the one-centre parameters (Dewar & Thiel, J. Am. Chem.
+// Soc. 99, 4899 (1977), the EISOL of MOPAC):
+//   E = ns U_ss + np U_pp + [ns = 2] G_ss + ns np G_sp - (ns np / 2) H_sp
+//       + a G_pp + b G_p2,
+// with (a, b) = (0, 0), (0, 0), (-1/2, 3/2), (-3/2, 9/2), (-1/2, 13/2), (0, 10)
+// for np = 0..5 (the p-shell pairs' Coulomb G_p2 and exchange
+// H_pp = (G_pp - G_p2) / 2 counted for the high-spin configuration).  It gives
+// the published AM1 and PM3 atomic energies (e.g. AM1 C -120.815794 eV,
+// PM3 O -289.342 eV) and MNDO's for H, C and F.
+double eisol_ev(int Z, int theory) {
+    const Element* e = find(Z);
+    if (!e || theory < 0 || theory > 2) return 0.0;
+    const Published& p = theory == 0 ? e->mndo : (theory == 1 ? e->am1 : e->pm3);
+    const int ns = Z == 1 ? 1 : 2, np = Z == 1 ? 0 : (int)e->core - 2;
+    static const double a[6] = {0.0, 0.0, -0.5, -1.5, -0.5, 0.0}, b[6] = {0.0, 0.0, 1.5, 4.5, 6.5, 10.0};
+    return ns * p.uss + np * p.upp + (ns == 2 ? p.gss : 0.0) + ns * np * p.gsp - 0.5 * ns * np * p.hsp +
+           a[np] * p.gpp + b[np] * p.gpp2;
+}
+
+double atom_hof_offset(int Z, int theory) {
+    const Element* e = find(Z);
+    if (!e) return 0.0;
+    if (theory == 0) return mndo_atom_hof_offset(Z);   // MolDS's MNDO tables (Mndo.cpp:395-404)
+    return -eisol_ev(Z, theory) * kEV2AU + e->mndo_dhf_kcal * kKcal2AU;
+}
+
 bool element_supported(int Z) { return find(Z) != nullptr; }
 
 }  // namespace nddo

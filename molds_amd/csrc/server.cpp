@@ -556,7 +556,7 @@ int nddo_submit(nddo_engine* eh, const nddo_job_desc* d, nddo_job_id* id) {
     if (s == NDDO_OK) {
         j->footprint = nddo::job_footprint_bytes(d->natoms, n, d->opts.diis_num_error_vect, d->opts.max_iter) +
                        nddo::result_bytes(n, d->opts.max_iter);
-        for (int a = 0; a < d->natoms; ++a) j->hof_offset += nddo::mndo_atom_hof_offset(d->Z[a]);
+        for (int a = 0; a < d->natoms; ++a) j->hof_offset += nddo::atom_hof_offset(d->Z[a], d->theory);
         j->res.status = NDDO_PENDING;
     }
     std::lock_guard<std::mutex> lk(e->smu);

@@ -682,12 +682,34 @@ double orc_elec_energy(void* h, const double* P)
     return 0.5 * (e + ec);
 }
 
-/* MNDO heat of formation minus E_total: sum_A (dHf_A - E_isol_A) (Mndo.cpp:395-404). */
+/* Isolated-atom electronic energy (eV) of the ground configuration s^ns p^np
+ * from the one-centre parameters (Dewar & Thiel 1977; MOPAC's EISOL):
+ * ns Uss + np Upp + [ns = 2] Gss + ns np Gsp - ns np Hsp / 2 + a Gpp + b Gp2,
+ * (a, b) per np = 0..5: (0,0) (0,0) (-1/2,3/2) (-3/2,9/2) (-1/2,13/2) (0,10). */
+double orc_eisol_ev(int Z, int theory)
+{
+    const orc_raw_elem* r = NULL;
+    for (int i = 0; i < ORC_NRAW; i++)
+        if (RAW[i].Z == Z) r = &RAW[i];
+    if (!r || theory < 0 || theory > 2) return NAN;
+    const orc_raw* p = &r->th[theory];
+    const int ns = (Z == 1) ? 1 : 2, np = (Z == 1) ? 0 : (int)r->core - 2;
+    static const double a[6] = {0.0, 0.0, -0.5, -1.5, -0.5, 0.0}, b[6] = {0.0, 0.0, 1.5, 4.5, 6.5, 10.0};
+    double e = ns * p->Uss + np * p->Upp + ns * np * p->Gsp - 0.5 * ns * np * p->Hsp + a[np] * p->Gpp + b[np] * p->Gpp2;
+    if (ns == 2) e += p->Gss;
+    return e;
+}
+
+/* Heat of formation minus E_total: sum_A (dHf_A - E_isol_A); MNDO with the
+ * tabulated E_isol (Mndo.cpp:395-404), AM1 / PM3 with orc_eisol_ev. */
 double orc_hof_offset(void* h)
 {
     orc_mol* m = (orc_mol*)h;
     double s = 0.0;
-    for (int a = 0; a < m->natoms; a++) s += -m->el[a].eisol + m->el[a].dhf;
+    for (int a = 0; a < m->natoms; a++) {
+        const double eis = m->theory == 0 ? m->el[a].eisol : orc_eisol_ev(m->el[a].Z, m->theory) * ORC_EV2AU;
+        s += -eis + m->el[a].dhf;
+    }
     return s;
 }
 

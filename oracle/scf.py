@@ -62,6 +62,8 @@ def lib():
         for f in ("orc_ecore", "orc_hof_offset"):
             getattr(L, f).restype = ctypes.c_double
             getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.orc_eisol_ev.restype = ctypes.c_double
+        L.orc_eisol_ev.argtypes = [ctypes.c_int, ctypes.c_int]
         L.orc_get_overlap.argtypes = [ctypes.c_void_p, dp]
         L.orc_get_integrals.argtypes = [ctypes.c_void_p, dp]
         L.orc_fock.argtypes = [ctypes.c_void_p, dp, ctypes.c_int, dp]
@@ -208,8 +210,7 @@ def scf(theory, Z, xyz_bohr, requires_guess=True, P0=None, log=None, **opts):
             res = dict(eps=eps, C=C, P=P, F_last=F, e_el=e_el, e_core=mol.ecore,
                        energy=e_el + mol.ecore, iterations=it + 1, rms=rms, traj=traj,
                        nao=n, nocc=nocc)
-            if theory == "mndo":
-                res["hof"] = res["energy"] + mol.hof_offset()
+            res["hof"] = res["energy"] + mol.hof_offset()
             break
         if not guess:
             if k > 0:

@@ -42,8 +42,9 @@ def test_mixed_matches_fp64_and_oracle(engine, theory, tag):
     assert abs(rmx["energy"] - o["energy"]) * EV <= TOL_MIXED_EV
     assert np.max(np.abs(rmx["eps"] - r64["eps"])) * EV <= TOL_MIXED_EV
     assert np.sqrt(np.mean((rmx["P"] - r64["P"]) ** 2)) <= 1e-7
-    if theory == "mndo":
-        assert abs(rmx["hof"] - r64["hof"]) * EV <= TOL_MIXED_EV
+    # heat of formation for every theory (AM1 / PM3 with E_isol from the one-centre parameters)
+    assert abs(rmx["hof"] - r64["hof"]) * EV <= TOL_MIXED_EV
+    assert abs(r64["hof"] - o["hof"]) * EV <= 1e-6
     # some iterations ran in FP32, and the FP64 polish finished the solve
     assert 1 <= rmx["fp64_iterations"] < rmx["iterations"]
     assert r64["fp64_iterations"] == r64["iterations"]

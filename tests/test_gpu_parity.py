@@ -177,6 +177,7 @@ def test_scf_fp64_vs_oracle(engine, theory, tag):
     assert r["status"] == 0
     assert abs(r["energy"] - o["energy"]) * EV <= TOL_E_EV
     assert abs(r["e_core"] - o["e_core"]) * EV <= 1e-9
+    assert abs(r["hof"] - o["hof"]) * EV <= TOL_E_EV
     n = r["nao"]
     assert np.sqrt(np.mean((r["P"] - o["P"]) ** 2)) <= 1e-8
     assert np.max(np.abs(r["eps"] - o["eps"])) <= 1e-6

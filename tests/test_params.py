@@ -86,3 +86,42 @@ def test_fixture_covers_the_scf_fields():
                 assert f in names, (Z, th, f)
             if th != "mndo":
                 assert "ParameterK[0]" in names and "ParameterM[1]" in names
+
+
+# Published isolated-atom energies (eV): AM1 [DZHS_1985] and PM3 [S_1989]
+# (the EISOL values of those parameter sets).
+EISOL_PUBLISHED = {
+    "am1": {1: -11.396427, 6: -120.815794, 7: -202.407743, 8: -316.099520, 9: -482.290583},
+    "pm3": {1: -13.073321, 6: -111.229917, 7: -157.6137755, 9: -437.517169},
+}
+
+
+@pytest.mark.parametrize("theory", ["am1", "pm3"])
+def test_eisol_from_one_centre_parameters_matches_published(theory):
+    """The AM1 / PM3 heat of formation subtracts E_isol derived from the
+    one-centre parameters; the derivation reproduces the published atomic
+    energies of both parameter sets, in the engine and in the oracle."""
+    from molds_amd.nddo import LIBPATH
+    from oracle import scf as O
+    eng = ctypes.CDLL(LIBPATH)
+    th = THEORIES.index(theory)
+    for Z, ref in EISOL_PUBLISHED[theory].items():
+        v = ctypes.c_double()
+        assert eng.nddo_element_eisol(Z, th, ctypes.byref(v)) == 0
+        assert abs(v.value - ref) <= 2e-6, (theory, Z, v.value, ref)
+        assert abs(O.lib().orc_eisol_ev(Z, th) - v.value) <= 1e-12 * abs(ref)
+
+
+def test_eisol_derivation_against_molds_mndo_table():
+    """The same derivation against MolDS's tabulated MNDO atomic energies
+    (mndoElecEnergyAtom): exact for H, C and F; the tabulated N and O values
+    differ by 0.015 and 0.0024 eV (MolDS keeps MNDO's table, which the MNDO
+    heat of formation uses unchanged)."""
+    from molds_amd.nddo import LIBPATH
+    eng = ctypes.CDLL(LIBPATH)
+    el = _fixture()
+    for Z, tol in ((1, 1e-9), (6, 1e-9), (9, 1e-9), (7, 0.016), (8, 0.0025)):
+        v = ctypes.c_double()
+        assert eng.nddo_element_eisol(Z, 0, ctypes.byref(v)) == 0
+        tab = el[str(Z)]["mndo"]["ElecEnergyAtom"][0]
+        assert abs(v.value - tab) <= tol, (Z, v.value, tab)
