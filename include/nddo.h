@@ -127,6 +127,10 @@ typedef struct {
 /* Keep the default launch shapes instead of timing the candidates at engine
  * creation (see nddo_engine_tuning). */
 #define NDDO_ENGINE_NO_AUTOTUNE 4
+/* Large molecules (the device-wide eigensolver class) use two-sided block
+ * Jacobi on FP64 MFMA instead of the default device-wide Householder
+ * tridiagonalization + divide and conquer + compact-WY back-transformation. */
+#define NDDO_ENGINE_BLOCK_JACOBI 8
 
 int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out);
 int nddo_engine_destroy(nddo_engine* e);
@@ -244,6 +248,9 @@ int nddo_dbg_set_tuning(nddo_engine* e, int fock_wpb, const int* bt_rows4);
  * returning the batch's device memory. */
 int nddo_dbg_host_times(nddo_engine* e, double* t5, int reset);
 int nddo_dbg_graph_stats(nddo_engine* e, long long* captured, long long* replays);
+/* Test hook: workgroups of the large-molecule tridiagonalization (0 = the
+ * engine's choice by size). */
+int nddo_dbg_set_xl_grid(nddo_engine* e, int tri_workgroups);
 
 #ifdef __cplusplus
 }

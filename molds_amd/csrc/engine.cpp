@@ -373,11 +373,12 @@ int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao) {
             list_xl.push_back(j);
             jobs[j].xl = (int)list_xl.size();
             xl_off.push_back(xlws_size);
-            xlws_size += xl_ws_doubles(n);
+            xlws_size += std::max(xl_ws_doubles(n), xlh_ws_doubles(n));
             maxn_xl = std::max(maxn_xl, n);
         }
     }
     xl_chunks = xl_nchunk(maxn_xl);
+    xl_householder = use_xl_householder && xlh_supported(maxn_xl);
     xl_grid = 1;
     for (size_t g = 0; g < list_xl.size(); g += xl_max_jobs()) {
         std::vector<int> ns;
@@ -422,7 +423,7 @@ void Batch::plan(bool want_sdbg, bool all, std::vector<std::pair<void**, size_t>
         add(&d_xl_off, std::max<size_t>(list_xl.size(), 1) * sizeof(long long));
         add(&d_xlws, std::max<long long>(xlws_size, 1) * sizeof(double));
         add(&d_xlpart, std::max<size_t>(list_xl.size(), 1) * (size_t)xl_chunks * XL_REC * sizeof(double));
-        add(&d_bar, 2 * sizeof(unsigned));
+        add(&d_bar, std::max<size_t>(2 * sizeof(unsigned), xlh_bar_bytes()));
     }
     if (want_sdbg || all) add(&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double));
 }
@@ -538,7 +539,7 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
     launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
     // the warm-start rotation is for the block-Jacobi (large) jobs only
-    if (nxl) launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
+    if (nxl && !xl_householder) launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
     const bool fork = fork_stream && !list_lds.empty() && !list_glb.empty();
     if (fork) {
         hipEventRecord(fork_ev, st);
@@ -606,8 +607,12 @@ void Batch::launch_xl_eigen_all(Engine* e, hipStream_t st) {
     if (e->xl_chain) hipStreamWaitEvent(st, e->xl_chain, 0);
     for (size_t g = 0; g < list_xl.size(); g += xl_max_jobs()) {
         const int cnt = (int)std::min<size_t>(xl_max_jobs(), list_xl.size() - g);
-        launch_xl_eigen(st, xl_grid, d_jobs, d_state, d_list_xl + g, d_xl_off + g, cnt, d_arena, d_eps, d_perm, d_xlws,
-                        d_bar);
+        if (xl_householder)
+            launch_xlh_eigen(st, xlh_grid(e->num_cus), d_jobs, d_state, d_list_xl + g, d_xl_off + g, cnt, maxn_xl,
+                             d_arena, d_eps, d_perm, d_tri, d_xlws, d_bar);
+        else
+            launch_xl_eigen(st, xl_grid, d_jobs, d_state, d_list_xl + g, d_xl_off + g, cnt, d_arena, d_eps, d_perm,
+                            d_xlws, d_bar);
     }
     if (e->xl_chain) hipEventRecord(e->xl_chain, st);
 }
@@ -640,7 +645,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             launch_xl_energy(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
             launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
             HIPCHK(hipEventRecord(e->ev[1], st));
-            if (!list_xl.empty()) launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
+            if (!list_xl.empty() && !xl_householder) launch_warm(st, d_tiles, (int)tiles.size(), d_jobs, d_state, d_arena);
             if (fork) {
                 // the LDS class on the second stream, timed as one stage (all three parts)
                 HIPCHK(hipEventRecord(fork_ev, st));
@@ -910,6 +915,7 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     e->num_cus = prop.multiProcessorCount;
     e->xl_min_nao = (e->opts.flags & NDDO_ENGINE_DEVICE_WIDE_EIGEN) ? 513 : 1201;
     e->use_graphs = !(e->opts.flags & NDDO_ENGINE_NO_GRAPHS);
+    e->xl_householder = !(e->opts.flags & NDDO_ENGINE_BLOCK_JACOBI);
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&e->api_stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&e->eig_stream, hipStreamNonBlocking));
@@ -992,6 +998,7 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     b.build(descs, njobs, which, nullptr);
     // jobs whose caller takes no C back-transform only the occupied eigenvectors
     for (size_t jj = 0; jj < which.size(); ++jj) b.jobs[jj].want_c = res[which[jj]].C != nullptr ? 1 : 0;
+    b.use_xl_householder = e->xl_householder;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     const size_t need = b.device_bytes(false);
     int rc = ensure_workspace(e, need);
@@ -1081,6 +1088,7 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
     const std::vector<int> which{0};
     b.build(job, 1, which, nullptr);
     b.jobs[0].want_c = res->C != nullptr ? 1 : 0;
+    b.use_xl_householder = e->xl_householder;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     const size_t need = b.device_bytes(false);
     if (need > kNoAllocScratchBytes || !b.list_xl.empty()) {
@@ -1219,6 +1227,7 @@ static int one_job(Engine* e, Batch& b, nddo_job_desc& d, int theory, int natoms
     if (s) return s;
     std::vector<int> which{0};
     b.build(&d, 1, which, nullptr);
+    b.use_xl_householder = e->xl_householder;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     int rc = b.allocate(sdbg);
     if (rc) return rc;
@@ -1333,6 +1342,7 @@ static int matrix_batch(Engine* e, Batch& b, int n, int nocc, int K) {
     b.arena_size = (long long)K * ARENA_FIXED * n * n;
     b.src.assign(K, 0);
     b.max_iter_all = 1;
+    b.use_xl_householder = e->xl_householder;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     int rc = b.allocate(false);
     if (rc) return rc;
@@ -1411,6 +1421,7 @@ static int autotune(Engine* e) {
         for (int k = 0; k < 48; ++k) which[k] = k;
         Batch b;
         int rc = b.build(d.data(), 48, which, nullptr);
+        b.use_xl_householder = e->xl_householder;
         if (rc == NDDO_OK) rc = b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
         if (rc == NDDO_OK) rc = b.allocate(false);
         if (rc == NDDO_OK) rc = b.upload(e->stream, d.data());
@@ -1515,6 +1526,17 @@ int nddo_dbg_set_tuning(nddo_engine* eh, int fock_wpb, const int* bt_rows4) {
     }
     std::lock_guard<std::mutex> lk(e->run_mu);
     HIPCHK(hipSetDevice(e->device));
+    set_tuning(t);
+    return NDDO_OK;
+}
+
+int nddo_dbg_set_xl_grid(nddo_engine* eh, int tri_workgroups) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || tri_workgroups < 0) return NDDO_BAD_ARG;
+    std::lock_guard<std::mutex> lk(e->run_mu);
+    HIPCHK(hipSetDevice(e->device));
+    Tuning t = get_tuning();
+    t.xl_tri_wg = tri_workgroups;
     set_tuning(t);
     return NDDO_OK;
 }

@@ -59,6 +59,9 @@ struct Batch {
     std::vector<long long> xl_off;   // per list_xl entry: offset in d_xlws
     int maxn_lds = 0, maxn_glb = 0, maxn_xl = 0, max_iter_all = 0;
     int xl_grid = 1, xl_chunks = 1;
+    // large molecules: device-wide Householder + divide and conquer (kernels_xlh.hip),
+    // or block Jacobi (kernels_xl.hip); classify() decides from use_xl_householder
+    bool use_xl_householder = true, xl_householder = true;
     long long xlws_size = 0;
     long long nslots = 0, nints = 0, arena_size = 0, nvec = 0, ntraj = 0;
 
@@ -155,6 +158,7 @@ struct Engine {
     int num_cus = 0;
     int xl_min_nao = 1201;   // 513 with NDDO_ENGINE_DEVICE_WIDE_EIGEN
     bool use_graphs = true;  // false with NDDO_ENGINE_NO_GRAPHS
+    bool xl_householder = true;   // false with NDDO_ENGINE_BLOCK_JACOBI
     std::atomic<long long> graphs_captured{0}, graph_replays{0};
     size_t hbm_budget = 0;
     int* h_remaining = nullptr;

@@ -1,0 +1,845 @@
+// Device-wide FP64 symmetric eigensolver for large molecules (replaces
+// LAPACKE_dsyevd, src/wrappers/Lapack.cpp:102-161, where one workgroup per
+// molecule would leave the chip idle and its tables no longer fit LDS):
+//
+//   k_xtri   Householder tridiagonalization T = Q^T F Q (Golub & Van Loan,
+//            Alg. 8.3.1) by the whole grid: rows of the trailing block are
+//            dealt to the wavefronts, one row per wavefront at a time; step k
+//            updates every row by the rank-2 correction of reflector k and, in
+//            the same pass over the row, forms its part of A v_{k+1} for the
+//            next reflector, so the trailing block is read and written once
+//            per step.  The pivot row of the next reflector is rebuilt by every
+//            workgroup from its stored copy and the previous step's v and w
+//            (so nobody writes it while others read it), with the same
+//            fixed-order reductions everywhere, hence bitwise the same v in
+//            every workgroup.  One grid barrier per step.
+//   k_xdc    Cuppen divide and conquer on T, bottom-up from 1 x 1 leaves, all
+//            merges of a level at once, as the one-workgroup k_dc
+//            (kernels_eig.hip) with its per-row tables in device memory and
+//            every phase spread over the grid: deflation (Dongarra-Sorensen),
+//            secular roots (rational model with bisection safeguard), Loewner
+//            eigenvector components (Gu-Eisenstat) and the merge products on
+//            FP64 MFMA.  One grid barrier per phase.
+//   k_xbt    eigenvector rows <- rows * H_{n-3} ... H_0 in compact WY blocks of
+//            16 reflectors (T from k_wy_t), one workgroup per 16 rows streaming
+//            them through FP64 MFMA; no synchronisation between workgroups.
+//
+// Output as the one-workgroup path: rows of AR_C are the eigenvectors in
+// ascending eigenvalue order, eps[] ascending, perm[] = identity.
+//
+// The persistent kernels use a counter/generation barrier in device memory
+// with agent-scope release/acquire; all their workgroups are resident (at most
+// one per CU, and the engine runs one such launch at a time); a wait that runs
+// out (20 s) marks the solve failed instead of hanging.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+
+#include "eig_device.h"
+#include "launch.h"
+
+namespace nddo {
+
+namespace {
+
+constexpr int XT = 256;             // threads per workgroup
+constexpr int XW = XT / 64;         // wavefronts per workgroup
+constexpr int XGRID_MAX = 1024;     // workgroups per persistent launch
+constexpr int XWAVE_MAX = XGRID_MAX * XW;
+constexpr int kXlhMaxN = 6400;      // k_xtri keeps v, w and the next v in LDS: 3 n doubles
+
+// Grid barrier in two levels, so that no counter sees more than 16 atomic
+// arrivals: the workgroups of each group of XB count on the group's counter,
+// the last of a group counts on the top counter, the last group releases the
+// generation word every workgroup waits on.  Words 128 bytes apart:
+// bar[0] generation, bar[32] top counter, bar[64 + 32 g] group g's counter
+// (XBAR_WORDS in all, cleared by k_xbar_reset before each launch).  Only
+// thread 0 makes agent-scope atomics: its release (L2 write-back) and acquire
+// (cache invalidation) act on the caches its workgroup shares, after the
+// workgroup barrier has made every wavefront's stores visible there.  Returns
+// false when the wait ran out (20 s).
+constexpr int XB = 16;
+constexpr int XBAR_WORDS = 64 + 32 * (XGRID_MAX / XB);
+__device__ bool xsync(unsigned* bar, unsigned nblk, unsigned* gen, int* sh_ok) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned target = *gen + 1;
+        const unsigned grp = blockIdx.x / XB, ngrp = (nblk + XB - 1) / XB;
+        const unsigned gsize = min((unsigned)XB, nblk - grp * XB);
+        unsigned* cnt = bar + 64 + 32 * grp;
+        bool release = false;
+        if (__hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
+            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(bar + 32, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == ngrp - 1) {
+                __hip_atomic_store(bar + 32, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(bar, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                release = true;
+            }
+        }
+        if (!release) {
+            const unsigned long long t0 = wall_clock64();
+            while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
+                __builtin_amdgcn_s_sleep(1);
+                if (wall_clock64() - t0 > 2000000000ull) { ok = 0; break; }   // 20 s of the 100 MHz clock
+            }
+        }
+        *gen = target;
+        *sh_ok = ok;
+    }
+    __syncthreads();
+    return *sh_ok != 0;
+}
+
+// Sum of one double per wavefront of the grid (q[0 .. nwaves)) in a fixed
+// order, the same in every workgroup; every thread gets it.
+__device__ inline double grid_partials_sum(const double* q, int nwaves, double* sh) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid < 64) {
+        double s = 0.0;
+        for (int g = lane; g < nwaves; g += 64) s += q[g];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) *sh = s;
+    }
+    __syncthreads();
+    return *sh;
+}
+
+// Workspace of one job in xlws (doubles): p[2][n], per-wave partials [2][XWAVE_MAX],
+// then the divide-and-conquer tables (12 n doubles, 9 n ints).
+__host__ __device__ inline long long xw_p(int) { return 0; }
+__host__ __device__ inline long long xw_q(int n) { return 2LL * n; }
+__host__ __device__ inline long long xw_dc(int n) { return 2LL * n + 2LL * XWAVE_MAX; }
+__host__ __device__ inline long long xw_size(int n) { return xw_dc(n) + 12LL * n + (9LL * n + 1) / 2 + 64; }
+
+// ------------------------------------------------------------------ tridiagonalization
+constexpr int TT = 1024;            // threads per workgroup of k_xtri (16 wavefronts: memory parallelism)
+constexpr int XU = 16;              // row elements loaded at once per lane in k_xtri's row pass
+constexpr int TW = TT / 64;
+
+__global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                             const int* __restrict__ joblist, const long long* __restrict__ xloff,
+                                             int nlist, double* __restrict__ arena, double* __restrict__ tri,
+                                             double* __restrict__ xlws, unsigned* __restrict__ bar) {
+    extern __shared__ double dyn[];
+    __shared__ double red[TW], red2[TW], bc[2];
+    __shared__ int sh_ok;
+    const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int gw = wg * TW + wave, GW = G * TW;
+    unsigned gen = 0;
+    bool alive = true;
+    for (int jl = 0; jl < nlist; ++jl) {
+        const int job = joblist[jl];
+        if (state[job].phase != PH_ACTIVE) continue;   // read-only here: the same in every workgroup
+        const unsigned long long t0 = wall_clock64();
+        const JobDev jb = jobs[job];
+        const int n = jb.nao;
+        const long long ln = n;
+        double* A = amat(arena, jb, AR_U);   // the trailing block, full and symmetric
+        double* Wm = amat(arena, jb, AR_W);
+        const double* F = amat(arena, jb, AR_F);
+        double* d = eig_ws(tri, jb, job);
+        double* e = d + n;
+        double* tau = e + n;
+        double* ws = xlws + xloff[jl];
+        double* pb[2] = {ws + xw_p(n), ws + xw_p(n) + n};
+        // one partial of p.v per workgroup (its wavefronts' partials summed in a fixed order)
+        double* qb[2] = {ws + xw_q(n), ws + xw_q(n) + XGRID_MAX};
+        double* vcur = dyn;
+        double* vnext = dyn + n;
+        double* wv = dyn + 2 * n;
+        if (!alive) {
+            if (wg == 0 && tid == 0) state[job].sweeps = -1;
+            continue;
+        }
+        if (wg == 0)
+            for (int i = tid; i < n; i += TT) { e[i] = 0.0; tau[i] = 0.0; }
+        if (n < 3) {
+            if (wg == 0 && tid == 0) {
+                d[0] = F[0];
+                if (n == 2) { e[0] = F[1]; d[1] = F[3]; }
+                state[job].prof[0] = (double)(wall_clock64() - t0);
+            }
+            continue;
+        }
+        // the workgroup's partial of p.v: its wavefronts' sums added in order
+        auto wg_partial = [&](double pacc, double* q) {
+            if (lane == 0) red2[wave] = pacc;
+            __syncthreads();
+            if (tid == 0) {
+                double s = 0.0;
+                for (int w = 0; w < TW; ++w) s += red2[w];
+                q[wg] = s;
+            }
+        };
+        // reflector of row 0 from F, then A = F (rows 1 .. n-1) and p = tau_0 A v_0
+        double tk;
+        {
+            double ss = 0.0;
+            for (int t = 2 + tid; t < n; t += TT) { const double x = F[t]; ss += x * x; }
+            ss = block_sum(ss, red, tid, TW);
+            const double alpha = F[1];
+            double beta = alpha, scale = 0.0;
+            tk = 0.0;
+            if (ss > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+                tk = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            for (int t = 1 + tid; t < n - 1; t += TT) vcur[t] = ss > 0.0 ? F[1 + t] * scale : 0.0;
+            if (tid == 0) vcur[0] = 1.0;
+            __syncthreads();
+            if (wg == 0) {
+                for (int t = tid; t < n - 1; t += TT) Wm[1 + t] = vcur[t];
+                if (tid == 0) { d[0] = F[0]; e[0] = beta; tau[0] = tk; }
+            }
+            double pv = 0.0;
+            for (int r = 1 + gw; r < n; r += GW) {
+                double s = 0.0;
+                int c = 1 + lane;
+                for (; c + 64 * 7 < n; c += 64 * 8) {
+                    double a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = F[(long long)r * ln + c + 64 * u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        A[(long long)r * ln + c + 64 * u] = a[u];
+                        s += a[u] * vcur[c + 64 * u - 1];
+                    }
+                }
+                for (; c < n; c += 64) {
+                    const double a = F[(long long)r * ln + c];
+                    A[(long long)r * ln + c] = a;
+                    s += a * vcur[c - 1];
+                }
+                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+                const double pn = tk * s;
+                if (lane == 0) pb[0][r - 1] = pn;
+                pv += pn * vcur[r - 1];
+            }
+            wg_partial(pv, qb[0]);
+        }
+        alive = xsync(bar, G, &gen, &sh_ok);
+        int cur = 0;
+        for (int k = 0; alive && k <= n - 3; ++k) {
+            const int m = n - 1 - k;   // length of v_k (indices t <-> row / column k + 1 + t)
+            const double pv = grid_partials_sum(qb[cur], G, &bc[0]);
+            const double K = -0.5 * tk * pv;   // w = p + K v
+            for (int t = tid; t < m; t += TT) wv[t] = pb[cur][t] + K * vcur[t];
+            __syncthreads();
+            const double w0 = wv[0];
+            if (k == n - 3) {   // last trailing 2 x 2 block (rows / columns n-2, n-1)
+                if (wg == 0 && tid == 0) {
+                    const double w1 = wv[1];
+                    d[n - 2] = A[(long long)(n - 2) * ln + n - 2] - 2.0 * vcur[0] * w0;
+                    e[n - 2] = A[(long long)(n - 2) * ln + n - 1] - (vcur[0] * w1 + w0 * vcur[1]);
+                    d[n - 1] = A[(long long)(n - 1) * ln + n - 1] - 2.0 * vcur[1] * w1;
+                }
+                break;
+            }
+            // row k+1 after the update of step k; its tail defines reflector k+1
+            const double* rowp = A + (long long)(k + 1) * ln + (k + 1);
+            double ss = 0.0;
+            for (int t = tid; t < m; t += TT) {
+                const double a = rowp[t] - (vcur[0] * wv[t] + w0 * vcur[t]);
+                if (t >= 2) ss += a * a;
+                else if (t == 1) bc[1] = a;
+                else if (wg == 0) d[k + 1] = a;
+                if (t >= 1) vnext[t - 1] = a;   // scaled below
+            }
+            ss = block_sum(ss, red, tid, TW);   // its barriers also publish bc[1] and vnext
+            const double alpha = bc[1];
+            double beta = alpha, tn = 0.0, scale = 0.0;
+            if (ss > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+                tn = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            const int m2 = m - 1;   // length of v_{k+1}
+            for (int t = 1 + tid; t < m2; t += TT) vnext[t] = ss > 0.0 ? vnext[t] * scale : 0.0;
+            if (tid == 0) vnext[0] = 1.0;
+            __syncthreads();
+            if (wg == 0) {
+                double* wr = Wm + (long long)(k + 1) * ln + (k + 2);
+                for (int t = tid; t < m2; t += TT) wr[t] = vnext[t];
+                if (tid == 0) { e[k + 1] = beta; tau[k + 1] = tn; }
+            }
+            // rows r = k+2 .. n-1: rank-2 update of step k and (A v_{k+1})_r in one
+            // pass, XU independent loads in flight per lane (the pass is bound by
+            // the bytes a CU keeps in flight)
+            const int nx = cur ^ 1;
+            double pacc = 0.0;
+            for (int r = k + 2 + gw; r < n; r += GW) {
+                const int tr = r - k - 1;
+                const double vr = vcur[tr], wr = wv[tr];
+                double* row = A + (long long)r * ln;
+                double s = 0.0;
+                int c = k + 2 + lane;
+                for (; c + 64 * (XU - 1) < n; c += 64 * XU) {
+                    double a[XU];
+#pragma unroll
+                    for (int u = 0; u < XU; ++u) a[u] = row[c + 64 * u];
+#pragma unroll
+                    for (int u = 0; u < XU; ++u) {
+                        const int tc = c + 64 * u - k - 1;
+                        a[u] -= vr * wv[tc] + wr * vcur[tc];
+                        row[c + 64 * u] = a[u];
+                        s += a[u] * vnext[tc - 1];
+                    }
+                }
+                for (; c < n; c += 64) {
+                    const int tc = c - k - 1;
+                    const double a = row[c] - (vr * wv[tc] + wr * vcur[tc]);
+                    row[c] = a;
+                    s += a * vnext[tc - 1];
+                }
+                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+                const double pn = tn * s;
+                if (lane == 0) pb[nx][tr - 1] = pn;
+                pacc += pn * vnext[tr - 1];
+            }
+            wg_partial(pacc, qb[nx]);
+            alive = xsync(bar, G, &gen, &sh_ok);
+            double* tmp = vcur; vcur = vnext; vnext = tmp;
+            cur = nx;
+            tk = tn;
+        }
+        if (!alive) {
+            if (wg == 0 && tid == 0) state[job].sweeps = -1;
+            continue;
+        }
+        // keep the grid together before the next job reuses the LDS vectors
+        alive = xsync(bar, G, &gen, &sh_ok);
+        if (wg == 0 && tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
+    }
+}
+
+// ------------------------------------------------------------------ divide and conquer
+// The one-workgroup k_dc (kernels_eig.hip) over the whole grid: gt / GT index
+// the grid's threads, gwv / GWV its wavefronts, and every __syncthreads of
+// k_dc between phases is a grid barrier.  Tables live in device memory.
+__global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                            const int* __restrict__ joblist, const long long* __restrict__ xloff,
+                                            int nlist, double* __restrict__ arena, const double* __restrict__ tri,
+                                            double* __restrict__ eps, int* __restrict__ perm,
+                                            double* __restrict__ xlws, unsigned* __restrict__ bar) {
+    __shared__ int sh_ok;
+    const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int gt = wg * XT + tid, GT = G * XT;
+    const int gwv = wg * XW + wave, GWV = G * XW;
+    unsigned gen = 0;
+    bool alive = true;
+    for (int jl = 0; jl < nlist; ++jl) {
+        const int job = joblist[jl];
+        if (state[job].phase != PH_ACTIVE) continue;
+        if (state[job].sweeps < 0) continue;   // the tridiagonalization gave up (set before this launch)
+        const unsigned long long t0 = wall_clock64();
+        const JobDev jb = jobs[job];
+        const int n = jb.nao;
+        const double ue = kEps;
+        double* tb = xlws + xloff[jl] + xw_dc(n);
+        double* Dc = tb;
+        double* ds = Dc + n;
+        double* zs = ds + n;
+        double* dl = zs + n;
+        double* zl = dl + n;
+        double* vals = zl + n;
+        double* rtau = vals + n;
+        double* zhat = rtau + n;
+        double* cnorm = zhat + n;
+        double* rc = cnorm + n;
+        double* rs = rc + n;
+        double* rhoq = rs + n;
+        int* col = (int*)(rhoq + n);
+        int* flag = col + n;
+        int* cl = flag + n;
+        int* rorig = cl + n;
+        int* rank = rorig + n;
+        int* r1 = rank + n;
+        int* r2 = r1 + n;
+        int* kkq = r2 + n;
+        int* nrot = kkq + n;
+        const double* d = eig_ws(const_cast<double*>(tri), jb, job);
+        const double* e = d + n;
+        double* QC = amat(arena, jb, AR_C);
+        double* QX = amat(arena, jb, AR_Q2);
+        double* U = amat(arena, jb, AR_U);
+        if (!alive) {
+            if (wg == 0 && tid == 0) state[job].sweeps = -1;
+            continue;
+        }
+        // a non-finite tridiagonal fails the solve (every workgroup checks all of it)
+        int bad = 0;
+        for (int i = tid; i < n; i += XT) bad |= !isfinite(d[i]) || (i < n - 1 && !isfinite(e[i]));
+        if (__syncthreads_or(bad)) {
+            for (int i = gt; i < n; i += GT) {
+                eps[jb.vec0 + i] = 0.0;
+                perm[jb.ivec0 + i] = i;
+            }
+            if (wg == 0 && tid == 0) state[job].sweeps = -1;
+            continue;
+        }
+        int L = 0;
+        while ((1 << L) < n) ++L;
+        {
+            double* Q0 = (L & 1) ? QX : QC;
+            for (int i = gt; i < n; i += GT) {
+                const double el = i > 0 ? fabs(e[i - 1]) : 0.0, er = i < n - 1 ? fabs(e[i]) : 0.0;
+                Dc[i] = d[i] - el - er;
+                Q0[(long long)i * n + i] = 1.0;
+            }
+        }
+        alive = xsync(bar, G, &gen, &sh_ok);
+        for (int l = 1; alive && l <= L; ++l) {
+            const int s = 1 << l, h = s >> 1, nm = (n + s - 1) >> l;
+            double* QO = ((L - l + 1) & 1) ? QX : QC;
+            double* QN = ((L - l) & 1) ? QX : QC;
+            // A: z and the stable merge order of the two sorted blocks
+            for (int i = gt; i < n; i += GT) {
+                const int a = (i >> l) << l, m = min(a + h, n), b = min(a + s, n);
+                const long long ri = (long long)i * n;
+                if (m == b) {
+                    for (int c = a; c < b; ++c) QN[ri + c] = QO[ri + c];
+                    continue;
+                }
+                double z = i < m ? QO[ri + m - 1] : QO[ri + m];
+                if (i >= m && e[m - 1] < 0.0) z = -z;
+                z *= 0.70710678118654752440;
+                const double di = Dc[i];
+                int pos = 0;
+                for (int t = a; t < b; ++t) pos += dc_before(Dc[t], t, di, i);
+                ds[a + pos] = di;
+                zs[a + pos] = z;
+                col[a + pos] = i;
+            }
+            // z comes from the diagonal blocks of QO; only its off-diagonal blocks are cleared here
+            for (long long x = gt; x < ((long long)n << (l - 1)); x += GT) {
+                const int i = (int)(x >> (l - 1)), c0 = (int)(x & (h - 1));
+                const int a = (i >> l) << l, m = min(a + h, n), b = min(a + s, n);
+                if (m == b) continue;
+                const int c = i < m ? m + c0 : a + c0;
+                if (c < (i < m ? b : m)) QO[(long long)i * n + c] = 0.0;
+            }
+            alive = xsync(bar, G, &gen, &sh_ok);
+            if (!alive) break;
+            // B: deflation, one wavefront per merge (the scan on lane 0)
+            for (int q = gwv; q < nm; q += GWV) {
+                const int a = q << l, m = min(a + h, n), b = min(a + s, n);
+                if (m == b) {
+                    if (lane == 0) kkq[q] = -1;
+                    continue;
+                }
+                const double rho = 2.0 * fabs(e[m - 1]);
+                double dmax = 0.0, zmax = 0.0;
+                for (int jj = a + lane; jj < b; jj += 64) { dmax = fmax(dmax, fabs(ds[jj])); zmax = fmax(zmax, fabs(zs[jj])); }
+                for (int off = 32; off > 0; off >>= 1) {
+                    dmax = fmax(dmax, __shfl_xor(dmax, off));
+                    zmax = fmax(zmax, __shfl_xor(zmax, off));
+                }
+                const double tol = 8.0 * ue * fmax(dmax, zmax);
+                if (rho * zmax <= tol) {
+                    for (int jj = a + lane; jj < b; jj += 64) flag[jj] = 1;
+                    if (lane == 0) nrot[q] = 0;
+                } else if (lane == 0) {
+                    int pj = -1, nr = 0, ps = 0;
+                    double zp = 0.0, dp = 0.0;
+                    double zn = zs[a], dn = ds[a];
+                    int cn = col[a];
+                    for (int jj = a; jj < b; ++jj) {
+                        double zj = zn, dj = dn;
+                        int js = cn < m ? 1 : 2;
+                        if (jj + 1 < b) { zn = zs[jj + 1]; dn = ds[jj + 1]; cn = col[jj + 1]; }
+                        if (rho * fabs(zj) <= tol) { flag[jj] = 1; continue; }
+                        flag[jj] = 0;
+                        if (pj >= 0) {
+                            const double t = dj - dp, zz = zj * zj + zp * zp;
+                            if (fabs(t * zj * zp) <= tol * zz) {
+                                const double tz = sqrt(zz);
+                                const double C = zj / tz, S = -zp / tz;
+                                zs[jj] = tz;
+                                zs[pj] = 0.0;
+                                r1[a + nr] = col[pj];
+                                r2[a + nr] = col[jj];
+                                rc[a + nr] = C;
+                                rs[a + nr] = S;
+                                ++nr;
+                                ds[pj] = dp * C * C + dj * S * S;
+                                dj = dp * S * S + dj * C * C;
+                                ds[jj] = dj;
+                                zj = tz;
+                                flag[pj] = 1;
+                                js |= ps;
+                                if (js == 3) flag[jj] = 2;
+                            }
+                        }
+                        pj = jj;
+                        ps = js;
+                        zp = zj;
+                        dp = dj;
+                    }
+                    nrot[q] = nr;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+                __builtin_amdgcn_wave_barrier();
+                int k = 0;
+                for (int c0 = a; c0 < b; c0 += 64) {
+                    const int jj = c0 + lane;
+                    const int fj = jj < b ? flag[jj] : 1;
+                    const bool keep = fj != 1;
+                    const unsigned long long mk = __ballot(keep);
+                    const int pre = __popcll(mk & ((1ull << lane) - 1ull));
+                    if (keep) {
+                        dl[a + k + pre] = ds[jj];
+                        zl[a + k + pre] = zs[jj];
+                        cl[a + k + pre] = col[jj];
+                        rorig[a + k + pre] = fj;
+                    }
+                    k += __popcll(mk);
+                }
+                int t2 = k;
+                for (int c0 = a; c0 < b; c0 += 64) {
+                    const int jj = c0 + lane;
+                    const bool dfl = jj < b && flag[jj] == 1;
+                    const unsigned long long mk = __ballot(dfl);
+                    const int pre = __popcll(mk & ((1ull << lane) - 1ull));
+                    if (dfl) { dl[a + t2 + pre] = ds[jj]; zl[a + t2 + pre] = 0.0; cl[a + t2 + pre] = col[jj]; }
+                    t2 += __popcll(mk);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+                __builtin_amdgcn_wave_barrier();
+                int kl = -1, ke = 0;
+                if (l >= 5) {
+                    int nl = 0, nr = 0;
+                    for (int c0 = 0; c0 < k; c0 += 64) {
+                        const int t = c0 + lane;
+                        const int sd = t < k ? (rorig[a + t] == 2 ? 3 : (cl[a + t] < m ? 1 : 2)) : 0;
+                        nl += __popcll(__ballot(sd & 1));
+                        nr += __popcll(__ballot(sd & 2));
+                    }
+                    if (nl + nr <= b - a) {
+                        kl = 0;
+                        ke = nl;
+                        for (int c0 = 0; c0 < k; c0 += 64) {
+                            const int t = c0 + lane;
+                            const int sd = t < k ? (rorig[a + t] == 2 ? 3 : (cl[a + t] < m ? 1 : 2)) : 0;
+                            const unsigned long long ml = __ballot(sd & 1), mr = __ballot(sd & 2);
+                            const unsigned long long below = (1ull << lane) - 1ull;
+                            if (sd & 1) flag[a + kl + __popcll(ml & below)] = t;
+                            if (sd & 2) flag[a + ke + __popcll(mr & below)] = t;
+                            kl += __popcll(ml);
+                            ke += __popcll(mr);
+                        }
+                    }
+                }
+                if (lane == 0) {
+                    kkq[q] = k;
+                    rhoq[q] = rho;
+                    if (l >= 5) {
+                        kkq[(n >> 1) + q] = kl;
+                        kkq[(n >> 1) + (n >> 2) + q] = ke;
+                    }
+                }
+            }
+            alive = xsync(bar, G, &gen, &sh_ok);
+            if (!alive) break;
+            // C: deflation rotations on the rows of QO (thread = column); D: secular roots
+            for (int i = gt; i < n; i += GT) {
+                const int q = i >> l, a = q << l;
+                const int kk = kkq[q];
+                if (kk < 0) continue;
+                const int nr = nrot[q];
+                for (int t = 0; t < nr; ++t) {
+                    const long long p1 = (long long)r1[a + t] * n + i, p2 = (long long)r2[a + t] * n + i;
+                    const double x = QO[p1], y = QO[p2], C = rc[a + t], S = rs[a + t];
+                    QO[p1] = C * x + S * y;
+                    QO[p2] = C * y - S * x;
+                }
+                const int j = i - a;
+                if (j < kk) {
+                    int o;
+                    double tau;
+                    dc_secular(dl + a, zl + a, kk, rhoq[q], j, &o, &tau, ue);
+                    rorig[i] = a + o;
+                    rtau[i] = tau;
+                    vals[i] = dl[a + o] + tau;
+                } else {
+                    vals[i] = dl[i];
+                }
+            }
+            alive = xsync(bar, G, &gen, &sh_ok);
+            if (!alive) break;
+            // E: Loewner z-hat and the rank of every new eigenvalue in its block
+            for (int i = gt; i < n; i += GT) {
+                const int q = i >> l, a = q << l;
+                const int kk = kkq[q];
+                if (kk < 0) continue;
+                const int b = min(a + s, n), j = i - a;
+                if (j < kk) {
+                    const double di = dl[i];
+                    double wv = (di - dl[rorig[i]]) - rtau[i];
+                    for (int t = 0; t < kk; ++t) {
+                        if (t == j) continue;
+                        const int ct = a + t;
+                        wv *= ((di - dl[rorig[ct]]) - rtau[ct]) / (di - dl[ct]);
+                    }
+                    zhat[i] = copysign(sqrt(fmax(-wv, 0.0)), zl[i]);
+                }
+                const double v = vals[i];
+                int rk = 0;
+                for (int t = a; t < b; ++t) rk += dc_before(vals[t], t, v, i);
+                rank[i] = rk;
+            }
+            alive = xsync(bar, G, &gen, &sh_ok);
+            if (!alive) break;
+            // F: inverse norms of the columns of U
+            for (int i = gt; i < n; i += GT) {
+                const int q = i >> l, a = q << l;
+                const int kk = kkq[q];
+                if (i - a >= kk) continue;
+                const double dO = dl[rorig[i]], tj = rtau[i];
+                double s2 = 0.0;
+                for (int t = 0; t < kk; ++t) {
+                    const double u = zhat[a + t] / ((dl[a + t] - dO) - tj);
+                    s2 += u * u;
+                }
+                cnorm[i] = 1.0 / sqrt(s2);
+            }
+            alive = xsync(bar, G, &gen, &sh_ok);
+            if (!alive) break;
+            // G: U[t][j] = zhat_t / (dl_t - lambda_j), columns normalized
+            const long long tot = (long long)n << l;
+            for (long long x = gt; x < tot; x += GT) {
+                const int i = (int)(x >> l), j = (int)(x & (s - 1));
+                const int q = i >> l, a = q << l, kk = kkq[q];
+                if (i - a >= kk || j >= kk) continue;
+                const int cj = a + j;
+                U[(long long)i * n + cj] = zhat[i] / ((dl[i] - dl[rorig[cj]]) - rtau[cj]) * cnorm[cj];
+            }
+            alive = xsync(bar, G, &gen, &sh_ok);
+            if (!alive) break;
+            // H: new eigenvector rows QN[a + rank] = U^T QO[cl]; deflated rows copied
+            const bool small = l < 5;
+            for (long long x = gt; x < tot; x += GT) {
+                const int u = (int)(x >> l), rr = (int)(x & (s - 1));
+                const int q = u >> l, a = q << l, kk = kkq[q];
+                if (kk < 0) continue;
+                const int b = min(a + s, n), r = a + rr, j = u - a;
+                if (r >= b) continue;
+                double v;
+                if (j < kk) {
+                    if (!small) continue;
+                    v = 0.0;
+                    for (int t = 0; t < kk; ++t) v += U[(long long)(a + t) * n + u] * QO[(long long)cl[a + t] * n + r];
+                } else {
+                    v = QO[(long long)cl[u] * n + r];
+                }
+                QN[(long long)(a + rank[u]) * n + r] = v;
+            }
+            if (!small) {
+                const int tpm = s >> 5;
+                const long long ntile = (long long)nm * tpm * tpm;
+                const int lr = lane & 15, lk = lane >> 4;
+                for (long long ti = gwv; ti < ntile; ti += GWV) {
+                    const int q = (int)(ti / ((long long)tpm * tpm));
+                    const int rem = (int)(ti - (long long)q * tpm * tpm);
+                    const int jt = rem / tpm, rt = rem - jt * tpm;
+                    const int kk = kkq[q];
+                    const int a = q << l, b = min(a + s, n), m = min(a + (s >> 1), n);
+                    const int j0 = jt * 32, r0 = a + rt * 32;
+                    if (kk <= 0 || j0 >= kk || r0 >= b) continue;
+                    const int kl = kkq[(n >> 1) + q], ke = kkq[(n >> 1) + (n >> 2) + q];
+                    const bool lh = kl >= 0 && r0 + 32 <= m, rh = kl >= 0 && r0 >= m;
+                    const int i0 = rh ? kl : 0, i1 = lh ? kl : (rh ? ke : kk);
+                    dbl4 acc[2][2];
+#pragma unroll
+                    for (int x = 0; x < 2; ++x)
+#pragma unroll
+                        for (int y = 0; y < 2; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
+                    for (int k0 = i0; k0 < i1; k0 += 4) {
+                        const int ii = k0 + lk;
+                        const bool tv = ii < i1;
+                        const int t = (lh || rh) ? (tv ? flag[a + ii] : 0) : ii;
+                        const long long urow = (long long)(a + t) * n + a;
+                        const long long qrow = tv ? (long long)cl[a + t] * n : 0;
+                        double av[2], bv[2];
+#pragma unroll
+                        for (int x = 0; x < 2; ++x) {
+                            const int j = j0 + 16 * x + lr;
+                            av[x] = (tv && j < kk) ? U[urow + j] : 0.0;
+                            const int r = r0 + 16 * x + lr;
+                            bv[x] = (tv && r < b) ? QO[qrow + r] : 0.0;
+                        }
+#pragma unroll
+                        for (int x = 0; x < 2; ++x)
+#pragma unroll
+                            for (int y = 0; y < 2; ++y)
+                                acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int x = 0; x < 2; ++x)
+#pragma unroll
+                        for (int y = 0; y < 2; ++y)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int j = j0 + 16 * x + 4 * v + lk, r = r0 + 16 * y + lr;
+                                if (j < kk && r < b) QN[(long long)(a + rank[a + j]) * n + r] = acc[x][y][v];
+                            }
+                }
+            }
+            // the new eigenvalues in block order (vals / rank of this level are read only here)
+            for (int i = gt; i < n; i += GT) {
+                const int q = i >> l, a = q << l;
+                if (kkq[q] >= 0) Dc[a + rank[i]] = vals[i];
+            }
+            alive = xsync(bar, G, &gen, &sh_ok);
+        }
+        if (!alive) {
+            if (wg == 0 && tid == 0) state[job].sweeps = -1;
+            continue;
+        }
+        for (int i = gt; i < n; i += GT) {
+            eps[jb.vec0 + i] = Dc[i];
+            perm[jb.ivec0 + i] = i;
+        }
+        if (wg == 0 && tid == 0) {
+            state[job].sweeps = 1;
+            state[job].warm = 0;
+            state[job].prof[1] = (double)(wall_clock64() - t0);
+            state[job].prof[2] = 0.0;
+        }
+        // the next job reuses nothing of this one's tables, but its phase A
+        // writes QN rows of its own arena only: no barrier needed before it
+    }
+}
+
+// ------------------------------------------------------------------ back-transform
+// One workgroup (4 wavefronts) per (job, 16 eigenvector rows): Z <- Z H_{n-3}
+// ... H_0 applied as compact-WY blocks of 16 reflectors from the last block
+// down, Z <- Z - (Z V) T^T V^T.  Per block: Y = Z V (16 x 16, each wavefront a
+// quarter of the columns, partials summed in LDS), Y2 = -Y T^T, then the
+// column tiles Z[:, j] += Y2 V^T on MFMA, read and written in place.
+// V[j][c] = W[k0 + c][j] for j > k0 + c (v[0] = 1 is stored).
+constexpr int kXbt = 16;
+__global__ void __launch_bounds__(XT) k_xbt(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                            const int* __restrict__ joblist, double* __restrict__ arena,
+                                            const double* __restrict__ tri) {
+    __shared__ double Yp[XW][16 * 16];
+    __shared__ double Ys[16 * 16];
+    __shared__ double Ts[16 * 16];
+    const int job = joblist[blockIdx.y];
+    const JobDev jb = jobs[job];
+    const JobState& st = state[job];
+    if (st.phase != PH_ACTIVE || st.sweeps < 0) return;
+    const int n = jb.nao, nref = n - 2, r0 = blockIdx.x * 16;
+    const int nrow = jb.want_c ? n : jb.nocc;
+    if (nref <= 0 || r0 >= nrow) return;
+    const unsigned long long t0 = wall_clock64();
+    const long long ln = n;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, li = lane & 15, lk = lane >> 4;
+    double* Z = amat(arena, jb, AR_C);
+    const double* Wm = amat(arena, jb, AR_W);
+    const double* Tall = eig_ws(const_cast<double*>(tri), jb, job) + 3 * n;
+    const int nblk = (nref + kXbt - 1) / kXbt;
+    const bool rowok = r0 + li < nrow;
+    const long long zrow = (long long)(r0 + li) * ln;
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+        const int k0 = bi * kXbt, nb = min(kXbt, nref - k0);
+        const int jlo = k0 + 1;
+        if (tid < 256) Ts[tid] = Tall[(long long)bi * 256 + tid];
+        // Y[r][c] = sum_j Z[r][j] V[j][c]: A = Z (i = row li, k = column), B = V (k, j = reflector li)
+        dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int j0 = jlo + 4 * wave; j0 < n; j0 += 4 * XW) {
+            const int j = j0 + lk;
+            const double a = (rowok && j < n) ? Z[zrow + j] : 0.0;
+            const double b = (li < nb && j < n && j > k0 + li) ? Wm[(long long)(k0 + li) * ln + j] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Yp[wave][(4 * v + lk) * 16 + li] = acc[v];
+        __syncthreads();
+        {   // Y (row-major 16 x 16) and Y2[r][c] = -sum_{c2 >= c} T[c][c2] Y[r][c2]
+            const int r = tid >> 4, c = tid & 15;
+            double y = 0.0;
+#pragma unroll
+            for (int w = 0; w < XW; ++w) y += Yp[w][r * 16 + c];
+            Ys[r * 16 + c] = y;
+        }
+        __syncthreads();
+        double y2 = 0.0;
+        {
+            const int r = tid >> 4, c = tid & 15;
+            for (int c2 = c; c2 < kXbt; ++c2) y2 -= Ts[c * 16 + c2] * Ys[r * 16 + c2];
+        }
+        __syncthreads();
+        Ys[tid] = y2;
+        __syncthreads();
+        // Z[r][j] += sum_c Y2[r][c] V[j][c] over the column tiles j >= jlo
+        const int tlo = jlo >> 4, ntj = (n + 15) >> 4;
+        for (int tj = tlo + wave; tj < ntj; tj += XW) {
+            dbl4 u = dbl4{0.0, 0.0, 0.0, 0.0};
+            const int j = 16 * tj + li;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = 4 * q + lk;
+                const double a = Ys[li * 16 + c];   // A[i = row li][k = c]
+                const double b = (c < nb && j < n && j > k0 + c) ? Wm[(long long)(k0 + c) * ln + j] : 0.0;
+                u = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, u, 0, 0, 0);
+            }
+            // D[i][j] sits in lane j + 16 (i % 4), element i / 4
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = r0 + 4 * v + lk;
+                if (r < nrow && j < n && j >= jlo) {
+                    double* zp = Z + (long long)r * ln + j;
+                    *zp += u[v];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && blockIdx.x == 0) state[job].prof[3] = (double)(wall_clock64() - t0);
+}
+
+__global__ void k_xbar_reset(unsigned* bar) {
+    for (int i = threadIdx.x; i < XBAR_WORDS; i += blockDim.x) bar[i] = 0u;
+}
+
+}  // namespace
+
+bool xlh_supported(int n) { return n <= kXlhMaxN; }
+
+size_t xlh_bar_bytes() { return (size_t)XBAR_WORDS * sizeof(unsigned); }
+
+long long xlh_ws_doubles(int n) { return (xw_size(n) + 31) / 32 * 32; }
+
+int xlh_grid(int num_cus) { return std::max(1, std::min(num_cus, XGRID_MAX)); }
+
+// Measured on MI355X (256 CUs; ms per cold tridiagonalization at 16 / 32 / 64 /
+// 128 / 256 workgroups): n = 1201: 20.8 / 16.3 / 16.2 / 24.5 / 52.7;
+// n = 3000: 183 / 113 / 84 / 96 / 215; n = 6000: 1251 / 701 / 442 / 403 / 754.
+// A CU streams its rows at about 57 GB/s, while the per-step grid barrier
+// costs more the more workgroups release and invalidate their caches.
+int xlh_tri_grid(int num_cus, int max_n) {
+    const int forced = get_tuning().xl_tri_wg;
+    const int g = forced > 0 ? forced : (max_n > 4096 ? 128 : 64);
+    return std::max(1, std::min(g, std::min(num_cus, XGRID_MAX)));
+}
+
+void launch_xlh_eigen(hipStream_t st, int grid, const JobDev* jobs, JobState* state, const int* joblist,
+                      const long long* xloff, int nlist, int max_n, double* arena, double* eps, int* perm,
+                      double* tri, double* xlws, unsigned* bar) {
+    if (nlist <= 0) return;
+    const size_t lt = (size_t)3 * max_n * sizeof(double);
+    raise_dynamic_lds((const void*)k_xtri, lt);
+    hipLaunchKernelGGL(k_xbar_reset, dim3(1), dim3(256), 0, st, bar);
+    hipLaunchKernelGGL(k_xtri, dim3(xlh_tri_grid(grid, max_n)), dim3(TT), lt, st, jobs, state, joblist, xloff, nlist,
+                       arena, tri, xlws, bar);
+    hipLaunchKernelGGL(k_xbar_reset, dim3(1), dim3(256), 0, st, bar);
+    hipLaunchKernelGGL(k_xdc, dim3(grid), dim3(XT), 0, st, jobs, state, joblist, xloff, nlist, arena, tri, eps, perm,
+                       xlws, bar);
+    launch_wy_t(st, jobs, state, joblist, nlist, max_n, arena, tri);
+    hipLaunchKernelGGL(k_xbt, dim3((max_n + 15) / 16, nlist), dim3(XT), 0, st, jobs, state, joblist, arena, tri);
+}
+
+}  // namespace nddo

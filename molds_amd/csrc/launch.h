@@ -12,6 +12,7 @@ namespace nddo {
 struct Tuning {
     int fock_wpb = 4;                      // atom pairs (wavefronts) per workgroup of the pair Fock kernel
     int bt_rows[4] = {64, 64, 64, 64};     // back-transformation rows per workgroup, n <= 128 / 256 / 384 / 512
+    int xl_tri_wg = 0;                     // workgroups of the device-wide tridiagonalization (0: by size)
 };
 Tuning get_tuning();               // of the current device
 void set_tuning(const Tuning& t);  // for the current device
@@ -83,6 +84,26 @@ void launch_apply(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev
 
 void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
                    int* remaining, const double* xlpart, int nchunk);
+
+// Compact-WY T blocks of the Householder reflectors in W (kernels_eig.hip), any n.
+void launch_wy_t(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
+                 double* arena, double* tri);
+
+// Large molecules (kernels_xlh.hip): device-wide Householder tridiagonalization,
+// divide and conquer and back-transformation, persistent launches of `grid`
+// workgroups (at most one per CU) over the jobs of joblist one after another;
+// xloff[i] is job i's offset in xlws (xlh_ws_doubles(n) each); bar: two
+// unsigned words, reset by the launches themselves.  n <= 6400 (xlh_supported).
+bool xlh_supported(int n);
+size_t xlh_bar_bytes();   // the barrier words the device-wide launches need
+long long xlh_ws_doubles(int n);
+int xlh_grid(int num_cus);
+// Workgroups of k_xtri for the largest n of a launch (the grid barrier's cost
+// grows with the workgroup count, the row work shrinks with it).
+int xlh_tri_grid(int num_cus, int max_n);
+void launch_xlh_eigen(hipStream_t st, int grid, const JobDev* jobs, JobState* state, const int* joblist,
+                      const long long* xloff, int nlist, int max_n, double* arena, double* eps, int* perm,
+                      double* tri, double* xlws, unsigned* bar);
 
 // Large molecules (kernels_xl.hip): device-wide block-Jacobi eigensolver, one
 // persistent launch of `grid` workgroups (at most one per CU) over up to

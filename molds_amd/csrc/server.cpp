@@ -217,6 +217,7 @@ int start_cohort(Engine* e, Cohort* c, Staging& stage) {
     c->descs.resize(c->members.size());
     for (size_t i = 0; i < c->members.size(); ++i) c->descs[i] = c->members[i]->desc;
     c->b.build(c->descs.data(), (int)c->descs.size(), which, nullptr);
+    c->b.use_xl_householder = e->xl_householder;
     c->b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     int rc = stage.ensure(c->b.upload_bytes(), st);
     if (rc) return rc;

@@ -159,6 +159,7 @@ class Engine:
     DEVICE_WIDE_EIGEN = 1   # NDDO_ENGINE_DEVICE_WIDE_EIGEN
     NO_GRAPHS = 2           # NDDO_ENGINE_NO_GRAPHS
     NO_AUTOTUNE = 4         # NDDO_ENGINE_NO_AUTOTUNE
+    BLOCK_JACOBI = 8        # NDDO_ENGINE_BLOCK_JACOBI
 
     def __init__(self, device=0, hbm_budget_bytes=0, max_batch_jobs=0, flags=0):
         L = load_library()

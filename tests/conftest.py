@@ -56,3 +56,13 @@ def engine_dw():
     e = Engine(0, flags=Engine.DEVICE_WIDE_EIGEN)
     yield e
     e.close()
+
+
+@pytest.fixture(scope="session")
+def engine_bj():
+    """Large molecules (above 512 AOs) on the block-Jacobi eigensolver instead
+    of the default device-wide Householder + divide and conquer."""
+    from molds_amd.nddo import Engine
+    e = Engine(0, flags=Engine.DEVICE_WIDE_EIGEN | Engine.BLOCK_JACOBI | Engine.NO_AUTOTUNE)
+    yield e
+    e.close()

@@ -1,7 +1,9 @@
 """Molecules above 512 AOs: the one-workgroup Householder + divide-and-conquer
-solver (up to 1200 AOs), the device-wide block-Jacobi eigensolver (above 1200,
-or above 512 with NDDO_ENGINE_DEVICE_WIDE_EIGEN), and the chunked DIIS / energy
-reductions of the device-wide class (kernels_eig.hip, kernels_xl.hip).
+solver (up to 1200 AOs), the device-wide eigensolver (above 1200, or above 512
+with NDDO_ENGINE_DEVICE_WIDE_EIGEN): Householder + divide and conquer over the
+whole grid (kernels_xlh.hip), or block Jacobi with NDDO_ENGINE_BLOCK_JACOBI
+(kernels_xl.hip), and the chunked DIIS / energy reductions of the device-wide
+class.
 
 Tolerances are those of the smaller sizes: eigenvalues within 1e-13 ||A||_F
 of LAPACK, residual and orthonormality 1e-12; FP64 SCF within 1e-6 eV total
@@ -59,6 +61,32 @@ def test_eigh_large(engine, n):
 def test_eigh_device_wide(engine_dw, n):
     print(f"eigh device-wide n={n}", flush=True)
     _check_eigh(engine_dw, _random_sym(n), "device-wide")
+
+
+# the device-wide Householder + divide-and-conquer path well past the
+# one-workgroup limit: odd sizes, several merge levels with unpaired blocks
+@pytest.mark.parametrize("n", [2049, 3001, 4100])
+def test_eigh_device_wide_householder_sizes(engine, n):
+    print(f"eigh device-wide Householder n={n}", flush=True)
+    _check_eigh(engine, _random_sym(n), "householder")
+
+
+# the block-Jacobi alternative (NDDO_ENGINE_BLOCK_JACOBI) stays available
+@pytest.mark.parametrize("n", [600, 1536])
+def test_eigh_block_jacobi(engine_bj, n):
+    print(f"eigh block Jacobi n={n}", flush=True)
+    _check_eigh(engine_bj, _random_sym(n), "block-jacobi")
+
+
+def test_scf_block_jacobi_equals_householder(engine_dw, engine_bj):
+    """(H2O)_90, 540 AOs, on both device-wide eigensolvers: the same SCF (energy
+    within 1e-9 eV, same iteration count)."""
+    Z, xyz = _water(90, 11)
+    a = engine_dw.scf("pm3", Z, xyz, precision="fp64", **SCF)
+    b = engine_bj.scf("pm3", Z, xyz, precision="fp64", **SCF)
+    assert a["status"] == 0 and b["status"] == 0
+    assert abs(a["energy"] - b["energy"]) * EV <= 1e-9
+    assert a["iterations"] == b["iterations"]
 
 
 @pytest.mark.parametrize("which", ["default", "device-wide"])
