@@ -202,6 +202,45 @@ def large_leg(engine):
     return out
 
 
+def water1000_leg(engine, iterations=6):
+    """(H2O)_1000 PM3, 6000 AOs, one molecule (the largest single-molecule
+    workload): seconds per SCF iteration in FP64 and mixed precision, stage
+    times of the FP64 run, and the density-RMS trajectory of the first four
+    iterations against the oracle's (tests/golden/large_vectors.json; this
+    randomly packed cluster does not converge under MolDS's DIIS / damping, in
+    the oracle either, so energies at convergence do not exist)."""
+    from molds_amd import workloads as W
+    Z, x = W.water_cluster(1000, seed=4, jitter=0.0)
+    xyz = W.to_bohr(x)
+    opts = dict(rms_density=1e-8, diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0,
+                damping_weight=0.0, max_iter=iterations)
+    try:
+        with open(os.path.join(HERE, "tests", "golden", "large_vectors.json")) as f:
+            ref = np.array(json.load(f)["water1000_pm3_first4"]["traj"])
+    except (OSError, KeyError, ValueError):
+        ref = None
+    out = {"workload": f"(H2O)_1000 PM3, 6000 AOs, first {iterations} SCF iterations (stored integrals, device-wide "
+                       f"Householder + divide-and-conquer eigensolver)"}
+    for prec in ("fp64", "mixed"):
+        if prec == "fp64":
+            engine.profile(True)
+            engine.stage_times(reset=True)
+        t0 = time.perf_counter()
+        r = engine.scf("pm3", Z, xyz, precision=prec, want_C=False, want_P=False, want_traj=True, **opts)
+        dt = time.perf_counter() - t0
+        leg = dict(seconds=round(dt, 3), iterations=r["iterations"], seconds_per_iteration=round(dt / max(r["iterations"], 1), 4),
+                   fp64_iterations=r["fp64_iterations"])
+        if prec == "fp64":
+            st = engine.stage_times(reset=True)
+            engine.profile(False)
+            leg["stage_ms"] = {k: round(v[0], 2) for k, v in st.items() if v[0] > 0}
+        if ref is not None:
+            traj = np.array([[t[1], t[2]] for t in r["traj"][:len(ref)]])
+            leg["max_rel_dev_first4_vs_oracle"] = float(np.max(np.abs(traj[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])))
+        out[prec] = leg
+    return out
+
+
 def headline_latency(served, rate):
     """p50/p99 (submit -> retire) of the mixed-precision served leg at `rate`."""
     leg = min(served["mixed"], key=lambda g: abs(g["rate_per_s"] - rate))
@@ -462,10 +501,13 @@ def main():
                 served[prec].append(leg)
                 log(f"served {prec} {rate:g}/s: {leg}")
         engine.trim()
-    large = None
+    large = water1000 = None
     if not args.no_large:
         large = large_leg(engine)
         log(f"large: {large}")
+        water1000 = water1000_leg(engine)
+        log(f"(H2O)_1000: {water1000}")
+        engine.trim()
     plain = plain_served = None
     if not args.no_plain:
         plain = plain_leg(engine, make_jobs(mols, "fp64"), args.plain_seconds)
@@ -506,6 +548,7 @@ def main():
         "latency_s": (headline_latency(served, args.latency_rate) if served else None),
         "served": served or None,
         "large_single": large,
+        "water1000_pm3": water1000,
         "plain_fp64_gpu": plain,
         "plain_fp64_gpu_served_mix": plain_served,
         "speedup_vs_plain_fp64": ({"batched": round(best["solves_per_s"] / plain["solves_per_s"], 2)}
