@@ -54,19 +54,23 @@ constexpr int kXlhMaxN = 6400;      // k_xtri keeps v, w and the next v in LDS: 
 // the last of a group counts on the top counter, the last group releases the
 // generation word every workgroup waits on.  Words 128 bytes apart:
 // bar[0] generation, bar[32] top counter, bar[64 + 32 g] group g's counter
-// (XBAR_WORDS in all, cleared by k_xbar_reset before each launch).  Only
+// (XBAR_WORDS per barrier, XMAX_GROUPS barriers, cleared by k_xbar_reset
+// before each launch).  Only
 // thread 0 makes agent-scope atomics: its release (L2 write-back) and acquire
 // (cache invalidation) act on the caches its workgroup shares, after the
 // workgroup barrier has made every wavefront's stores visible there.  Returns
 // false when the wait ran out (20 s).
 constexpr int XB = 16;
-constexpr int XBAR_WORDS = 64 + 32 * (XGRID_MAX / XB);
-__device__ bool xsync(unsigned* bar, unsigned nblk, unsigned* gen, int* sh_ok) {
+constexpr int XBAR_WORDS = 64 + 32 * (XGRID_MAX / XB);   // one barrier
+constexpr int XMAX_GROUPS = 32;                           // molecules solved side by side per launch
+constexpr int XMIN_GROUP = 8;                             // fewest workgroups per molecule
+// blk: the workgroup's index among the nblk of its barrier
+__device__ bool xsync(unsigned* bar, unsigned nblk, unsigned blk, unsigned* gen, int* sh_ok) {
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
         const unsigned target = *gen + 1;
-        const unsigned grp = blockIdx.x / XB, ngrp = (nblk + XB - 1) / XB;
+        const unsigned grp = blk / XB, ngrp = (nblk + XB - 1) / XB;
         const unsigned gsize = min((unsigned)XB, nblk - grp * XB);
         unsigned* cnt = bar + 64 + 32 * grp;
         bool release = false;
@@ -90,6 +94,26 @@ __device__ bool xsync(unsigned* bar, unsigned nblk, unsigned* gen, int* sh_ok) {
     }
     __syncthreads();
     return *sh_ok != 0;
+}
+
+// The molecules of a launch are solved side by side: the grid is split into
+// ngroups = min(nlist, grid / XMIN_GROUP) groups of consecutive workgroups,
+// group g takes the jobs g, g + ngroups, ... one after another, with a
+// barrier of its own.
+struct XGroup {
+    int ngroups, grp, G, wg;
+    unsigned* bar;
+};
+__device__ inline XGroup x_group(int nlist, unsigned* bar) {
+    XGroup x;
+    const int grid = (int)gridDim.x;
+    x.ngroups = max(1, min(min(nlist, XMAX_GROUPS), grid / XMIN_GROUP));
+    const int gsz = grid / x.ngroups;
+    x.grp = min((int)blockIdx.x / gsz, x.ngroups - 1);
+    x.G = x.grp == x.ngroups - 1 ? grid - x.grp * gsz : gsz;
+    x.wg = (int)blockIdx.x - x.grp * gsz;
+    x.bar = bar + (long long)x.grp * XBAR_WORDS;
+    return x;
 }
 
 // Sum of one double per wavefront of the grid (q[0 .. nwaves)) in a fixed
@@ -125,11 +149,12 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
     extern __shared__ double dyn[];
     __shared__ double red[TW], red2[TW], bc[2];
     __shared__ int sh_ok;
-    const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const XGroup xg = x_group(nlist, bar);
+    const int G = xg.G, wg = xg.wg, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int gw = wg * TW + wave, GW = G * TW;
     unsigned gen = 0;
     bool alive = true;
-    for (int jl = 0; jl < nlist; ++jl) {
+    for (int jl = xg.grp; jl < nlist; jl += xg.ngroups) {
         const int job = joblist[jl];
         if (state[job].phase != PH_ACTIVE) continue;   // read-only here: the same in every workgroup
         const unsigned long long t0 = wall_clock64();
@@ -220,7 +245,7 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
             }
             wg_partial(pv, qb[0]);
         }
-        alive = xsync(bar, G, &gen, &sh_ok);
+        alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
         int cur = 0;
         for (int k = 0; alive && k <= n - 3; ++k) {
             const int m = n - 1 - k;   // length of v_k (indices t <-> row / column k + 1 + t)
@@ -300,7 +325,7 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
                 pacc += pn * vnext[tr - 1];
             }
             wg_partial(pacc, qb[nx]);
-            alive = xsync(bar, G, &gen, &sh_ok);
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             double* tmp = vcur; vcur = vnext; vnext = tmp;
             cur = nx;
             tk = tn;
@@ -310,7 +335,7 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
             continue;
         }
         // keep the grid together before the next job reuses the LDS vectors
-        alive = xsync(bar, G, &gen, &sh_ok);
+        alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
         if (wg == 0 && tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
     }
 }
@@ -325,12 +350,13 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                                             double* __restrict__ eps, int* __restrict__ perm,
                                             double* __restrict__ xlws, unsigned* __restrict__ bar) {
     __shared__ int sh_ok;
-    const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const XGroup xg = x_group(nlist, bar);
+    const int G = xg.G, wg = xg.wg, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int gt = wg * XT + tid, GT = G * XT;
     const int gwv = wg * XW + wave, GWV = G * XW;
     unsigned gen = 0;
     bool alive = true;
-    for (int jl = 0; jl < nlist; ++jl) {
+    for (int jl = xg.grp; jl < nlist; jl += xg.ngroups) {
         const int job = joblist[jl];
         if (state[job].phase != PH_ACTIVE) continue;
         if (state[job].sweeps < 0) continue;   // the tridiagonalization gave up (set before this launch)
@@ -390,7 +416,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                 Q0[(long long)i * n + i] = 1.0;
             }
         }
-        alive = xsync(bar, G, &gen, &sh_ok);
+        alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
         for (int l = 1; alive && l <= L; ++l) {
             const int s = 1 << l, h = s >> 1, nm = (n + s - 1) >> l;
             double* QO = ((L - l + 1) & 1) ? QX : QC;
@@ -421,7 +447,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                 const int c = i < m ? m + c0 : a + c0;
                 if (c < (i < m ? b : m)) QO[(long long)i * n + c] = 0.0;
             }
-            alive = xsync(bar, G, &gen, &sh_ok);
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
             // B: deflation, one wavefront per merge (the scan on lane 0)
             for (int q = gwv; q < nm; q += GWV) {
@@ -541,7 +567,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                     }
                 }
             }
-            alive = xsync(bar, G, &gen, &sh_ok);
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
             // C: deflation rotations on the rows of QO (thread = column); D: secular roots
             for (int i = gt; i < n; i += GT) {
@@ -567,7 +593,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                     vals[i] = dl[i];
                 }
             }
-            alive = xsync(bar, G, &gen, &sh_ok);
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
             // E: Loewner z-hat and the rank of every new eigenvalue in its block
             for (int i = gt; i < n; i += GT) {
@@ -590,7 +616,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                 for (int t = a; t < b; ++t) rk += dc_before(vals[t], t, v, i);
                 rank[i] = rk;
             }
-            alive = xsync(bar, G, &gen, &sh_ok);
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
             // F: inverse norms of the columns of U
             for (int i = gt; i < n; i += GT) {
@@ -605,7 +631,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                 }
                 cnorm[i] = 1.0 / sqrt(s2);
             }
-            alive = xsync(bar, G, &gen, &sh_ok);
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
             // G: U[t][j] = zhat_t / (dl_t - lambda_j), columns normalized
             const long long tot = (long long)n << l;
@@ -616,7 +642,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                 const int cj = a + j;
                 U[(long long)i * n + cj] = zhat[i] / ((dl[i] - dl[rorig[cj]]) - rtau[cj]) * cnorm[cj];
             }
-            alive = xsync(bar, G, &gen, &sh_ok);
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
             // H: new eigenvector rows QN[a + rank] = U^T QO[cl]; deflated rows copied
             const bool small = l < 5;
@@ -692,7 +718,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                 const int q = i >> l, a = q << l;
                 if (kkq[q] >= 0) Dc[a + rank[i]] = vals[i];
             }
-            alive = xsync(bar, G, &gen, &sh_ok);
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
         }
         if (!alive) {
             if (wg == 0 && tid == 0) state[job].sweeps = -1;
@@ -802,14 +828,14 @@ __global__ void __launch_bounds__(XT) k_xbt(const JobDev* __restrict__ jobs, Job
 }
 
 __global__ void k_xbar_reset(unsigned* bar) {
-    for (int i = threadIdx.x; i < XBAR_WORDS; i += blockDim.x) bar[i] = 0u;
+    for (int i = threadIdx.x; i < XBAR_WORDS * XMAX_GROUPS; i += blockDim.x) bar[i] = 0u;
 }
 
 }  // namespace
 
 bool xlh_supported(int n) { return n <= kXlhMaxN; }
 
-size_t xlh_bar_bytes() { return (size_t)XBAR_WORDS * sizeof(unsigned); }
+size_t xlh_bar_bytes() { return (size_t)XBAR_WORDS * XMAX_GROUPS * sizeof(unsigned); }
 
 long long xlh_ws_doubles(int n) { return (xw_size(n) + 31) / 32 * 32; }
 
@@ -820,9 +846,11 @@ int xlh_grid(int num_cus) { return std::max(1, std::min(num_cus, XGRID_MAX)); }
 // n = 3000: 183 / 113 / 84 / 96 / 215; n = 6000: 1251 / 701 / 442 / 403 / 754.
 // A CU streams its rows at about 57 GB/s, while the per-step grid barrier
 // costs more the more workgroups release and invalidate their caches.
-int xlh_tri_grid(int num_cus, int max_n) {
+// Several molecules of a launch share the grid, each with its own group.
+int xlh_tri_grid(int num_cus, int max_n, int nlist) {
     const int forced = get_tuning().xl_tri_wg;
-    const int g = forced > 0 ? forced : (max_n > 4096 ? 128 : 64);
+    const int per = max_n > 4096 ? 128 : (max_n > 2048 ? 64 : (nlist > 1 ? 32 : 64));
+    const int g = forced > 0 ? forced : per * std::max(1, nlist);
     return std::max(1, std::min(g, std::min(num_cus, XGRID_MAX)));
 }
 
@@ -833,7 +861,7 @@ void launch_xlh_eigen(hipStream_t st, int grid, const JobDev* jobs, JobState* st
     const size_t lt = (size_t)3 * max_n * sizeof(double);
     raise_dynamic_lds((const void*)k_xtri, lt);
     hipLaunchKernelGGL(k_xbar_reset, dim3(1), dim3(256), 0, st, bar);
-    hipLaunchKernelGGL(k_xtri, dim3(xlh_tri_grid(grid, max_n)), dim3(TT), lt, st, jobs, state, joblist, xloff, nlist,
+    hipLaunchKernelGGL(k_xtri, dim3(xlh_tri_grid(grid, max_n, nlist)), dim3(TT), lt, st, jobs, state, joblist, xloff, nlist,
                        arena, tri, xlws, bar);
     hipLaunchKernelGGL(k_xbar_reset, dim3(1), dim3(256), 0, st, bar);
     hipLaunchKernelGGL(k_xdc, dim3(grid), dim3(XT), 0, st, jobs, state, joblist, xloff, nlist, arena, tri, eps, perm,

@@ -100,7 +100,7 @@ long long xlh_ws_doubles(int n);
 int xlh_grid(int num_cus);
 // Workgroups of k_xtri for the largest n of a launch (the grid barrier's cost
 // grows with the workgroup count, the row work shrinks with it).
-int xlh_tri_grid(int num_cus, int max_n);
+int xlh_tri_grid(int num_cus, int max_n, int nlist);
 void launch_xlh_eigen(hipStream_t st, int grid, const JobDev* jobs, JobState* state, const int* joblist,
                       const long long* xloff, int nlist, int max_n, double* arena, double* eps, int* perm,
                       double* tri, double* xlws, unsigned* bar);

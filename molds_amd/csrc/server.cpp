@@ -99,10 +99,18 @@ double secs_since(Clock::time_point t) { return std::chrono::duration<double>(Cl
 
 constexpr int kMaxCohorts = 16;
 // AO count <= 64, <= 137 (eigensolver matrix in LDS), <= 256, <= 512, up to
-// the device-wide eigensolver's threshold (one workgroup per molecule), larger
-// (device-wide block Jacobi, whose launches of different cohorts run one after
-// another; at most xl_max_jobs() molecules per cohort)
+// the device-wide eigensolver's threshold (one workgroup per molecule; empty
+// in served mode, whose threshold is 513), larger (device-wide eigensolver,
+// whose launches of different cohorts run one after another; at most
+// xl_max_jobs() molecules per cohort, solved side by side)
 constexpr int kSizeClasses = 6;
+
+// Served requests above 512 AOs take the device-wide eigensolver: one
+// workgroup per molecule would hold such a request for 50-190 ms per SCF
+// iteration on one CU, the device-wide solver takes about 20-30 ms with the
+// molecules of a cohort (at most xl_max_jobs()) side by side.
+constexpr int kServedXlMin = 513;
+int served_xl_min(const Engine* e) { return std::min(e->xl_min_nao, kServedXlMin); }
 
 int size_class(int nao, int xl_min_nao) {
     if (nao >= xl_min_nao) return 5;
@@ -218,7 +226,7 @@ int start_cohort(Engine* e, Cohort* c, Staging& stage) {
     for (size_t i = 0; i < c->members.size(); ++i) c->descs[i] = c->members[i]->desc;
     c->b.build(c->descs.data(), (int)c->descs.size(), which, nullptr);
     c->b.use_xl_householder = e->xl_householder;
-    c->b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+    c->b.classify(e->lds_max, e->num_cus, served_xl_min(e));
     int rc = stage.ensure(c->b.upload_bytes(), st);
     if (rc) return rc;
     rc = c->b.allocate(false);
@@ -363,7 +371,7 @@ void Engine::start_server() {
                 const size_t room = std::min(room_dev, room_budget);
                 for (auto it = queue.begin(); it != queue.end();) {
                     ServedJob* j = *it;
-                    const int k = size_class(j->res.nao, xl_min_nao);
+                    const int k = size_class(j->res.nao, served_xl_min(this));
                     if (!bucket[k]) {
                         if (nb >= free_slots.size()) { ++it; continue; }
                         bucket[k].reset(new Cohort());
