@@ -102,6 +102,10 @@ typedef struct {
     double* trajectory;      /* [max_iter*4]: rms, diis_error, diis_on, damping_on per iteration */
     double* mulliken;        /* [natoms] Mulliken charges of the final density: core charge minus the
                                 atom's sum of P_mu,mu (Cndo2.cpp:733, 1187-1199, 1733-1747) */
+    double* dipole;          /* [9] ground-state dipole moments (a.u.): total x y z, electronic x y z, core
+                                x y z, about the centre of core masses, with the STO-6G position integrals
+                                MolDS prints (Cndo2.cpp:748-753, 1126-1184, 1871-2131); filled by
+                                nddo_scf_batch / nddo_scf_once for converged jobs, left alone by served jobs */
 } nddo_result;
 
 typedef struct nddo_engine nddo_engine;

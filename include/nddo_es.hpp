@@ -21,6 +21,7 @@
 #ifndef MOLDS_AMD_NDDO_ES_HPP
 #define MOLDS_AMD_NDDO_ES_HPP
 
+#include <array>
 #include <algorithm>
 #include <memory>
 #include <sstream>
@@ -150,6 +151,7 @@ public:
         r.eps = eps_.data();
         r.C = fock_.data();
         r.P = P_.data();
+        r.dipole = dipole_.data();
         const int st = nddo_scf_once(engine_->get(), &d, &r);
         if (st == NDDO_NOT_CONVERGED) {
             std::ostringstream ss;
@@ -177,6 +179,11 @@ public:
     double GetCoreRepulsionEnergy() const { return e_core_; }
     // Mndo.cpp:395-404 for MNDO; AM1 / PM3 with E_isol from the one-centre parameters
     double GetHeatsFormation() const { return hof_; }
+    // ground-state dipole moments (a.u.) as Cndo2::OutputSCFDipole prints them
+    // (Cndo2.cpp:748-753, 1126-1184): total, electronic, core
+    double const* GetTotalDipoleMoment() const { return dipole_.data(); }
+    double const* GetElectronicDipoleMoment() const { return dipole_.data() + 3; }
+    double const* GetCoreDipoleMoment() const { return dipole_.data() + 6; }
 
     // GetForce(elecState) (ElectronicStructure.h:36; Mndo::CalcForce, Mndo.cpp:2711-2986):
     // force[a][c] in Hartree/bohr of the ground state at the last DoSCF's
@@ -247,6 +254,7 @@ private:
     std::vector<double*> frow_;
     bool have_density_ = false;
     double energy_ = 0.0, e_core_ = 0.0, hof_ = 0.0;
+    std::array<double, 9> dipole_{};
     int iterations_ = 0;
 };
 

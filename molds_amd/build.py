@@ -17,7 +17,7 @@ OBJ = os.path.join(PKG, "lib", "obj")
 LIB = os.path.join(PKG, "lib", "libnddo.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["kernels_setup.hip", "kernels_scf.hip", "kernels_eig.hip", "kernels_xl.hip", "kernels_xlh.hip",
-           "engine.cpp", "server.cpp", "params.cpp"]
+           "kernels_props.hip", "engine.cpp", "server.cpp", "params.cpp"]
 # Packed FP32 instructions (v_pk_*_f32) are disabled for all device code: some of
 # their forms have given silently wrong results on this hardware generation.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",

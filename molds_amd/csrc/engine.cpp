@@ -426,6 +426,24 @@ void Batch::plan(bool want_sdbg, bool all, std::vector<std::pair<void**, size_t>
         add(&d_bar, std::max<size_t>(2 * sizeof(unsigned), xlh_bar_bytes()));
     }
     if (want_sdbg || all) add(&d_sdbg, std::max<long long>(arena_size, 1) * sizeof(double));
+    if (!list_dip.empty() || all) {
+        add(&d_list_dip, std::max<size_t>(list_dip.size(), 1) * sizeof(int));
+        add(&d_dip_off, std::max<size_t>(list_dip.size(), 1) * sizeof(long long));
+        add(&d_dippart, std::max<long long>(dip_size, 1) * sizeof(double));
+    }
+}
+
+// The jobs (batch positions) whose caller wants the dipole moment.
+void Batch::set_dipole_jobs(const std::vector<int>& jj) {
+    list_dip = jj;
+    dip_off.clear();
+    dip_size = 0;
+    maxn_dip = 0;
+    for (int j : jj) {
+        dip_off.push_back(dip_size);
+        dip_size += 4LL * jobs[j].nao;
+        maxn_dip = std::max(maxn_dip, jobs[j].nao);
+    }
 }
 
 size_t Batch::device_bytes(bool want_sdbg) {
@@ -499,6 +517,10 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs, char* stage) {
     if (!list_xl.empty()) {
         HIPCHK(h2d(d_list_xl, list_xl.data(), list_xl.size() * sizeof(int)));
         HIPCHK(h2d(d_xl_off, xl_off.data(), xl_off.size() * sizeof(long long)));
+    }
+    if (!list_dip.empty()) {
+        HIPCHK(h2d(d_list_dip, list_dip.data(), list_dip.size() * sizeof(int)));
+        HIPCHK(h2d(d_dip_off, dip_off.data(), dip_off.size() * sizeof(long long)));
     }
     HIPCHK(hipMemsetAsync(d_arena, 0, std::max<long long>(arena_size, 1) * sizeof(double), st));
     HIPCHK(hipMemsetAsync(d_traj, 0, std::max<long long>(ntraj, 1) * sizeof(double), st));
@@ -761,6 +783,37 @@ void mulliken_charges(int natoms, const int* Z, const double* pdiag, long long s
     }
 }
 
+// Dipole moments (a.u.) from the per-row partial sums of k_dipole, as MolDS
+// prints them (Cndo2.cpp:748-753, 1871-1934): about the centre of core masses
+// (Molecule::CalcXyzCOC: atomic mass in electron masses minus the valence
+// electrons), electronic = -sum P r + centre sum P S, core = sum_A Z_A (R_A -
+// centre); out = total, electronic, core (x, y, z each).
+void dipole_moments(const nddo_job_desc& d, const double* rows, int n, double* out) {
+    const double gmol2au = 1.0e5 / (6.0221415 * 9.1095);   // Parameters.cpp:52
+    double centre[3] = {0.0, 0.0, 0.0}, msum = 0.0;
+    for (int a = 0; a < d.natoms; ++a) {
+        const double core = d.Z[a] == 1 ? 1.0 : (double)(d.Z[a] - 2);
+        const double m = atomic_mass_gmol(d.Z[a]) * gmol2au - core;
+        msum += m;
+        for (int c = 0; c < 3; ++c) centre[c] += m * d.xyz_bohr[3 * a + c];
+    }
+    for (int c = 0; c < 3; ++c) centre[c] /= msum;
+    double pr[3] = {0.0, 0.0, 0.0}, ps = 0.0;
+    for (int mu = 0; mu < n; ++mu) {
+        for (int c = 0; c < 3; ++c) pr[c] += rows[4 * mu + c];
+        ps += rows[4 * mu + 3];
+    }
+    for (int c = 0; c < 3; ++c) {
+        double core = 0.0;
+        for (int a = 0; a < d.natoms; ++a)
+            core += (d.Z[a] == 1 ? 1.0 : (double)(d.Z[a] - 2)) * (d.xyz_bohr[3 * a + c] - centre[c]);
+        const double el = -pr[c] + centre[c] * ps;
+        out[c] = el + core;
+        out[3 + c] = el;
+        out[6 + c] = core;
+    }
+}
+
 int Batch::fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* const* res, const double* hof_offsets) {
     const size_t nj = jobs.size();
     std::vector<JobState> s(nj);
@@ -797,7 +850,16 @@ int Batch::fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* co
             HIPCHK(hipMemcpyAsync(r->trajectory, d_traj + jb.traj0, 4LL * jb.max_iter * sizeof(double),
                                   hipMemcpyDeviceToHost, st));
     }
+    std::vector<double> dip(std::max<long long>(dip_size, 1));
+    if (!list_dip.empty())
+        HIPCHK(hipMemcpyAsync(dip.data(), d_dippart, dip_size * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    for (size_t k = 0; k < list_dip.size(); ++k) {
+        const int jj = list_dip[k];
+        nddo_result* r = res[jj];
+        if (!r->dipole || r->status != NDDO_OK) continue;
+        dipole_moments(descs[src[jj]], dip.data() + dip_off[k], jobs[jj].nao, r->dipole);
+    }
     for (size_t jj = 0; jj < nj; ++jj) {
         nddo_result* r = res[jj];
         if (r->C) order_and_phase(jobs[jj].nao, VT[jj].data(), perm.data() + jobs[jj].ivec0, r->C);
@@ -1000,6 +1062,12 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     for (size_t jj = 0; jj < which.size(); ++jj) b.jobs[jj].want_c = res[which[jj]].C != nullptr ? 1 : 0;
     b.use_xl_householder = e->xl_householder;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+    {
+        std::vector<int> dj;
+        for (size_t jj = 0; jj < which.size(); ++jj)
+            if (res[which[jj]].dipole) dj.push_back((int)jj);
+        b.set_dipole_jobs(dj);
+    }
     const size_t need = b.device_bytes(false);
     int rc = ensure_workspace(e, need);
     if (rc == NDDO_ADMISSION_DENIED) {
@@ -1030,6 +1098,11 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     e->host_s[1] += lap(tp);
     rc = b.run_scf(e, e->stream);
     if (rc) return rc;
+    if (!b.list_dip.empty()) {
+        launch_dipole(e->stream, b.d_jobs, b.d_state, b.d_list_dip, b.d_dip_off, (int)b.list_dip.size(), b.maxn_dip,
+                      b.d_atoms, e->d_elems, b.d_arena, b.d_dippart);
+        HIPCHK(hipGetLastError());
+    }
     e->host_s[2] += lap(tp);
     const double wall = std::chrono::duration<double>(Clock::now() - t0).count();
     std::vector<nddo_result*> rp(which.size());
@@ -1090,6 +1163,7 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
     b.jobs[0].want_c = res->C != nullptr ? 1 : 0;
     b.use_xl_householder = e->xl_householder;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+    if (res->dipole) b.set_dipole_jobs(std::vector<int>{0});
     const size_t need = b.device_bytes(false);
     if (need > kNoAllocScratchBytes || !b.list_xl.empty()) {
         set_error("nddo_scf_once_noalloc: the job needs " + std::to_string(need) +
@@ -1106,6 +1180,11 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
     if (rc) return rc;
     rc = b.run_scf(e, e->stream);
     if (rc) return rc;
+    if (!b.list_dip.empty()) {
+        launch_dipole(e->stream, b.d_jobs, b.d_state, b.d_list_dip, b.d_dip_off, 1, b.maxn_dip, b.d_atoms, e->d_elems,
+                      b.d_arena, b.d_dippart);
+        HIPCHK(hipGetLastError());
+    }
     double hof = 0.0;
     for (int a = 0; a < job->natoms; ++a) hof += atom_hof_offset(job->Z[a], job->theory);
     nddo_result* rp = res;

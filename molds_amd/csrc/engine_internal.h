@@ -94,6 +94,14 @@ struct Batch {
     double* d_xlcoef = nullptr;  // per job: DIIS coefficients and what to apply (XL_COEF doubles)
     unsigned* d_bar = nullptr;
     double* d_sdbg = nullptr;
+    // jobs whose caller asked for the dipole moment (set before allocate)
+    std::vector<int> list_dip;
+    std::vector<long long> dip_off;   // per list_dip entry: offset of its 4 n row partials
+    long long dip_size = 0;
+    int maxn_dip = 0;
+    int* d_list_dip = nullptr;
+    long long* d_dip_off = nullptr;
+    double* d_dippart = nullptr;
     hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream
     // set: every device buffer is carved out of [region, region + region_cap)
     // (the engine's batch workspace, or the fixed scratch array of
@@ -133,7 +141,11 @@ struct Batch {
     bool eager_only = false;   // never capture (nddo_scf_once_noalloc: instantiation may allocate)
     int run_scf(Engine* e, hipStream_t st);
     int fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* const* res, const double* hof_offsets);
+    void set_dipole_jobs(const std::vector<int>& jj);
 };
+
+// Total, electronic and core dipole moments of a job from k_dipole's row partials.
+void dipole_moments(const nddo_job_desc& d, const double* rows, int n, double* out);
 
 struct ServedJob;
 

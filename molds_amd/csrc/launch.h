@@ -85,6 +85,13 @@ void launch_apply(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev
 void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
                    int* remaining, const double* xlpart, int nchunk);
 
+// Ground-state dipole partial sums of converged jobs (kernels_props.hip): per
+// AO row mu of job joblist[i], 4 doubles at part[doff[i] + 4 mu]: sum over the
+// row's upper part of P r_x, P r_y, P r_z and P S (off-diagonal terms doubled).
+void launch_dipole(hipStream_t st, const JobDev* jobs, const JobState* state, const int* joblist,
+                   const long long* doff, int nlist, int max_n, const AtomDev* atoms, const ElemDev* elems,
+                   const double* arena, double* part);
+
 // Compact-WY T blocks of the Householder reflectors in W (kernels_eig.hip), any n.
 void launch_wy_t(hipStream_t st, const JobDev* jobs, JobState* state, const int* joblist, int nlist, int max_n,
                  double* arena, double* tri);

@@ -166,4 +166,15 @@ double atom_hof_offset(int Z, int theory) {
 
 bool element_supported(int Z) { return find(Z) != nullptr; }
 
+double atomic_mass_gmol(int Z) {
+    switch (Z) {
+        case 1: return 1.00794;
+        case 6: return 12.0107;
+        case 7: return 14.00674;
+        case 8: return 15.9994;
+        case 9: return 18.9984032;
+        default: return 0.0;
+    }
+}
+
 }  // namespace nddo

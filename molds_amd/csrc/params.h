@@ -15,4 +15,6 @@ double eisol_ev(int Z, int theory);
 // AM1 / PM3 heat of formation, src/am1/Am1.cpp:261-263).
 double atom_hof_offset(int Z, int theory);
 bool element_supported(int Z);
+// Standard atomic weight (g/mol) as MolDS's atom classes set it (Hatom.cpp:52, Catom.cpp:49, ...).
+double atomic_mass_gmol(int Z);
 }  // namespace nddo

@@ -632,12 +632,14 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
     double* P = res->P;
     double* tr = res->trajectory;
     double* q = res->mulliken;
+    double* dip = res->dipole;
     *res = j->res;
     res->eps = eps;
     res->C = C;
     res->P = P;
     res->trajectory = tr;
     res->mulliken = q;
+    res->dipole = dip;
     const size_t n = (size_t)std::max(j->res.nao, 0);
     if (j->on_host) {
         if (eps) std::memcpy(eps, j->eps.data(), n * sizeof(double));

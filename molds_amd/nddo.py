@@ -46,7 +46,7 @@ class Result(ctypes.Structure):
                 ("energy_elec", ctypes.c_double), ("energy_core", ctypes.c_double),
                 ("heat_of_formation", ctypes.c_double), ("final_rms", ctypes.c_double),
                 ("latency_s", ctypes.c_double), ("eps", _dp), ("C", _dp), ("P", _dp),
-                ("trajectory", _dp), ("mulliken", _dp)]
+                ("trajectory", _dp), ("mulliken", _dp), ("dipole", _dp)]
 
 
 class EngineOpts(ctypes.Structure):
@@ -189,7 +189,7 @@ class Engine:
         self.close()
 
     # --------------------------------------------------------------- SCF
-    def _result_buffers(self, job, want_C, want_P, want_traj, want_mulliken=False, r=None):
+    def _result_buffers(self, job, want_C, want_P, want_traj, want_mulliken=False, r=None, want_dipole=False):
         """Caller buffers for one result; fills r (a Result, possibly an element
         of a results array) and leaves the pointers of unwanted arrays NULL."""
         n = job.nao
@@ -210,6 +210,9 @@ class Engine:
         if want_mulliken:
             bufs["mulliken"] = np.zeros(len(job.Z))
             r.mulliken = bufs["mulliken"].ctypes.data_as(_dp)
+        if want_dipole:
+            bufs["dipole"] = np.full(9, np.nan)
+            r.dipole = bufs["dipole"].ctypes.data_as(_dp)
         return r, bufs
 
     @staticmethod
@@ -222,6 +225,9 @@ class Engine:
                         iterations=r.iterations, fp64_iterations=r.fp64_iterations, energy=r.energy_total, e_el=r.energy_elec, e_core=r.energy_core,
                         hof=r.heat_of_formation, rms=r.final_rms, latency_s=r.latency_s, eps=bufs["eps"],
                         C=bufs.get("C"), P=bufs.get("P"), mulliken=bufs.get("mulliken"))
+        if "dipole" in bufs:
+            d = bufs["dipole"]
+            out["dipole_total"], out["dipole_electronic"], out["dipole_core"] = d[0:3], d[3:6], d[6:9]
         if "traj" in bufs:
             it = max(r.iterations, 0)
             t = bufs["traj"][:it]
@@ -229,16 +235,17 @@ class Engine:
         return out
 
     def scf(self, theory, Z, xyz_bohr, precision="fp64", requires_guess=True, P0=None, want_C=True, want_P=True,
-            want_traj=True, want_mulliken=True, **opts):
+            want_traj=True, want_mulliken=True, want_dipole=False, **opts):
         return self.scf_batch([Job(theory, Z, xyz_bohr, precision, requires_guess, P0, **opts)],
-                              want_C=want_C, want_P=want_P, want_traj=want_traj, want_mulliken=want_mulliken)[0]
+                              want_C=want_C, want_P=want_P, want_traj=want_traj, want_mulliken=want_mulliken,
+                              want_dipole=want_dipole)[0]
 
-    def scf_batch(self, jobs, want_C=False, want_P=False, want_traj=False, want_mulliken=False):
+    def scf_batch(self, jobs, want_C=False, want_P=False, want_traj=False, want_mulliken=False, want_dipole=False):
         """All jobs share one launch per SCF stage.  Returns per-job results."""
         descs = (JobDesc * len(jobs))(*[j.desc for j in jobs])
         results = (Result * len(jobs))()
-        if want_C or want_P or want_traj or want_mulliken:
-            keep = [self._result_buffers(j, want_C, want_P, want_traj, want_mulliken, results[i])[1]
+        if want_C or want_P or want_traj or want_mulliken or want_dipole:
+            keep = [self._result_buffers(j, want_C, want_P, want_traj, want_mulliken, results[i], want_dipole)[1]
                     for i, j in enumerate(jobs)]
         else:
             # only the MO energies: one array for all jobs, its addresses written

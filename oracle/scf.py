@@ -16,6 +16,7 @@ restates the SCF driver of MolDS around them:
 * ``DoDIIS`` / ``DoDamp``             src/cndo/Cndo2.cpp:920-1061
 * ``CalcSCFProperties``               src/cndo/Cndo2.cpp:732-754, 1324-1416
 * ``Mndo::CalcHeatsFormation``        src/mndo/Mndo.cpp:395-404
+* dipole moments (STO-6G)             src/cndo/Cndo2.cpp:748-753, 1871-2131
 """
 from __future__ import annotations
 
@@ -35,7 +36,7 @@ KCAL2AU = 0.00159360175
 
 # SCF defaults, src/base/Parameters.cpp:132-144
 DEFAULT_OPTS = dict(max_iter=100, rms_density=1.0e-8, damping_thresh=1.0, damping_weight=0.8,
-                    diis_k=5, diis_start=1.0e-2, diis_end=1.0e-8)
+                    diis_k=5, diis_start=1.0e-2, diis_end=1.0e-8, dipole=False)
 
 _lib = None
 
@@ -144,6 +145,92 @@ class Molecule:
         return lib().orc_hof_offset(self.h)
 
 
+# STO-6G expansions of unit-exponent Slater orbitals (Stewart, J. Chem. Phys.
+# 52, 431 (1970); the table MolDS keeps in src/base/GTOExpansionSTO.cpp):
+# (exponents, coefficients of normalized primitive Gaussians) for 1s, 2s, 2p.
+STO6G = {
+    "1s": ((23.10303149, 4.235915534, 1.185056519, 0.4070988982, 0.1580884151, 0.06510953954),
+           (0.00916359628, 0.04936149294, 0.1685383049, 0.3705627997, 0.4164915298, 0.1303340841)),
+    "2s": ((27.68496241, 5.077140627, 1.42678605, 0.2040335729, 0.09260298399, 0.04416183978),
+           (-0.004151277819, -0.02067024148, -0.05150303337, 0.3346271174, 0.5621061301, 0.1712994697)),
+    "2p": ((5.868285913, 1.530329631, 0.5475665231, 0.2288932733, 0.1046655969, 0.04948220127),
+           (0.007924233646, 0.05144104825, 0.189840006, 0.4049863191, 0.4012362861, 0.1051855189)),
+}
+# standard atomic weights (g/mol) as the atom classes set them (Hatom.cpp:52, Catom.cpp:49, ...)
+ATOMIC_MASS = {1: 1.00794, 6: 12.0107, 7: 14.00674, 8: 15.9994, 9: 18.9984032}
+GMOL2AU = 1.0e5 / (6.0221415 * 9.1095)   # Parameters.cpp:52
+DEBYE2AU = 0.393430191                   # Parameters.cpp:57
+
+
+def _ov1d(a, b, A, B):
+    """1-D overlaps of (x-A)^i e^{-a(x-A)^2} and (x-B)^j e^{-b(x-B)^2} for
+    i <= 2, j <= 1 (Obara-Saika recurrence); a, b arrays of primitive pairs."""
+    p = a + b
+    P = (a * A + b * B) / p
+    e00 = np.sqrt(np.pi / p) * np.exp(-a * b / p * (A - B) ** 2)
+    xa, xb, h = P - A, P - B, 0.5 / p
+    e10 = xa * e00
+    e01 = xb * e00
+    e11 = xb * e10 + h * e00
+    e20 = xa * e10 + h * e00
+    e21 = xb * e20 + 2.0 * h * e10
+    return {(0, 0): e00, (1, 0): e10, (0, 1): e01, (1, 1): e11, (2, 0): e20, (2, 1): e21}
+
+
+def cartesian_matrix(theory, Z, xyz_bohr):
+    """<mu| r_c |nu> (c = x, y, z; absolute coordinates) between the valence
+    Slater AOs expanded in STO-6G (Cndo2::CalcCartesianMatrixByGTOExpansion,
+    Cndo2.cpp:1946-2131): AO order s, py, pz, px; normalized primitives;
+    exponents zeta^2 * alpha_i with the theory's Slater exponents."""
+    Z = list(Z)
+    xyz = np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+    prm = np.zeros(14)
+    aos = []   # (atom, shell key, cartesian direction or -1, zeta)
+    for a, z in enumerate(Z):
+        lib().orc_param(int(z), THEORIES[theory], _dp(prm))
+        zs, zp = prm[6], prm[7]
+        if z == 1:
+            aos.append((a, "1s", -1, zs))
+        else:
+            aos += [(a, "2s", -1, zs), (a, "2p", 1, zp), (a, "2p", 2, zp), (a, "2p", 0, zp)]
+    n = len(aos)
+    X = np.zeros((3, n, n))
+    for mu, (A, ka, da, za) in enumerate(aos):
+        ea, ca = STO6G[ka]
+        ea = za * za * np.asarray(ea)
+        na = (2 * ea / np.pi) ** 0.75 * (np.sqrt(4 * ea) if da >= 0 else 1.0) * np.asarray(ca)
+        for nu, (B, kb, db, zb) in enumerate(aos):
+            eb, cb = STO6G[kb]
+            eb = zb * zb * np.asarray(eb)
+            nb = (2 * eb / np.pi) ** 0.75 * (np.sqrt(4 * eb) if db >= 0 else 1.0) * np.asarray(cb)
+            a2, b2 = np.meshgrid(ea, eb, indexing="ij")
+            w = np.outer(na, nb)
+            E = [_ov1d(a2, b2, xyz[A, c], xyz[B, c]) for c in range(3)]
+            la = [1 if c == da else 0 for c in range(3)]
+            lb = [1 if c == db else 0 for c in range(3)]
+            ov = [E[c][(la[c], lb[c])] for c in range(3)]
+            for c in range(3):
+                raised = E[c][(la[c] + 1, lb[c])] * ov[(c + 1) % 3] * ov[(c + 2) % 3]
+                X[c, mu, nu] = np.sum(w * (raised + xyz[A, c] * ov[0] * ov[1] * ov[2]))
+    return X
+
+
+def dipole(theory, Z, xyz_bohr, P, S):
+    """Ground-state dipole moments in a.u. as MolDS computes them
+    (Cndo2.cpp:748-753, 1871-1934): about the centre of core masses
+    (Molecule::CalcXyzCOC, core mass = atomic mass - valence electrons),
+    electronic = -sum P_mu,nu <mu|r|nu> + centre * sum P_mu,nu S_mu,nu,
+    core = sum_A Z_A (R_A - centre); returns (total, electronic, core)."""
+    xyz = np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+    core = np.array([1.0 if z == 1 else float(z - 2) for z in Z])
+    mass = np.array([ATOMIC_MASS[int(z)] * GMOL2AU for z in Z]) - core
+    centre = (mass[:, None] * xyz).sum(axis=0) / mass.sum()
+    X = cartesian_matrix(theory, Z, xyz)
+    el = -np.einsum("ij,cij->c", P, X) + centre * np.sum(P * S)
+    cd = (core[:, None] * (xyz - centre)).sum(axis=0)
+    return el + cd, el, cd
+
+
 def dsyevd_rows(F):
     """Lapack::Dsyevd: eigenvalues ascending, row i = eigenvector i, and each
     row's sign flipped so that its element sum is >= 0 (Lapack.cpp:117-150)."""
@@ -210,6 +297,9 @@ def scf(theory, Z, xyz_bohr, requires_guess=True, P0=None, log=None, **opts):
             res = dict(eps=eps, C=C, P=P, F_last=F, e_el=e_el, e_core=mol.ecore,
                        energy=e_el + mol.ecore, iterations=it + 1, rms=rms, traj=traj,
                        nao=n, nocc=nocc)
+            if o.get("dipole"):
+                res["dipole_total"], res["dipole_electronic"], res["dipole_core"] = dipole(
+                    theory, Z, xyz_bohr, P, mol.overlap())
             res["hof"] = res["energy"] + mol.hof_offset()
             break
         if not guess:

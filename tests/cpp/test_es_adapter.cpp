@@ -72,6 +72,10 @@ int main(int argc, char** argv) {
     }
     std::printf("energy %.15e core %.15e iterations %d nao %d eps0 %.15e epsN %.15e\n", es.GetElectronicEnergy(0),
                 es.GetCoreRepulsionEnergy(), es.GetNumberIterationsSCF(), n, eps[0], eps[n - 1]);
+    {
+        double const* dt = es.GetTotalDipoleMoment();
+        std::printf("dipole %.15e %.15e %.15e\n", dt[0], dt[1], dt[2]);
+    }
     const double e0 = es.GetElectronicEnergy(0);
     es.DoSCF(false);   // from the converged density: few iterations, same energy
     std::printf("warm_iterations %d warm_energy %.15e\n", es.GetNumberIterationsSCF(), es.GetElectronicEnergy(0));

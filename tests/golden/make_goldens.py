@@ -4,7 +4,7 @@ Reads the reference test inputs (test/<case>.in) and golden outputs
 (test/<case>.dat) as text and keeps only data: the SCF knobs, the geometry,
 and the printed results of the ground-state SCF (iteration lines, MO energies,
 electronic energy incl. core repulsion, core repulsion, MNDO heat of formation,
-Mulliken charges).  Run where the reference tree is available (the machines
+Mulliken charges, total / electronic / core dipole moments in a.u.).  Run where the reference tree is available (the machines
 that run the GPU tests have none):  python tests/golden/make_goldens.py /root/reference/test
 """
 import glob
@@ -82,6 +82,14 @@ def parse_dat(path, theory):
         m = re.match(r"Heats of formation:\s+(\S+)\s+(\S+)", s)
         if m:
             res["hof"] = float(m.group(1))
+            continue
+        m = re.match(r"(Total|Electronic) Dipole moment\(SCF\):\s+(\S+)\s+(\S+)\s+(\S+)", s)
+        if m:
+            res["dipole_" + m.group(1).lower()] = [float(m.group(k)) for k in (2, 3, 4)]
+            continue
+        m = re.match(r"Core Dipole moment:\s+(\S+)\s+(\S+)\s+(\S+)", s)
+        if m:
+            res["dipole_core"] = [float(m.group(k)) for k in (1, 2, 3)]
             continue
         m = re.match(r"Mulliken charge\(SCF\):\s+\d+\s+\d+\s+\S+\s+(\S+)\s+(\S+)", s)
         if m:

@@ -38,11 +38,15 @@ def test_cpp_adapter_on_goldens(tmp_path, name):
     r = _run_case(tmp_path, c)
     assert r.returncode == 0, r.stdout + r.stderr
     vals = dict(zip(*[iter(r.stdout.split("\n")[0].split())] * 2))
-    warm = dict(zip(*[iter(r.stdout.split("\n")[1].split())] * 2))
+    warm = dict(zip(*[iter([l for l in r.stdout.split("\n") if l.startswith("warm_")][0].split())] * 2))
     e = c["expect"]
     assert abs(float(vals["energy"]) - e["energy"]) <= 5.01e-7 * abs(e["energy"])
     assert abs(float(vals["core"]) - e["ecore"]) <= 5.01e-7 * abs(e["ecore"])
     assert int(warm["warm_iterations"]) <= 3
+    dline = [l for l in r.stdout.split("\n") if l.startswith("dipole ")][0]
+    dip = np.array([float(v) for v in dline.split()[1:]])
+    g = np.array(e["dipole_total"])
+    assert np.all(np.abs(dip - g) <= 5.01e-7 * np.abs(g) + 5.01e-7 * max(np.max(np.abs(g)), 1e-3) + 1e-12)
     # GetForce(0): the same central differences on the oracle's SCF energies
     from oracle import scf as O
     from molds_amd.forces import stencil_forces

@@ -50,6 +50,12 @@ def _read_block(text):
         m = re.match(r"Heats of formation:\s+(\S+)\s+(\S+)", s)
         if m:
             out["hof"] = float(m.group(1))
+        m = re.match(r"(Total|Electronic) Dipole moment\(SCF\):\s+(\S+)\s+(\S+)\s+(\S+)", s)
+        if m:
+            out["dipole_" + m.group(1).lower()] = [float(m.group(k)) for k in (2, 3, 4)]
+        m = re.match(r"Core Dipole moment:\s+(\S+)\s+(\S+)\s+(\S+)", s)
+        if m:
+            out["dipole_core"] = [float(m.group(k)) for k in (1, 2, 3)]
     return out
 
 
@@ -66,6 +72,10 @@ def _check_against_golden(block, c):
         assert abs(block["hof"] - e["hof"]) <= REL * abs(e["hof"])
     if c["scf"]["rms_density"] >= 1e-8:
         assert len(block["iters"]) == len(e["iters"])
+    for k in ("dipole_total", "dipole_electronic", "dipole_core"):
+        g = np.array(e[k])
+        v = np.array(block[k])
+        assert np.all(np.abs(v - g) <= REL * np.abs(g) + 5.01e-7 * max(np.max(np.abs(g)), 1e-3) + 1e-12), k
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -107,7 +117,7 @@ def test_printed_block_of_oracle_results(name):
     o = golden_opts(_case(name))
     r = O.scf(inp["theory"], inp["Z"], inp["xyz_bohr"], max_iter=o["max_iter"], rms_density=o["rms_density"],
               damping_thresh=o["damping_thresh"], damping_weight=o["damping_weight"], diis_k=o["diis_num_error_vect"],
-              diis_start=o["diis_start_error"], diis_end=o["diis_end_error"])
+              diis_start=o["diis_start_error"], diis_end=o["diis_end_error"], dipole=True)
     res = dict(r, status=0, status_name="OK", e_core=r["e_core"], mulliken=O.mulliken(inp["Z"], r["P"]),
                hof=r.get("hof", float("nan")))
     text = format_scf_results(inp["theory"], inp["Z"], res)

@@ -249,3 +249,40 @@ def test_fock_linearity_and_energy_identity():
     # guess Fock: U on the diagonal, beta_bar S between atoms, nothing else (Mndo.cpp:434-435, 511-513)
     Fg = mol.fock(P1, guess=True)
     assert np.max(np.abs(Fg - mol.fock(np.zeros((n, n)), guess=True))) == 0.0
+
+
+def _dipole_close(v, e):
+    # 7 printed digits of each component, with an absolute floor for components
+    # that print as rounding noise (1e-15 on symmetric molecules)
+    v, e = np.asarray(v), np.asarray(e)
+    return np.all(np.abs(v - e) <= REL * np.abs(e) + 5.01e-7 * max(np.max(np.abs(e)), 1e-3) + 1e-12)
+
+
+@pytest.mark.parametrize("case", _cases(), ids=lambda c: c["name"])
+def test_oracle_dipole_matches_goldens(case):
+    """Total, electronic and core dipole moments against MolDS's printed values
+    (STO-6G position integrals, centre of core masses, Cndo2.cpp:1871-2131)."""
+    e = case["expect"]
+    r = O.scf(case["theory"], case["Z"], case_xyz_bohr(case), dipole=True, **oracle_opts(case))
+    for k in ("dipole_total", "dipole_electronic", "dipole_core"):
+        assert _dipole_close(r[k], e[k]), (k, r[k], e[k])
+
+
+def test_position_integrals_closed_forms():
+    """STO-6G position integrals: <s|x|s> on one atom is the atom's coordinate
+    (normalized expansion), <s_A|x|p_x,A> on one centre is the s-p dipole
+    length of the fitted Slater pair, the matrix is symmetric, and a rigid
+    translation by t adds t times the overlap matrix."""
+    Z = [8, 1]
+    xyz = np.array([[0.3, -0.2, 0.5], [1.1, 1.2, -0.4]])
+    X = O.cartesian_matrix("pm3", Z, xyz)
+    for c in range(3):
+        assert np.allclose(X[c], X[c].T, atol=1e-14)
+        assert abs(X[c, 0, 0] - xyz[0, c]) <= 2e-5 * max(1.0, abs(xyz[0, c]))   # STO-6G norm ~ 1 - 1e-5
+    t = np.array([0.7, -1.3, 2.1])
+    X2 = O.cartesian_matrix("pm3", Z, xyz + t)
+    # S from the same STO-6G expansion: X(x + t) - X(x) = t S_gauss
+    Sg = (X2[0] - X[0]) / t[0]
+    for c in range(3):
+        assert np.allclose(X2[c] - X[c], t[c] * Sg, atol=1e-12)
+    assert np.allclose(np.diag(Sg), 1.0, atol=2e-5)

@@ -125,3 +125,21 @@ def test_eisol_derivation_against_molds_mndo_table():
         assert eng.nddo_element_eisol(Z, 0, ctypes.byref(v)) == 0
         tab = el[str(Z)]["mndo"]["ElecEnergyAtom"][0]
         assert abs(v.value - tab) <= tol, (Z, v.value, tab)
+
+
+def test_sto6g_tables_match_molds():
+    """The STO-6G expansion of the dipole integrals (the oracle's STO6G table and
+    the engine's kernels_props.hip constants) against MolDS's own table
+    (src/base/GTOExpansionSTO.cpp, extracted by make_sto6g_fixture.py)."""
+    import re
+    from oracle import scf as O
+    with open(os.path.join(ROOT, "tests", "golden", "sto6g_fixture.json")) as f:
+        ref = json.load(f)["expansions"]
+    src = open(os.path.join(ROOT, "molds_amd", "csrc", "kernels_props.hip")).read()
+    blocks = [list(map(float, re.findall(r"[-+]?\d+\.\d+(?:[eE][-+]?\d+)?", b)))
+              for b in re.findall(r"c_sto_(?:exp|coef)\[3\]\[6\] = \{(.*?)\};", src, re.S)]
+    eng_exp, eng_coef = blocks[0], blocks[1]
+    for i, key in enumerate(("1s", "2s", "2p")):
+        exps, coefs = ref[key]["exponents"], ref[key]["coefficients"]
+        assert list(O.STO6G[key][0]) == exps and list(O.STO6G[key][1]) == coefs, key
+        assert eng_exp[6 * i:6 * i + 6] == exps and eng_coef[6 * i:6 * i + 6] == coefs, key
