@@ -408,6 +408,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched legs (profiling runs of the other legs; the line then has no value)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -437,6 +439,12 @@ def main():
 
     from molds_amd.nddo import Engine
     engine = Engine(0)
+    if args.no_batched:   # profiling run of the single-molecule legs only
+        out = {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "note": "single-molecule legs only",
+               "large_single": large_leg(engine), "water1000_pm3": water1000_leg(engine)}
+        engine.close()
+        print(json.dumps(out), flush=True)
+        return
     legs = {}
     for mode in args.modes.split(","):
         jobs = make_jobs(mols, mode)
