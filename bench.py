@@ -14,11 +14,14 @@ at the start of each step and every integral is rebuilt on the GPU inside the
 timed region.
 
 Beside it, in the same run:
-  * served legs: Poisson arrivals at 50..2000 requests/s (2 s of arrivals,
-    at most 400 requests per leg, sizes log-uniform on 5..600 atoms over the
-    alkane / water generators, retired at density RMS < 1e-7) through the
-    continuous-batching request loop, mixed and FP64, with p50/p99 latency
-    (submit -> retire);
+  * served legs: Poisson arrivals at 5..2000 requests/s (2 s of arrivals,
+    at least 50 and at most 400 requests per leg, sizes log-uniform on 5..600
+    atoms over the alkane / water generators, retired at density RMS < 1e-7)
+    through the continuous-batching request loop, mixed and FP64, with
+    p50/p99 latency (submit -> retire), overall and by request size; the 5/s
+    leg runs below saturation;
+  * the requests of the mixed 500/s served leg as one nddo_scf_batch, the
+    batched rate of the same work;
   * (H2O)_256 AM1 (1536 AOs), one molecule, FP64 and mixed precision;
   * the plain FP64 GPU path: the same 4096-molecule mix solved one molecule at
     a time (nddo_scf_once), on a bounded prefix;
@@ -131,7 +134,7 @@ def served_leg(engine, rate, count, precision, seed=3, hi=SERVED_HI):
             time.sleep(t - now)
         ids.append((engine.submit(job), time.perf_counter() - t0, job))
     from molds_amd.nddo import NddoError
-    lat, done_at, ok, fp64_it, it, errors = [], [], 0, [], [], []
+    lat, done_at, ok, fp64_it, it, errors, nao = [], [], 0, [], [], [], []
     for jid, ts, job in ids:
         engine.wait(jid, 300.0)
         try:
@@ -144,20 +147,43 @@ def served_leg(engine, rate, count, precision, seed=3, hi=SERVED_HI):
         if r["status"] == 0:
             ok += 1
         lat.append(r["latency_s"])
+        nao.append(len(r["eps"]))
         done_at.append(ts + r["latency_s"])
         it.append(r["iterations"])
         fp64_it.append(r["fp64_iterations"])
     if errors:
         log(f"served {precision} {rate:g}/s: {len(errors)} failed requests, first: {errors[0]}")
     span = max(done_at)
-    lat = np.array(lat)
+    lat, nao = np.array(lat), np.array(nao)
+    by_size = {}
+    for name, n0, n1 in (("<=137", 0, 137), ("138-512", 138, 512), (">512", 513, 1 << 30)):
+        m = (nao >= n0) & (nao <= n1)
+        if m.any():
+            by_size[name] = dict(requests=int(m.sum()), p50_s=round(float(np.percentile(lat[m], 50)), 4))
     return dict(precision=precision, rate_per_s=rate, requests=count, converged=ok, failed=len(errors),
                 first_error=errors[0] if errors else None,
                 solves_per_s=round(count / span, 3), span_s=round(span, 3),
                 p50_s=round(float(np.percentile(lat, 50)), 5), p99_s=round(float(np.percentile(lat, 99)), 5),
                 mean_iterations=round(float(np.mean(it)), 2), mean_fp64_iterations=round(float(np.mean(fp64_it)), 2),
+                p50_by_AOs=by_size,
                 sizes=f"log-uniform {SERVED_LO}..{hi} atoms, alkane / water clusters",
                 rms_density=SERVED_SCF["rms_density"])
+
+
+def served_as_batch_leg(engine, rate, count, precision, seed=3):
+    """The requests of a served leg solved as ONE nddo_scf_batch (all present
+    at once, one launch per stage): the batched rate of the same work, beside
+    which the served rate is read."""
+    from molds_amd import workloads as W
+    from molds_amd.nddo import Job
+    jobs = [Job("pm3", Z, W.to_bohr(x), precision=precision, **SERVED_SCF)
+            for _, _, _, Z, x in W.poisson_requests(rate, count, seed=seed, lo=SERVED_LO, hi=SERVED_HI)]
+    t0 = time.perf_counter()
+    res = engine.scf_batch(jobs)
+    dt = time.perf_counter() - t0
+    return dict(precision=precision, requests=count, solves_per_s=round(count / dt, 3), seconds=round(dt, 3),
+                converged=int(sum(r["status"] == 0 for r in res)),
+                note=f"the {count} requests of the served {precision} {rate:g}/s leg as one batch")
 
 
 def plain_served_leg(engine, seconds, seed=3):
@@ -393,7 +419,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--modes", default="fp64,mixed")
-    ap.add_argument("--served-rates", default="50,500,2000", help="mixed-precision served legs (requests/s)")
+    ap.add_argument("--served-rates", default="5,50,500,2000", help="mixed-precision served legs (requests/s)")
     ap.add_argument("--served-rates-fp64", default="500", help="FP64 served legs (requests/s)")
     ap.add_argument("--served-seconds", type=float, default=2.0, help="seconds of arrivals per served leg")
     ap.add_argument("--served-max", type=int, default=400, help="at most this many requests per served leg")
@@ -508,6 +534,10 @@ def main():
                 leg = served_leg(engine, rate, count, prec)
                 served[prec].append(leg)
                 log(f"served {prec} {rate:g}/s: {leg}")
+        if args.latency_rate in [float(r) for r in args.served_rates.split(",") if r]:
+            count = min(args.served_max, max(50, int(round(args.latency_rate * args.served_seconds))))
+            served["mixed_as_one_batch"] = served_as_batch_leg(engine, args.latency_rate, count, "mixed")
+            log(f"served mix as one batch: {served['mixed_as_one_batch']}")
         engine.trim()
     large = water1000 = None
     if not args.no_large:

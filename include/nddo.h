@@ -120,9 +120,12 @@ typedef struct {
 
 /* Eigensolver placement.  By default a molecule of up to 1200 AOs gets one
  * workgroup (Householder + divide and conquer), which gives the most solves/s
- * when many molecules share the GPU; larger ones use the device-wide block-Jacobi
- * solver.  With this flag every molecule above 512 AOs uses the device-wide
- * solver: lower latency when only a few large molecules are in flight. */
+ * when many molecules share the GPU; larger ones use the device-wide solver
+ * (the same steps spread over the GPU, several molecules side by side).  A
+ * batch with at most 64 molecules above 512 AOs, and every served request
+ * above 512 AOs, sends those to the device-wide solver too.  With this flag
+ * every molecule above 512 AOs uses the device-wide solver: lower latency when
+ * only a few large molecules are in flight. */
 #define NDDO_ENGINE_DEVICE_WIDE_EIGEN 1
 /* Launch every SCF iteration stage by stage instead of replaying the
  * iteration's hipGraph (captured once per batch or served cohort after its
@@ -186,13 +189,15 @@ int nddo_release(nddo_engine* e, nddo_job_id id);
  * and the parts of the eigen stage: 6 tridiagonalization of the matrices kept
  * in LDS, 7 tridiagonalization of the matrices in device memory, 8 divide and
  * conquer, 9 back-transformation (these four up to 512 AOs), 10 the
- * device-wide block-Jacobi solver of the larger molecules.  In a batch
+ * device-wide eigensolver of the larger molecules (Householder + divide and
+ * conquer + back-transformation, or block Jacobi under
+ * NDDO_ENGINE_BLOCK_JACOBI).  In a batch
  * (nddo_scf_batch) with matrices of both kinds, the two classes' eigensolves
  * run concurrently on two streams: stage 6 then times the whole eigensolve of
  * the LDS class on its stream, and stages 8 and 9 the device-memory class. */
 enum { NDDO_STAGE_SETUP = 0, NDDO_STAGE_FOCK, NDDO_STAGE_EIGEN, NDDO_STAGE_DENSITY, NDDO_STAGE_STEP,
        NDDO_STAGE_ENERGY, NDDO_STAGE_TRIDIAG_LDS, NDDO_STAGE_TRIDIAG_GLB, NDDO_STAGE_DC,
-       NDDO_STAGE_BACKTRANSFORM, NDDO_STAGE_BLOCK_JACOBI, NDDO_STAGE_COUNT };
+       NDDO_STAGE_BACKTRANSFORM, NDDO_STAGE_XL_EIGEN, NDDO_STAGE_COUNT };
 int nddo_engine_profile(nddo_engine* e, int enable);
 int nddo_engine_stage_times(nddo_engine* e, double* ms, long long* launches, int reset);
 

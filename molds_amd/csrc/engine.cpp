@@ -347,9 +347,17 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
 }
 
 // Eigensolver classes: the one-workgroup solvers with the matrix in LDS, the
-// same in device memory (up to 512 AOs), and the device-wide block Jacobi for
-// larger molecules (kernels_xl.hip).
-int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao) {
+// same in device memory, and the device-wide solver (kernels_xlh.hip, or the
+// block Jacobi of kernels_xl.hip) for molecules of at least xl_min_nao AOs.
+// With route_few_large, a batch with few molecules above 512 AOs (at most
+// kFewLarge) sends those to the device-wide solver whatever xl_min_nao says: one workgroup would hold
+// each for 50-190 ms per iteration on one CU while the rest of the GPU idles
+// (measured on the 400 requests of the served 5-600-atom mix, 61 of them
+// above 512 AOs: 14.4 s as one batch with one workgroup each up to 1200 AOs,
+// 8.8 s with the device-wide solver above 512).  Many such molecules fill the
+// GPU one workgroup each, which gives more solves/s.
+constexpr int kFewLarge = 64;
+int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao, bool route_few_large) {
     list_lds.clear();
     list_glb.clear();
     list_xl.clear();
@@ -359,6 +367,11 @@ int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao) {
     std::vector<int> order(jobs.size());
     for (size_t i = 0; i < jobs.size(); ++i) order[i] = (int)i;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return jobs[a].nao > jobs[b].nao; });
+    if (route_few_large && xl_min_nao > 513) {
+        int nlarge = 0;
+        for (const JobDev& jb : jobs) nlarge += jb.nao > 512;
+        if (nlarge <= kFewLarge) xl_min_nao = 513;
+    }
     for (int j : order) {
         int n = jobs[j].nao;
         // the Jacobi kernels also hold up to 8 KB of static LDS (reduction buffers)
@@ -592,7 +605,11 @@ void Batch::iterate(Engine* e, hipStream_t st) {
         hipGraph_t g = nullptr;
         bool ok;
         {
-            std::lock_guard<std::mutex> cap(e->capture_mu);
+            std::lock_guard<std::recursive_mutex> cap(e->capture_mu);
+            if (served && e->sync_busy.load() > 0) {   // a synchronous call runs: capture at a later iteration
+                launch_iteration(e, st);
+                return;
+            }
             ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
                 launch_iteration(e, st);
@@ -718,7 +735,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             const int map[10][3] = {{NDDO_STAGE_FOCK, 0, 9},          {NDDO_STAGE_EIGEN, 1, 10},
                                     {NDDO_STAGE_TRIDIAG_LDS, l0, l1}, {NDDO_STAGE_TRIDIAG_GLB, 2, 3},
                                     {NDDO_STAGE_DC, 3, 4},            {NDDO_STAGE_BACKTRANSFORM, 4, 5},
-                                    {NDDO_STAGE_BLOCK_JACOBI, 5, 13}, {NDDO_STAGE_DENSITY, 10, 6},
+                                    {NDDO_STAGE_XL_EIGEN, 5, 13},     {NDDO_STAGE_DENSITY, 10, 6},
                                     {NDDO_STAGE_STEP, 6, 7},          {NDDO_STAGE_ENERGY, 9, 1}};
             for (int k = 0; k < 10; ++k) {
                 float ms = 0.f;
@@ -1061,7 +1078,7 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     // jobs whose caller takes no C back-transform only the occupied eigenvectors
     for (size_t jj = 0; jj < which.size(); ++jj) b.jobs[jj].want_c = res[which[jj]].C != nullptr ? 1 : 0;
     b.use_xl_householder = e->xl_householder;
-    b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
+    b.classify(e->lds_max, e->num_cus, e->xl_min_nao, true);
     {
         std::vector<int> dj;
         for (size_t jj = 0; jj < which.size(); ++jj)
@@ -1123,7 +1140,7 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || njobs < 0 || (njobs > 0 && (!descs || !res))) { set_error("bad arguments"); return NDDO_BAD_ARG; }
     auto t0 = std::chrono::steady_clock::now();
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     std::vector<int> ok;
     for (int i = 0; i < njobs; ++i) {
@@ -1149,7 +1166,7 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !job || !res) { set_error("bad arguments"); return NDDO_BAD_ARG; }
     auto t0 = std::chrono::steady_clock::now();
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     int n = 0, no = 0;
     const int s = validate_job(*job, &n, &no);
@@ -1222,7 +1239,7 @@ int nddo_element_params(int Z, int theory, double* out) {
 int nddo_engine_trim(nddo_engine* eh) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     free_workspace(e);
     trim_pool(e->device);
@@ -1235,7 +1252,10 @@ int nddo_engine_memory(nddo_engine* eh, size_t* free_bytes, size_t* total_bytes,
     if (!e) return NDDO_BAD_ARG;
     HIPCHK(hipSetDevice(e->device));
     size_t f = 0, t = 0;
-    HIPCHK(hipMemGetInfo(&f, &t));
+    {
+        std::lock_guard<std::recursive_mutex> cap(e->capture_mu);
+        HIPCHK(hipMemGetInfo(&f, &t));
+    }
     if (free_bytes) *free_bytes = f;
     if (total_bytes) *total_bytes = t;
     if (workspace_bytes) {
@@ -1263,7 +1283,7 @@ int nddo_engine_info(nddo_engine* eh, size_t* lds_bytes, int* num_cus, size_t* h
 int nddo_engine_profile(nddo_engine* eh, int enable) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     e->profile = enable != 0;
     return NDDO_OK;
 }
@@ -1271,7 +1291,7 @@ int nddo_engine_profile(nddo_engine* eh, int enable) {
 int nddo_engine_stage_times(nddo_engine* eh, double* ms, long long* launches, int reset) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     for (int i = 0; i < NDDO_STAGE_COUNT; ++i) {
         if (ms) ms[i] = e->stage_ms[i];
         if (launches) launches[i] = e->stage_launches[i];
@@ -1317,7 +1337,7 @@ int nddo_dbg_integrals(nddo_engine* eh, int theory, int natoms, const int* Z, co
                        double* g_dense, double* S, double* H, double* e_core) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     Batch b;
     nddo_job_desc d;
@@ -1372,7 +1392,7 @@ int nddo_dbg_fock(nddo_engine* eh, int theory, int natoms, const int* Z, const d
                   int guess, double* F) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !F) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     Batch b;
     nddo_job_desc d;
@@ -1603,7 +1623,7 @@ int nddo_dbg_set_tuning(nddo_engine* eh, int fock_wpb, const int* bt_rows4) {
         if (bt_rows4[i] != 32 && bt_rows4[i] != 64) return NDDO_BAD_ARG;
         t.bt_rows[i] = bt_rows4[i];
     }
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     set_tuning(t);
     return NDDO_OK;
@@ -1612,7 +1632,7 @@ int nddo_dbg_set_tuning(nddo_engine* eh, int fock_wpb, const int* bt_rows4) {
 int nddo_dbg_set_xl_grid(nddo_engine* eh, int tri_workgroups) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || tri_workgroups < 0) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     Tuning t = get_tuning();
     t.xl_tri_wg = tri_workgroups;
@@ -1623,7 +1643,7 @@ int nddo_dbg_set_xl_grid(nddo_engine* eh, int tri_workgroups) {
 int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* C, int* sweeps) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || n <= 0 || !F) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     Batch b;
     int rc = matrix_job(e, b, n, 0);
@@ -1703,7 +1723,7 @@ int nddo_dbg_eigh_dc_phases(nddo_engine* eh, double* ticks7) {
 int nddo_dbg_density(nddo_engine* eh, int n, int nocc, const double* C, double* P) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || n <= 0 || nocc < 0 || nocc > n || !C || !P) return NDDO_BAD_ARG;
-    std::lock_guard<std::mutex> lk(e->run_mu);
+    SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
     Batch b;
     int rc = matrix_job(e, b, n, nocc);

@@ -124,7 +124,9 @@ struct Batch {
     size_t device_bytes(bool want_sdbg);
     int build(const nddo_job_desc* descs, int nd, const std::vector<int>& which, const int* unused);
     // xl_min_nao: smallest molecule of the device-wide eigensolver class
-    int classify(size_t lds_max, int num_cus, int xl_min_nao);
+    // route_few_large: molecules above 512 AOs take the device-wide solver
+    // when the batch holds few of them (nddo_scf_batch / nddo_scf_once)
+    int classify(size_t lds_max, int num_cus, int xl_min_nao, bool route_few_large = false);
     void launch_xl_eigen_all(Engine* e, hipStream_t st);
     int allocate(bool want_sdbg);
     size_t upload_bytes() const;
@@ -139,6 +141,7 @@ struct Batch {
     int iterations_launched = 0;
     bool graph_off = false;
     bool eager_only = false;   // never capture (nddo_scf_once_noalloc: instantiation may allocate)
+    bool served = false;       // a served cohort: no capture while a synchronous call runs
     int run_scf(Engine* e, hipStream_t st);
     int fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* const* res, const double* hof_offsets);
     void set_dipole_jobs(const std::vector<int>& jj);
@@ -175,10 +178,15 @@ struct Engine {
     size_t hbm_budget = 0;
     int* h_remaining = nullptr;
     std::mutex run_mu;   // one synchronous batch at a time
-    // held while a stream captures an iteration graph; HIP calls made from a
-    // caller's thread (served-result copies and frees) take it, so none of them
-    // runs while another thread's capture is open
-    std::mutex capture_mu;
+    // Held while a stream captures an iteration graph, and by every HIP call
+    // that may run beside another thread's capture: the served scheduler's
+    // launches, queries, copies and frees, and the caller threads' served-result
+    // copies and frees.  The synchronous entry points instead keep the
+    // scheduler from opening a capture while they run (sync_busy, SyncRun).
+    // Recursive: the scheduler holds it around a cohort iteration whose
+    // launch may itself begin a capture.
+    std::recursive_mutex capture_mu;
+    std::atomic<int> sync_busy{0};
     bool profile = false;
     double stage_ms[NDDO_STAGE_COUNT] = {};
     long long stage_launches[NDDO_STAGE_COUNT] = {};
@@ -216,6 +224,23 @@ struct Engine {
 
     void start_server();
     void stop_server();
+};
+
+// A synchronous entry point (nddo_scf_batch, nddo_scf_once, the stage hooks):
+// one at a time (run_mu); while it runs the served scheduler opens no graph
+// capture, so none of its HIP calls on the caller's thread falls inside
+// another thread's capture window, and a capture already open ends before it
+// starts.
+struct SyncRun {
+    Engine* e;
+    std::lock_guard<std::mutex> run;
+    explicit SyncRun(Engine* eng) : e(eng), run(eng->run_mu) {
+        std::lock_guard<std::recursive_mutex> cap(e->capture_mu);
+        e->sync_busy.fetch_add(1);
+    }
+    ~SyncRun() { e->sync_busy.fetch_sub(1); }
+    SyncRun(const SyncRun&) = delete;
+    SyncRun& operator=(const SyncRun&) = delete;
 };
 
 }  // namespace nddo

@@ -220,6 +220,7 @@ struct Staging {
 int start_cohort(Engine* e, Cohort* c, Staging& stage) {
     hipStream_t st = c->st;
     c->b.alloc_stream = st;
+    c->b.served = true;
     std::vector<int> which(c->members.size());
     for (size_t i = 0; i < which.size(); ++i) which[i] = (int)i;
     c->descs.resize(c->members.size());
@@ -280,7 +281,7 @@ int materialize(Engine* e, ServedJob* j) {
         for (;;) {
             hipError_t q;
             {
-                std::lock_guard<std::mutex> cap(e->capture_mu);
+                std::lock_guard<std::recursive_mutex> cap(e->capture_mu);
                 q = hipEventQuery(j->res_ev->ev);
             }
             if (q == hipSuccess) break;
@@ -293,7 +294,7 @@ int materialize(Engine* e, ServedJob* j) {
     }
     std::vector<char> h(result_bytes(n, mi));
     {
-        std::lock_guard<std::mutex> cap(e->capture_mu);
+        std::lock_guard<std::recursive_mutex> cap(e->capture_mu);
         hipError_t err = hipMemcpyAsync(h.data(), j->d_res, h.size(), hipMemcpyDeviceToHost, e->api_stream);
         if (err == hipSuccess) err = hipStreamSynchronize(e->api_stream);
         if (err == hipSuccess) err = hipFreeAsync(j->d_res, e->api_stream);
@@ -408,10 +409,17 @@ void Engine::start_server() {
                 f->st = spool[k];
                 f->ev = epool[k];
                 f->h_state = hstate[k];
-                const int rc = start_cohort(this, f.get(), stage[k]);
+                int rc;
+                {
+                    std::lock_guard<std::recursive_mutex> cap(capture_mu);
+                    rc = start_cohort(this, f.get(), stage[k]);
+                }
                 if (rc != NDDO_OK) {
                     fail_cohort(this, f.get(), rc, "cohort start: " + nddo::get_error());
-                    free_cohort(f.get());
+                    {
+                        std::lock_guard<std::recursive_mutex> cap(capture_mu);
+                        free_cohort(f.get());
+                    }
                     free_slots.push_back(k);
                 } else {
                     resident += f->bytes;
@@ -424,6 +432,7 @@ void Engine::start_server() {
             // 2. the next SCF iteration of every idle cohort, then its jobs' scalar state
             for (auto& c : cohorts) {
                 if (c->inflight || c->live <= 0) continue;
+                std::lock_guard<std::recursive_mutex> cap(capture_mu);
                 c->b.iterate(this, c->st);
                 hipMemcpy2DAsync(c->h_state, kScalarBytes, c->b.d_state, sizeof(JobState), kScalarBytes,
                                  c->members.size(), hipMemcpyDeviceToHost, c->st);
@@ -438,7 +447,11 @@ void Engine::start_server() {
             for (;;) {
                 for (auto& c : cohorts) {
                     if (!c->inflight) continue;
-                    const hipError_t q = hipEventQuery(c->ev);
+                    hipError_t q;
+                    {
+                        std::lock_guard<std::recursive_mutex> cap(capture_mu);
+                        q = hipEventQuery(c->ev);
+                    }
                     if (q == hipSuccess) {
                         ready.push_back(c.get());
                     } else if (q != hipErrorNotReady) {
@@ -469,14 +482,20 @@ void Engine::start_server() {
                     // every job ends within max_iter + 1 iterations (k_step marks NOT_CONVERGED)
                     if (phase == PH_DONE || c->turns > c->b.max_iter_all + 2) done.push_back((int)jj);
                 }
-                if (!done.empty()) retire_jobs(this, c, done);
+                if (!done.empty()) {
+                    std::lock_guard<std::recursive_mutex> cap(capture_mu);
+                    retire_jobs(this, c, done);
+                }
             }
             server_s[3] += secs_since(t_phase);
             // 4. free finished cohorts
             for (size_t i = 0; i < cohorts.size();) {
                 if (cohorts[i]->live <= 0 && !cohorts[i]->inflight) {
                     resident -= cohorts[i]->bytes;
-                    free_cohort(cohorts[i].get());
+                    {
+                        std::lock_guard<std::recursive_mutex> cap(capture_mu);
+                        free_cohort(cohorts[i].get());
+                    }
                     free_slots.push_back(cohorts[i]->slot);
                     cohorts.erase(cohorts.begin() + i);
                 } else {
@@ -678,7 +697,7 @@ int nddo_release(nddo_engine* eh, nddo_job_id id) {
         std::lock_guard<std::mutex> jl(j->mu);
         if (j->d_res) {   // on the API stream, after the retire copies (never on the cohort stream)
             hipSetDevice(e->device);
-            std::lock_guard<std::mutex> cap(e->capture_mu);
+            std::lock_guard<std::recursive_mutex> cap(e->capture_mu);
             if (j->res_ev && j->res_ev->ev) hipStreamWaitEvent(e->api_stream, j->res_ev->ev, 0);
             hipFreeAsync(j->d_res, e->api_stream);
             e->result_resident -= (long long)nddo::result_bytes(j->res.nao, j->desc.opts.max_iter);

@@ -14,9 +14,11 @@
 * format_scf_results(...)  the lines MolDS prints for that SCF: iteration
                            table (Cndo2.cpp:1505-1510), convergence line, MO
                            energies (:1063-1081), electronic and core-repulsion
-                           energies (:1083-1107), Mulliken charges (:1187-1199)
-                           and, for MNDO, the heat of formation (Mndo.cpp:412-420).
-                           The STO-6G dipole moments are not computed.
+                           energies (:1083-1107), the STO-6G dipole moments
+                           (:1126-1184), Mulliken charges (:1187-1199) and, for
+                           MNDO, the heat of formation (Mndo.cpp:412-420; MolDS
+                           prints none for AM1/PM3, Am1.cpp:261-263, although
+                           the engine returns it).
 * main()                   python -m molds_amd.molds_io input.in: solve on GPU 0
                            and print the block.
 """

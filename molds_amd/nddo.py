@@ -272,7 +272,7 @@ class Engine:
 
     # --------------------------------------------------------------- profiling
     STAGES = ("setup", "fock", "eigen", "density", "step", "energy", "tridiag_lds", "tridiag_glb", "divide_conquer",
-              "backtransform", "block_jacobi")
+              "backtransform", "xl_eigen")
 
     def server_stats(self, reset=False):
         t = np.zeros(4)

@@ -115,6 +115,53 @@ def test_served_results_taken_while_cohorts_capture(engine):
         assert r["iterations"] == alone["iterations"]
 
 
+def test_batch_concurrent_with_served(engine):
+    """nddo_scf_batch on the caller's thread while another thread submits,
+    fetches and releases served requests whose cohorts start, capture their
+    iteration graphs and retire: no call fails, every served result equals the
+    alone solve, and every batch equals the first one bit for bit."""
+    import threading
+    print("batch concurrent with served", flush=True)
+    served_jobs = [_job("water", 3 + (5 * k) % 17, 200 + k) if k % 2 else _job("alkane", 2 + (7 * k) % 19, 200 + k)
+                   for k in range(24)]
+    batch_jobs = [_job("alkane", 3 + k, 300 + k, precision="mixed") if k % 2 else _job("water", 4 + k, 300 + k)
+                  for k in range(12)]
+    out, errors = {}, []
+
+    def client():
+        try:
+            for w in range(4):
+                ids = [(engine.submit(j), j) for j in served_jobs[6 * w:6 * w + 6]]
+                for jid, j in ids:
+                    assert engine.wait(jid, 120.0) == 0
+                    out[jid] = (engine.get_result(jid, j), j)
+                    assert engine.release(jid) == 0
+                time.sleep(0.01)
+        except Exception as ex:   # reported by the main thread
+            errors.append(repr(ex))
+
+    th = threading.Thread(target=client)
+    th.start()
+    batches = []
+    try:
+        while th.is_alive() or len(batches) < 2:
+            batches.append(engine.scf_batch(batch_jobs))
+    finally:
+        th.join()
+    assert not errors, errors[0]
+    assert len(out) == 24
+    for b in batches[1:]:
+        for r, r0 in zip(b, batches[0]):
+            assert r["status"] == 0 and r["energy"] == r0["energy"] and r["iterations"] == r0["iterations"]
+            assert np.array_equal(r["eps"], r0["eps"])
+    for jid in list(out)[::3]:
+        r, j = out[jid]
+        alone = engine.scf_batch([j])[0]
+        assert r["status"] == 0 and alone["status"] == 0
+        assert abs(r["energy"] - alone["energy"]) * EV <= 1e-9
+        assert r["iterations"] == alone["iterations"]
+
+
 def test_served_statuses(engine):
     from molds_amd.nddo import Engine, Job
     odd = Job("pm3", [6, 1, 1, 1], np.eye(4, 3) * 2.0)

@@ -270,8 +270,8 @@ def test_oracle_dipole_matches_goldens(case):
 
 def test_position_integrals_closed_forms():
     """STO-6G position integrals: <s|x|s> on one atom is the atom's coordinate
-    (normalized expansion), <s_A|x|p_x,A> on one centre is the s-p dipole
-    length of the fitted Slater pair, the matrix is symmetric, and a rigid
+    (normalized expansion), <s_A|x|p_x,A> on one centre is the Slater s-p
+    dipole length to the STO-6G fit's accuracy, the matrix is symmetric, and a rigid
     translation by t adds t times the overlap matrix."""
     Z = [8, 1]
     xyz = np.array([[0.3, -0.2, 0.5], [1.1, 1.2, -0.4]])
@@ -286,3 +286,13 @@ def test_position_integrals_closed_forms():
     for c in range(3):
         assert np.allclose(X2[c] - X[c], t[c] * Sg, atol=1e-12)
     assert np.allclose(np.diag(Sg), 1.0, atol=2e-5)
+    # one-centre <2s|x_c|2p_c> of Slater AOs (zeta_s, zeta_p):
+    # N_s N_p 5! / (zeta_s + zeta_p)^6 / sqrt(3), N = sqrt((2 zeta)^5 / 4!)
+    prm = np.zeros(14)
+    O.lib().orc_param(8, O.THEORIES["pm3"], O._dp(prm))
+    zs, zp = prm[6], prm[7]
+    ns, npn = np.sqrt((2 * zs) ** 5 / 24), np.sqrt((2 * zp) ** 5 / 24)
+    ref = ns * npn * 120.0 / (zs + zp) ** 6 / np.sqrt(3.0)
+    for c, k in ((0, 3), (1, 1), (2, 2)):   # AO order s, py, pz, px
+        assert abs(X[c][0, k] - ref) <= 1e-4 * ref, (c, X[c][0, k], ref)
+        assert abs(X[(c + 1) % 3][0, k]) <= 1e-14
