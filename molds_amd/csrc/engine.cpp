@@ -725,7 +725,8 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             HIPCHK(hipEventRecord(e->ev[7], st));
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(h_rem, d_remaining, sizeof(int), hipMemcpyDeviceToHost, st));
+        launch_publish(st, d_remaining, sizeof(int), e->d_h_remaining, sizeof(int), sizeof(int), 1);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         if (prof) {
             // (stage, first event, last event)
@@ -1013,7 +1014,9 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
         set_error("overlap table upload failed");
         return NDDO_HIP_ERROR;
     }
-    HIPCHK(hipHostMalloc((void**)&e->h_remaining, sizeof(int), hipHostMallocDefault));
+    // fine-grained (coherent) pinned memory: written by a kernel each iteration
+    HIPCHK(hipHostMalloc((void**)&e->h_remaining, sizeof(int), hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&e->d_h_remaining, e->h_remaining, 0));
     for (int i = 0; i < 16; ++i) HIPCHK(hipEventCreate(&e->ev[i]));
     HIPCHK(hipEventCreateWithFlags(&e->xl_chain, hipEventDisableTiming));
     {

@@ -176,7 +176,8 @@ struct Engine {
     bool xl_householder = true;   // false with NDDO_ENGINE_BLOCK_JACOBI
     std::atomic<long long> graphs_captured{0}, graph_replays{0};
     size_t hbm_budget = 0;
-    int* h_remaining = nullptr;
+    int* h_remaining = nullptr;     // pinned, fine-grained
+    int* d_h_remaining = nullptr;   // its device address
     std::mutex run_mu;   // one synchronous batch at a time
     // Held while a stream captures an iteration graph, and by every HIP call
     // that may run beside another thread's capture: the served scheduler's

@@ -1062,4 +1062,24 @@ void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* stat
     hipLaunchKernelGGL(k_energy, dim3(njobs), dim3(256), 0, st, jobs, state, arena, remaining, xlpart, nchunk);
 }
 
+// count records of `words` 32-bit words, src_stride / dst_stride bytes apart,
+// from device memory into mapped pinned host memory, with plain vector stores.
+__global__ void __launch_bounds__(256) k_publish(const unsigned* __restrict__ src, long long src_stride_w,
+                                                 unsigned* __restrict__ dst, long long dst_stride_w, int words,
+                                                 int count) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)words * count) return;
+    const long long r = i / words, w = i - r * words;
+    dst[r * dst_stride_w + w] = src[r * src_stride_w + w];
+}
+
+void launch_publish(hipStream_t st, const void* src, size_t src_stride, void* dst_dev, size_t dst_stride,
+                    size_t bytes, int count) {
+    if (count <= 0 || bytes == 0) return;
+    const int words = (int)(bytes / 4);
+    const long long total = (long long)words * count;
+    hipLaunchKernelGGL(k_publish, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, (const unsigned*)src,
+                       (long long)(src_stride / 4), (unsigned*)dst_dev, (long long)(dst_stride / 4), words, count);
+}
+
 }  // namespace nddo

@@ -85,6 +85,18 @@ void launch_apply(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev
 void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
                    int* remaining, const double* xlpart, int nchunk);
 
+// Per-iteration readback of small device records (the jobs' scalar state, the
+// batch's remaining count) into pinned host memory by a kernel on stream st.
+// A copy-engine transfer (hipMemcpyAsync to the host) would queue behind the
+// copies of other streams, each waiting for its own stream's kernels: a served
+// cohort of small molecules then advanced one SCF iteration per iteration of
+// the slowest resident cohort (measured: 19 ms instead of 0.5 ms per
+// iteration beside an 864-AO request).  bytes and both strides are multiples
+// of 4; dst_dev is the device address of the pinned buffer
+// (hipHostGetDevicePointer).
+void launch_publish(hipStream_t st, const void* src, size_t src_stride, void* dst_dev, size_t dst_stride,
+                    size_t bytes, int count);
+
 // Ground-state dipole partial sums of converged jobs (kernels_props.hip): per
 // AO row mu of job joblist[i], 4 doubles at part[doff[i] + 4 mu]: sum over the
 // row's upper part of P r_x, P r_y, P r_z and P S (off-diagonal terms doubled).

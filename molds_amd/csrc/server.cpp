@@ -48,6 +48,7 @@ namespace {
 // Leading part of JobState the scheduler reads every iteration: the int
 // fields and rms, diis_error, e_el, e_core.
 constexpr size_t kScalarBytes = offsetof(JobState, prof);
+static_assert(kScalarBytes % 4 == 0 && sizeof(JobState) % 4 == 0, "launch_publish copies 32-bit words");
 
 struct EventRef {
     hipEvent_t ev = nullptr;
@@ -83,7 +84,8 @@ struct Cohort {
     int slot = -1;   // index into the scheduler's stream / event / pinned-buffer pools
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
-    char* h_state = nullptr;   // pinned: kScalarBytes per job
+    char* h_state = nullptr;   // pinned, fine-grained: kScalarBytes per job
+    char* d_hstate = nullptr;  // its device address (written by launch_publish)
     bool inflight = false;
     std::vector<nddo_job_desc> descs;
     std::vector<ServedJob*> members;
@@ -336,13 +338,14 @@ void Engine::start_server() {
         const int cap = admit_limit(this);
         std::vector<hipStream_t> spool(kMaxCohorts, nullptr);
         std::vector<hipEvent_t> epool(kMaxCohorts, nullptr);
-        std::vector<char*> hstate(kMaxCohorts, nullptr);
+        std::vector<char*> hstate(kMaxCohorts, nullptr), hstate_dev(kMaxCohorts, nullptr);
         std::vector<Staging> stage(kMaxCohorts);
         std::vector<int> free_slots;
         for (int k = kMaxCohorts - 1; k >= 0; --k) {
             if (hipStreamCreateWithFlags(&spool[k], hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&epool[k], hipEventDisableTiming) != hipSuccess ||
-                hipHostMalloc((void**)&hstate[k], (size_t)cap * kScalarBytes, hipHostMallocDefault) != hipSuccess)
+                hipHostMalloc((void**)&hstate[k], (size_t)cap * kScalarBytes, hipHostMallocCoherent) != hipSuccess ||
+                hipHostGetDevicePointer((void**)&hstate_dev[k], hstate[k], 0) != hipSuccess)
                 continue;
             free_slots.push_back(k);
         }
@@ -409,6 +412,7 @@ void Engine::start_server() {
                 f->st = spool[k];
                 f->ev = epool[k];
                 f->h_state = hstate[k];
+                f->d_hstate = hstate_dev[k];
                 int rc;
                 {
                     std::lock_guard<std::recursive_mutex> cap(capture_mu);
@@ -434,8 +438,8 @@ void Engine::start_server() {
                 if (c->inflight || c->live <= 0) continue;
                 std::lock_guard<std::recursive_mutex> cap(capture_mu);
                 c->b.iterate(this, c->st);
-                hipMemcpy2DAsync(c->h_state, kScalarBytes, c->b.d_state, sizeof(JobState), kScalarBytes,
-                                 c->members.size(), hipMemcpyDeviceToHost, c->st);
+                launch_publish(c->st, c->b.d_state, sizeof(JobState), c->d_hstate, kScalarBytes, kScalarBytes,
+                               (int)c->members.size());
                 hipEventRecord(c->ev, c->st);
                 c->inflight = true;
                 ++c->turns;
