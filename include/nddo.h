@@ -106,6 +106,10 @@ typedef struct {
                                 x y z, about the centre of core masses, with the STO-6G position integrals
                                 MolDS prints (Cndo2.cpp:748-753, 1126-1184, 1871-2131); filled by
                                 nddo_scf_batch / nddo_scf_once for converged jobs, left alone by served jobs */
+    double* forces;          /* [natoms*3] ground-state forces -dE/dR (Hartree/bohr) of the converged
+                                density, analytic (Mndo::CalcForce, Mndo.cpp:2711-2986; GetForce(0),
+                                ElectronicStructure.h:36); filled by nddo_scf_batch / nddo_scf_once for
+                                converged jobs (NaN otherwise), left alone by served jobs */
 } nddo_result;
 
 typedef struct nddo_engine nddo_engine;

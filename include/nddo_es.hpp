@@ -187,55 +187,37 @@ public:
 
     // GetForce(elecState) (ElectronicStructure.h:36; Mndo::CalcForce, Mndo.cpp:2711-2986):
     // force[a][c] in Hartree/bohr of the ground state at the last DoSCF's
-    // geometry, by the fourth-order central difference of SCF energies at
-    // +-h, +-2h (h = 1e-3 bohr) per coordinate; the 12 N displaced SCFs run as
-    // one nddo_scf_batch, each started from the last DoSCF's density
-    // (DoSCF(false)) and converged to density RMS 1e-10.
+    // geometry, analytic (nddo_result.forces).  The SCF is first taken from
+    // the last DoSCF's density (DoSCF(false)) to density RMS 1e-10, so the
+    // forces carry no error from a looser SCF threshold; the adapter's energy,
+    // orbitals and density stay those of the last DoSCF.
     double const* const* GetForce(int elecState) {
         if (elecState != 0) throw MolDSException("excited states are not part of this engine", NDDO_BAD_ARG);
         if (!molecule_ || !have_density_)
             throw MolDSException("Error in cndo::Cndo2::GetForce: DoSCF has not run.\n", NDDO_BAD_ARG);
-        const int N = molecule_->GetAtomVectSize(), nj = 12 * N;
-        const double h = 1.0e-3, step[4] = {-2.0, -1.0, 1.0, 2.0};
-        std::vector<double> geo((size_t)nj * 3 * N);
-        std::vector<nddo_job_desc> d(nj);
-        std::vector<nddo_result> r(nj);
-        std::vector<double> P0 = P_;
-        nddo_scf_opts o = params_.o;
-        o.rms_density = std::min(o.rms_density, 1.0e-10);
-        o.max_iter = std::max(o.max_iter, 200);
-        for (int a = 0, j = 0; a < N; ++a)
-            for (int c = 0; c < 3; ++c)
-                for (int s = 0; s < 4; ++s, ++j) {
-                    double* g = &geo[(size_t)j * 3 * N];
-                    std::copy(molecule_->xyz_bohr.begin(), molecule_->xyz_bohr.end(), g);
-                    g[3 * a + c] += step[s] * h;
-                    d[j] = nddo_job_desc{};
-                    d[j].theory = (int)theory_;
-                    d[j].precision = precision_;
-                    d[j].natoms = N;
-                    d[j].Z = molecule_->Z.data();
-                    d[j].xyz_bohr = g;
-                    d[j].opts = o;
-                    d[j].requires_guess = 0;
-                    d[j].P0 = P0.data();
-                    r[j] = nddo_result{};
-                }
-        const int rc = nddo_scf_batch(engine_->get(), nj, d.data(), r.data());
-        if (rc != NDDO_OK) throw MolDSException(std::string("Error in GetForce: ") + nddo_last_error(), rc);
+        const int N = molecule_->GetAtomVectSize();
+        nddo_job_desc d{};
+        d.theory = (int)theory_;
+        d.precision = precision_;
+        d.natoms = N;
+        d.Z = molecule_->Z.data();
+        d.xyz_bohr = molecule_->xyz_bohr.data();
+        d.opts = params_.o;
+        d.opts.rms_density = std::min(d.opts.rms_density, 1.0e-10);
+        d.opts.max_iter = std::max(d.opts.max_iter, 200);
+        d.requires_guess = 0;
+        std::vector<double> P0 = P_, eps(n_);
+        d.P0 = P0.data();
         force_.assign((size_t)3 * N, 0.0);
+        nddo_result r{};
+        r.eps = eps.data();
+        r.forces = force_.data();
+        const int st = nddo_scf_once(engine_->get(), &d, &r);
+        if (st != NDDO_OK)
+            throw MolDSException(std::string("Error in GetForce: the SCF for the forces ended with: ") + nddo_last_error(),
+                                 st);
         frow_.resize(N);
-        for (int a = 0; a < N; ++a) {
-            frow_[a] = &force_[(size_t)3 * a];
-            for (int c = 0; c < 3; ++c) {
-                const nddo_result* e = &r[(size_t)(3 * a + c) * 4];
-                for (int s = 0; s < 4; ++s)
-                    if (e[s].status != NDDO_OK)
-                        throw MolDSException("Error in GetForce: a displaced SCF did not converge\n", e[s].status);
-                force_[(size_t)3 * a + c] = -(e[0].energy_total - 8.0 * e[1].energy_total + 8.0 * e[2].energy_total -
-                                              e[3].energy_total) / (12.0 * h);
-            }
-        }
+        for (int a = 0; a < N; ++a) frow_[a] = &force_[(size_t)3 * a];
         return frow_.data();
     }
     int GetNumberIterationsSCF() const { return iterations_; }

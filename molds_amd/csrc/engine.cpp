@@ -444,6 +444,25 @@ void Batch::plan(bool want_sdbg, bool all, std::vector<std::pair<void**, size_t>
         add(&d_dip_off, std::max<size_t>(list_dip.size(), 1) * sizeof(long long));
         add(&d_dippart, std::max<long long>(dip_size, 1) * sizeof(double));
     }
+    if (any_forces || all) {
+        add(&d_grad, std::max<long long>(nslots, 1) * 4 * sizeof(double));
+        add(&d_forces, std::max<size_t>(atoms.size(), 1) * 3 * sizeof(double));
+    }
+}
+
+void Batch::set_force_jobs(const std::vector<int>& jj) {
+    for (JobDev& jb : jobs) jb.want_f = 0;
+    for (int j : jj) jobs[j].want_f = 1;
+    any_forces = !jj.empty();
+}
+
+void Batch::launch_properties(Engine* e, hipStream_t st) {
+    if (!list_dip.empty())
+        launch_dipole(st, d_jobs, d_state, d_list_dip, d_dip_off, (int)list_dip.size(), maxn_dip, d_atoms, e->d_elems,
+                      d_arena, d_dippart);
+    if (any_forces)
+        launch_forces(st, d_pairs, nslots, d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_arena, d_grad,
+                      d_forces);
 }
 
 // The jobs (batch positions) whose caller wants the dipole moment.
@@ -871,7 +890,21 @@ int Batch::fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* co
     std::vector<double> dip(std::max<long long>(dip_size, 1));
     if (!list_dip.empty())
         HIPCHK(hipMemcpyAsync(dip.data(), d_dippart, dip_size * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> frc;
+    if (any_forces) {
+        frc.resize(3 * atoms.size());
+        HIPCHK(hipMemcpyAsync(frc.data(), d_forces, frc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     HIPCHK(hipStreamSynchronize(st));
+    for (size_t jj = 0; jj < nj && any_forces; ++jj) {
+        nddo_result* r = res[jj];
+        if (!r->forces || !jobs[jj].want_f) continue;
+        const int na = jobs[jj].natoms;
+        if (r->status == NDDO_OK)
+            std::memcpy(r->forces, frc.data() + 3LL * jobs[jj].atom0, 3 * (size_t)na * sizeof(double));
+        else
+            for (int k = 0; k < 3 * na; ++k) r->forces[k] = std::nan("");
+    }
     for (size_t k = 0; k < list_dip.size(); ++k) {
         const int jj = list_dip[k];
         nddo_result* r = res[jj];
@@ -1083,10 +1116,13 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     b.use_xl_householder = e->xl_householder;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao, true);
     {
-        std::vector<int> dj;
-        for (size_t jj = 0; jj < which.size(); ++jj)
+        std::vector<int> dj, fj;
+        for (size_t jj = 0; jj < which.size(); ++jj) {
             if (res[which[jj]].dipole) dj.push_back((int)jj);
+            if (res[which[jj]].forces) fj.push_back((int)jj);
+        }
         b.set_dipole_jobs(dj);
+        b.set_force_jobs(fj);
     }
     const size_t need = b.device_bytes(false);
     int rc = ensure_workspace(e, need);
@@ -1118,11 +1154,8 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     e->host_s[1] += lap(tp);
     rc = b.run_scf(e, e->stream);
     if (rc) return rc;
-    if (!b.list_dip.empty()) {
-        launch_dipole(e->stream, b.d_jobs, b.d_state, b.d_list_dip, b.d_dip_off, (int)b.list_dip.size(), b.maxn_dip,
-                      b.d_atoms, e->d_elems, b.d_arena, b.d_dippart);
-        HIPCHK(hipGetLastError());
-    }
+    b.launch_properties(e, e->stream);
+    HIPCHK(hipGetLastError());
     e->host_s[2] += lap(tp);
     const double wall = std::chrono::duration<double>(Clock::now() - t0).count();
     std::vector<nddo_result*> rp(which.size());
@@ -1184,6 +1217,7 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
     b.use_xl_householder = e->xl_householder;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao);
     if (res->dipole) b.set_dipole_jobs(std::vector<int>{0});
+    if (res->forces) b.set_force_jobs(std::vector<int>{0});
     const size_t need = b.device_bytes(false);
     if (need > kNoAllocScratchBytes || !b.list_xl.empty()) {
         set_error("nddo_scf_once_noalloc: the job needs " + std::to_string(need) +
@@ -1200,11 +1234,8 @@ int nddo_scf_once_noalloc(nddo_engine* eh, const nddo_job_desc* job, nddo_result
     if (rc) return rc;
     rc = b.run_scf(e, e->stream);
     if (rc) return rc;
-    if (!b.list_dip.empty()) {
-        launch_dipole(e->stream, b.d_jobs, b.d_state, b.d_list_dip, b.d_dip_off, 1, b.maxn_dip, b.d_atoms, e->d_elems,
-                      b.d_arena, b.d_dippart);
-        HIPCHK(hipGetLastError());
-    }
+    b.launch_properties(e, e->stream);
+    HIPCHK(hipGetLastError());
     double hof = 0.0;
     for (int a = 0; a < job->natoms; ++a) hof += atom_hof_offset(job->Z[a], job->theory);
     nddo_result* rp = res;

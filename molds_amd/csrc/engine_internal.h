@@ -102,6 +102,9 @@ struct Batch {
     int* d_list_dip = nullptr;
     long long* d_dip_off = nullptr;
     double* d_dippart = nullptr;
+    bool any_forces = false;
+    double* d_grad = nullptr;     // 4 doubles per pair slot: dE_AB/dR, E_AB
+    double* d_forces = nullptr;   // 3 doubles per atom
     hipStream_t alloc_stream = nullptr;   // set: stream-ordered allocation on this stream
     // set: every device buffer is carved out of [region, region + region_cap)
     // (the engine's batch workspace, or the fixed scratch array of
@@ -145,6 +148,11 @@ struct Batch {
     int run_scf(Engine* e, hipStream_t st);
     int fetch_all(hipStream_t st, const nddo_job_desc* descs, nddo_result* const* res, const double* hof_offsets);
     void set_dipole_jobs(const std::vector<int>& jj);
+    // Jobs (batch positions) whose caller wants forces: sets JobDev::want_f
+    // (before upload()).
+    void set_force_jobs(const std::vector<int>& jj);
+    // After run_scf: the dipole partials and the forces the jobs asked for.
+    void launch_properties(Engine* e, hipStream_t st);
 };
 
 // Total, electronic and core dipole moments of a job from k_dipole's row partials.

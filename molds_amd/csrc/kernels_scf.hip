@@ -39,9 +39,6 @@ __device__ inline double* mat(double* arena, const JobDev& jb, int kind) {
     return arena + jb.arena + (long long)kind * jb.nao * jb.nao;
 }
 
-__device__ inline long long tri_slot(int a, int b, int N) {   // a < b, job-local
-    return (long long)a * N - (long long)a * (a + 1) / 2 + (b - a - 1);
-}
 
 // One-centre Coulomb g and exchange h integrals (Mndo.cpp:583-639).
 __device__ inline double g1c(const ElemDev& E, int a, int b) {

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <math.h>
 
+#include "dual.h"
 #include "nddo_device.h"
 #include "launch.h"
 
@@ -66,8 +67,11 @@ __device__ int charges(int t, double D, Charge* c) {
     return 0;
 }
 
-__device__ __forceinline__ double multipole_pair(const ElemDev& A, const ElemDev& B, int ta, int tb, double r) {
-    if (parity_x(ta) != parity_x(tb) || parity_y(ta) != parity_y(tb)) return 0.0;
+// The distance-dependent functions below are templates on the type of r:
+// double for the integrals, D3 for their analytic derivatives (k_pair_grad).
+template <class T>
+__device__ __forceinline__ T multipole_pair(const ElemDev& A, const ElemDev& B, int ta, int tb, T r) {
+    if (parity_x(ta) != parity_x(tb) || parity_y(ta) != parity_y(tb)) return T(0.0);
     int ka = mp_kind(ta), kb = mp_kind(tb);
     double a = A.rho[ka] + B.rho[kb];
     double a2 = a * a;
@@ -77,20 +81,22 @@ __device__ __forceinline__ double multipole_pair(const ElemDev& A, const ElemDev
 #pragma unroll
     for (int i = 0; i < 4; ++i) ca[i] = cb[i] = Charge{0.0, 0.0, 0.0, 0.0};
     const int na = charges(ta, A.D[ka], ca), nb = charges(tb, B.D[kb], cb);
-    double v = 0.0;
+    T v = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (i >= na || j >= nb) continue;
-            double dx = cb[j].x - ca[i].x, dy = cb[j].y - ca[i].y, dz = r + cb[j].z - ca[i].z;
+            const double dx = cb[j].x - ca[i].x, dy = cb[j].y - ca[i].y;
+            const T dz = r + cb[j].z - ca[i].z;
             v += ca[i].q * cb[j].q / sqrt(dz * dz + dx * dx + dy * dy + a2);
         }
     return v;
 }
 
-__device__ __forceinline__ double local_integral_sum(const ElemDev& A, const ElemDev& B, int i, int j, double r) {
-    double v = 0.0;
+template <class T>
+__device__ __forceinline__ T local_integral_sum(const ElemDev& A, const ElemDev& B, int i, int j, T r) {
+    T v = 0.0;
     for (int u = 0; u < 2; ++u) {
         int ta = c_dist[i][u];
         if (ta < 0) continue;
@@ -104,11 +110,12 @@ __device__ __forceinline__ double local_integral_sum(const ElemDev& A, const Ele
 }
 
 // Diatomic-frame (mu nu|lam sig) for packed indices i (on A), j (on B).
-__device__ __forceinline__ double local_integral(const ElemDev& A, const ElemDev& B, int i, int j, double r) {
+template <class T>
+__device__ __forceinline__ T local_integral(const ElemDev& A, const ElemDev& B, int i, int j, T r) {
     if (i == 6 || j == 6) {
         if (i == 6 && j == 6)   // (px py|px py) = 1/2[(px px|px px) - (px px|py py)]
             return 0.5 * (local_integral_sum(A, B, 9, 9, r) - local_integral_sum(A, B, 9, 4, r));
-        return 0.0;
+        return T(0.0);
     }
     return local_integral_sum(A, B, i, j, r);
 }
@@ -129,8 +136,10 @@ __device__ void rotation(double dx, double dy, double dz, double R[4][4]) {
 
 // A_k(p) = int_1^inf x^k e^{-px} dx and B_k(q) = int_-1^1 x^k e^{-qx} dx
 // (Pople-Beveridge B.22/B.23; src/cndo/Cndo2.cpp:6385-6447).
-__device__ double aux_A(int k, double p) {
-    double s = 0.0, fk = 1.0, pm = 1.0;
+template <class T>
+__device__ T aux_A(int k, T p) {
+    T s = 0.0, pm = 1.0;
+    double fk = 1.0;
     for (int i = 2; i <= k; ++i) fk *= i;
     for (int mu = 1; mu <= k + 1; ++mu) {
         pm *= p;
@@ -141,9 +150,11 @@ __device__ double aux_A(int k, double p) {
     return exp(-p) * fk * s;
 }
 
-__device__ double aux_B(int k, double q) {
-    if (fabs(q) > 0.0) {
-        double t1 = 0.0, t2 = 0.0, fk = 1.0, qm = 1.0;
+template <class T>
+__device__ T aux_B(int k, T q) {
+    if (fabs(val(q)) > 0.0) {
+        T t1 = 0.0, t2 = 0.0, qm = 1.0;
+        double fk = 1.0;
         for (int i = 2; i <= k; ++i) fk *= i;
         for (int mu = 1; mu <= k + 1; ++mu) {
             qm *= q;
@@ -156,46 +167,48 @@ __device__ double aux_B(int k, double q) {
         }
         return -exp(-q) * t1 - exp(q) * t2;
     }
-    return (k % 2 == 0) ? 2.0 / (1.0 + k) : 0.0;
+    return T((k % 2 == 0) ? 2.0 / (1.0 + k) : 0.0);
 }
 
-__device__ __forceinline__ double overlap_local(const ElemDev& A, const ElemDev& B, int la, int lb, int m, double r) {
+template <class T>
+__device__ __forceinline__ T overlap_local(const ElemDev& A, const ElemDev& B, int la, int lb, int m, T r) {
     double za = A.zeta[la], zb = B.zeta[lb];
     int na = A.nshell, nb = B.nshell;
     int cid = ((((na - 1) * 2 + la) * 2 + (nb - 1)) * 2 + lb) * 2 + m;
     const double* c = &c_ovl[cid * 25];
     // alpha = zeta_a R and beta = zeta_b R are rounded separately (no FMA), so
     // equal exponents give q == 0 exactly and take the B_k(0) closed form
-    const double al = __dmul_rn(za, r), be = __dmul_rn(zb, r);
-    double p = 0.5 * (al + be), q = 0.5 * (al - be);
-    double Ak[5], Bk[5];
+    const T al = mul_rn(za, r), be = mul_rn(zb, r);
+    const T p = 0.5 * (al + be), q = 0.5 * (al - be);
+    T Ak[5], Bk[5];
     for (int k = 0; k < 5; ++k) { Ak[k] = aux_A(k, p); Bk[k] = aux_B(k, q); }
-    double s = 0.0;
+    T s = 0.0;
     for (int i = 0; i < 5; ++i)
         for (int j = 0; j < 5; ++j)
             if (c[i * 5 + j] != 0.0) s += c[i * 5 + j] * Ak[i] * Bk[j];
     double fa = 1.0, fb = 1.0;
     for (int i = 2; i <= 2 * na; ++i) fa *= i;
     for (int i = 2; i <= 2 * nb; ++i) fb *= i;
-    double pre = pow(2.0 * za, na + 0.5) * pow(2.0 * zb, nb + 0.5) / sqrt(fa * fb);
-    pre *= pow(r / 2.0, na + nb + 1.0);
-    return pre * s;
+    const double pre = pow(2.0 * za, na + 0.5) * pow(2.0 * zb, nb + 0.5) / sqrt(fa * fb);
+    const T prer = pre * pow(r / 2.0, na + nb + 1.0);
+    return prer * s;
 }
 
 // Pair core repulsion (Mndo.cpp:223-240, 319-327; Am1.cpp:145-170) from the
 // rotated (ss|ss) integral gss.
-__device__ __forceinline__ double pair_core_repulsion(const ElemDev& A, const ElemDev& B, double r, double gss, int theory) {
+template <class T>
+__device__ __forceinline__ T pair_core_repulsion(const ElemDev& A, const ElemDev& B, T r, T gss, int theory) {
     const double ang = 1.0 / 0.5291772;
-    double f;
+    T f;
     if (A.Z == 1 && (B.Z == 7 || B.Z == 8))
         f = 1.0 + (r / ang) * exp(-B.alpha * r) + exp(-A.alpha * r);
     else if (B.Z == 1 && (A.Z == 7 || A.Z == 8))
         f = 1.0 + (r / ang) * exp(-A.alpha * r) + exp(-B.alpha * r);
     else
         f = 1.0 + exp(-A.alpha * r) + exp(-B.alpha * r);
-    double e = A.core * B.core * gss * f;
+    T e = A.core * B.core * gss * f;
     if (theory != 0) {
-        double t = 0.0;
+        T t = 0.0;
         for (int k = 0; k < 4; ++k) {
             t += A.K[k] * exp(-A.L[k] * (r - A.M[k]) * (r - A.M[k]));
             t += B.K[k] * exp(-B.L[k] * (r - B.M[k]) * (r - B.M[k]));
@@ -421,10 +434,6 @@ __global__ void __launch_bounds__(256) k_setup_light(
     ecore[pr.slot] = pair_core_repulsion(A, B, r, I[0], jb.theory);
 }
 
-__device__ inline long long tri_slot(int a, int b, int N) {   // a < b, job-local
-    return (long long)a * N - (long long)a * (a + 1) / 2 + (b - a - 1);
-}
-
 // Core Hamiltonian diagonal blocks: H_AA = diag(U) + sum_B V_AB  (one wavefront per atom,
 // deterministic fixed-order reduction over B).
 __global__ void __launch_bounds__(256) k_atom_hcore(
@@ -544,6 +553,226 @@ void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long lon
     if (nints <= 0) return;
     const long long blocks = std::min<long long>((nints + 255) / 256, 65536);
     hipLaunchKernelGGL(k_ints_f32, dim3((unsigned)blocks), dim3(256), 0, st, ints, ints32, nints);
+}
+
+// ---------------------------------------------------------------------------
+// Analytic SCF forces.  At convergence the SCF energy is stationary in the
+// density, and the NDDO secular equation has no overlap matrix, so the force
+// is the gradient of the energy at fixed P (no density response and no
+// energy-weighted-density term).  The geometry enters only the two-centre
+// terms of each pair A < B, through R = R_B - R_A:
+//   E_AB = 2 sum_{mu on A, nu on B} P_mu,nu beta_bar_mu,nu S_mu,nu
+//        + sum_{i on A} w_i P_A(i) (-Z_B) (i|s_B s_B) + sum_{j on B} w_j P_B(j) (-Z_A) (s_A s_A|j)
+//        + sum_{i on A, j on B} [w_i w_j P_A(i) P_B(j) - 1/2 K_ij] (i|j)
+//        + E_core(A, B),
+// with i, j packed AO pairs, w = 1 for mu = nu and 2 otherwise, and K_ij the
+// sum over the orderings (a, b) of i and (c, d) of j of P_ac P_bd (exchange).
+// k_pair_grad evaluates E_AB with the integral, overlap and core-repulsion
+// functions above in D3 dual numbers seeded on R, so dE_AB/dR is exact to
+// rounding; k_atom_forces sums F_A = -dE/dR_A over the pairs in a fixed order.
+// Reference: Mndo::CalcForce (src/mndo/Mndo.cpp:2711-2986), which builds the
+// same gradient from derivative integrals (GetNddoRepulsionIntegral1stDerivative,
+// :5245-5661) and the overlap and rotation derivatives of Cndo2.
+
+// Rotation R[space][local] for the gradient: local z along A->B, local x the
+// normalized part of a fixed space axis e orthogonal to z (e the axis of the
+// smallest |z_c|, picked from the values and then held fixed, so the frame is
+// a smooth function of R around the current geometry), local y = z x x.  The
+// local-frame integrals and overlaps are symmetric under rotations about z (the
+// (px py|px py) identity above), so every space-frame quantity equals that of
+// the setup kernels' Euler-angle frame, whose azimuth is singular for R along
+// the space z axis.
+__device__ void rotation_frame(const D3 u[3], D3 R[4][4]) {
+    const int sp[3] = {ORB_PX, ORB_PY, ORB_PZ};   // space x, y, z -> AO slot
+    int c0 = 0;
+    for (int c = 1; c < 3; ++c)
+        if (fabs(u[c].v) < fabs(u[c0].v)) c0 = c;
+    D3 x[3];
+    for (int c = 0; c < 3; ++c) x[c] = (c == c0 ? 1.0 : 0.0) - u[c0] * u[c];   // e - (e.u) u
+    const D3 nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    for (int c = 0; c < 3; ++c) x[c] = x[c] / nx;
+    const D3 y[3] = {u[1] * x[2] - u[2] * x[1], u[2] * x[0] - u[0] * x[2], u[0] * x[1] - u[1] * x[0]};
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) R[i][j] = D3(0.0);
+    R[ORB_S][ORB_S] = D3(1.0);
+    for (int c = 0; c < 3; ++c) {
+        R[sp[c]][ORB_PZ] = u[c];
+        R[sp[c]][ORB_PX] = x[c];
+        R[sp[c]][ORB_PY] = y[c];
+    }
+}
+
+__device__ inline void wave_sum4(double& a, double& b, double& c, double& d) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
+        d += __shfl_xor(d, off);
+    }
+}
+
+// One wavefront per pair slot of the batch; pairs of jobs without want_f are
+// skipped.  grad[4 slot + 0..2] = dE_AB/dR, grad[4 slot + 3] = E_AB.
+__global__ void __launch_bounds__(256) k_pair_grad(const PairDev* __restrict__ pairs, long long nslots,
+                                                   const AtomDev* __restrict__ atoms,
+                                                   const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs,
+                                                   const JobState* __restrict__ state,
+                                                   const double* __restrict__ arena, double* __restrict__ grad) {
+    // D3 has constructors, so its LDS arrays are declared as plain storage
+    __shared__ double sLraw[4][400], sTraw[4][400], sXraw[4][400], sRraw[4][64], sSlraw[4][64];
+    __shared__ double sG[4][100], sP[4][3][16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long slot = (long long)blockIdx.x * 4 + wave;
+    const PairDev pr = pairs[slot < nslots ? slot : 0];
+    const JobDev jb = jobs[pr.job];
+    const bool live = slot < nslots && jb.want_f;
+    const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
+    const ElemDev& A = elems[aA.elem];
+    const ElemDev& B = elems[aB.elem];
+    const int na = A.nao, nb = B.nao, nap = na == 4 ? 10 : 1, nbp = nb == 4 ? 10 : 1;
+    const int n = jb.nao;
+    // the converged density (the one the energy was evaluated with)
+    const double* P = arena + jb.arena + (long long)(AR_P0 + ((state[pr.job].iter + 1) & 1)) * n * n;
+    double* PAA = sP[wave][0];
+    double* PBB = sP[wave][1];
+    double* PAB = sP[wave][2];
+    if (lane < 16) {
+        const int i = lane >> 2, j = lane & 3;
+        PAA[lane] = live && i < na && j < na ? P[(long long)(aA.ao + i) * n + aA.ao + j] : 0.0;
+        PBB[lane] = live && i < nb && j < nb ? P[(long long)(aB.ao + i) * n + aB.ao + j] : 0.0;
+        PAB[lane] = live && i < na && j < nb ? P[(long long)(aA.ao + i) * n + aB.ao + j] : 0.0;
+    }
+    const D3 dx(aB.x - aA.x, 1.0, 0.0, 0.0), dy(aB.y - aA.y, 0.0, 1.0, 0.0), dz(aB.z - aA.z, 0.0, 0.0, 1.0);
+    const D3 r = sqrt(dx * dx + dy * dy + dz * dz);
+    D3* L = reinterpret_cast<D3*>(sLraw[wave]);
+    D3* T = reinterpret_cast<D3*>(sTraw[wave]);
+    D3* X = reinterpret_cast<D3*>(sXraw[wave]);
+    D3* Rw = reinterpret_cast<D3*>(sRraw[wave]);
+    D3* Slw = reinterpret_cast<D3*>(sSlraw[wave]);
+    double* G = sG[wave];
+    if (live)
+        for (int e = lane; e < nap * nbp; e += 64) L[e] = local_integral(A, B, e / nbp, e % nbp, r);
+    if (lane == 0) {
+        const D3 u[3] = {dx / r, dy / r, dz / r};
+        rotation_frame(u, reinterpret_cast<D3(*)[4]>(Rw));
+    }
+    __syncthreads();
+    // weights of the rotated integrals (i|j): Coulomb, exchange, electron-core
+    for (int e = lane; e < nap * nbp; e += 64) {
+        const int i = e / nbp, j = e % nbp;
+        const int mu = nap == 1 ? 0 : c_pk[i][0], nu = nap == 1 ? 0 : c_pk[i][1];
+        const int la = nbp == 1 ? 0 : c_pk[j][0], si = nbp == 1 ? 0 : c_pk[j][1];
+        const double wa = mu == nu ? 1.0 : 2.0, wb = la == si ? 1.0 : 2.0;
+        const double pa = PAA[mu * 4 + nu], pb = PBB[la * 4 + si];
+        double k = PAB[mu * 4 + la] * PAB[nu * 4 + si];
+        if (la != si) k += PAB[mu * 4 + si] * PAB[nu * 4 + la];
+        if (mu != nu) {
+            k += PAB[nu * 4 + la] * PAB[mu * 4 + si];
+            if (la != si) k += PAB[nu * 4 + si] * PAB[mu * 4 + la];
+        }
+        double g = wa * wb * pa * pb - 0.5 * k;
+        if (j == 0) g -= B.core * wa * pa;
+        if (i == 0) g -= A.core * wb * pb;
+        G[e] = g;
+    }
+    for (int e = lane; e < 100; e += 64) {
+        const int i = e / 10, k = e % 10;
+        const int mu = c_pk[i][0], nu = c_pk[i][1], al = c_pk[k][0], be = c_pk[k][1];
+        const D3* R = Rw;
+        D3 t = R[mu * 4 + al] * R[nu * 4 + be];
+        if (al != be) t += R[mu * 4 + be] * R[nu * 4 + al];
+        T[e] = t;
+    }
+    // overlap in the diatomic frame (m-matched pairs), as k_pair_setup
+    if (lane < 16) {
+        const int i = lane >> 2, j = lane & 3;
+        D3 v = 0.0;
+        if (live && i < na && j < nb) {
+            const int la = i == ORB_S ? 0 : 1, lb = j == ORB_S ? 0 : 1;
+            const int ma = i == ORB_PX ? 1 : (i == ORB_PY ? -1 : 0);
+            const int mb = j == ORB_PX ? 1 : (j == ORB_PY ? -1 : 0);
+            if (ma == mb) v = overlap_local(A, B, la, lb, ma != 0 ? 1 : 0, r);
+        }
+        Slw[lane] = v;
+    }
+    __syncthreads();
+    // X = T_A L
+    for (int e = lane; e < nap * nbp; e += 64) {
+        const int i = e / nbp, j = e % nbp;
+        D3 s = 0.0;
+        if (nap == 1) s = L[e];
+        else for (int k = 0; k < 10; ++k) s += T[i * 10 + k] * L[k * nbp + j];
+        X[e] = s;
+    }
+    __syncthreads();
+    D3 acc = 0.0;
+    // (i|j) = (X T_B^T)_ij against its weight
+    for (int e = lane; e < nap * nbp; e += 64) {
+        const int i = e / nbp, j = e % nbp;
+        D3 s = 0.0;
+        if (nbp == 1) s = X[e];
+        else for (int l = 0; l < 10; ++l) s += X[i * nbp + l] * T[j * 10 + l];
+        acc += G[e] * s;
+    }
+    // resonance: 2 P_AB beta_bar S with S = R S_loc R^T
+    if (lane < 16) {
+        const int i = lane >> 2, j = lane & 3;
+        if (i < na && j < nb) {
+            const D3* R = Rw;
+            const D3* Sl = Slw;
+            D3 sv = 0.0;
+            for (int k = 0; k < 4; ++k)
+                for (int l = 0; l < 4; ++l) sv += R[i * 4 + k] * Sl[k * 4 + l] * R[j * 4 + l];
+            const double bb = 0.5 * (A.beta[i == ORB_S ? 0 : 1] + B.beta[j == ORB_S ? 0 : 1]);
+            acc += (2.0 * bb * PAB[i * 4 + j]) * sv;
+        }
+    }
+    if (lane == 63 && live) acc += pair_core_repulsion(A, B, r, L[0], jb.theory);
+    double g0 = acc.d[0], g1 = acc.d[1], g2 = acc.d[2], ev = acc.v;
+    wave_sum4(g0, g1, g2, ev);
+    if (live && lane == 0) {
+        double* o = grad + 4 * slot;
+        o[0] = g0;
+        o[1] = g1;
+        o[2] = g2;
+        o[3] = ev;
+    }
+}
+
+// F_A = sum_B -dE/dR_A: +dE_AB/dR for the pairs (A, B), -dE_BA/dR for (B, A),
+// B in order.  forces[3 g + c] for every atom g of a job with want_f.
+__global__ void __launch_bounds__(256) k_atom_forces(const AtomDev* __restrict__ atoms, int natoms,
+                                                     const JobDev* __restrict__ jobs, const double* __restrict__ grad,
+                                                     double* __restrict__ forces) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= natoms) return;
+    const AtomDev at = atoms[g];
+    const JobDev jb = jobs[at.job];
+    if (!jb.want_f) return;
+    const int A = at.local, N = jb.natoms;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int b = 0; b < N; ++b) {
+        if (b == A) continue;
+        const double* gp = grad + 4 * (jb.pair0 + (A < b ? tri_slot(A, b, N) : tri_slot(b, A, N)));
+        const double sg = A < b ? 1.0 : -1.0;
+        f0 += sg * gp[0];
+        f1 += sg * gp[1];
+        f2 += sg * gp[2];
+    }
+    forces[3LL * g + 0] = f0;
+    forces[3LL * g + 1] = f1;
+    forces[3LL * g + 2] = f2;
+}
+
+void launch_forces(hipStream_t st, const PairDev* pairs, long long nslots, const AtomDev* atoms, int natoms,
+                   const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* arena, double* grad,
+                   double* forces) {
+    if (nslots > 0)
+        hipLaunchKernelGGL(k_pair_grad, dim3((unsigned)((nslots + 3) / 4)), dim3(256), 0, st, pairs, nslots, atoms,
+                           elems, jobs, state, arena, grad);
+    if (natoms > 0)
+        hipLaunchKernelGGL(k_atom_forces, dim3((natoms + 255) / 256), dim3(256), 0, st, atoms, natoms, jobs, grad,
+                           forces);
 }
 
 int upload_overlap_table(const double* table /* 32*25 */) {

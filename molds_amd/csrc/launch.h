@@ -85,6 +85,14 @@ void launch_apply(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev
 void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
                    int* remaining, const double* xlpart, int nchunk);
 
+// Analytic forces of the converged jobs with want_f (kernels_setup.hip): per
+// pair slot grad[4 slot + 0..2] = dE_AB/d(R_B - R_A) and grad[4 slot + 3] =
+// E_AB (the pair's two-centre energy), then forces[3 g + c] = -dE/dx_gc for
+// every atom g of those jobs (Hartree/bohr).
+void launch_forces(hipStream_t st, const PairDev* pairs, long long nslots, const AtomDev* atoms, int natoms,
+                   const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* arena, double* grad,
+                   double* forces);
+
 // Per-iteration readback of small device records (the jobs' scalar state, the
 // batch's remaining count) into pinned host memory by a kernel on stream st.
 // A copy-engine transfer (hipMemcpyAsync to the host) would queue behind the

@@ -7,6 +7,7 @@
 //   arena  : per job (ARENA_FIXED + 2k) * n^2 doubles, see ArenaKind
 // Matrices are row-major n x n with leading dimension n.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #define NDDO_MAX_DIIS 16
@@ -88,6 +89,7 @@ struct JobDev {
     int xl;               // 1 + position in the batch's large-molecule list (0: not large)
     int want_c;           // 1: back-transform every eigenvector; 0: only the nocc occupied ones
                           // (all the density needs; the caller asked for no C)
+    int want_f;           // 1: analytic forces of the converged density (k_pair_grad)
     double rms_thresh, damp_thresh, damp_weight, diis_start, diis_end;
     long long traj0;      // offset (doubles) into the trajectory buffer: max_iter * 4
 };
@@ -103,6 +105,11 @@ struct JobState {
 };
 
 struct TileDev { int job, i0, j0, pad; };
+
+// Slot of pair (a, b), a < b job-local, in the job's pair triangle.
+__host__ __device__ inline long long tri_slot(int a, int b, int N) {
+    return (long long)a * N - (long long)a * (a + 1) / 2 + (b - a - 1);
+}
 
 // DIIS partial records: NDDO_MAX_DIIS + 1 doubles (the error dot products in
 // logical ring order, then max |e|) per density tile (k_density) or, for large

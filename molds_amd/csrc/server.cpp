@@ -656,6 +656,7 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
     double* tr = res->trajectory;
     double* q = res->mulliken;
     double* dip = res->dipole;
+    double* frc = res->forces;
     *res = j->res;
     res->eps = eps;
     res->C = C;
@@ -663,6 +664,7 @@ int nddo_get_result(nddo_engine* eh, nddo_job_id id, nddo_result* res) {
     res->trajectory = tr;
     res->mulliken = q;
     res->dipole = dip;
+    res->forces = frc;
     const size_t n = (size_t)std::max(j->res.nao, 0);
     if (j->on_host) {
         if (eps) std::memcpy(eps, j->eps.data(), n * sizeof(double));

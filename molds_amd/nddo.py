@@ -46,7 +46,7 @@ class Result(ctypes.Structure):
                 ("energy_elec", ctypes.c_double), ("energy_core", ctypes.c_double),
                 ("heat_of_formation", ctypes.c_double), ("final_rms", ctypes.c_double),
                 ("latency_s", ctypes.c_double), ("eps", _dp), ("C", _dp), ("P", _dp),
-                ("trajectory", _dp), ("mulliken", _dp), ("dipole", _dp)]
+                ("trajectory", _dp), ("mulliken", _dp), ("dipole", _dp), ("forces", _dp)]
 
 
 class EngineOpts(ctypes.Structure):
@@ -189,7 +189,8 @@ class Engine:
         self.close()
 
     # --------------------------------------------------------------- SCF
-    def _result_buffers(self, job, want_C, want_P, want_traj, want_mulliken=False, r=None, want_dipole=False):
+    def _result_buffers(self, job, want_C, want_P, want_traj, want_mulliken=False, r=None, want_dipole=False,
+                        want_forces=False):
         """Caller buffers for one result; fills r (a Result, possibly an element
         of a results array) and leaves the pointers of unwanted arrays NULL."""
         n = job.nao
@@ -213,6 +214,9 @@ class Engine:
         if want_dipole:
             bufs["dipole"] = np.full(9, np.nan)
             r.dipole = bufs["dipole"].ctypes.data_as(_dp)
+        if want_forces:
+            bufs["forces"] = np.full((len(job.Z), 3), np.nan)
+            r.forces = bufs["forces"].ctypes.data_as(_dp)
         return r, bufs
 
     @staticmethod
@@ -228,6 +232,8 @@ class Engine:
         if "dipole" in bufs:
             d = bufs["dipole"]
             out["dipole_total"], out["dipole_electronic"], out["dipole_core"] = d[0:3], d[3:6], d[6:9]
+        if "forces" in bufs:
+            out["forces"] = bufs["forces"]
         if "traj" in bufs:
             it = max(r.iterations, 0)
             t = bufs["traj"][:it]
@@ -235,17 +241,20 @@ class Engine:
         return out
 
     def scf(self, theory, Z, xyz_bohr, precision="fp64", requires_guess=True, P0=None, want_C=True, want_P=True,
-            want_traj=True, want_mulliken=True, want_dipole=False, **opts):
+            want_traj=True, want_mulliken=True, want_dipole=False, want_forces=False, **opts):
         return self.scf_batch([Job(theory, Z, xyz_bohr, precision, requires_guess, P0, **opts)],
                               want_C=want_C, want_P=want_P, want_traj=want_traj, want_mulliken=want_mulliken,
-                              want_dipole=want_dipole)[0]
+                              want_dipole=want_dipole, want_forces=want_forces)[0]
 
-    def scf_batch(self, jobs, want_C=False, want_P=False, want_traj=False, want_mulliken=False, want_dipole=False):
-        """All jobs share one launch per SCF stage.  Returns per-job results."""
+    def scf_batch(self, jobs, want_C=False, want_P=False, want_traj=False, want_mulliken=False, want_dipole=False,
+                  want_forces=False):
+        """All jobs share one launch per SCF stage.  Returns per-job results
+        (with want_forces, "forces": natoms x 3, Hartree/bohr, analytic)."""
         descs = (JobDesc * len(jobs))(*[j.desc for j in jobs])
         results = (Result * len(jobs))()
-        if want_C or want_P or want_traj or want_mulliken or want_dipole:
-            keep = [self._result_buffers(j, want_C, want_P, want_traj, want_mulliken, results[i], want_dipole)[1]
+        if want_C or want_P or want_traj or want_mulliken or want_dipole or want_forces:
+            keep = [self._result_buffers(j, want_C, want_P, want_traj, want_mulliken, results[i], want_dipole,
+                                         want_forces)[1]
                     for i, j in enumerate(jobs)]
         else:
             # only the MO energies: one array for all jobs, its addresses written
@@ -262,9 +271,9 @@ class Engine:
             raise NddoError(rc, last_error())
         return [self._pack(results[i], keep[i], jobs[i]) for i in range(len(jobs))]
 
-    def scf_once_noalloc(self, job, want_C=True, want_P=True, want_traj=True, want_mulliken=True):
+    def scf_once_noalloc(self, job, want_C=True, want_P=True, want_traj=True, want_mulliken=True, want_forces=False):
         """nddo_scf_once_noalloc: one solve on the library's fixed device scratch."""
-        r, bufs = self._result_buffers(job, want_C, want_P, want_traj, want_mulliken)
+        r, bufs = self._result_buffers(job, want_C, want_P, want_traj, want_mulliken, want_forces=want_forces)
         rc = self.L.nddo_scf_once_noalloc(self.h, ctypes.byref(job.desc), ctypes.byref(r))
         if rc < 0 and rc not in (-1, -5):
             raise NddoError(rc, last_error())

@@ -111,3 +111,63 @@ def test_gpu_forces_and_md_step(engine, case):
     assert np.max(np.abs(F0 - F0o)) <= 1e-8
     x1, p1, F1, E1 = FM.md_step(forces_of, Z, x0, np.zeros_like(x0), F0, case["dt_fs"] * FM.FS2AU)
     _check_step(case, x0, x1, p1)
+
+
+def _pair_energy_sum(theory, Z, xyz, P):
+    """Sum over pairs A < B of the two-centre energy the GPU force kernel
+    differentiates (resonance 2 P beta_bar S, electron-core, Coulomb P_AA P_BB
+    (mu nu|lam sig), exchange -1/2 P_AB P_AB (mu nu|lam sig), core repulsion),
+    from the oracle's space-frame integrals and overlap."""
+    import ctypes
+    from oracle import scf as O
+    m = O.Molecule(theory, Z, xyz)
+    g = m.integrals()
+    S = m.overlap()
+    el = {}
+    for z in set(Z):
+        v = np.zeros(33)
+        O.lib().orc_elem_dump(int(z), O.THEORIES[theory], v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        el[z] = v
+    nao = [1 if z == 1 else 4 for z in Z]
+    off = np.concatenate(([0], np.cumsum(nao)))
+    beta = np.concatenate([[el[z][5]] if z == 1 else [el[z][5]] + [el[z][6]] * 3 for z in Z])
+    E = m.ecore
+    for A in range(len(Z)):
+        for B in range(A + 1, len(Z)):
+            a, b = slice(off[A], off[A + 1]), slice(off[B], off[B + 1])
+            na, nb = nao[A], nao[B]
+            I = g[A, B][:na, :na, :nb, :nb]
+            E += 2.0 * np.sum(P[a, b] * 0.5 * (beta[a][:, None] + beta[b][None, :]) * S[a, b])
+            E += np.einsum("mn,ls,mnls->", P[a, a], P[b, b], I)
+            E -= 0.5 * np.einsum("ml,ns,mnls->", P[a, b], P[a, b], I)
+            E -= el[Z[B]][0] * np.sum(P[a, a] * I[:, :, 0, 0]) + el[Z[A]][0] * np.sum(P[b, b] * I[0, 0, :, :])
+    return E
+
+
+@pytest.mark.parametrize("theory", ["mndo", "am1", "pm3"])
+def test_pair_energy_decomposition_and_frozen_density_gradient(theory):
+    """The analytic forces differentiate the pair energies at fixed density.
+    Pinned here on the oracle: (1) the energy at fixed P minus the pair sum is
+    the same at two geometries (the remainder is one-centre only), so the pair
+    terms and weights are complete; (2) the gradient of the energy at fixed P
+    equals the gradient of the SCF energy (stationarity), both by central
+    differences."""
+    from molds_amd.forces import stencil_forces
+    from oracle import scf as O
+    Z = [6, 7, 1, 1, 1, 1, 1] if theory != "mndo" else [8, 1, 1]
+    ang = 1.0 / 0.5291772
+    x0 = (np.array([[0.0, 0.0, 0.0], [1.47, 0.02, -0.01], [-0.37, 1.02, 0.03], [-0.36, -0.52, 0.89],
+                    [-0.38, -0.50, -0.88], [1.80, -0.93, 0.05], [1.79, 0.49, 0.83]]) if theory != "mndo" else
+          np.array([[0.0, 0.0, 0.0], [0.96, 0.05, 0.0], [-0.25, 0.93, 0.02]])) * ang
+    P = O.scf(theory, Z, x0, **_opts_oracle())["P"]
+
+    def frozen(x):
+        m = O.Molecule(theory, Z, x)
+        return m.elec_energy(P) + m.ecore
+    x1 = x0 + np.random.default_rng(3).normal(scale=0.05, size=x0.shape)
+    d0 = frozen(x0) - _pair_energy_sum(theory, Z, x0, P)
+    d1 = frozen(x1) - _pair_energy_sum(theory, Z, x1, P)
+    assert abs(d0 - d1) <= 1e-10 * abs(d0), (d0, d1)
+    Ff = stencil_forces(lambda gs: [frozen(x) for x in gs], x0, 1e-3)
+    Fs = _oracle_forces_of(theory, Z)(x0)[1]
+    assert np.max(np.abs(Ff - Fs)) <= 1e-8, np.max(np.abs(Ff - Fs))
