@@ -23,6 +23,9 @@ Beside it, in the same run:
   * the requests of the mixed 500/s served leg as one nddo_scf_batch, the
     batched rate of the same work;
   * (H2O)_256 AM1 (1536 AOs), one molecule, FP64 and mixed precision;
+    (H2O)_1000 PM3 (6000 AOs), seconds per SCF iteration;
+  * analytic ground-state forces of C_20H_42 and (H2O)_64 against the
+    central-difference stencil of 12 N displaced SCFs;
   * the plain FP64 GPU path: the same 4096-molecule mix solved one molecule at
     a time (nddo_scf_once), on a bounded prefix;
   * the CPU oracle on a bounded prefix of the same mix (cpu_baseline).
@@ -202,6 +205,32 @@ def plain_served_leg(engine, seconds, seed=3):
     return dict(solves_per_s=round(done / dt, 3),
                 sample=f"first {done} requests of the served mix ({SERVED_LO}..{SERVED_HI} atoms), one "
                        f"nddo_scf_once each, FP64 ({dt:.1f} s)")
+
+
+def forces_leg(engine):
+    """Ground-state forces of C_20H_42 (62 atoms, 122 AOs) and (H2O)_64 (192
+    atoms, 384 AOs), PM3: seconds per force evaluation (SCF to density RMS
+    1e-10 from the core-Hamiltonian guess, then the analytic forces), and the
+    same by the central-difference stencil of 12 N displaced SCFs in one
+    batch (warm-started), with the largest difference between the two."""
+    from molds_amd import workloads as W
+    from molds_amd.forces import scf_forces
+    out = {}
+    for name, (Z, x) in (("C20H42", W.alkane(20, seed=2)), ("(H2O)_64", W.water_cluster(64, seed=2, jitter=0.02))):
+        xyz = W.to_bohr(x)
+        opts = dict(diis_num_error_vect=5, diis_start_error=0.1, diis_end_error=0.0, damping_weight=0.7,
+                    damping_thresh=0.02, max_iter=200)
+        scf_forces(engine, "pm3", Z, xyz, **opts)   # warm-up
+        t0 = time.perf_counter()
+        a = scf_forces(engine, "pm3", Z, xyz, **opts)
+        ta = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s = scf_forces(engine, "pm3", Z, xyz, method="stencil", **opts)
+        ts = time.perf_counter() - t0
+        out[name] = dict(atoms=len(Z), analytic_s=round(ta, 4), stencil_s=round(ts, 3),
+                         stencil_scfs=12 * len(Z), speedup=round(ts / ta, 1),
+                         max_abs_diff_hartree_per_bohr=float(np.max(np.abs(a["forces"] - s["forces"]))))
+    return out
 
 
 def large_leg(engine):
@@ -467,7 +496,8 @@ def main():
     engine = Engine(0)
     if args.no_batched:   # profiling run of the single-molecule legs only
         out = {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "note": "single-molecule legs only",
-               "large_single": large_leg(engine), "water1000_pm3": water1000_leg(engine)}
+               "large_single": large_leg(engine), "water1000_pm3": water1000_leg(engine),
+               "forces_pm3": forces_leg(engine)}
         engine.close()
         print(json.dumps(out), flush=True)
         return
@@ -539,12 +569,14 @@ def main():
             served["mixed_as_one_batch"] = served_as_batch_leg(engine, args.latency_rate, count, "mixed")
             log(f"served mix as one batch: {served['mixed_as_one_batch']}")
         engine.trim()
-    large = water1000 = None
+    large = water1000 = forces = None
     if not args.no_large:
         large = large_leg(engine)
         log(f"large: {large}")
         water1000 = water1000_leg(engine)
         log(f"(H2O)_1000: {water1000}")
+        forces = forces_leg(engine)
+        log(f"forces: {forces}")
         engine.trim()
     plain = plain_served = None
     if not args.no_plain:
@@ -587,6 +619,7 @@ def main():
         "served": served or None,
         "large_single": large,
         "water1000_pm3": water1000,
+        "forces_pm3": forces,
         "plain_fp64_gpu": plain,
         "plain_fp64_gpu_served_mix": plain_served,
         "speedup_vs_plain_fp64": ({"batched": round(best["solves_per_s"] / plain["solves_per_s"], 2)}
