@@ -240,6 +240,43 @@ class Engine:
             out["traj"] = [(i, t[i, 0], t[i, 1], bool(t[i, 2]), bool(t[i, 3])) for i in range(len(t))]
         return out
 
+    # the scalar fields of a Result array read column by column (one numpy view)
+    _SCALARS = (("status", "status"), ("nao", "nao"), ("nocc", "nocc"), ("iterations", "iterations"),
+                ("fp64_iterations", "fp64_iterations"), ("energy", "energy_total"), ("e_el", "energy_elec"),
+                ("e_core", "energy_core"), ("hof", "heat_of_formation"), ("rms", "final_rms"),
+                ("latency_s", "latency_s"))
+
+    @classmethod
+    def _pack_all(cls, results, keep, jobs):
+        """_pack for a whole Result array: the scalars are read per field over
+        all jobs at once."""
+        nj = len(jobs)
+        if nj == 0:
+            return []
+        raw = np.frombuffer(results, dtype=np.uint8).reshape(nj, ctypes.sizeof(Result))
+        cols = {}
+        for key, field in cls._SCALARS:
+            f = getattr(Result, field)
+            dt = np.int32 if f.size == 4 else np.float64
+            cols[key] = np.ascontiguousarray(raw[:, f.offset:f.offset + f.size]).view(dt).reshape(nj).tolist()
+        out = [ScfResult(status=st, status_name=STATUS.get(st, str(st)), nao=na, nocc=no, iterations=it,
+                         fp64_iterations=i64, energy=et, e_el=ee, e_core=ec, hof=hf, rms=rm, latency_s=lt,
+                         eps=b["eps"], C=b.get("C"), P=b.get("P"), mulliken=b.get("mulliken"))
+               for st, na, no, it, i64, et, ee, ec, hf, rm, lt, b in
+               zip(cols["status"], cols["nao"], cols["nocc"], cols["iterations"], cols["fp64_iterations"],
+                   cols["energy"], cols["e_el"], cols["e_core"], cols["hof"], cols["rms"], cols["latency_s"], keep)]
+        for r, b in zip(out, keep):
+            if len(b) > 2:
+                if "dipole" in b:
+                    d = b["dipole"]
+                    r["dipole_total"], r["dipole_electronic"], r["dipole_core"] = d[0:3], d[3:6], d[6:9]
+                if "forces" in b:
+                    r["forces"] = b["forces"]
+                if "traj" in b:
+                    t = b["traj"][:max(r["iterations"], 0)]
+                    r["traj"] = [(k, t[k, 0], t[k, 1], bool(t[k, 2]), bool(t[k, 3])) for k in range(len(t))]
+        return out
+
     def scf(self, theory, Z, xyz_bohr, precision="fp64", requires_guess=True, P0=None, want_C=True, want_P=True,
             want_traj=True, want_mulliken=True, want_dipole=False, want_forces=False, **opts):
         return self.scf_batch([Job(theory, Z, xyz_bohr, precision, requires_guess, P0, **opts)],
@@ -269,7 +306,7 @@ class Engine:
         rc = self.L.nddo_scf_batch(self.h, len(jobs), descs, results)
         if rc != 0:
             raise NddoError(rc, last_error())
-        return [self._pack(results[i], keep[i], jobs[i]) for i in range(len(jobs))]
+        return self._pack_all(results, keep, jobs)
 
     def scf_once_noalloc(self, job, want_C=True, want_P=True, want_traj=True, want_mulliken=True, want_forces=False):
         """nddo_scf_once_noalloc: one solve on the library's fixed device scratch."""
