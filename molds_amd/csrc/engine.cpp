@@ -286,6 +286,11 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
     n_light = (int)(ncls[0] + ncls[1]);
     n_heavy = (int)ncls[2];
     const long long cbase[3] = {0, ncls[0], ncls[0] + ncls[1]};
+    for (int c = 0; c < 3; ++c) {
+        cls_pos[c].resize(nj + 1);
+        for (int jj = 0; jj < nj; ++jj) cls_pos[c][jj] = cbase[c] + off[jj].cls[c];
+        cls_pos[c][nj] = cbase[c] + ncls[c];
+    }
     auto fill = [&](int j0, int j1) {
         for (int jj = j0; jj < j1; ++jj) {
             const nddo_job_desc& d = descs[which[jj]];
@@ -357,6 +362,10 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
 // 8.8 s with the device-wide solver above 512).  Many such molecules fill the
 // GPU one workgroup each, which gives more solves/s.
 constexpr int kFewLarge = 64;
+bool lds_eig_class(int n, size_t lds_max) {
+    // the Jacobi kernels also hold up to 8 KB of static LDS (reduction buffers)
+    return n <= 512 && lds_max > 9216 && eig_lds_bytes(n, true) <= lds_max - 9216;
+}
 int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao, bool route_few_large) {
     list_lds.clear();
     list_glb.clear();
@@ -376,7 +385,7 @@ int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao, bool route_few_
         int n = jobs[j].nao;
         // the Jacobi kernels also hold up to 8 KB of static LDS (reduction buffers)
         jobs[j].xl = 0;
-        if (n <= 512 && lds_max > 9216 && eig_lds_bytes(n, true) <= lds_max - 9216) {
+        if (lds_eig_class(n, lds_max)) {
             list_lds.push_back(j);
             maxn_lds = std::max(maxn_lds, n);
         } else if (heig_supported(n) && n < xl_min_nao) {
@@ -392,6 +401,16 @@ int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao, bool route_few_
     }
     xl_chunks = xl_nchunk(maxn_xl);
     xl_householder = use_xl_householder && xlh_supported(maxn_xl);
+    split_job = 0;
+    if (split_classes && !list_lds.empty() && list_lds.size() < jobs.size()) {
+        bool prefix = true;   // the LDS-class jobs are exactly the first ones
+        for (int j : list_lds) prefix &= j < (int)list_lds.size();
+        if (prefix) {
+            split_job = (int)list_lds.size();
+            split_atom = jobs[split_job].atom0;
+            split_tile = jobs[split_job].tile0;
+        }
+    }
     xl_grid = 1;
     for (size_t g = 0; g < list_xl.size(); g += xl_max_jobs()) {
         std::vector<int> ns;
@@ -583,13 +602,69 @@ void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
     if (d_ints32) launch_ints_f32(st, d_ints, d_ints32, nints);
 }
 
+// The pairs of jobs [j0, j1) as pair-list ranges.
+FockPairs Batch::fock_pairs(int j0, int j1) const {
+    FockPairs fp;
+    if (jobs.empty() || j1 <= j0) return fp;
+    for (int c = 0; c < 2; ++c) {
+        fp.light[c] = d_plist + cls_pos[c][j0];
+        fp.nlight[c] = (int)(cls_pos[c][j1] - cls_pos[c][j0]);
+    }
+    fp.heavy = d_plist + cls_pos[2][j0];
+    fp.nheavy = (int)(cls_pos[2][j1] - cls_pos[2][j0]);
+    return fp;
+}
+
+bool Batch::split_ready() const { return split_job > 0 && fork_stream && aux_stream && aux_fork && aux_join; }
+
+// One iteration with the LDS-class jobs [0, split_job) on fork_stream and the
+// others on st, every stage restricted to its part's pairs, atoms, jobs and
+// tiles (a job's are contiguous), joined once at the end: neither part waits
+// for the other between stages.
+void Batch::launch_iteration_split(Engine* e, hipStream_t st) {
+    const int nj = (int)jobs.size(), js = split_job, nxl = (int)list_xl.size();
+    const int nt = (int)tiles.size(), na = (int)atoms.size();
+    hipStream_t s2 = fork_stream;
+    hipEventRecord(fork_ev, st);
+    hipStreamWaitEvent(s2, fork_ev, 0);
+    launch_fock(s2, 2, fock_pairs(0, js), d_pairs, d_atoms, 0, split_atom, e->d_elems, d_jobs, d_state, d_ints, d_ints32,
+                d_arena, d_part);
+    launch_energy(s2, d_jobs, js, d_state, d_arena, d_remaining, d_xlpart, xl_chunks, 0);
+    launch_eigen(s2, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri, true);
+    launch_density(s2, d_tiles, split_tile, d_jobs, d_state, d_arena, d_perm, d_rmspart, d_tpart);
+    launch_step(s2, d_jobs, js, d_state, d_rmspart, d_tpart, d_traj, d_remaining, d_xlpart, d_xlcoef, xl_chunks, 0);
+    launch_apply(s2, d_tiles, split_tile, d_jobs, d_arena, d_xlcoef);
+    hipEventRecord(join_ev, s2);
+
+    launch_fock(st, 2, fock_pairs(js, nj), d_pairs, d_atoms, split_atom, na - split_atom, e->d_elems, d_jobs, d_state,
+                d_ints, d_ints32, d_arena, d_part, aux_stream, aux_fork, aux_join);
+    launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
+    launch_energy(st, d_jobs, nj - js, d_state, d_arena, d_remaining, d_xlpart, xl_chunks, js);
+    if (nxl && !xl_householder)
+        launch_warm(st, d_tiles + split_tile, nt - split_tile, d_jobs, d_state, d_arena);
+    launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri, false);
+    launch_xl_eigen_all(e, st);
+    launch_density(st, d_tiles + split_tile, nt - split_tile, d_jobs, d_state, d_arena, d_perm, d_rmspart + split_tile,
+                   d_tpart + (long long)split_tile * XL_REC);
+    launch_xl_dots(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
+    launch_step(st, d_jobs, nj - js, d_state, d_rmspart, d_tpart, d_traj, d_remaining, d_xlpart, d_xlcoef, xl_chunks,
+                js);
+    launch_apply(st, d_tiles + split_tile, nt - split_tile, d_jobs, d_arena, d_xlcoef);
+    launch_xl_apply(st, d_jobs, d_list_xl, nxl, d_arena, d_xlcoef, xl_chunks);
+    hipStreamWaitEvent(st, join_ev, 0);
+}
+
 // One SCF iteration of every job: the Fock launch also rebuilds F from the final
 // density for jobs that converged in the previous iteration, whose energy is
 // then evaluated (Cndo2.cpp:1324-1416) before the eigensolve of the others.
 void Batch::launch_iteration(Engine* e, hipStream_t st) {
+    if (split_ready()) {
+        launch_iteration_split(e, st);
+        return;
+    }
     const int nxl = (int)list_xl.size();
-    launch_fock(st, 2, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state, d_ints, d_ints32,
-                d_arena, d_part, fork_stream, fork_ev2, join_ev2);
+    launch_fock(st, 2, fock_pairs(0, (int)jobs.size()), d_pairs, d_atoms, 0, (int)atoms.size(), e->d_elems, d_jobs, d_state,
+                d_ints, d_ints32, d_arena, d_part, fork_stream, fork_ev2, join_ev2);
     launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
     // the warm-start rotation is for the block-Jacobi (large) jobs only
@@ -697,8 +772,8 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             // the same launches as launch_iteration, with an event between stages
             // and between the parts of the eigensolver
             HIPCHK(hipEventRecord(e->ev[0], st));
-            launch_fock(st, 2, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs, d_state,
-                        d_ints, d_ints32, d_arena, d_part, fork_stream, fork_ev2, join_ev2);
+            launch_fock(st, 2, fock_pairs(0, (int)jobs.size()), d_pairs, d_atoms, 0, (int)atoms.size(), e->d_elems, d_jobs,
+                        d_state, d_ints, d_ints32, d_arena, d_part, fork_stream, fork_ev2, join_ev2);
             HIPCHK(hipEventRecord(e->ev[9], st));
             launch_xl_energy(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
             launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
@@ -1036,6 +1111,9 @@ int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine
     HIPCHK(hipEventCreateWithFlags(&e->eig_join, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&e->fock_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&e->fock_join, hipEventDisableTiming));
+    HIPCHK(hipStreamCreateWithFlags(&e->aux_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&e->aux_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->aux_join, hipEventDisableTiming));
     std::vector<ElemDev> el(3 * kNumElem);
     for (int t = 0; t < 3; ++t)
         for (int i = 0; i < kNumElem; ++i) elem_params(kElemZ[i], t, &el[t * kNumElem + i]);
@@ -1091,6 +1169,9 @@ int nddo_engine_destroy(nddo_engine* eh) {
     if (e->fock_fork) hipEventDestroy(e->fock_fork);
     if (e->fock_join) hipEventDestroy(e->fock_join);
     if (e->eig_stream) hipStreamDestroy(e->eig_stream);
+    if (e->aux_fork) hipEventDestroy(e->aux_fork);
+    if (e->aux_join) hipEventDestroy(e->aux_join);
+    if (e->aux_stream) hipStreamDestroy(e->aux_stream);
     delete e;
     return NDDO_OK;
 }
@@ -1114,6 +1195,7 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     // jobs whose caller takes no C back-transform only the occupied eigenvectors
     for (size_t jj = 0; jj < which.size(); ++jj) b.jobs[jj].want_c = res[which[jj]].C != nullptr ? 1 : 0;
     b.use_xl_householder = e->xl_householder;
+    b.split_classes = true;
     b.classify(e->lds_max, e->num_cus, e->xl_min_nao, true);
     {
         std::vector<int> dj, fj;
@@ -1145,6 +1227,9 @@ static int run_batch(Engine* e, const nddo_job_desc* descs, int njobs, nddo_resu
     b.join_ev = e->eig_join;
     b.fork_ev2 = e->fock_fork;
     b.join_ev2 = e->fock_join;
+    b.aux_stream = e->aux_stream;
+    b.aux_fork = e->aux_fork;
+    b.aux_join = e->aux_join;
     e->host_s[0] += lap(tp);
     rc = b.allocate(false);
     if (rc) return rc;
@@ -1178,15 +1263,17 @@ int nddo_scf_batch(nddo_engine* eh, int njobs, const nddo_job_desc* descs, nddo_
     auto t0 = std::chrono::steady_clock::now();
     SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
-    std::vector<int> ok;
+    std::vector<int> ok, rest;
     for (int i = 0; i < njobs; ++i) {
         int n = 0, no = 0;
         int s = validate_job(descs[i], &n, &no);
         res[i].status = s;
         res[i].nao = n;
         res[i].nocc = no;
-        if (s == NDDO_OK) ok.push_back(i);
+        // the jobs of the LDS eigensolver class first (the split iteration)
+        if (s == NDDO_OK) (lds_eig_class(n, e->lds_max) ? ok : rest).push_back(i);
     }
+    ok.insert(ok.end(), rest.begin(), rest.end());
     if (ok.empty()) return NDDO_OK;
     const int rc = run_batch(e, descs, njobs, res, ok, t0);
     for (int i : ok)
@@ -1440,8 +1527,8 @@ int nddo_dbg_fock(nddo_engine* eh, int theory, int natoms, const int* Z, const d
     }
     if (P) HIPCHK(hipMemcpy(b.d_arena + (long long)AR_P0 * nn, P, nn * sizeof(double), hipMemcpyHostToDevice));
     b.launch_setup_stage(e, e->stream);
-    launch_fock(e->stream, 0, b.d_pairs, b.d_plist, b.n_light, b.n_heavy, b.d_atoms, (int)b.atoms.size(), e->d_elems, b.d_jobs,
-                b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
+    launch_fock(e->stream, 0, b.fock_pairs(0, (int)b.jobs.size()), b.d_pairs, b.d_atoms, 0, (int)b.atoms.size(), e->d_elems,
+                b.d_jobs, b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(F, b.d_arena + (long long)AR_F * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
@@ -1575,11 +1662,12 @@ static int autotune(Engine* e) {
                 c.fock_wpb = cand[ci];
                 set_tuning(c);
                 float ms = 0.f;
+                FockPairs hp = b.fock_pairs(0, (int)b.jobs.size());
+                hp.nlight[0] = hp.nlight[1] = 0;
                 rc = time_ms(e, [&] {
                     for (int r = 0; r < 4; ++r)
-                        launch_fock(e->stream, 0, b.d_pairs, b.d_plist + b.n_light, 0, b.n_heavy, b.d_atoms,
-                                    (int)b.atoms.size(), e->d_elems, b.d_jobs, b.d_state, b.d_ints, b.d_ints32,
-                                    b.d_arena, b.d_part);
+                        launch_fock(e->stream, 0, hp, b.d_pairs, b.d_atoms, 0, (int)b.atoms.size(), e->d_elems,
+                                    b.d_jobs, b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
                 }, 3, &ms);
                 if (rc != NDDO_OK) return rc;
                 ms_of[ci] = std::min(ms_of[ci], ms);

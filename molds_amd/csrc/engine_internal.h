@@ -11,6 +11,7 @@
 #include <thread>
 #include <vector>
 
+#include "launch.h"
 #include "nddo.h"
 #include "nddo_device.h"
 
@@ -38,6 +39,8 @@ struct Engine;
 size_t device_available(int device);
 // Returns the default pool's unused reserve to the device.
 void trim_pool(int device);
+// Whether n AOs take the one-workgroup eigensolver with the matrix in LDS.
+bool lds_eig_class(int n, size_t lds_max);
 // Makes the engine's batch workspace at least `bytes` (caller holds run_mu);
 // NDDO_ADMISSION_DENIED when the device cannot provide it.
 int ensure_workspace(Engine* e, size_t bytes);
@@ -117,6 +120,20 @@ struct Batch {
     // and so do the light-pair and heavy-pair Fock kernels (fork_ev2, join_ev2)
     hipStream_t fork_stream = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr, fork_ev2 = nullptr, join_ev2 = nullptr;
+    // set with split_classes (synchronous batches whose jobs come with the LDS
+    // eigensolver class first): the whole iteration of those jobs runs on
+    // fork_stream beside the others' on the iteration's stream, joined once;
+    // the others' light-pair Fock kernels then fork to aux_stream
+    bool split_classes = false;
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t aux_fork = nullptr, aux_join = nullptr;
+    int split_job = 0, split_atom = 0, split_tile = 0;   // first job / atom / tile of the second part
+    // per job, the pair-list position of its first pair of each class (H-H,
+    // heavy-H, heavy-heavy); nj + 1 entries
+    std::vector<long long> cls_pos[3];
+    FockPairs fock_pairs(int j0, int j1) const;
+    bool split_ready() const;
+    void launch_iteration_split(Engine* e, hipStream_t st);
 
     ~Batch();
     int dalloc(void** p, size_t bytes);
@@ -176,6 +193,9 @@ struct Engine {
     // and join events of each)
     hipStream_t eig_stream = nullptr;
     hipEvent_t eig_fork = nullptr, eig_join = nullptr, fock_fork = nullptr, fock_join = nullptr;
+    // the third stream of a split iteration (the second part's light-pair Fock)
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t aux_fork = nullptr, aux_join = nullptr;
     ElemDev* d_elems = nullptr;
     size_t lds_max = 65536;
     int num_cus = 0;

@@ -412,25 +412,27 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
     }
 }
 
-void launch_fock(hipStream_t st, int mode, const PairDev* pairs, const int* plist, int nlight, int nheavy,
-                 const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state,
-                 const double* ints, const float* ints32, double* arena, double* part, hipStream_t st2,
-                 hipEvent_t fork_ev, hipEvent_t join_ev) {
+void launch_fock(hipStream_t st, int mode, const FockPairs& fp, const PairDev* pairs, const AtomDev* atoms, int atom0,
+                 int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
+                 const float* ints32, double* arena, double* part, hipStream_t st2, hipEvent_t fork_ev,
+                 hipEvent_t join_ev) {
     // the light-pair and heavy-pair kernels write disjoint F blocks and pair
     // slots: with a second stream they run concurrently, joined before the
     // per-atom reduction
-    const bool fork = st2 && nlight > 0 && nheavy > 0;
+    const int nl = fp.nlight[0] + fp.nlight[1];
+    const bool fork = st2 && nl > 0 && fp.nheavy > 0;
     if (fork) {
         hipEventRecord(fork_ev, st);
         hipStreamWaitEvent(st2, fork_ev, 0);
     }
-    if (nlight > 0)
-        hipLaunchKernelGGL(k_fock_light, dim3((nlight + 255) / 256), dim3(256), 0, fork ? st2 : st, mode, pairs, plist,
-                           nlight, atoms, elems, jobs, state, ints, ints32, arena, part);
+    for (int c = 0; c < 2; ++c)
+        if (fp.nlight[c] > 0)
+            hipLaunchKernelGGL(k_fock_light, dim3((fp.nlight[c] + 255) / 256), dim3(256), 0, fork ? st2 : st, mode,
+                               pairs, fp.light[c], fp.nlight[c], atoms, elems, jobs, state, ints, ints32, arena, part);
     if (fork) hipEventRecord(join_ev, st2);
-    if (nheavy > 0) {
-        const int npairs = nheavy;
-        const int* hl = plist + nlight;
+    if (fp.nheavy > 0) {
+        const int npairs = fp.nheavy;
+        const int* hl = fp.heavy;
         const int w = get_tuning().fock_wpb;
 #define NDDO_FOCK_PAIRS(W)                                                                                       \
     hipLaunchKernelGGL(k_fock_pairs<W>, dim3((npairs + (W)-1) / (W)), dim3(64 * (W)), 0, st, mode, pairs, hl, npairs, \
@@ -442,8 +444,9 @@ void launch_fock(hipStream_t st, int mode, const PairDev* pairs, const int* plis
 #undef NDDO_FOCK_PAIRS
     }
     if (fork) hipStreamWaitEvent(st, join_ev, 0);
-    hipLaunchKernelGGL(k_fock_atoms, dim3((natoms + 3) / 4), dim3(256), 0, st, mode, atoms, natoms, elems, jobs,
-                       state, arena, part);
+    if (natoms > 0)
+        hipLaunchKernelGGL(k_fock_atoms, dim3((natoms + 3) / 4), dim3(256), 0, st, mode, atoms + atom0, natoms, elems,
+                           jobs, state, arena, part);
 }
 
 // ------------------------------------------------------------------ eigensolver dispatch
@@ -846,8 +849,8 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
                                               const double* __restrict__ rmspart, const double* __restrict__ tpart,
                                               double* __restrict__ traj, int* __restrict__ remaining,
                                               const double* __restrict__ xlpart, double* __restrict__ xlcoef,
-                                              int nchunk) {
-    const int job = blockIdx.x;
+                                              int nchunk, int job0) {
+    const int job = job0 + (int)blockIdx.x;
     JobState* st = &state[job];
     const JobDev jb = jobs[job];
     const int tid = threadIdx.x, T = blockDim.x;
@@ -999,9 +1002,11 @@ __global__ void __launch_bounds__(256) k_step(const JobDev* __restrict__ jobs, J
 }
 
 void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, const double* rmspart,
-                 const double* tpart, double* traj, int* remaining, const double* xlpart, double* xlcoef, int nchunk) {
-    hipLaunchKernelGGL(k_step, dim3(njobs), dim3(256), 0, st, jobs, state, rmspart, tpart, traj, remaining, xlpart,
-                       xlcoef, nchunk);
+                 const double* tpart, double* traj, int* remaining, const double* xlpart, double* xlcoef, int nchunk,
+                 int job0) {
+    if (njobs > 0)
+        hipLaunchKernelGGL(k_step, dim3(njobs), dim3(256), 0, st, jobs, state, rmspart, tpart, traj, remaining, xlpart,
+                           xlcoef, nchunk, job0);
 }
 
 // ------------------------------------------------------------------ energy
@@ -1009,8 +1014,8 @@ void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state,
 // (Cndo2.cpp:1348-1381); compensated summation.
 __global__ void __launch_bounds__(256) k_energy(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
                                                 double* __restrict__ arena, int* __restrict__ remaining,
-                                                const double* __restrict__ xlpart, int nchunk) {
-    const int job = blockIdx.x;
+                                                const double* __restrict__ xlpart, int nchunk, int job0) {
+    const int job = job0 + (int)blockIdx.x;
     JobState* st = &state[job];
     if (st->phase != PH_ENERGY) return;
     const JobDev jb = jobs[job];
@@ -1055,8 +1060,10 @@ __global__ void __launch_bounds__(256) k_energy(const JobDev* __restrict__ jobs,
 }
 
 void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena, int* remaining,
-                   const double* xlpart, int nchunk) {
-    hipLaunchKernelGGL(k_energy, dim3(njobs), dim3(256), 0, st, jobs, state, arena, remaining, xlpart, nchunk);
+                   const double* xlpart, int nchunk, int job0) {
+    if (njobs > 0)
+        hipLaunchKernelGGL(k_energy, dim3(njobs), dim3(256), 0, st, jobs, state, arena, remaining, xlpart, nchunk,
+                           job0);
 }
 
 // count records of `words` 32-bit words, src_stride / dst_stride bytes apart,

@@ -34,13 +34,22 @@ void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long lon
 // of the batch is mixed-precision);
 // mode 1: non-guess FP64 Fock of the final density for PH_ENERGY jobs;
 // mode 2: both in one launch (what the SCF loop uses).
-// plist: the pair indices, first the nlight H-H and heavy-H pairs (one lane
-// each), then the nheavy heavy-heavy pairs (one wavefront each).
+// fp: the pairs to build, as pair-list ranges: the H-H and the heavy-H pairs
+// (one lane each), the heavy-heavy pairs (one wavefront each); atoms: the
+// batch's atoms (the pairs index them globally), of which atoms
+// [atom0, atom0 + natoms) get their diagonal blocks reduced (a job's pairs and
+// atoms are all in or all out).
 // st2 (optional): the light pairs run on it, forked from st by fork_ev and
 // joined by join_ev before the per-atom reduction on st.
-void launch_fock(hipStream_t st, int mode, const PairDev* pairs, const int* plist, int nlight, int nheavy,
-                 const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state,
-                 const double* ints, const float* ints32, double* arena, double* part, hipStream_t st2 = nullptr,
+struct FockPairs {
+    const int* light[2] = {nullptr, nullptr};
+    int nlight[2] = {0, 0};
+    const int* heavy = nullptr;
+    int nheavy = 0;
+};
+void launch_fock(hipStream_t st, int mode, const FockPairs& fp, const PairDev* pairs, const AtomDev* atoms, int atom0,
+                 int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
+                 const float* ints32, double* arena, double* part, hipStream_t st2 = nullptr,
                  hipEvent_t fork_ev = nullptr, hipEvent_t join_ev = nullptr);
 
 // Warm start: F <- VT F VT^T for PH_ACTIVE jobs whose VT holds a previous solve.
@@ -76,14 +85,14 @@ void launch_density(hipStream_t st, const TileDev* tiles, int ntiles, const JobD
 // leaves XL_COEF doubles per job in xlcoef for launch_apply / launch_xl_apply.
 void launch_step(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, const double* rmspart,
                  const double* tpart, double* traj, int* remaining, const double* xlpart, double* xlcoef,
-                 int nchunk);
+                 int nchunk, int job0 = 0);
 
 // DIIS extrapolation and damping of the jobs that are not large, per upper tile.
 void launch_apply(hipStream_t st, const TileDev* tiles, int ntiles, const JobDev* jobs, double* arena,
                   const double* coef);
 
 void launch_energy(hipStream_t st, const JobDev* jobs, int njobs, JobState* state, double* arena,
-                   int* remaining, const double* xlpart, int nchunk);
+                   int* remaining, const double* xlpart, int nchunk, int job0 = 0);
 
 // Analytic forces of the converged jobs with want_f (kernels_setup.hip): per
 // pair slot grad[4 slot + 0..2] = dE_AB/d(R_B - R_A) and grad[4 slot + 3] =

@@ -265,16 +265,17 @@ class Engine:
                for st, na, no, it, i64, et, ee, ec, hf, rm, lt, b in
                zip(cols["status"], cols["nao"], cols["nocc"], cols["iterations"], cols["fp64_iterations"],
                    cols["energy"], cols["e_el"], cols["e_core"], cols["hof"], cols["rms"], cols["latency_s"], keep)]
+        if "_all" in keep[0]:   # the MO energies only
+            return out
         for r, b in zip(out, keep):
-            if len(b) > 2:
-                if "dipole" in b:
-                    d = b["dipole"]
-                    r["dipole_total"], r["dipole_electronic"], r["dipole_core"] = d[0:3], d[3:6], d[6:9]
-                if "forces" in b:
-                    r["forces"] = b["forces"]
-                if "traj" in b:
-                    t = b["traj"][:max(r["iterations"], 0)]
-                    r["traj"] = [(k, t[k, 0], t[k, 1], bool(t[k, 2]), bool(t[k, 3])) for k in range(len(t))]
+            if "dipole" in b:
+                d = b["dipole"]
+                r["dipole_total"], r["dipole_electronic"], r["dipole_core"] = d[0:3], d[3:6], d[6:9]
+            if "forces" in b:
+                r["forces"] = b["forces"]
+            if "traj" in b:
+                t = b["traj"][:max(r["iterations"], 0)]
+                r["traj"] = [(k, t[k, 0], t[k, 1], bool(t[k, 2]), bool(t[k, 3])) for k in range(len(t))]
         return out
 
     def scf(self, theory, Z, xyz_bohr, precision="fp64", requires_guess=True, P0=None, want_C=True, want_P=True,
