@@ -74,9 +74,15 @@ __device__ bool xsync(unsigned* bar, unsigned nblk, unsigned blk, unsigned* gen,
         const unsigned gsize = min((unsigned)XB, nblk - grp * XB);
         unsigned* cnt = bar + 64 + 32 * grp;
         bool release = false;
-        if (__hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
+        // Arrivals are release RMWs; whoever goes on acquires with one fence
+        // afterwards (the RMWs on a counter form one release sequence), so the
+        // cache invalidation of an acquire happens once per barrier and
+        // workgroup, not once per poll of the generation word.
+        if (__hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__hip_atomic_fetch_add(bar + 32, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == ngrp - 1) {
+            if (__hip_atomic_fetch_add(bar + 32, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == ngrp - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 __hip_atomic_store(bar + 32, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(bar, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 release = true;
@@ -84,10 +90,11 @@ __device__ bool xsync(unsigned* bar, unsigned nblk, unsigned blk, unsigned* gen,
         }
         if (!release) {
             const unsigned long long t0 = wall_clock64();
-            while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
+            while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
                 __builtin_amdgcn_s_sleep(1);
                 if (wall_clock64() - t0 > 2000000000ull) { ok = 0; break; }   // 20 s of the 100 MHz clock
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         *gen = target;
         *sh_ok = ok;
@@ -140,6 +147,7 @@ __host__ __device__ inline long long xw_size(int n) { return xw_dc(n) + 12LL * n
 // ------------------------------------------------------------------ tridiagonalization
 constexpr int TT = 1024;            // threads per workgroup of k_xtri (16 wavefronts: memory parallelism)
 constexpr int XU = 16;              // row elements loaded at once per lane in k_xtri's row pass
+constexpr int XP = 8;               // pivot-row and p elements loaded at once per thread (n <= 8192 in one)
 constexpr int TW = TT / 64;
 
 __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
@@ -220,7 +228,7 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
                 if (tid == 0) { d[0] = F[0]; e[0] = beta; tau[0] = tk; }
             }
             double pv = 0.0;
-            for (int r = 1 + gw; r < n; r += GW) {
+            for (int r = 1 + (gw - 1 + GW) % GW; r < n; r += GW) {   // row r with wavefront r mod GW
                 double s = 0.0;
                 int c = 1 + lane;
                 for (; c + 64 * 7 < n; c += 64 * 8) {
@@ -233,10 +241,17 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
                         s += a[u] * vcur[c + 64 * u - 1];
                     }
                 }
-                for (; c < n; c += 64) {
-                    const double a = F[(long long)r * ln + c];
-                    A[(long long)r * ln + c] = a;
-                    s += a * vcur[c - 1];
+                if (c < n) {   // the last (partial) chunk
+                    double a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = c + 64 * u < n ? F[(long long)r * ln + c + 64 * u] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (c + 64 * u < n) {
+                            A[(long long)r * ln + c + 64 * u] = a[u];
+                            s += a[u] * vcur[c + 64 * u - 1];
+                        }
+                    }
                 }
                 for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
                 const double pn = tk * s;
@@ -251,7 +266,14 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
             const int m = n - 1 - k;   // length of v_k (indices t <-> row / column k + 1 + t)
             const double pv = grid_partials_sum(qb[cur], G, &bc[0]);
             const double K = -0.5 * tk * pv;   // w = p + K v
-            for (int t = tid; t < m; t += TT) wv[t] = pb[cur][t] + K * vcur[t];
+            for (int t0 = tid; t0 < m; t0 += TT * XP) {
+                double pr[XP];
+#pragma unroll
+                for (int u = 0; u < XP; ++u) pr[u] = t0 + TT * u < m ? pb[cur][t0 + TT * u] : 0.0;
+#pragma unroll
+                for (int u = 0; u < XP; ++u)
+                    if (t0 + TT * u < m) wv[t0 + TT * u] = pr[u] + K * vcur[t0 + TT * u];
+            }
             __syncthreads();
             const double w0 = wv[0];
             if (k == n - 3) {   // last trailing 2 x 2 block (rows / columns n-2, n-1)
@@ -266,12 +288,21 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
             // row k+1 after the update of step k; its tail defines reflector k+1
             const double* rowp = A + (long long)(k + 1) * ln + (k + 1);
             double ss = 0.0;
-            for (int t = tid; t < m; t += TT) {
-                const double a = rowp[t] - (vcur[0] * wv[t] + w0 * vcur[t]);
-                if (t >= 2) ss += a * a;
-                else if (t == 1) bc[1] = a;
-                else if (wg == 0) d[k + 1] = a;
-                if (t >= 1) vnext[t - 1] = a;   // scaled below
+            for (int t0 = tid; t0 < m; t0 += TT * XP) {   // XP loads per thread in flight
+                double ar[XP];
+#pragma unroll
+                for (int u = 0; u < XP; ++u) ar[u] = t0 + TT * u < m ? rowp[t0 + TT * u] : 0.0;
+#pragma unroll
+                for (int u = 0; u < XP; ++u) {
+                    const int t = t0 + TT * u;
+                    if (t < m) {
+                        const double a = ar[u] - (vcur[0] * wv[t] + w0 * vcur[t]);
+                        if (t >= 2) ss += a * a;
+                        else if (t == 1) bc[1] = a;
+                        else if (wg == 0) d[k + 1] = a;
+                        if (t >= 1) vnext[t - 1] = a;   // scaled below
+                    }
+                }
             }
             ss = block_sum(ss, red, tid, TW);   // its barriers also publish bc[1] and vnext
             const double alpha = bc[1];
@@ -295,7 +326,9 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
             // the bytes a CU keeps in flight)
             const int nx = cur ^ 1;
             double pacc = 0.0;
-            for (int r = k + 2 + gw; r < n; r += GW) {
+            // row r stays with wavefront r mod GW at every step (its workgroup's
+            // L2 keeps the rows it rewrites)
+            for (int r = k + 2 + (gw - (k + 2) % GW + GW) % GW; r < n; r += GW) {
                 const int tr = r - k - 1;
                 const double vr = vcur[tr], wr = wv[tr];
                 double* row = A + (long long)r * ln;
@@ -313,11 +346,19 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
                         s += a[u] * vnext[tc - 1];
                     }
                 }
-                for (; c < n; c += 64) {
-                    const int tc = c - k - 1;
-                    const double a = row[c] - (vr * wv[tc] + wr * vcur[tc]);
-                    row[c] = a;
-                    s += a * vnext[tc - 1];
+                if (c < n) {   // the last (partial) chunk: its loads in flight together
+                    double a[XU];
+#pragma unroll
+                    for (int u = 0; u < XU; ++u) a[u] = c + 64 * u < n ? row[c + 64 * u] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < XU; ++u) {
+                        if (c + 64 * u < n) {
+                            const int tc = c + 64 * u - k - 1;
+                            a[u] -= vr * wv[tc] + wr * vcur[tc];
+                            row[c + 64 * u] = a[u];
+                            s += a[u] * vnext[tc - 1];
+                        }
+                    }
                 }
                 for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
                 const double pn = tn * s;
@@ -841,15 +882,15 @@ long long xlh_ws_doubles(int n) { return (xw_size(n) + 31) / 32 * 32; }
 
 int xlh_grid(int num_cus) { return std::max(1, std::min(num_cus, XGRID_MAX)); }
 
-// Measured on MI355X (256 CUs; ms per cold tridiagonalization at 16 / 32 / 64 /
-// 128 / 256 workgroups): n = 1201: 20.8 / 16.3 / 16.2 / 24.5 / 52.7;
-// n = 3000: 183 / 113 / 84 / 96 / 215; n = 6000: 1251 / 701 / 442 / 403 / 754.
-// A CU streams its rows at about 57 GB/s, while the per-step grid barrier
-// costs more the more workgroups release and invalidate their caches.
+// Measured on MI355X (256 CUs; ms per cold tridiagonalization of one molecule
+// at 32 / 64 / 128 / 256 workgroups): n = 1201: 15.3 / 13.2 / 14.0 / 18.0;
+// n = 1536: 23.9 / 20.2 / 19.4 / 24.2; n = 3000: 105 / 73 / 62 / 64;
+// n = 6000: 676 / 409 / 296 / 291.  More workgroups stream the rows faster
+// while the per-step grid barrier costs more the more workgroups take part.
 // Several molecules of a launch share the grid, each with its own group.
 int xlh_tri_grid(int num_cus, int max_n, int nlist) {
     const int forced = get_tuning().xl_tri_wg;
-    const int per = max_n > 4096 ? 128 : (max_n > 2048 ? 64 : (nlist > 1 ? 32 : 64));
+    const int per = max_n > 4096 ? 256 : (max_n > 1400 ? 128 : (nlist > 1 ? 32 : 64));
     const int g = forced > 0 ? forced : per * std::max(1, nlist);
     return std::max(1, std::min(g, std::min(num_cus, XGRID_MAX)));
 }
