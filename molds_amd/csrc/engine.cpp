@@ -14,6 +14,7 @@
 #include <functional>
 #include <cstdlib>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -362,10 +363,16 @@ int Batch::build(const nddo_job_desc* descs, int nd, const std::vector<int>& whi
 // 8.8 s with the device-wide solver above 512).  Many such molecules fill the
 // GPU one workgroup each, which gives more solves/s.
 constexpr int kFewLarge = 64;
+// LDS kept free beside the dynamic LDS of k_tridiag (its static reduction words)
+constexpr size_t kLdsStaticReserve = 1024;
 bool lds_eig_class(int n, size_t lds_max) {
-    // the Jacobi kernels also hold up to 8 KB of static LDS (reduction buffers)
-    return n <= 512 && lds_max > 9216 && eig_lds_bytes(n, true) <= lds_max - 9216;
+    return n <= 512 && lds_max > kLdsStaticReserve && eig_lds_bytes(n, true) <= lds_max - kLdsStaticReserve;
 }
+// Largest molecule of each LDS-class span but the last: with k_tridiag's LDS
+// (n^2 + 4 n + max(threads, n) doubles) and threads (256 up to 44 AOs, 512 up
+// to 96, else 1024), spans up to 44 / 64 / 96 AOs fit 8 / 4 / 2 workgroups per
+// CU, the last span (up to 137 AOs on 160 KB) one.
+constexpr int kLdsSpanMax[] = {44, 64, 96};
 int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao, bool route_few_large) {
     list_lds.clear();
     list_glb.clear();
@@ -399,6 +406,19 @@ int Batch::classify(size_t lds_max, int num_cus, int xl_min_nao, bool route_few_
             maxn_xl = std::max(maxn_xl, n);
         }
     }
+    lds_spans.clear();
+    for (int i = 0; i < (int)list_lds.size(); ++i) {
+        const int n = jobs[list_lds[i]].nao;
+        int cap = INT_MAX;
+        for (int c : kLdsSpanMax)
+            if (n <= c) { cap = c; break; }
+        if (lds_spans.empty() || cap != lds_spans.back().maxn) {
+            if (!lds_spans.empty()) lds_spans.back().maxn = jobs[list_lds[lds_spans.back().start]].nao;
+            lds_spans.push_back(Span{i, 0, cap});
+        }
+        ++lds_spans.back().count;
+    }
+    if (!lds_spans.empty()) lds_spans.back().maxn = jobs[list_lds[lds_spans.back().start]].nao;
     xl_chunks = xl_nchunk(maxn_xl);
     xl_householder = use_xl_householder && xlh_supported(maxn_xl);
     split_job = 0;
@@ -615,6 +635,19 @@ FockPairs Batch::fock_pairs(int j0, int j1) const {
     return fp;
 }
 
+void Batch::launch_lds_eigen(int part, hipStream_t st) {
+    // the tridiagonalization span by span (its LDS grows as n^2); divide and
+    // conquer and the back-transform once over the class
+    if (part <= 0)
+        for (const Span& sp : lds_spans)
+            launch_heig_part(0, st, d_jobs, d_state, d_list_lds + sp.start, sp.count, sp.maxn, d_arena, d_eps, d_perm,
+                             d_tri, true);
+    for (int p = 1; p < 3; ++p)
+        if (part < 0 || part == p)
+            launch_heig_part(p, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
+                             d_perm, d_tri, true);
+}
+
 bool Batch::split_ready() const { return split_job > 0 && fork_stream && aux_stream && aux_fork && aux_join; }
 
 // One iteration with the LDS-class jobs [0, split_job) on fork_stream and the
@@ -630,7 +663,7 @@ void Batch::launch_iteration_split(Engine* e, hipStream_t st) {
     launch_fock(s2, 2, fock_pairs(0, js), d_pairs, d_atoms, 0, split_atom, e->d_elems, d_jobs, d_state, d_ints, d_ints32,
                 d_arena, d_part);
     launch_energy(s2, d_jobs, js, d_state, d_arena, d_remaining, d_xlpart, xl_chunks, 0);
-    launch_eigen(s2, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps, d_perm, d_tri, true);
+    launch_lds_eigen(-1, s2);
     launch_density(s2, d_tiles, split_tile, d_jobs, d_state, d_arena, d_perm, d_rmspart, d_tpart);
     launch_step(s2, d_jobs, js, d_state, d_rmspart, d_tpart, d_traj, d_remaining, d_xlpart, d_xlcoef, xl_chunks, 0);
     launch_apply(s2, d_tiles, split_tile, d_jobs, d_arena, d_xlcoef);
@@ -674,8 +707,7 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
         hipEventRecord(fork_ev, st);
         hipStreamWaitEvent(fork_stream, fork_ev, 0);
     }
-    launch_eigen(fork ? fork_stream : st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
-                 d_perm, d_tri, true);
+    launch_lds_eigen(-1, fork ? fork_stream : st);
     if (fork) hipEventRecord(join_ev, fork_stream);
     launch_eigen(st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm, d_tri, false);
     launch_xl_eigen_all(e, st);
@@ -784,23 +816,18 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
                 HIPCHK(hipEventRecord(fork_ev, st));
                 HIPCHK(hipStreamWaitEvent(fork_stream, fork_ev, 0));
                 HIPCHK(hipEventRecord(e->ev[11], fork_stream));
-                for (int part = 0; part < 3; ++part)
-                    launch_heig_part(part, fork_stream, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds,
-                                     d_arena, d_eps, d_perm, d_tri, true);
+                launch_lds_eigen(-1, fork_stream);
                 HIPCHK(hipEventRecord(e->ev[12], fork_stream));
                 HIPCHK(hipEventRecord(join_ev, fork_stream));
             } else {
-                launch_heig_part(0, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena, d_eps,
-                                 d_perm, d_tri, true);
+                launch_lds_eigen(0, st);
             }
             HIPCHK(hipEventRecord(e->ev[2], st));
             launch_heig_part(0, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps, d_perm,
                              d_tri, false);
             HIPCHK(hipEventRecord(e->ev[3], st));
             for (int part = 1; part < 3; ++part) {
-                if (!fork)
-                    launch_heig_part(part, st, d_jobs, d_state, d_list_lds, (int)list_lds.size(), maxn_lds, d_arena,
-                                     d_eps, d_perm, d_tri, true);
+                if (!fork) launch_lds_eigen(part, st);
                 launch_heig_part(part, st, d_jobs, d_state, d_list_glb, (int)list_glb.size(), maxn_glb, d_arena, d_eps,
                                  d_perm, d_tri, false);
                 HIPCHK(hipEventRecord(e->ev[3 + part], st));
@@ -1395,7 +1422,7 @@ int nddo_engine_info(nddo_engine* eh, size_t* lds_bytes, int* num_cus, size_t* h
     if (hbm_budget) *hbm_budget = e->hbm_budget;
     if (lds_tridiag_max_n) {
         int n = 0;
-        while (n < 512 && e->lds_max > 9216 && eig_lds_bytes(n + 1, true) <= e->lds_max - 9216) ++n;
+        while (n < 512 && lds_eig_class(n + 1, e->lds_max)) ++n;
         *lds_tridiag_max_n = n;
     }
     return NDDO_OK;
@@ -1772,8 +1799,7 @@ int nddo_dbg_eigh(nddo_engine* eh, int n, const double* F, double* eps, double* 
     if (rc) return rc;
     const long long nn = (long long)n * n;
     HIPCHK(hipMemcpy(b.d_arena + (long long)AR_F * nn, F, nn * sizeof(double), hipMemcpyHostToDevice));
-    launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_lds, (int)b.list_lds.size(), b.maxn_lds, b.d_arena, b.d_eps,
-                 b.d_perm, b.d_tri, true);
+    b.launch_lds_eigen(-1, e->stream);
     launch_eigen(e->stream, b.d_jobs, b.d_state, b.d_list_glb, (int)b.list_glb.size(), b.maxn_glb, b.d_arena, b.d_eps,
                  b.d_perm, b.d_tri, false);
     b.launch_xl_eigen_all(e, e->stream);

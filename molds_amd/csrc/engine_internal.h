@@ -61,6 +61,13 @@ struct Batch {
     int n_light = 0, n_heavy = 0;
     std::vector<long long> xl_off;   // per list_xl entry: offset in d_xlws
     int maxn_lds = 0, maxn_glb = 0, maxn_xl = 0, max_iter_all = 0;
+    // The LDS class in spans of similar size (list_lds is sorted by size,
+    // largest first): each span's launches size their LDS and threads for its
+    // own largest molecule, so small molecules share a CU several at a time.
+    struct Span {
+        int start, count, maxn;
+    };
+    std::vector<Span> lds_spans;
     int xl_grid = 1, xl_chunks = 1;
     // large molecules: device-wide Householder + divide and conquer (kernels_xlh.hip),
     // or block Jacobi (kernels_xl.hip); classify() decides from use_xl_householder
@@ -148,6 +155,9 @@ struct Batch {
     // when the batch holds few of them (nddo_scf_batch / nddo_scf_once)
     int classify(size_t lds_max, int num_cus, int xl_min_nao, bool route_few_large = false);
     void launch_xl_eigen_all(Engine* e, hipStream_t st);
+    // The LDS-class eigensolver span by span: part 0 / 1 / 2 of launch_heig_part,
+    // or -1 for all three parts of each span.
+    void launch_lds_eigen(int part, hipStream_t st);
     int allocate(bool want_sdbg);
     size_t upload_bytes() const;
     int upload(hipStream_t st, const nddo_job_desc* descs, char* stage = nullptr);

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -33,9 +34,13 @@
 
 namespace nddo {
 
+// Threads of k_tridiag for molecules of at most max_n AOs.
+static int tridiag_threads(int max_n) { return max_n <= 44 ? 256 : (max_n <= 96 ? 512 : 1024); }
+
 size_t tridiag_lds_bytes(int n, bool a_in_lds) {
     size_t d = 4 * (size_t)n;   // vcur vnext p pnext
-    if (a_in_lds) d += (size_t)n * n;
+    // the matrix and the segment partials of A v (at most max(threads, n))
+    if (a_in_lds) d += (size_t)n * n + (size_t)std::max(tridiag_threads(n), n);
     return d * 8;
 }
 
@@ -48,13 +53,13 @@ constexpr int kHeigMaxN = 1200;
 // ------------------------------------------------------------------ 1. tridiagonalize
 // Reflector k acts on indices k+1..n-1: H_k = I - tau_k v v^T with v[0] = 1,
 // kept in row k of W (columns k+1..n-1); d, e, tau go to the job's workspace.
-__global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
-                                                  const int* __restrict__ joblist, double* __restrict__ arena,
-                                                  double* __restrict__ tri) {
+// Up to 64 VGPRs (8 wavefronts per SIMD), so that eight 256-thread or four
+// 512-thread workgroups of small molecules share a CU.
+__global__ void __launch_bounds__(1024, 8) k_tridiag(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                                     const int* __restrict__ joblist, double* __restrict__ arena,
+                                                     double* __restrict__ tri) {
     extern __shared__ double dyn[];
     __shared__ double red[16], red2[16], sh_alpha;
-    // segment partials of A v (S * m2 <= max(blockDim.x, m2), m2 <= 136)
-    __shared__ double part[1024];
     const int job = joblist[blockIdx.x];
     const JobDev jb = jobs[job];
     if (state[job].phase != PH_ACTIVE) return;
@@ -67,6 +72,8 @@ __global__ void __launch_bounds__(1024) k_tridiag(const JobDev* __restrict__ job
     double* pnext = p + n;
     double* W = amat(arena, jb, AR_W);
     double* A = pnext + n;
+    // segment partials of A v after A (S * m2 <= max(blockDim.x, m2) doubles)
+    double* part = A + (long long)n * n;
     const double* F = amat(arena, jb, AR_F);
     double* d = eig_ws(tri, jb, job);
     double* e = d + n;
@@ -1225,7 +1232,7 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
                       int max_n, double* arena, double* eps, int* perm, double* tri, bool a_in_lds) {
     if (nlist <= 0) return;
     if (part == 0) {
-        const int threads = max_n <= 32 ? 256 : (max_n <= 96 ? 512 : 1024);
+        const int threads = tridiag_threads(max_n);
         const size_t lt = tridiag_lds_bytes(max_n, a_in_lds);
         if (a_in_lds) {
             raise_dynamic_lds((const void*)k_tridiag, lt);

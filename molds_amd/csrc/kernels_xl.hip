@@ -127,11 +127,13 @@ __device__ bool grid_sync(unsigned* bar, unsigned nblk, unsigned* gen, int* sh_o
             __hip_atomic_store(&bar[1], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const unsigned long long t0 = wall_clock64();
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
+            // relaxed polls, one acquire afterwards (one cache invalidation per barrier)
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
                 __builtin_amdgcn_s_sleep(2);
                 // 100 MHz constant clock: give up after 20 s
                 if (wall_clock64() - t0 > 2000000000ull) { ok = 0; break; }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         *gen = target;
         *sh_ok = ok;
