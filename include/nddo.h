@@ -264,6 +264,11 @@ int nddo_dbg_graph_stats(nddo_engine* e, long long* captured, long long* replays
 /* Test hook: workgroups of the large-molecule tridiagonalization (0 = the
  * engine's choice by size). */
 int nddo_dbg_set_xl_grid(nddo_engine* e, int tri_workgroups);
+/* Test hook: the one-workgroup tridiagonalization of matrices in device memory
+ * (138 to 512 AOs) on this engine's device: blocked, panels of 8 reflectors
+ * with a rank-16 FP64 MFMA update per panel (1, the default), or one rank-2
+ * update per reflector (0).  Both store the same reflectors and tridiagonal. */
+int nddo_dbg_set_tridiag(nddo_engine* e, int blocked);
 
 #ifdef __cplusplus
 }

@@ -1766,14 +1766,24 @@ int nddo_engine_tuning(nddo_engine* eh, int* fock_wpb, int* bt_rows4) {
 int nddo_dbg_set_tuning(nddo_engine* eh, int fock_wpb, const int* bt_rows4) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !bt_rows4 || !(fock_wpb == 1 || fock_wpb == 2 || fock_wpb == 4 || fock_wpb == 8)) return NDDO_BAD_ARG;
-    Tuning t;
-    t.fock_wpb = fock_wpb;
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 4; ++i)
         if (bt_rows4[i] != 32 && bt_rows4[i] != 64) return NDDO_BAD_ARG;
-        t.bt_rows[i] = bt_rows4[i];
-    }
     SyncRun lk(e);
     HIPCHK(hipSetDevice(e->device));
+    Tuning t = get_tuning();
+    t.fock_wpb = fock_wpb;
+    for (int i = 0; i < 4; ++i) t.bt_rows[i] = bt_rows4[i];
+    set_tuning(t);
+    return NDDO_OK;
+}
+
+int nddo_dbg_set_tridiag(nddo_engine* eh, int blocked) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || (blocked != 0 && blocked != 1)) return NDDO_BAD_ARG;
+    SyncRun lk(e);
+    HIPCHK(hipSetDevice(e->device));
+    Tuning t = get_tuning();
+    t.tri_blocked = blocked;
     set_tuning(t);
     return NDDO_OK;
 }

@@ -579,6 +579,304 @@ __global__ void __launch_bounds__(1024, 8) k_tridiag_sym(const JobDev* __restric
     if (tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
 }
 
+// Blocked (deferred-update) variant for matrices in device memory, n <= 512:
+// LAPACK's dsytrd / dlatrd scheme (Dongarra, Hammarling & Sorensen, J. Comput.
+// Appl. Math. 27, 215 (1989)) on the upper triangle. The reflectors form
+// panels of kTriNb. Within a panel the stored trailing matrix A_p (the matrix
+// at the panel's first step) is only read: step k forms row k of the current
+// matrix A_k = A_p - sum_l (V_l W_l^T + W_l V_l^T) over the panel's earlier
+// reflectors l, the reflector v_k from it, then
+//   p = tau_k (A_p v_k - sum_l V_l (W_l . v_k) + W_l (V_l . v_k)),
+//   w_k = p - (tau_k / 2) (p . v_k) v_k,
+// and keeps V_l = v_l, W_l = w_l (full length, zero before the reflector's
+// first index) in the job's AR_Q2 matrix, free until divide and conquer. At
+// the next panel's first step one rank-2 kTriNb update of the upper triangle
+// on FP64 MFMA brings the trailing block to A_{p + kTriNb}. A step reads the
+// trailing triangle once and writes nothing to it, against one read and one
+// write per step in k_tridiag_sym; the reflectors, d, e and tau are stored as
+// there, so divide and conquer and the back-transform are unchanged.
+constexpr int kTriNb = 8;
+
+// Trailing block k0..n-1 of the upper triangle minus sum_l (V_l W_l^T + W_l V_l^T)
+// over the kTriNb vectors of VW (V_0..V_7 then W_0..W_7, n doubles each); one
+// wave per 16 x 16 tile, the rank-16 product as four v_mfma_f64_16x16x4f64 with
+// the tile itself as the accumulator (element v of a lane is row 4 v + lane / 16,
+// column lane % 16 of the tile).
+template <typename TS>
+__device__ inline void tridiag_panel_update(__amdgpu_buffer_rsrc_t Ab, double* __restrict__ VW, int n, int k0,
+                                            int wave, int nw, int lane) {
+    constexpr u32 ES = sizeof(TS);
+    const int m0 = n - k0, nt = (m0 + 15) >> 4, ntiles = nt * (nt + 1) / 2;
+    const int lr = lane & 15, lk = lane >> 4;
+    // V/W element (kk, j) at byte (kk n + j) 8: the lane's part (lk n + j) 8 in
+    // a VGPR, the k-step's 4 q n 8 in a scalar offset
+    const __amdgpu_buffer_rsrc_t Vb =
+        __builtin_amdgcn_make_buffer_rsrc(VW, (short)0, 2 * kTriNb * n * (int)sizeof(double), 0x00020000);
+    const u32 nbytes = (u32)n * 8u;
+    for (int tile = wave; tile < ntiles; tile += nw) {
+        // tile = J (J + 1) / 2 + I with 0 <= I <= J < nt
+        int J = (int)((sqrtf(8.0f * (float)tile + 1.0f) - 1.0f) * 0.5f);
+        while (J * (J + 1) / 2 > tile) --J;
+        while ((J + 1) * (J + 2) / 2 <= tile) ++J;
+        const int I = tile - J * (J + 1) / 2;
+        const int rb = k0 + 16 * I, cb = k0 + 16 * J, c = cb + lr;
+        const u32 cbyte = (u32)(c < n ? c : 0) * ES;
+        dbl4 acc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = rb + 4 * v + lk;
+            acc[v] = (c < n && r <= c) ? sym_load<TS>(Ab, (u32)r * (u32)n * ES + cbyte, 0) : 0.0;
+        }
+        const int ra = rb + lr;
+        const u32 va = (u32)lk * nbytes + (u32)(ra < n ? ra : 0) * 8u, vb = (u32)lk * nbytes + (u32)(c < n ? c : 0) * 8u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            // k = 4 q + lk: V_k (k < 8) or W_{k-8} against W_k or V_{k-8}
+            const double a = buf_load_d(Vb, va, (u32)(4 * q) * nbytes);
+            const double b = buf_load_d(Vb, vb, (u32)((4 * q + kTriNb) & (2 * kTriNb - 1)) * nbytes);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ra < n ? -a : 0.0, c < n ? b : 0.0, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = rb + 4 * v + lk;
+            if (c < n && r <= c) sym_store<TS>(acc[v], Ab, (u32)r * (u32)n * ES + cbyte, 0);
+        }
+    }
+}
+
+template <typename TS>
+__device__ void tridiag_blk_body(TS* __restrict__ As, double* __restrict__ W, const double* __restrict__ F,
+                                 double* __restrict__ VW, double* dyn, int n, double* d, double* e, double* tau,
+                                 double* red, double* red2, double* dots) {
+    const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63, nw = T >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr u32 ES = sizeof(TS);
+    // v and p of the current step (indexed from the reflector's first index)
+    // and of the previous one, ping-pong
+    double* vb0 = dyn;
+    double* vb1 = dyn + n;
+    double* pb0 = dyn + 2 * n;
+    double* pb1 = dyn + 3 * n;
+    double* colpart = dyn + 4 * n;
+    const int ccap = sym_col_cap(n);
+    double* rowpart = colpart + ccap;
+    // row k of the current matrix between its formation and its reflector
+    double* xrow = colpart;
+    const __amdgpu_buffer_rsrc_t Ab = __builtin_amdgcn_make_buffer_rsrc(As, (short)0, n * n * (int)ES, 0x00020000);
+    for (int r = wave; r < n; r += nw)
+        for (int c = r + lane; c < n; c += 64) As[(long long)r * n + c] = (TS)F[(long long)r * n + c];
+    __syncthreads();
+    double Kp = 0.0;   // -tau/2 (p . v) of the previous step: w = p + Kp v
+    for (int k = 0; k <= n - 3; ++k) {
+        const int i = k % kTriNb;
+        double* vcur = (k & 1) ? vb1 : vb0;
+        double* pcur = (k & 1) ? pb1 : pb0;
+        const double* vprev = (k & 1) ? vb0 : vb1;   // v_{k-1}, indices k .. n-1
+        const double* pprev = (k & 1) ? pb0 : pb1;
+        if (k > 0 && i == 0) {
+            // last W of the panel, then the panel's rank-2 kTriNb update
+            for (int c = k + tid; c < n; c += T)
+                VW[(long long)(2 * kTriNb - 1) * n + c] = pprev[c - k] + Kp * vprev[c - k];
+            __syncthreads();
+            tridiag_panel_update<TS>(Ab, VW, n, k, wave, nw, lane);
+            __syncthreads();
+        }
+        // row k of A_k, columns k .. n-1; W_{i-1} is formed here from p and v
+        const int m0 = n - k;
+        double ss = 0.0;
+        for (int t = tid; t < m0; t += T) {
+            const int c = k + t;
+            double x = (double)As[(long long)k * n + c];
+            for (int l = 0; l + 1 < i; ++l) {
+                const double* V = VW + (long long)l * n;
+                const double* Wl = VW + (long long)(kTriNb + l) * n;
+                x -= V[k] * Wl[c] + Wl[k] * V[c];
+            }
+            if (i > 0) {
+                const double wc = pprev[t] + Kp * vprev[t], wk = pprev[0] + Kp * vprev[0];
+                VW[(long long)(kTriNb + i - 1) * n + c] = wc;
+                x -= vprev[0] * wc + wk * vprev[t];
+            }
+            xrow[t] = x;
+            if (t >= 2) ss += x * x;
+        }
+        ss = group_sum(ss, 64);
+        if (lane == 0) red2[wave] = ss;
+        __syncthreads();   // 1
+        ss = 0.0;
+        for (int w = 0; w < nw; ++w) ss += red2[w];
+        const double alpha = xrow[1];
+        double beta = alpha, tn = 0.0, scale = 0.0;
+        if (ss > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            tn = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        const int m = m0 - 1;   // length of v_k (indices k+1 .. n-1)
+        const long long wrow = (long long)k * n + k + 1;
+        double* Vi = VW + (long long)i * n + k + 1;
+        for (int t = 1 + tid; t < m; t += T) {
+            const double v = ss > 0.0 ? xrow[1 + t] * scale : 0.0;
+            vcur[t] = v;
+            W[wrow + t] = v;
+            Vi[t] = v;
+        }
+        if (tid == 0) {
+            vcur[0] = 1.0;
+            W[wrow] = 1.0;
+            Vi[0] = 1.0;
+            d[k] = xrow[0];
+            e[k] = beta;
+            tau[k] = tn;
+        }
+        __syncthreads();   // 2
+        const int base = k + 1;
+        // W_l . v_k and V_l . v_k of the panel's earlier reflectors, one wave each
+        if (wave < 2 * i) {
+            const int l = wave >> 1;
+            const double* src = VW + (long long)((wave & 1) ? l : kTriNb + l) * n + base;
+            double s = 0.0;
+            for (int t = lane; t < m; t += 64) s += src[t] * vcur[t];
+            s = group_sum(s, 64);
+            if (lane == 0) dots[wave] = s;
+        }
+        // A_p v_k over the upper triangle of rows/columns base .. n-1 (read only)
+        const int nct = (m + 63) >> 6;
+        const int BR = ((m + 7) >> 3) * m <= ccap ? 8 : (((m + 15) >> 4) * m <= ccap ? 16 : 32);
+        const int per_tile = 64 / BR, nb = (m + BR - 1) / BR;
+        int nitems = 0;
+        for (int ct = 0; ct < nct; ++ct) nitems += min(per_tile * (ct + 1), nb);
+        for (int it = wave; it < nitems; it += nw) {
+            int ct = 0, b = it;
+            while (b >= min(per_tile * (ct + 1), nb)) { b -= min(per_tile * (ct + 1), nb); ++ct; }
+            const int c = base + 64 * ct + lane;
+            const bool cv = c < n;
+            const u32 cbyte = (u32)(cv ? c : 0) * ES;
+            const double vc = cv ? vcur[c - base] : 0.0;
+            double colacc = 0.0;
+            const int r0 = base + b * BR, rend = min(r0 + BR, n);
+            // two 4-row slices per pass, their eight loads issued together (BR is
+            // a multiple of 8, so a pass never crosses into the next band)
+            for (int rs = r0; rs < rend; rs += 8) {
+                const u32 voff = (u32)rs * (u32)n * ES + cbyte;
+                double a[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    a[u] = (cv && rs + u <= c && rs + u < rend) ? sym_load<TS>(Ab, voff, (u32)u * (u32)n * ES)
+                                                                : 0.0;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int rh = rs + 4 * h;
+                    if (rh >= rend) break;
+                    double x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int r = rh + u;
+                        const double vr = r < n ? vcur[r - base] : 0.0;
+                        colacc += a[4 * h + u] * vr;
+                        x[u] = r < c ? a[4 * h + u] * vc : 0.0;
+                    }
+                    const double s4 = rows4_reduce(x[0], x[1], x[2], x[3]);
+                    const int rr = rh + ((lane >> 4) & 3);
+                    if ((lane & 15) == 0 && rr < n) rowpart[ct * m + (rr - base)] = s4;
+                }
+            }
+            if (cv) colpart[b * m + (c - base)] = colacc;
+        }
+        __syncthreads();   // 3
+        int S = 1;
+        while (S < 64 && 2 * S * m <= T) S *= 2;
+        double pv = 0.0;
+        for (int t0s = 0; t0s < m; t0s += T / S) {
+            const int t = t0s + tid / S, sl = tid & (S - 1);
+            double q = 0.0;
+            if (t < m) {
+                for (int bb = sl; bb <= t / BR; bb += S) q += colpart[bb * m + t];
+                for (int cc = (t >> 6) + sl; cc < nct; cc += S) q += rowpart[cc * m + t];
+            }
+            q = group_sum(q, S);
+            if (sl == 0 && t < m) {
+                double corr = 0.0;
+                for (int l = 0; l < i; ++l)
+                    corr += VW[(long long)l * n + base + t] * dots[2 * l] +
+                            VW[(long long)(kTriNb + l) * n + base + t] * dots[2 * l + 1];
+                const double pn = tn * (q - corr);
+                pcur[t] = pn;
+                pv += pn * vcur[t];
+            }
+        }
+        pv = group_sum(pv, 64);
+        if (lane == 0) red[wave] = pv;
+        __syncthreads();   // 4
+        pv = 0.0;
+        for (int w = 0; w < nw; ++w) pv += red[w];
+        Kp = -0.5 * tn * pv;
+        if (k == n - 3) {
+            // the last 2 x 2 block: rows n-2, n-1 of A_{n-2}
+            if (tid == 0) {
+                double a00 = (double)As[(long long)(n - 2) * n + n - 2];
+                double a01 = (double)As[(long long)(n - 2) * n + n - 1];
+                double a11 = (double)As[(long long)(n - 1) * n + n - 1];
+                for (int l = 0; l < i; ++l) {
+                    const double* V = VW + (long long)l * n;
+                    const double* Wl = VW + (long long)(kTriNb + l) * n;
+                    a00 -= 2.0 * V[n - 2] * Wl[n - 2];
+                    a01 -= V[n - 2] * Wl[n - 1] + Wl[n - 2] * V[n - 1];
+                    a11 -= 2.0 * V[n - 1] * Wl[n - 1];
+                }
+                const double w0 = pcur[0] + Kp * vcur[0], w1 = pcur[1] + Kp * vcur[1];
+                a00 -= 2.0 * vcur[0] * w0;
+                a01 -= vcur[0] * w1 + w0 * vcur[1];
+                a11 -= 2.0 * vcur[1] * w1;
+                d[n - 2] = a00;
+                e[n - 2] = a01;
+                d[n - 1] = a11;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024, 8) k_tridiag_blk(const JobDev* __restrict__ jobs,
+                                                       JobState* __restrict__ state, const int* __restrict__ joblist,
+                                                       double* __restrict__ arena, double* __restrict__ tri) {
+    extern __shared__ double dyn[];
+    __shared__ double red[16], red2[16], dots[2 * kTriNb];
+    const int job = joblist[blockIdx.x];
+    const JobDev jb = jobs[job];
+    if (state[job].phase != PH_ACTIVE) return;
+    const unsigned long long t0 = wall_clock64();
+    const int n = jb.nao, T = blockDim.x, tid = threadIdx.x;
+    double* W = amat(arena, jb, AR_W);
+    const double* F = amat(arena, jb, AR_F);
+    double* VW = amat(arena, jb, AR_Q2);
+    double* d = eig_ws(tri, jb, job);
+    double* e = d + n;
+    double* tau = e + n;
+    for (int i = tid; i < n; i += T) { e[i] = 0.0; tau[i] = 0.0; }
+    if (n < 3) {
+        __syncthreads();
+        if (tid == 0) {
+            d[0] = F[0];
+            if (n == 2) { e[0] = F[1]; d[1] = F[3]; }
+            state[job].prof[0] = (double)(wall_clock64() - t0);
+        }
+        return;
+    }
+    if (n < 2 * kTriNb) {
+        // the panel vectors need 16 n <= n^2 doubles of AR_Q2; molecules this
+        // small take the LDS class on gfx950 (160 KB of LDS), so this is not
+        // reached: a NaN diagonal makes divide and conquer fail the job loudly
+        if (tid == 0) d[0] = __builtin_nan("");
+    } else {
+        // FP64 storage also for mixed-precision jobs in their FP32 iterations:
+        // this kernel's steps are bound by load round trips, not bytes, and an
+        // FP32-storage form of it ran slower (328 against 294 ms of
+        // tridiagonalization per 4096-molecule pass)
+        tridiag_blk_body<double>(W, W, F, VW, dyn, n, d, e, tau, red, red2, dots);
+    }
+    if (tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
+}
+
 // ------------------------------------------------------------------ 2. divide and conquer
 // Rows of QT (AR_C / AR_Q2, ping-pong) are eigenvectors of the current blocks;
 // the block [a, b) of level l only touches rows and columns a..b-1.  U (AR_U)
@@ -1237,6 +1535,10 @@ void launch_heig_part(int part, hipStream_t st, const JobDev* jobs, JobState* st
         if (a_in_lds) {
             raise_dynamic_lds((const void*)k_tridiag, lt);
             hipLaunchKernelGGL(k_tridiag, dim3(nlist), dim3(threads), lt, st, jobs, state, joblist, arena, tri);
+        } else if (max_n <= kSymMaxLdsN && get_tuning().tri_blocked) {
+            const size_t ls = tridiag_sym_lds_bytes(max_n, false);
+            raise_dynamic_lds((const void*)k_tridiag_blk, ls);
+            hipLaunchKernelGGL(k_tridiag_blk, dim3(nlist), dim3(1024), ls, st, jobs, state, joblist, arena, tri);
         } else if (max_n <= kSymMaxLdsN) {
             const size_t ls = tridiag_sym_lds_bytes(max_n, false);
             raise_dynamic_lds((const void*)k_tridiag_sym<false>, ls);

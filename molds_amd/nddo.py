@@ -89,6 +89,7 @@ def load_library(path=LIBPATH):
     L.nddo_dbg_eigh_dc_phases.argtypes = [vp, _dp]
     L.nddo_engine_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.nddo_dbg_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.nddo_dbg_set_tridiag.argtypes = [vp, ctypes.c_int]
     L.nddo_dbg_host_times.argtypes = [vp, _dp, ctypes.c_int]
     L.nddo_dbg_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.nddo_engine_trim.argtypes = [vp]
@@ -432,6 +433,13 @@ class Engine:
     def set_tuning(self, fock_wpb, bt_rows):
         rows = (ctypes.c_int * 4)(*bt_rows)
         rc = self.L.nddo_dbg_set_tuning(self.h, fock_wpb, rows)
+        if rc != 0:
+            raise NddoError(rc, last_error())
+
+    def set_tridiag(self, blocked):
+        """Blocked (True, the default) or per-reflector (False) tridiagonalization
+        of the 138-512 AO class on this engine's device."""
+        rc = self.L.nddo_dbg_set_tridiag(self.h, 1 if blocked else 0)
         if rc != 0:
             raise NddoError(rc, last_error())
 
