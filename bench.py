@@ -54,7 +54,7 @@ HBM_PEAK_TBPS = 8.0           # MI355X HBM3E
 # HBM bytes per launch of the dominant kernel on the 4096-molecule pass of a
 # precision mode, from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised
 # in these files
-TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_tridiag_sym_batched_pm3_4096_{mode}.md")
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_tridiag_blk_batched_pm3_4096_{mode}.md")
 
 
 def measured_traffic(mode):
@@ -556,20 +556,20 @@ def main():
             best_mode, best = mode, lg
 
     # roofline of the dominant kernel of the step: the tridiagonalization of
-    # the matrices that do not fit LDS (k_tridiag_sym), timed with hipEvents on
-    # the engine's stream around its launches in the profiled pass (the same
-    # kernels and inputs as the timed passes); algorithmic work (4/3) n^3 FP64
-    # flops per eigensolve, one eigensolve per SCF iteration per molecule.
-    # The same launches seen as HBM traffic: the upper triangle of the trailing
-    # block read and written once per Householder step, 8 n^3 / 3 bytes with
-    # FP64 storage, 4 n^3 / 3 in the FP32 iterations of a mixed-precision job.
+    # the matrices that do not fit LDS (k_tridiag_blk, blocked), timed with
+    # hipEvents on the engine's stream around its launches in the profiled
+    # pass (the same kernels and inputs as the timed passes); algorithmic work
+    # (4/3) n^3 FP64 flops per eigensolve, one eigensolve per SCF iteration per
+    # molecule. The same launches seen as HBM traffic: the upper triangle of
+    # the trailing block read once per Householder step (4 n^3 / 3 bytes) and
+    # read and written once per panel of 8 steps (n^3 / 3), FP64 storage in
+    # both precision modes: 5 n^3 / 3 bytes, 0.8 flop per byte.
     info = engine.info()
     glb = nao > info["lds_tridiag_max_n"]
     ms_tri, launches_tri = best["stages"]["tridiag_glb"]
     flops = float(np.sum(best["iters"][glb] * (4.0 / 3.0) * nao[glb] ** 3))
     achieved = flops / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
-    it64 = best["it64"][glb]
-    alg_bytes = float(np.sum((it64 * 8.0 + (best["iters"][glb] - it64) * 4.0) / 3.0 * nao[glb] ** 3))
+    alg_bytes = float(np.sum(best["iters"][glb] * (5.0 / 3.0) * nao[glb] ** 3))
     hbm_tbs = alg_bytes / (ms_tri * 1e-3) / 1e12 if ms_tri > 0 else 0.0
     traffic = measured_traffic(best_mode) if args.count == 4096 else None
     stages = {m: stage_rooflines(lg, nao, nocc, nints, info["lds_tridiag_max_n"]) for m, lg in legs.items()}
@@ -643,9 +643,9 @@ def main():
         "plain_fp64_gpu_served_mix": plain_served,
         "speedup_vs_plain_fp64": ({"batched": round(best["solves_per_s"] / plain["solves_per_s"], 2)}
                                   if plain else None),
-        # the unblocked Householder step reads and writes the trailing triangle
-        # once per (4/3) n^3 / (8/3) n^3 = 0.5 flop per byte: HBM bounds it
-        "roofline": {"bound": "hbm", "kernel": "k_tridiag_sym (Householder tridiagonalization, device memory)",
+        # (4/3) n^3 flops over (5/3) n^3 bytes, 0.8 flop per byte, far below
+        # the FP64 ridge point (78.6 TF/s / 8 TB/s ~ 10): HBM bounds it
+        "roofline": {"bound": "hbm", "kernel": "k_tridiag_blk (blocked Householder tridiagonalization, device memory)",
                      "achieved": round(hbm_tbs * 1e3, 2), "peak": HBM_PEAK_TBPS * 1e3, "unit": "GB/s",
                      "frac": round(hbm_tbs / HBM_PEAK_TBPS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": round(alg_bytes / max(launches_tri, 1)),
