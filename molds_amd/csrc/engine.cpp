@@ -1110,8 +1110,8 @@ static int autotune(Engine* e);
 int nddo_engine_create(int hip_device, const nddo_engine_opts* opts, nddo_engine** out) {
     if (!out) { set_error("null engine pointer"); return NDDO_BAD_ARG; }
     *out = nullptr;
-    // The serving scheduler runs up to 16 cohorts on streams of their own; ask
-    // HIP for as many hardware queues (its default is 4) unless the caller chose.
+    // The serving scheduler runs up to 8 cohorts on streams of their own; ask
+    // HIP for 16 hardware queues (its default is 4) unless the caller chose.
     // Only effective when this is the process's first HIP call.
     setenv("GPU_MAX_HW_QUEUES", "16", 0);
     int ndev = 0;

@@ -7,8 +7,9 @@
 // with a stream, an event and pinned buffers from the scheduler's pools, so
 // the kernels of different cohorts overlap on the device.  The loop:
 //   1. admits queued requests as new cohorts (one per size class) when the HBM
-//      budget (the sum of job_footprint_bytes over resident jobs) and the pool
-//      allow; uploads through pinned staging and runs the setup stage;
+//      budget (the sum of job_footprint_bytes over resident jobs), the pool and
+//      the cohort slots allow (slots kept free for small molecules, see
+//      kSmallReserve); uploads through pinned staging and runs the setup stage;
 //   2. launches the next SCF iteration of every cohort that is not running one
 //      (each job keeps its geometry, integrals, density and DIIS ring resident
 //      in its cohort's arena), then copies the jobs' scalar state (phase,
@@ -82,6 +83,7 @@ struct ServedJob {
 struct Cohort {
     Batch b;
     int slot = -1;   // index into the scheduler's stream / event / pinned-buffer pools
+    int cls = 0;     // size class of its members
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
     char* h_state = nullptr;   // pinned, fine-grained: kScalarBytes per job
@@ -99,7 +101,19 @@ namespace {
 
 double secs_since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
 
-constexpr int kMaxCohorts = 16;
+// Cohorts run on streams of their own, and HIP places streams on a few
+// hardware queues (GPU_MAX_HW_QUEUES, 4 by default), where a kernel waits for
+// every kernel queued before it. A cohort of large molecules holds its slot for
+// seconds (up to max_iter iterations of tens of ms), one of small molecules
+// for milliseconds, so at most kMaxCohorts - kSmallReserve (one per hardware
+// queue) resident cohorts may hold molecules above 137 AOs, and the rest of the
+// slots are kept for the small size classes. Measured on the bench's Poisson
+// mix at 500 requests/s (mixed precision): 16 slots, no reserve: 33.8 solves/s,
+// p50 2.26 s; 12 slots, 4 reserved: 34.6, 0.87 s; 8 slots, 4 reserved: 38.3,
+// 0.40 s (requests up to 137 AOs 0.14 s); 32 slots, 8 reserved: 27.5, 3.2 s.
+constexpr int kMaxCohorts = 8;
+constexpr int kSmallReserve = 4;
+constexpr int kFirstLargeClass = 2;
 // AO count <= 64, <= 137 (eigensolver matrix in LDS), <= 256, <= 512, up to
 // the device-wide eigensolver's threshold (one workgroup per molecule; empty
 // in served mode, whose threshold is 513), larger (device-wide eigensolver,
@@ -373,12 +387,17 @@ void Engine::start_server() {
                 const size_t used = resident + results;
                 const size_t room_budget = hbm_budget > used ? hbm_budget - used : 0;
                 const size_t room = std::min(room_dev, room_budget);
+                int nlarge = 0;   // resident and new cohorts of the large size classes
+                for (auto& c : cohorts) nlarge += c->cls >= kFirstLargeClass;
                 for (auto it = queue.begin(); it != queue.end();) {
                     ServedJob* j = *it;
                     const int k = size_class(j->res.nao, served_xl_min(this));
                     if (!bucket[k]) {
                         if (nb >= free_slots.size()) { ++it; continue; }
+                        if (k >= kFirstLargeClass && nlarge >= kMaxCohorts - kSmallReserve) { ++it; continue; }
                         bucket[k].reset(new Cohort());
+                        bucket[k]->cls = k;
+                        nlarge += k >= kFirstLargeClass;
                         ++nb;
                     }
                     Cohort* c = bucket[k].get();
