@@ -201,7 +201,8 @@ size_t job_footprint_bytes(int natoms, int nao, int diis_k, int max_iter) {
     const size_t n2 = (size_t)nao * nao;
     const size_t npairs = (size_t)natoms * (natoms - 1) / 2;
     size_t b = (ARENA_FIXED + 2 * (size_t)diis_k) * n2 * 8;   // arena
-    b += npairs * (100 * 8 + 20 * 8 + 8 + sizeof(PairDev));   // integrals (upper bound), partials, ecore
+    // integrals (upper bound), partials, ecore, the pair records
+    b += npairs * (100 * 8 + 20 * 8 + 8 + sizeof(PairDev) + sizeof(int) + sizeof(FockPairDev));
     const size_t tpd = ((size_t)nao + 63) / 64;
     b += tpd * tpd * (sizeof(TileDev) + (1 + XL_REC) * 8);           // density tiles and their partials
     b += (size_t)nao * 12 + (size_t)max_iter * 32 + sizeof(JobDev) + sizeof(JobState) + XL_COEF * 8;
@@ -449,6 +450,7 @@ void Batch::plan(bool want_sdbg, bool all, std::vector<std::pair<void**, size_t>
     add(&d_atoms, std::max<size_t>(atoms.size(), 1) * sizeof(AtomDev));
     add(&d_pairs, std::max<long long>(npairs_all, 1) * sizeof(PairDev));
     add(&d_plist, std::max<long long>(npairs_all, 1) * sizeof(int));
+    add(&d_fpairs, std::max<long long>(npairs_all, 1) * sizeof(FockPairDev));
     add(&d_arows, std::max<size_t>(arows.size(), 1) * sizeof(AtomRowDev));
     add(&d_jobs, nj * sizeof(JobDev));
     add(&d_state, nj * sizeof(JobState));
@@ -616,7 +618,7 @@ int Batch::upload(hipStream_t st, const nddo_job_desc* descs, char* stage) {
 }
 
 void Batch::launch_setup_stage(Engine* e, hipStream_t st) {
-    launch_build_pairs(st, d_atoms, d_arows, (int)atoms.size(), d_jobs, e->d_elems, d_pairs, d_plist);
+    launch_build_pairs(st, d_atoms, d_arows, (int)atoms.size(), d_jobs, e->d_elems, d_pairs, d_plist, d_fpairs);
     launch_setup(st, d_pairs, d_plist, n_light, n_heavy, d_atoms, (int)atoms.size(), e->d_elems, d_jobs,
                  (int)jobs.size(), d_ints, d_arena, d_part, d_ecore, d_state, d_sdbg);
     if (d_ints32) launch_ints_f32(st, d_ints, d_ints32, nints);
@@ -627,10 +629,10 @@ FockPairs Batch::fock_pairs(int j0, int j1) const {
     FockPairs fp;
     if (jobs.empty() || j1 <= j0) return fp;
     for (int c = 0; c < 2; ++c) {
-        fp.light[c] = d_plist + cls_pos[c][j0];
+        fp.light[c] = d_fpairs + cls_pos[c][j0];
         fp.nlight[c] = (int)(cls_pos[c][j1] - cls_pos[c][j0]);
     }
-    fp.heavy = d_plist + cls_pos[2][j0];
+    fp.heavy = d_fpairs + cls_pos[2][j0];
     fp.nheavy = (int)(cls_pos[2][j1] - cls_pos[2][j0]);
     return fp;
 }
@@ -660,7 +662,7 @@ void Batch::launch_iteration_split(Engine* e, hipStream_t st) {
     hipStream_t s2 = fork_stream;
     hipEventRecord(fork_ev, st);
     hipStreamWaitEvent(s2, fork_ev, 0);
-    launch_fock(s2, 2, fock_pairs(0, js), d_pairs, d_atoms, 0, split_atom, e->d_elems, d_jobs, d_state, d_ints, d_ints32,
+    launch_fock(s2, 2, fock_pairs(0, js), d_atoms, 0, split_atom, e->d_elems, d_jobs, d_state, d_ints, d_ints32,
                 d_arena, d_part);
     launch_energy(s2, d_jobs, js, d_state, d_arena, d_remaining, d_xlpart, xl_chunks, 0);
     launch_lds_eigen(-1, s2);
@@ -669,7 +671,7 @@ void Batch::launch_iteration_split(Engine* e, hipStream_t st) {
     launch_apply(s2, d_tiles, split_tile, d_jobs, d_arena, d_xlcoef);
     hipEventRecord(join_ev, s2);
 
-    launch_fock(st, 2, fock_pairs(js, nj), d_pairs, d_atoms, split_atom, na - split_atom, e->d_elems, d_jobs, d_state,
+    launch_fock(st, 2, fock_pairs(js, nj), d_atoms, split_atom, na - split_atom, e->d_elems, d_jobs, d_state,
                 d_ints, d_ints32, d_arena, d_part, aux_stream, aux_fork, aux_join);
     launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_energy(st, d_jobs, nj - js, d_state, d_arena, d_remaining, d_xlpart, xl_chunks, js);
@@ -696,7 +698,7 @@ void Batch::launch_iteration(Engine* e, hipStream_t st) {
         return;
     }
     const int nxl = (int)list_xl.size();
-    launch_fock(st, 2, fock_pairs(0, (int)jobs.size()), d_pairs, d_atoms, 0, (int)atoms.size(), e->d_elems, d_jobs, d_state,
+    launch_fock(st, 2, fock_pairs(0, (int)jobs.size()), d_atoms, 0, (int)atoms.size(), e->d_elems, d_jobs, d_state,
                 d_ints, d_ints32, d_arena, d_part, fork_stream, fork_ev2, join_ev2);
     launch_xl_energy(st, d_jobs, d_state, d_list_xl, nxl, d_arena, d_xlpart, xl_chunks);
     launch_energy(st, d_jobs, (int)jobs.size(), d_state, d_arena, d_remaining, d_xlpart, xl_chunks);
@@ -804,7 +806,7 @@ int Batch::run_scf(Engine* e, hipStream_t st) {
             // the same launches as launch_iteration, with an event between stages
             // and between the parts of the eigensolver
             HIPCHK(hipEventRecord(e->ev[0], st));
-            launch_fock(st, 2, fock_pairs(0, (int)jobs.size()), d_pairs, d_atoms, 0, (int)atoms.size(), e->d_elems, d_jobs,
+            launch_fock(st, 2, fock_pairs(0, (int)jobs.size()), d_atoms, 0, (int)atoms.size(), e->d_elems, d_jobs,
                         d_state, d_ints, d_ints32, d_arena, d_part, fork_stream, fork_ev2, join_ev2);
             HIPCHK(hipEventRecord(e->ev[9], st));
             launch_xl_energy(st, d_jobs, d_state, d_list_xl, (int)list_xl.size(), d_arena, d_xlpart, xl_chunks);
@@ -1554,7 +1556,7 @@ int nddo_dbg_fock(nddo_engine* eh, int theory, int natoms, const int* Z, const d
     }
     if (P) HIPCHK(hipMemcpy(b.d_arena + (long long)AR_P0 * nn, P, nn * sizeof(double), hipMemcpyHostToDevice));
     b.launch_setup_stage(e, e->stream);
-    launch_fock(e->stream, 0, b.fock_pairs(0, (int)b.jobs.size()), b.d_pairs, b.d_atoms, 0, (int)b.atoms.size(), e->d_elems,
+    launch_fock(e->stream, 0, b.fock_pairs(0, (int)b.jobs.size()), b.d_atoms, 0, (int)b.atoms.size(), e->d_elems,
                 b.d_jobs, b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -1693,7 +1695,7 @@ static int autotune(Engine* e) {
                 hp.nlight[0] = hp.nlight[1] = 0;
                 rc = time_ms(e, [&] {
                     for (int r = 0; r < 4; ++r)
-                        launch_fock(e->stream, 0, hp, b.d_pairs, b.d_atoms, 0, (int)b.atoms.size(), e->d_elems,
+                        launch_fock(e->stream, 0, hp, b.d_atoms, 0, (int)b.atoms.size(), e->d_elems,
                                     b.d_jobs, b.d_state, b.d_ints, b.d_ints32, b.d_arena, b.d_part);
                 }, 3, &ms);
                 if (rc != NDDO_OK) return rc;

@@ -96,6 +96,7 @@ struct Batch {
     int* d_list_lds = nullptr;
     int* d_list_glb = nullptr;
     int* d_plist = nullptr;
+    FockPairDev* d_fpairs = nullptr;   // the pairs in d_plist order, as the Fock kernels read them
     // large molecules (kernels_xl.hip)
     int* d_list_xl = nullptr;
     long long* d_xl_off = nullptr;

@@ -63,8 +63,7 @@ __device__ inline double h1c(const ElemDev& E, int a, int b) {
 // work does not depend on it.
 template <int WPB>
 __global__ void __launch_bounds__(64 * WPB) k_fock_pairs(
-    int mode, const PairDev* __restrict__ pairs, const int* __restrict__ plist, int npairs,
-    const AtomDev* __restrict__ atoms, const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs,
+    int mode, const FockPairDev* __restrict__ fpl, int npairs, const JobDev* __restrict__ jobs,
     const JobState* __restrict__ state, const double* __restrict__ ints, const float* __restrict__ ints32,
     double* __restrict__ arena, double* __restrict__ part) {
     __shared__ double sI[WPB][100], sPab[WPB][16], sWa[WPB][10], sWb[WPB][10];
@@ -72,11 +71,11 @@ __global__ void __launch_bounds__(64 * WPB) k_fock_pairs(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int pidx = blockIdx.x * WPB + wave;
     bool act = pidx < npairs;
-    PairDev pr;
+    FockPairDev pr;
     JobDev jb;
     int phase = -1, iter = 0;
     if (act) {
-        pr = pairs[plist[pidx]];
+        pr = fpl[pidx];
         jb = jobs[pr.job];
         phase = state[pr.job].phase;
         iter = state[pr.job].iter;
@@ -88,13 +87,12 @@ __global__ void __launch_bounds__(64 * WPB) k_fock_pairs(
     if (act) {
         const bool scf = phase == PH_ACTIVE;   // else the final-density Fock of the energy
         f32 = scf && ints32 != nullptr && state[pr.job].fp32 != 0;
-        const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
-        nA = elems[aA.elem].nao;
-        nB = elems[aB.elem].nao;
+        nA = fp_nao(pr.ao_a);
+        nB = fp_nao(pr.ao_b);
         nap = nA == 4 ? 10 : 1;
         nbp = nB == 4 ? 10 : 1;
-        aoA = aA.ao;
-        aoB = aB.ao;
+        aoA = fp_ao(pr.ao_a);
+        aoB = fp_ao(pr.ao_b);
         n = jb.nao;
         guess = scf && iter == 0 && jb.requires_guess;
         int pk = scf ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1));
@@ -312,20 +310,18 @@ __device__ inline void fock_light_pair(int nA, int nB, const R* I, const double*
 }
 
 __global__ void __launch_bounds__(256) k_fock_light(
-    int mode, const PairDev* __restrict__ pairs, const int* __restrict__ plist, int nlight,
-    const AtomDev* __restrict__ atoms, const ElemDev* __restrict__ elems, const JobDev* __restrict__ jobs,
+    int mode, const FockPairDev* __restrict__ fpl, int nlight, const JobDev* __restrict__ jobs,
     const JobState* __restrict__ state, const double* __restrict__ ints, const float* __restrict__ ints32,
     double* __restrict__ arena, double* __restrict__ part) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= nlight) return;
-    const PairDev pr = pairs[plist[idx]];
+    const FockPairDev pr = fpl[idx];
     const JobState& st = state[pr.job];
     const int phase = st.phase, iter = st.iter;
     if (!((mode != 1 && phase == PH_ACTIVE) || (mode != 0 && phase == PH_ENERGY))) return;
     const bool scf = phase == PH_ACTIVE;   // else the final-density Fock of the energy
     const JobDev& jb = jobs[pr.job];
-    const AtomDev aA = atoms[pr.a], aB = atoms[pr.b];
-    const int nA = elems[aA.elem].nao, nB = elems[aB.elem].nao;
+    const int nA = fp_nao(pr.ao_a), nB = fp_nao(pr.ao_b), aoA = fp_ao(pr.ao_a), aoB = fp_ao(pr.ao_b);
     const bool guess = scf && iter == 0 && jb.requires_guess;
     const bool f32 = scf && ints32 != nullptr && st.fp32 != 0;
     const double* P = mat(arena, jb, scf ? (AR_P0 + (iter & 1)) : (AR_P0 + ((iter + 1) & 1)));
@@ -337,12 +333,12 @@ __global__ void __launch_bounds__(256) k_fock_light(
         float I[10];
 #pragma unroll
         for (int e = 0; e < 10; ++e) I[e] = e < ni ? ints32[pr.ioff + e] : 0.0f;
-        fock_light_pair<float>(nA, nB, I, P, jb.nao, aA.ao, aB.ao, H, F, pp, false);
+        fock_light_pair<float>(nA, nB, I, P, jb.nao, aoA, aoB, H, F, pp, false);
     } else {
         double I[10];
 #pragma unroll
         for (int e = 0; e < 10; ++e) I[e] = (e < ni && !guess) ? ints[pr.ioff + e] : 0.0;
-        fock_light_pair<double>(nA, nB, I, P, jb.nao, aA.ao, aB.ao, H, F, pp, guess);
+        fock_light_pair<double>(nA, nB, I, P, jb.nao, aoA, aoB, H, F, pp, guess);
     }
 }
 
@@ -412,7 +408,7 @@ __global__ void __launch_bounds__(256) k_fock_atoms(
     }
 }
 
-void launch_fock(hipStream_t st, int mode, const FockPairs& fp, const PairDev* pairs, const AtomDev* atoms, int atom0,
+void launch_fock(hipStream_t st, int mode, const FockPairs& fp, const AtomDev* atoms, int atom0,
                  int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
                  const float* ints32, double* arena, double* part, hipStream_t st2, hipEvent_t fork_ev,
                  hipEvent_t join_ev) {
@@ -428,15 +424,15 @@ void launch_fock(hipStream_t st, int mode, const FockPairs& fp, const PairDev* p
     for (int c = 0; c < 2; ++c)
         if (fp.nlight[c] > 0)
             hipLaunchKernelGGL(k_fock_light, dim3((fp.nlight[c] + 255) / 256), dim3(256), 0, fork ? st2 : st, mode,
-                               pairs, fp.light[c], fp.nlight[c], atoms, elems, jobs, state, ints, ints32, arena, part);
+                               fp.light[c], fp.nlight[c], jobs, state, ints, ints32, arena, part);
     if (fork) hipEventRecord(join_ev, st2);
     if (fp.nheavy > 0) {
         const int npairs = fp.nheavy;
-        const int* hl = fp.heavy;
+        const FockPairDev* hl = fp.heavy;
         const int w = get_tuning().fock_wpb;
-#define NDDO_FOCK_PAIRS(W)                                                                                       \
-    hipLaunchKernelGGL(k_fock_pairs<W>, dim3((npairs + (W)-1) / (W)), dim3(64 * (W)), 0, st, mode, pairs, hl, npairs, \
-                       atoms, elems, jobs, state, ints, ints32, arena, part)
+#define NDDO_FOCK_PAIRS(W)                                                                                 \
+    hipLaunchKernelGGL(k_fock_pairs<W>, dim3((npairs + (W)-1) / (W)), dim3(64 * (W)), 0, st, mode, hl, npairs, \
+                       jobs, state, ints, ints32, arena, part)
         if (w == 1) NDDO_FOCK_PAIRS(1);
         else if (w == 2) NDDO_FOCK_PAIRS(2);
         else if (w == 8) NDDO_FOCK_PAIRS(8);

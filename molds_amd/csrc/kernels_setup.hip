@@ -505,7 +505,7 @@ __global__ void __launch_bounds__(256) k_build_pairs(const AtomDev* __restrict__
                                                      const AtomRowDev* __restrict__ rows, int natoms,
                                                      const JobDev* __restrict__ jobs,
                                                      const ElemDev* __restrict__ elems, PairDev* __restrict__ pairs,
-                                                     int* __restrict__ plist) {
+                                                     int* __restrict__ plist, FockPairDev* __restrict__ fpairs) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= natoms) return;
     const AtomDev at = atoms[g];
@@ -513,20 +513,24 @@ __global__ void __launch_bounds__(256) k_build_pairs(const AtomDev* __restrict__
     const AtomRowDev rw = rows[g];
     long long slot = rw.slot0, io = rw.ioff0, c0 = rw.cpos[0], c1 = rw.cpos[1], c2 = rw.cpos[2];
     const bool ha = elems[at.elem].nao == 1;
+    const int aoa = at.ao | (elems[at.elem].nao << 24);
     for (int b = a + 1; b < nat; ++b, ++slot) {
-        const bool hb = elems[atoms[atom0 + b].elem].nao == 1;
+        const AtomDev atb = atoms[atom0 + b];
+        const int nb = elems[atb.elem].nao;
+        const bool hb = nb == 1;
         pairs[slot] = PairDev{g, atom0 + b, at.job, (int)slot, io};
-        io += (ha ? 1 : 10) * (hb ? 1 : 10);
         const long long pos = (ha && hb) ? c0++ : ((ha || hb) ? c1++ : c2++);
         plist[pos] = (int)slot;
+        fpairs[pos] = FockPairDev{at.job, aoa, atb.ao | (nb << 24), (int)slot, io};
+        io += (ha ? 1 : 10) * (hb ? 1 : 10);
     }
 }
 
 void launch_build_pairs(hipStream_t st, const AtomDev* atoms, const AtomRowDev* rows, int natoms, const JobDev* jobs,
-                        const ElemDev* elems, PairDev* pairs, int* plist) {
+                        const ElemDev* elems, PairDev* pairs, int* plist, FockPairDev* fpairs) {
     if (natoms > 0)
         hipLaunchKernelGGL(k_build_pairs, dim3((natoms + 255) / 256), dim3(256), 0, st, atoms, rows, natoms, jobs,
-                           elems, pairs, plist);
+                           elems, pairs, plist, fpairs);
 }
 
 void launch_setup(hipStream_t st, const PairDev* pairs, const int* plist, int nlight, int nheavy,

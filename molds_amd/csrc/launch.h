@@ -22,7 +22,7 @@ int upload_overlap_table(const double* table /* 32*25 */);
 
 // The batch's pair list and its class-sorted index from the per-atom row starts.
 void launch_build_pairs(hipStream_t st, const AtomDev* atoms, const AtomRowDev* rows, int natoms, const JobDev* jobs,
-                        const ElemDev* elems, PairDev* pairs, int* plist);
+                        const ElemDev* elems, PairDev* pairs, int* plist, FockPairDev* fpairs);
 void launch_setup(hipStream_t st, const PairDev* pairs, const int* plist, int nlight, int nheavy,
                   const AtomDev* atoms, int natoms, const ElemDev* elems, const JobDev* jobs, int njobs, double* ints,
                   double* arena, double* part, double* ecore, JobState* state, double* sdbg);
@@ -43,12 +43,12 @@ void launch_ints_f32(hipStream_t st, const double* ints, float* ints32, long lon
 // st2 (optional): the light pairs run on it, forked from st by fork_ev and
 // joined by join_ev before the per-atom reduction on st.
 struct FockPairs {
-    const int* light[2] = {nullptr, nullptr};
+    const FockPairDev* light[2] = {nullptr, nullptr};
     int nlight[2] = {0, 0};
-    const int* heavy = nullptr;
+    const FockPairDev* heavy = nullptr;
     int nheavy = 0;
 };
-void launch_fock(hipStream_t st, int mode, const FockPairs& fp, const PairDev* pairs, const AtomDev* atoms, int atom0,
+void launch_fock(hipStream_t st, int mode, const FockPairs& fp, const AtomDev* atoms, int atom0,
                  int natoms, const ElemDev* elems, const JobDev* jobs, const JobState* state, const double* ints,
                  const float* ints32, double* arena, double* part, hipStream_t st2 = nullptr,
                  hipEvent_t fork_ev = nullptr, hipEvent_t join_ev = nullptr);

@@ -47,6 +47,19 @@ struct PairDev {
 
 // Row a of a job's pair triangle (pairs (a, b), b > a, in order): where its
 // entries go, from which k_build_pairs writes the pair list on the device.
+// A pair as the Fock kernels read it, in the class-sorted order of the pair
+// list (written by k_build_pairs): everything but the job's own state and
+// matrices in one record, so a pair's density and integrals are one dependent
+// load away. ao_a / ao_b: the atom's job-local first AO, its AO count in bits 24+.
+struct FockPairDev {
+    int job;
+    int ao_a, ao_b;
+    int slot;          // global pair slot (Coulomb partials)
+    long long ioff;    // offset of the packed integrals (doubles)
+};
+__host__ __device__ inline int fp_ao(int x) { return x & 0xFFFFFF; }
+__host__ __device__ inline int fp_nao(int x) { return x >> 24; }
+
 struct AtomRowDev {
     long long slot0;    // global slot of pair (a, a+1)
     long long ioff0;    // integral offset (doubles) of that pair
