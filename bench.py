@@ -344,13 +344,15 @@ def _host_cpu():
 
 
 def cpu_leg(mols, seconds, workers):
-    """The oracle (C restatement of MolDS's integrals / Fock / core repulsion +
-    numpy LAPACK dsyevd SCF driver) on the same molecules, in order, on the
-    host: (a) one thread, one molecule after another; (b) one molecule after
-    another with LAPACK threaded over every core (as MolDS runs); (c) `workers`
-    processes, one molecule per process, one thread each.  Each runs until `seconds` of
-    wall time: a bounded sample of the workload.  Runs before the engine
-    touches the GPU, so the pool's processes start from a GPU-free parent."""
+    """The oracle (C restatement of MolDS's integrals / Fock / core repulsion,
+    OpenMP over atom rows and pairs as MolDS, + numpy LAPACK dsyevd SCF
+    driver) on the same molecules, in order, on the host: (a) one thread, one
+    molecule after another; (b) one molecule after another with the OpenMP
+    Fock and LAPACK threaded over every core, as MolDS runs (the reported
+    `value`); (c) `workers` processes, one molecule per process, one thread
+    each.  Each runs until `seconds` of wall time: a bounded sample of the
+    workload.  Runs before the engine touches the GPU, so the pool's processes
+    start from a GPU-free parent."""
     import multiprocessing as mp
     from threadpoolctl import threadpool_limits
     from oracle import scf as O
@@ -386,9 +388,9 @@ def cpu_leg(mols, seconds, workers):
             if time.perf_counter() - t0 > seconds:
                 break
     dt = time.perf_counter() - t0
-    out["sequential_all_cores"] = dict(value=round(done / dt, 3), unit="solves/s", cores=nthr,
-                                       sample=f"first {done} molecules of the mix ({conv} converged), in order, "
-                                              f"LAPACK on {nthr} threads (the C Fock is serial), {dt:.1f} s")
+    seq = dict(value=round(done / dt, 3), unit="solves/s", cores=nthr,
+               sample=f"first {done} molecules of the mix ({conv} converged), in order, OpenMP Fock and LAPACK on "
+                      f"{nthr} threads (MolDS's way: one molecule at a time, every core on it), {dt:.1f} s")
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     done = conv = 0
@@ -400,10 +402,12 @@ def cpu_leg(mols, seconds, workers):
                 break
         pool.terminate()
     dt = time.perf_counter() - t0
-    out.update(value=round(done / dt, 3), unit="solves/s", cores=workers,
-               sample=f"first {done} molecules of the same PM3 mix ({conv} converged), one molecule per process on "
-                      f"{workers} processes of one thread each, {dt:.1f} s (oracle: C Fock/integrals + numpy LAPACK "
-                      f"dsyevd); one_thread: the same code on one core, molecules in order")
+    out["molecule_parallel"] = dict(value=round(done / dt, 3), unit="solves/s", cores=workers,
+                                    sample=f"first {done} molecules of the mix ({conv} converged), one molecule per "
+                                           f"process on {workers} processes of one thread each, {dt:.1f} s")
+    out.update(seq)
+    out["sample"] += (" (oracle: C Fock / integrals + numpy LAPACK dsyevd); one_thread: the same code on one core; "
+                      "molecule_parallel: one molecule per single-threaded process")
     return out
 
 

@@ -459,8 +459,9 @@ void* orc_create(int theory, int natoms, const int* Z, const double* xyz_bohr, i
             m->dist[a * natoms + b] = d; m->dist[b * natoms + a] = d;
         }
 
-    /* overlap, Cndo2.cpp:4030-4146 */
+    /* overlap, Cndo2.cpp:4030-4146; atom pairs in parallel (each writes its own block) */
     m->S = (double*)calloc((size_t)nao * nao, sizeof(double));
+#pragma omp parallel for schedule(dynamic)
     for (int a = 0; a < natoms; a++)
         for (int b = a + 1; b < natoms; b++) {
             const orc_elem *A = &m->el[a], *B = &m->el[b];
@@ -493,6 +494,7 @@ void* orc_create(int theory, int natoms, const int* Z, const double* xyz_bohr, i
 
     /* two-centre integrals, Mndo.cpp:3703-3820, 3948-4025, 4223-4293 */
     m->g = (double*)calloc((size_t)natoms * natoms * 256, sizeof(double));
+#pragma omp parallel for schedule(dynamic)
     for (int a = 0; a < natoms; a++)
         for (int b = a + 1; b < natoms; b++) {
             const orc_elem *A = &m->el[a], *B = &m->el[b];
@@ -593,13 +595,17 @@ static double orc_h1(const orc_elem* e, int a, int b)
 }
 
 /* Fock matrix (Cndo2.cpp:1526-1636 with Mndo.cpp:422-650).  The upper triangle
- * is computed element by element as the reference does, then mirrored.
- * P is n x n row-major, F is written as a full symmetric n x n. */
+ * is computed element by element as the reference does, then mirrored; the
+ * atom rows A in parallel as the reference's OpenMP loop (Cndo2.cpp:1555),
+ * each element by one thread in a fixed order, so the result does not depend
+ * on the thread count.  P is n x n row-major, F is written as a full
+ * symmetric n x n. */
 void orc_fock(void* h, const double* P, int is_guess, double* F)
 {
     orc_mol* m = (orc_mol*)h;
     int n = m->nao, N = m->natoms;
     memset(F, 0, sizeof(double) * n * n);
+#pragma omp parallel for schedule(dynamic)
     for (int A = 0; A < N; A++) {
         const orc_elem* eA = &m->el[A];
         int fA = m->first[A];
@@ -660,6 +666,7 @@ void orc_fock(void* h, const double* P, int is_guess, double* F)
             }
         }
     }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++)
         for (int j = i + 1; j < n; j++) F[(size_t)j * n + i] = F[(size_t)i * n + j];
 }

@@ -51,6 +51,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             build()
+        # the C oracle's OpenMP threads and numpy's BLAS threads alternate
+        # (Fock, then eigensolve): threads that spin while idle would take the
+        # cores from the other pool, so they sleep instead (read when the
+        # OpenMP runtime loads, with the library below)
+        os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
         L = ctypes.CDLL(LIBPATH)
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int)
