@@ -269,6 +269,12 @@ int nddo_dbg_set_xl_grid(nddo_engine* e, int tri_workgroups);
  * with a rank-16 FP64 MFMA update per panel (1, the default), or one rank-2
  * update per reflector (0).  Both store the same reflectors and tridiagonal. */
 int nddo_dbg_set_tridiag(nddo_engine* e, int blocked);
+/* Test hook: the device-wide tridiagonalization of large molecules on this
+ * engine's device: blocked (panels of 8 reflectors whose columns only read the
+ * trailing block, one rank-16 FP64 MFMA update per panel) from 5600 AOs and
+ * one rank-2 update per reflector below (1, the default), always blocked (2),
+ * or never (0).  Both store the same reflectors and tridiagonal. */
+int nddo_dbg_set_xl_tridiag(nddo_engine* e, int mode);
 
 #ifdef __cplusplus
 }

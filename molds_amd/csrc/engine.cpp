@@ -1790,6 +1790,17 @@ int nddo_dbg_set_tridiag(nddo_engine* eh, int blocked) {
     return NDDO_OK;
 }
 
+int nddo_dbg_set_xl_tridiag(nddo_engine* eh, int mode) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || mode < 0 || mode > 2) return NDDO_BAD_ARG;
+    SyncRun lk(e);
+    HIPCHK(hipSetDevice(e->device));
+    Tuning t = get_tuning();
+    t.xl_tri_blocked = mode;
+    set_tuning(t);
+    return NDDO_OK;
+}
+
 int nddo_dbg_set_xl_grid(nddo_engine* eh, int tri_workgroups) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || tri_workgroups < 0) return NDDO_BAD_ARG;

@@ -47,7 +47,7 @@ constexpr int XT = 256;             // threads per workgroup
 constexpr int XW = XT / 64;         // wavefronts per workgroup
 constexpr int XGRID_MAX = 1024;     // workgroups per persistent launch
 constexpr int XWAVE_MAX = XGRID_MAX * XW;
-constexpr int kXlhMaxN = 6400;      // k_xtri keeps v, w and the next v in LDS: 3 n doubles
+constexpr int kXlhMaxN = 6144;      // k_xtri(_blk) keep v, w and the next v in LDS: 3 n doubles
 
 // Grid barrier in two levels, so that no counter sees more than 16 atomic
 // arrivals: the workgroups of each group of XB count on the group's counter,
@@ -128,8 +128,15 @@ __device__ inline XGroup x_group(int nlist, unsigned* bar) {
 __device__ inline double grid_partials_sum(const double* q, int nwaves, double* sh) {
     const int tid = threadIdx.x, lane = tid & 63;
     if (tid < 64) {
-        double s = 0.0;
-        for (int g = lane; g < nwaves; g += 64) s += q[g];
+        // four loads in flight per lane (one pass up to 256 partials)
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int g = lane; g < nwaves; g += 256) {
+            s0 += q[g];
+            if (g + 64 < nwaves) s1 += q[g + 64];
+            if (g + 128 < nwaves) s2 += q[g + 128];
+            if (g + 192 < nwaves) s3 += q[g + 192];
+        }
+        double s = (s0 + s1) + (s2 + s3);
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
         if (lane == 0) *sh = s;
     }
@@ -137,12 +144,26 @@ __device__ inline double grid_partials_sum(const double* q, int nwaves, double* 
     return *sh;
 }
 
+constexpr int XNB = 8;              // reflectors per panel of the blocked k_xtri_blk
+constexpr int XSR = 6 * XNB;        // k_xtri_blk's per-workgroup record of panel dot partials
+// Size from which k_xtri_blk is the default.  Measured on MI355X (ms per
+// tridiagonalization, per-reflector / blocked): n = 4608: 149 / 165; n = 5200:
+// 198 / 203; n = 6000: 293 / 273.  The row passes of both are bound by the
+// loads one wavefront keeps in flight on its one or two rows, not by HBM
+// bytes, so halving the bytes pays only where the trailing blocks are large.
+constexpr int kXtriBlkMinN = 5600;
+
 // Workspace of one job in xlws (doubles): p[2][n], per-wave partials [2][XWAVE_MAX],
-// then the divide-and-conquer tables (12 n doubles, 9 n ints).
+// the divide-and-conquer tables (12 n doubles, 9 n ints), then k_xtri_blk's
+// w vectors of the current panel (XNB rows of n, by global index), its next
+// pivot row [2][n] and its panel dot partials [2][XSR][XGRID_MAX].
 __host__ __device__ inline long long xw_p(int) { return 0; }
 __host__ __device__ inline long long xw_q(int n) { return 2LL * n; }
 __host__ __device__ inline long long xw_dc(int n) { return 2LL * n + 2LL * XWAVE_MAX; }
-__host__ __device__ inline long long xw_size(int n) { return xw_dc(n) + 12LL * n + (9LL * n + 1) / 2 + 64; }
+__host__ __device__ inline long long xw_wp(int n) { return xw_dc(n) + 12LL * n + (9LL * n + 1) / 2 + 64; }
+__host__ __device__ inline long long xw_xq(int n) { return xw_wp(n) + (long long)XNB * n; }
+__host__ __device__ inline long long xw_xs(int n) { return xw_xq(n) + 2LL * n; }
+__host__ __device__ inline long long xw_size(int n) { return xw_xs(n) + 2LL * XGRID_MAX * XSR; }
 
 // ------------------------------------------------------------------ tridiagonalization
 constexpr int TT = 1024;            // threads per workgroup of k_xtri (16 wavefronts: memory parallelism)
@@ -376,6 +397,482 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
             continue;
         }
         // keep the grid together before the next job reuses the LDS vectors
+        alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
+        if (wg == 0 && tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
+    }
+}
+
+// ------------------------------------------------------------------ blocked tridiagonalization
+// k_xtri_blk: the reduction of k_xtri in panels of XNB reflectors (the
+// two-sided rank-2 form of LAPACK dsytrd / dlatrd).  Within a panel that
+// starts at reflector k0 the stored trailing block stays A_p, the matrix after
+// reflector k0 - 1, and the current one is
+// A_p - sum_{k0 <= j <= k} (v_j w_j^T + w_j v_j^T).  Step k:
+//   - every workgroup forms w_k = p_k + K v_k, then the pivot row k+1 from
+//     x'[c] (row k+1 of A_p less the panel's terms before reflector k, by
+//     symmetry column k+1, formed by the row owners in the previous step's
+//     row pass) and the rank-2 term of reflector k;
+//   - the panel's dot products with the new reflector, w_j . v_{k+1} and
+//     v_j . v_{k+1}, come from grid partials of v_j . x', v_j . v_k and
+//     v_j . p_k (and the same for w_j) that the row owners also accumulated,
+//     plus the two of reflector k from LDS;
+//   - the row pass only reads A_p: (A v_{k+1})_r = (A_p v_{k+1})_r
+//     - sum_j (v_j[r] (w_j . v_{k+1}) + w_j[r] (v_j . v_{k+1})), with one
+//     panel vector per lane (lane l: v_l, lane XNB + l: w_l);
+//   - after the panel's last reflector a pass over the trailing block applies
+//     the panel's rank-2 XNB update on FP64 MFMA, then a grid barrier, then
+//     the row pass against the new A_p.
+// A column thus reads the trailing block (8 m^2 bytes) where k_xtri reads and
+// writes it (16 m^2), plus one read and write per panel.  v_j are rows of AR_W
+// (stored there for the back-transform anyway), w_j rows of the panel buffer
+// in the job's workspace; the newest w stays in LDS.  Every workgroup makes the
+// same fixed-order sums, so all hold bitwise the same reflectors.
+
+// Raw buffer loads of the panel vectors: the element's byte offset within a
+// row in a VGPR, the row's in a scalar register (no 64-bit address per row).
+typedef unsigned int xu32;
+__device__ inline __amdgpu_buffer_rsrc_t xrsrc(const double* base, long long ndoubles) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, (int)(ndoubles * 8), 0x00020000);
+}
+__device__ inline double xbuf_load(__amdgpu_buffer_rsrc_t r, xu32 voff, xu32 soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// Three block-wide sums at once (fixed order); every thread gets them in out[].
+__device__ inline void block_sum3(double a, double b, double c, double* red3, double* out, int tid) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
+    }
+    if ((tid & 63) == 0) {
+        red3[3 * (tid >> 6)] = a;
+        red3[3 * (tid >> 6) + 1] = b;
+        red3[3 * (tid >> 6) + 2] = c;
+    }
+    __syncthreads();
+    if (tid < 3) {
+        double s = 0.0;
+        for (int w = 0; w < TW; ++w) s += red3[3 * w + tid];
+        out[tid] = s;
+    }
+    __syncthreads();
+}
+
+// A[r][c] -= sum_{l < XNB} (V_l[r] W_l[c] + W_l[r] V_l[c]) for r, c in [r0, n):
+// V_l = row k0 + l of Wm, W_l = row l of Wp (both by global index), the last
+// W_l from LDS (wl[t] <-> index r0 - 1 + t).  One wavefront per 16 x 16 tile,
+// K = 2 XNB = 16 in four 16x16x4 FP64 MFMAs: A operand row rb + lane % 16,
+// k = lane / 16 of the step (V, then W); B operand column cb + lane % 16 (W,
+// then V); the accumulator holds rows rb + 4 v + lane / 16.
+__device__ inline void xtri_panel_update(double* __restrict__ A, const double* __restrict__ Wm,
+                                         const double* __restrict__ Wp, const double* wl, int n, int k0, int r0,
+                                         int gw, int GW, int lane) {
+    static_assert(2 * XNB == 16, "four MFMA k-steps");
+    const long long ln = n;
+    const int nt = (n - r0 + 15) >> 4;
+    const int ntiles = nt * nt;
+    const int lr = lane & 15, lk = lane >> 4;
+    // two tiles per pass (tile and tile + GW), their loads in flight together
+    for (int tile0 = gw; tile0 < ntiles; tile0 += 2 * GW) {
+        int rbs[2], cbs[2];
+        bool on[2];
+        dbl4 acc[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int tile = tile0 + h * GW;
+            on[h] = tile < ntiles;
+            const int I = on[h] ? tile / nt : 0, J = on[h] ? tile - I * nt : 0;
+            rbs[h] = r0 + 16 * I;
+            cbs[h] = r0 + 16 * J;
+            const int c = cbs[h] + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rbs[h] + 4 * v + lk;
+                acc[h][v] = (on[h] && r < n && c < n) ? A[(long long)r * ln + c] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kk = 4 * q + lk;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int ra = rbs[h] + lr, c = cbs[h] + lr;
+                double a = 0.0, b = 0.0;
+                if (q < 2) {   // V_kk (rows) against W_kk (columns)
+                    if (ra < n) a = Wm[(long long)(k0 + kk) * ln + ra];
+                    if (c < n) b = kk == XNB - 1 ? wl[c - r0 + 1] : Wp[(long long)kk * ln + c];
+                } else {       // W_l (rows) against V_l (columns)
+                    const int l = kk - XNB;
+                    if (ra < n) a = l == XNB - 1 ? wl[ra - r0 + 1] : Wp[(long long)l * ln + ra];
+                    if (c < n) b = Wm[(long long)(k0 + l) * ln + c];
+                }
+                acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a, b, acc[h], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = cbs[h] + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rbs[h] + 4 * v + lk;
+                if (on[h] && r < n && c < n) A[(long long)r * ln + c] = acc[h][v];
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(TT) k_xtri_blk(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
+                                                 const int* __restrict__ joblist, const long long* __restrict__ xloff,
+                                                 int nlist, double* __restrict__ arena, double* __restrict__ tri,
+                                                 double* __restrict__ xlws, unsigned* __restrict__ bar) {
+    extern __shared__ double dyn[];
+    __shared__ double red[TW], red2[TW], bc[2], red3[3 * TW], sum3[3];
+    // v_j and w_j of the panel's earlier reflectors at indices k+1 and k+2
+    __shared__ double s1v[XNB], s1w[XNB], s2v[XNB], s2w[XNB];
+    // al[l] = w_l . v_{k+1}, be[l] = v_l . v_{k+1} over indices >= k+2 (l <= i)
+    __shared__ double al[XNB], be[XNB];
+    // grid sums of the panel partials (per panel lane L: P_L . x', P_L . v_k,
+    // P_L . p_k at L, 2 XNB + L, 4 XNB + L) and their per-wavefront terms
+    __shared__ double sS[XSR], redS[TW * XSR];
+    __shared__ int sh_ok;
+    const XGroup xg = x_group(nlist, bar);
+    const int G = xg.G, wg = xg.wg, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int gw = wg * TW + wave, GW = G * TW;
+    unsigned gen = 0;
+    bool alive = true;
+    for (int jl = xg.grp; jl < nlist; jl += xg.ngroups) {
+        const int job = joblist[jl];
+        if (state[job].phase != PH_ACTIVE) continue;   // read-only here: the same in every workgroup
+        const unsigned long long t0 = wall_clock64();
+        const JobDev jb = jobs[job];
+        const int n = jb.nao;
+        const long long ln = n;
+        double* A = amat(arena, jb, AR_U);   // A_p, full and symmetric
+        double* Wm = amat(arena, jb, AR_W);
+        const double* F = amat(arena, jb, AR_F);
+        double* d = eig_ws(tri, jb, job);
+        double* e = d + n;
+        double* tau = e + n;
+        double* ws = xlws + xloff[jl];
+        double* pb[2] = {ws + xw_p(n), ws + xw_p(n) + n};
+        double* qb[2] = {ws + xw_q(n), ws + xw_q(n) + XGRID_MAX};
+        double* xq[2] = {ws + xw_xq(n), ws + xw_xq(n) + n};
+        double* xs[2] = {ws + xw_xs(n), ws + xw_xs(n) + (long long)XGRID_MAX * XSR};
+        double* Wp = ws + xw_wp(n);
+        double* vcur = dyn;
+        double* vnext = dyn + n;
+        double* wv = dyn + 2 * n;
+        if (!alive) {
+            if (wg == 0 && tid == 0) state[job].sweeps = -1;
+            continue;
+        }
+        if (wg == 0)
+            for (int i = tid; i < n; i += TT) { e[i] = 0.0; tau[i] = 0.0; }
+        if (n < 3) {
+            if (wg == 0 && tid == 0) {
+                d[0] = F[0];
+                if (n == 2) { e[0] = F[1]; d[1] = F[3]; }
+                state[job].prof[0] = (double)(wall_clock64() - t0);
+            }
+            continue;
+        }
+        // reflector of row 0 from F, then A_p = F (rows 1 .. n-1), p = tau_0 A v_0
+        // and x' = column 1 of F
+        double tk;
+        {
+            double ss = 0.0;
+            for (int t = 2 + tid; t < n; t += TT) { const double x = F[t]; ss += x * x; }
+            ss = block_sum(ss, red, tid, TW);
+            const double alpha = F[1];
+            double beta = alpha, scale = 0.0;
+            tk = 0.0;
+            if (ss > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+                tk = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            for (int t = 1 + tid; t < n - 1; t += TT) vcur[t] = ss > 0.0 ? F[1 + t] * scale : 0.0;
+            if (tid == 0) vcur[0] = 1.0;
+            __syncthreads();
+            if (wg == 0) {
+                for (int t = tid; t < n - 1; t += TT) Wm[1 + t] = vcur[t];
+                if (tid == 0) { d[0] = F[0]; e[0] = beta; tau[0] = tk; }
+            }
+            double pv = 0.0;
+            for (int r = 1 + (gw - 1 + GW) % GW; r < n; r += GW) {
+                double s = 0.0;
+                int c = 1 + lane;
+                for (; c + 64 * 7 < n; c += 64 * 8) {
+                    double a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = F[(long long)r * ln + c + 64 * u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        A[(long long)r * ln + c + 64 * u] = a[u];
+                        s += a[u] * vcur[c + 64 * u - 1];
+                    }
+                }
+                if (c < n) {
+                    double a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = c + 64 * u < n ? F[(long long)r * ln + c + 64 * u] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (c + 64 * u < n) {
+                            A[(long long)r * ln + c + 64 * u] = a[u];
+                            s += a[u] * vcur[c + 64 * u - 1];
+                        }
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+                const double pn = tk * s;
+                if (lane == 0) {
+                    pb[0][r - 1] = pn;
+                    xq[0][r] = F[(long long)r * ln + 1];
+                }
+                pv += pn * vcur[r - 1];
+            }
+            if (lane == 0) red2[wave] = pv;
+            __syncthreads();
+            if (tid == 0) {
+                double s = 0.0;
+                for (int w = 0; w < TW; ++w) s += red2[w];
+                qb[0][wg] = s;
+            }
+        }
+        alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
+        int cur = 0, k0 = 0;
+        for (int k = 0; alive && k <= n - 3; ++k) {
+            const int m = n - 1 - k;   // length of v_k (indices t <-> row / column k + 1 + t)
+            const int i = k - k0;      // earlier reflectors of the panel: k0 .. k-1
+            // the panel's v_l, w_l at indices k+1, k+2 (l < i), and the grid
+            // sums of its dot partials (wavefront 2 + l: v_l, 9 + l: w_l),
+            // loaded beside the partials of p . v
+            if (tid >= 64 && tid < 64 + i) {
+                const int l = tid - 64;
+                const long long vrow = (long long)(k0 + l) * ln, wrow = (long long)l * ln;
+                s1v[l] = Wm[vrow + k + 1];
+                s1w[l] = Wp[wrow + k + 1];
+                s2v[l] = Wm[vrow + k + 2];
+                s2w[l] = Wp[wrow + k + 2];
+            }
+            if (wave >= 2) {
+                const int l = wave < 9 ? wave - 2 : wave - 9;
+                const int L = wave < 9 ? l : XNB + l;
+                if (l < i) {
+                    double a1 = 0.0, a2 = 0.0, a3 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+                    const double* q1 = xs[cur] + (long long)L * XGRID_MAX;
+                    const double* q2 = xs[cur] + (long long)(2 * XNB + L) * XGRID_MAX;
+                    const double* q3 = xs[cur] + (long long)(4 * XNB + L) * XGRID_MAX;
+                    for (int g = lane; g < G; g += 128) {   // two partials of each in flight per lane
+                        a1 += q1[g];
+                        a2 += q2[g];
+                        a3 += q3[g];
+                        if (g + 64 < G) {
+                            b1 += q1[g + 64];
+                            b2 += q2[g + 64];
+                            b3 += q3[g + 64];
+                        }
+                    }
+                    a1 += b1;
+                    a2 += b2;
+                    a3 += b3;
+                    for (int off = 32; off > 0; off >>= 1) {
+                        a1 += __shfl_xor(a1, off);
+                        a2 += __shfl_xor(a2, off);
+                        a3 += __shfl_xor(a3, off);
+                    }
+                    if (lane == 0) {
+                        sS[L] = a1;
+                        sS[2 * XNB + L] = a2;
+                        sS[4 * XNB + L] = a3;
+                    }
+                }
+            }
+            const double pv = grid_partials_sum(qb[cur], G, &bc[0]);   // its barrier publishes the above
+            const double K = -0.5 * tk * pv;   // w = p + K v
+            for (int t0 = tid; t0 < m; t0 += TT * XP) {
+                double pr[XP];
+#pragma unroll
+                for (int u = 0; u < XP; ++u) pr[u] = t0 + TT * u < m ? pb[cur][t0 + TT * u] : 0.0;
+#pragma unroll
+                for (int u = 0; u < XP; ++u)
+                    if (t0 + TT * u < m) wv[t0 + TT * u] = pr[u] + K * vcur[t0 + TT * u];
+            }
+            __syncthreads();
+            if (wg == 0)   // w_k for the later steps of the panel (read after the next grid barrier)
+                for (int t = tid; t < m; t += TT) Wp[(long long)i * ln + k + 1 + t] = wv[t];
+            if (k == n - 3) {   // last trailing 2 x 2 block (rows / columns n-2, n-1)
+                if (wg == 0 && tid == 0) {
+                    double d0 = A[(long long)(n - 2) * ln + n - 2];
+                    double e0 = A[(long long)(n - 2) * ln + n - 1];
+                    double d1 = A[(long long)(n - 1) * ln + n - 1];
+                    for (int l = 0; l < i; ++l) {
+                        d0 -= 2.0 * s1v[l] * s1w[l];
+                        e0 -= s1v[l] * s2w[l] + s1w[l] * s2v[l];
+                        d1 -= 2.0 * s2v[l] * s2w[l];
+                    }
+                    d[n - 2] = d0 - 2.0 * vcur[0] * wv[0];
+                    e[n - 2] = e0 - (vcur[0] * wv[1] + wv[0] * vcur[1]);
+                    d[n - 1] = d1 - 2.0 * vcur[1] * wv[1];
+                }
+                break;
+            }
+            // row k+1 of the current matrix: x' less reflector k's rank-2 term;
+            // its tail norm and its dots with v_k and w_k past index k+2
+            const double v0 = vcur[0], w0 = wv[0];
+            double ss = 0.0, dv = 0.0, dw = 0.0;
+            for (int t0 = tid; t0 < m; t0 += TT * XP) {
+                double xr[XP];
+#pragma unroll
+                for (int u = 0; u < XP; ++u) xr[u] = t0 + TT * u < m ? xq[cur][k + 1 + t0 + TT * u] : 0.0;
+#pragma unroll
+                for (int u = 0; u < XP; ++u) {
+                    const int t = t0 + TT * u;
+                    if (t < m) {
+                        const double vt = vcur[t], wt = wv[t];
+                        const double x = xr[u] - (v0 * wt + w0 * vt);
+                        if (t >= 2) {
+                            ss += x * x;
+                            dv += vt * x;
+                            dw += wt * x;
+                        } else if (t == 1) {
+                            bc[1] = x;
+                        } else if (wg == 0) {
+                            d[k + 1] = x;
+                        }
+                        if (t >= 1) vnext[t - 1] = x;   // scaled below
+                    }
+                }
+            }
+            block_sum3(ss, dv, dw, red3, sum3, tid);   // its barriers also publish bc[1] and vnext
+            const double alpha = bc[1];
+            double beta = alpha, tn = 0.0, scale = 0.0;
+            if (sum3[0] > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + sum3[0]), alpha);
+                tn = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            const int m2 = m - 1;   // length of v_{k+1}
+            for (int t = 1 + tid; t < m2; t += TT) vnext[t] = sum3[0] > 0.0 ? vnext[t] * scale : 0.0;
+            if (tid == 0) vnext[0] = 1.0;
+            if (tid < XNB) {
+                // P . x = P . x' - v0 (P . p_k + K P . v_k) - w0 P . v_k over indices >= k+3
+                const int l = tid;
+                if (l < i) {
+                    const double xv = sS[l] - v0 * (sS[4 * XNB + l] + K * sS[2 * XNB + l]) - w0 * sS[2 * XNB + l];
+                    const double xw = sS[XNB + l] - v0 * (sS[5 * XNB + l] + K * sS[3 * XNB + l]) -
+                                      w0 * sS[3 * XNB + l];
+                    al[l] = s2w[l] + scale * xw;
+                    be[l] = s2v[l] + scale * xv;
+                } else if (l == i) {
+                    al[l] = wv[1] + scale * sum3[2];
+                    be[l] = vcur[1] + scale * sum3[1];
+                }
+            }
+            __syncthreads();
+            if (wg == 0) {
+                double* wr = Wm + (long long)(k + 1) * ln + (k + 2);
+                for (int t = tid; t < m2; t += TT) wr[t] = vnext[t];
+                if (tid == 0) { e[k + 1] = beta; tau[k + 1] = tn; }
+            }
+            // the panel's last reflector: A_p <- A_p - V W^T - W V^T on the
+            // trailing block, which every workgroup's row pass then reads
+            const bool pend = i == XNB - 1;
+            if (pend) {
+                xtri_panel_update(A, Wm, Wp, wv, n, k0, k + 2, gw, GW, lane);
+                alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
+                if (!alive) break;
+            }
+            // rows r = k+2 .. n-1: (A v_{k+1})_r from A_p and the panel, x'_r
+            // for the next pivot row (A_p[r][k+2] less the panel's terms) and
+            // this lane's panel vector's dot partials; XU loads in flight per lane
+            const __amdgpu_buffer_rsrc_t Vr = xrsrc(Wm + (long long)k0 * ln, (long long)XNB * ln);
+            const __amdgpu_buffer_rsrc_t Wr = xrsrc(Wp, (long long)XNB * ln);
+            const xu32 nbytes = (xu32)n * 8u;
+            const int pl = lane & (XNB - 1);
+            const bool isv = lane < XNB;
+            const bool plane = !pend && lane < 2 * XNB && pl <= i;
+            // lane v_l: -v_l[r] (al[l], w_l[k+2]); lane w_l: -w_l[r] (be[l], v_l[k+2])
+            double cs = 0.0, cx = 0.0;
+            if (plane) {
+                cs = isv ? al[pl] : be[pl];
+                cx = isv ? (pl < i ? s2w[pl] : wv[1]) : (pl < i ? s2v[pl] : vcur[1]);
+            }
+            const int nx = cur ^ 1;
+            double pacc = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
+            for (int r = k + 2 + (gw - (k + 2) % GW + GW) % GW; r < n; r += GW) {
+                const int tr = r - k - 1;
+                const double* row = A + (long long)r * ln;
+                double P = 0.0;
+                if (plane) {
+                    if (pl == i) P = isv ? vcur[tr] : wv[tr];
+                    else if (isv) P = xbuf_load(Vr, (xu32)r * 8u, (xu32)pl * nbytes);
+                    else P = xbuf_load(Wr, (xu32)r * 8u, (xu32)pl * nbytes);
+                }
+                double s = -P * cs, xp = -P * cx;
+                if (lane == 0) xp += row[k + 2];
+                int c = k + 2 + lane;
+                for (; c + 64 * (XU - 1) < n; c += 64 * XU) {
+                    double a[XU];
+#pragma unroll
+                    for (int u = 0; u < XU; ++u) a[u] = row[c + 64 * u];
+#pragma unroll
+                    for (int u = 0; u < XU; ++u) s += a[u] * vnext[c + 64 * u - k - 2];
+                }
+                if (c < n) {
+                    double a[XU];
+#pragma unroll
+                    for (int u = 0; u < XU; ++u) a[u] = c + 64 * u < n ? row[c + 64 * u] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < XU; ++u)
+                        if (c + 64 * u < n) s += a[u] * vnext[c + 64 * u - k - 2];
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    s += __shfl_xor(s, off);
+                    xp += __shfl_xor(xp, off);
+                }
+                const double pn = tn * s;
+                const double vr = vnext[tr - 1];
+                if (lane == 0) {
+                    pb[nx][tr - 1] = pn;
+                    xq[nx][r] = xp;
+                }
+                pacc += pn * vr;
+                if (r >= k + 4) {
+                    S1 += P * xp;
+                    S2 += P * vr;
+                    S3 += P * pn;
+                }
+            }
+            if (lane == 0) red2[wave] = pacc;
+            if (lane < 2 * XNB) {
+                redS[wave * XSR + lane] = S1;
+                redS[wave * XSR + 2 * XNB + lane] = S2;
+                redS[wave * XSR + 4 * XNB + lane] = S3;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double sp = 0.0;
+                for (int w = 0; w < TW; ++w) sp += red2[w];
+                qb[nx][wg] = sp;
+            }
+            if (!pend && tid < XSR) {
+                double sp = 0.0;
+                for (int w = 0; w < TW; ++w) sp += redS[w * XSR + tid];
+                xs[nx][(long long)tid * XGRID_MAX + wg] = sp;
+            }
+            alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
+            double* tmp = vcur; vcur = vnext; vnext = tmp;
+            cur = nx;
+            tk = tn;
+            if (pend) k0 = k + 1;
+        }
+        if (!alive) {
+            if (wg == 0 && tid == 0) state[job].sweeps = -1;
+            continue;
+        }
         alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
         if (wg == 0 && tid == 0) state[job].prof[0] = (double)(wall_clock64() - t0);
     }
@@ -900,10 +1397,17 @@ void launch_xlh_eigen(hipStream_t st, int grid, const JobDev* jobs, JobState* st
                       double* tri, double* xlws, unsigned* bar) {
     if (nlist <= 0) return;
     const size_t lt = (size_t)3 * max_n * sizeof(double);
-    raise_dynamic_lds((const void*)k_xtri, lt);
+    const int mode = get_tuning().xl_tri_blocked;
+    const bool blocked = mode == 2 || (mode == 1 && max_n >= kXtriBlkMinN);
+    const void* ktri = blocked ? (const void*)k_xtri_blk : (const void*)k_xtri;
+    raise_dynamic_lds(ktri, lt);
     hipLaunchKernelGGL(k_xbar_reset, dim3(1), dim3(256), 0, st, bar);
-    hipLaunchKernelGGL(k_xtri, dim3(xlh_tri_grid(grid, max_n, nlist)), dim3(TT), lt, st, jobs, state, joblist, xloff, nlist,
-                       arena, tri, xlws, bar);
+    if (blocked)
+        hipLaunchKernelGGL(k_xtri_blk, dim3(xlh_tri_grid(grid, max_n, nlist)), dim3(TT), lt, st, jobs, state, joblist,
+                           xloff, nlist, arena, tri, xlws, bar);
+    else
+        hipLaunchKernelGGL(k_xtri, dim3(xlh_tri_grid(grid, max_n, nlist)), dim3(TT), lt, st, jobs, state, joblist,
+                           xloff, nlist, arena, tri, xlws, bar);
     hipLaunchKernelGGL(k_xbar_reset, dim3(1), dim3(256), 0, st, bar);
     hipLaunchKernelGGL(k_xdc, dim3(grid), dim3(XT), 0, st, jobs, state, joblist, xloff, nlist, arena, tri, eps, perm,
                        xlws, bar);

@@ -14,6 +14,7 @@ struct Tuning {
     int bt_rows[4] = {64, 64, 64, 64};     // back-transformation rows per workgroup, n <= 128 / 256 / 384 / 512
     int xl_tri_wg = 0;                     // workgroups of the device-wide tridiagonalization (0: by size)
     int tri_blocked = 1;                   // device-memory tridiagonalization up to 512 AOs: blocked (1) or not (0)
+    int xl_tri_blocked = 1;                // device-wide tridiagonalization: by size (1), always blocked (2), never (0)
 };
 Tuning get_tuning();               // of the current device
 void set_tuning(const Tuning& t);  // for the current device

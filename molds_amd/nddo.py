@@ -90,6 +90,7 @@ def load_library(path=LIBPATH):
     L.nddo_engine_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.nddo_dbg_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.nddo_dbg_set_tridiag.argtypes = [vp, ctypes.c_int]
+    L.nddo_dbg_set_xl_tridiag.argtypes = [vp, ctypes.c_int]
     L.nddo_dbg_host_times.argtypes = [vp, _dp, ctypes.c_int]
     L.nddo_dbg_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.nddo_engine_trim.argtypes = [vp]
@@ -440,6 +441,14 @@ class Engine:
         """Blocked (True, the default) or per-reflector (False) tridiagonalization
         of the 138-512 AO class on this engine's device."""
         rc = self.L.nddo_dbg_set_tridiag(self.h, 1 if blocked else 0)
+        if rc != 0:
+            raise NddoError(rc, last_error())
+
+    def set_xl_tridiag(self, blocked=None):
+        """Device-wide tridiagonalization of large molecules on this engine's
+        device: blocked from 5600 AOs (None, the default), always blocked (True)
+        or never (False)."""
+        rc = self.L.nddo_dbg_set_xl_tridiag(self.h, 1 if blocked is None else (2 if blocked else 0))
         if rc != 0:
             raise NddoError(rc, last_error())
 

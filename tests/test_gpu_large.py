@@ -71,6 +71,62 @@ def test_eigh_device_wide_householder_sizes(engine, n):
     _check_eigh(engine, _random_sym(n), "householder")
 
 
+@pytest.fixture
+def dw_tri(engine_dw):
+    yield engine_dw
+    engine_dw.set_xl_tridiag(None)
+
+
+# both device-wide tridiagonalizations, forced at sizes where the engine picks
+# the per-reflector one: blocked (panels of 8, the default from 5600 AOs) and
+# per reflector; sizes with a ragged last panel and last MFMA tile
+@pytest.mark.parametrize("blocked", [True, False])
+@pytest.mark.parametrize("n", [513, 530, 777, 2049])
+def test_eigh_device_wide_tridiagonalizations(dw_tri, n, blocked):
+    print(f"eigh device-wide blocked={blocked} n={n}", flush=True)
+    dw_tri.set_xl_tridiag(blocked)
+    _check_eigh(dw_tri, _random_sym(n), f"device-wide blocked={blocked}")
+
+
+@pytest.mark.parametrize("blocked", [True, False])
+def test_eigh_device_wide_structured(dw_tri, blocked):
+    """Zero reflector tails (tau = 0) inside and at the ends of panels: a block
+    diagonal matrix with 1x1, 7x7, 9x9 and 16x16 blocks, a diagonal one, and a
+    graded one (entries over 12 orders of magnitude)."""
+    print(f"eigh device-wide structured blocked={blocked}", flush=True)
+    dw_tri.set_xl_tridiag(blocked)
+    rng = np.random.default_rng(17)
+    sizes = [1, 7, 9, 8, 16, 1, 33, 64, 40, 24, 200, 130]
+    n = sum(sizes)
+    A = np.zeros((n, n))
+    i = 0
+    for s in sizes:
+        B = rng.standard_normal((s, s))
+        A[i:i + s, i:i + s] = B + B.T
+        i += s
+    _check_eigh(dw_tri, A, "block diagonal")
+    _check_eigh(dw_tri, np.diag(rng.standard_normal(600)), "diagonal")
+    g = 10.0 ** np.linspace(0, -12, 560)
+    B = rng.standard_normal((560, 560))
+    _check_eigh(dw_tri, (B + B.T) * np.outer(g, g) + np.diag(g), "graded")
+
+
+def test_scf_device_wide_blocked_equals_unblocked(dw_tri):
+    """(H2O)_90, 540 AOs, on the device-wide solver with either
+    tridiagonalization: the same SCF (energy within 1e-9 eV, same iteration
+    count)."""
+    print("scf device-wide blocked vs unblocked", flush=True)
+    Z, xyz = _water(90, 11)
+    dw_tri.set_xl_tridiag(True)
+    a = dw_tri.scf("pm3", Z, xyz, precision="fp64", **SCF)
+    dw_tri.set_xl_tridiag(False)
+    b = dw_tri.scf("pm3", Z, xyz, precision="fp64", **SCF)
+    assert a["status"] == 0 and b["status"] == 0
+    assert abs(a["energy"] - b["energy"]) * EV <= 1e-9
+    assert a["iterations"] == b["iterations"]
+    assert np.max(np.abs(a["eps"] - b["eps"])) * EV <= 1e-9
+
+
 # the block-Jacobi alternative (NDDO_ENGINE_BLOCK_JACOBI) stays available
 @pytest.mark.parametrize("n", [600, 1536])
 def test_eigh_block_jacobi(engine_bj, n):
