@@ -131,7 +131,7 @@ void launch_wy_t(hipStream_t st, const JobDev* jobs, JobState* state, const int*
 // divide and conquer and back-transformation, persistent launches of `grid`
 // workgroups (at most one per CU) over the jobs of joblist one after another;
 // xloff[i] is job i's offset in xlws (xlh_ws_doubles(n) each); bar: two
-// unsigned words, reset by the launches themselves.  n <= 6400 (xlh_supported).
+// unsigned words, reset by the launches themselves.  n <= 6144 (xlh_supported).
 bool xlh_supported(int n);
 size_t xlh_bar_bytes();   // the barrier words the device-wide launches need
 long long xlh_ws_doubles(int n);
