@@ -169,6 +169,7 @@ __host__ __device__ inline long long xw_size(int n) { return xw_xs(n) + 2LL * XG
 constexpr int TT = 1024;            // threads per workgroup of k_xtri (16 wavefronts: memory parallelism)
 constexpr int XU = 16;              // row elements loaded at once per lane in k_xtri's row pass
 constexpr int XP = 8;               // pivot-row and p elements loaded at once per thread (n <= 8192 in one)
+static_assert(kXlhMaxN <= TT * XP, "one pass of XP loads per thread covers the pivot row and p");
 constexpr int TW = TT / 64;
 
 __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
@@ -285,15 +286,22 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
         int cur = 0;
         for (int k = 0; alive && k <= n - 3; ++k) {
             const int m = n - 1 - k;   // length of v_k (indices t <-> row / column k + 1 + t)
+            // this thread's p and pivot-row elements (t = tid + TT u, m <= TT XP),
+            // loaded before the wait for the partials so the three loads overlap
+            const double* rowp = A + (long long)(k + 1) * ln + (k + 1);
+            double pr[XP], ar[XP];
+#pragma unroll
+            for (int u = 0; u < XP; ++u) {
+                const int t = tid + TT * u;
+                pr[u] = t < m ? pb[cur][t] : 0.0;
+                ar[u] = t < m ? rowp[t] : 0.0;
+            }
             const double pv = grid_partials_sum(qb[cur], G, &bc[0]);
             const double K = -0.5 * tk * pv;   // w = p + K v
-            for (int t0 = tid; t0 < m; t0 += TT * XP) {
-                double pr[XP];
 #pragma unroll
-                for (int u = 0; u < XP; ++u) pr[u] = t0 + TT * u < m ? pb[cur][t0 + TT * u] : 0.0;
-#pragma unroll
-                for (int u = 0; u < XP; ++u)
-                    if (t0 + TT * u < m) wv[t0 + TT * u] = pr[u] + K * vcur[t0 + TT * u];
+            for (int u = 0; u < XP; ++u) {
+                const int t = tid + TT * u;
+                if (t < m) wv[t] = pr[u] + K * vcur[t];
             }
             __syncthreads();
             const double w0 = wv[0];
@@ -307,22 +315,16 @@ __global__ void __launch_bounds__(TT) k_xtri(const JobDev* __restrict__ jobs, Jo
                 break;
             }
             // row k+1 after the update of step k; its tail defines reflector k+1
-            const double* rowp = A + (long long)(k + 1) * ln + (k + 1);
             double ss = 0.0;
-            for (int t0 = tid; t0 < m; t0 += TT * XP) {   // XP loads per thread in flight
-                double ar[XP];
 #pragma unroll
-                for (int u = 0; u < XP; ++u) ar[u] = t0 + TT * u < m ? rowp[t0 + TT * u] : 0.0;
-#pragma unroll
-                for (int u = 0; u < XP; ++u) {
-                    const int t = t0 + TT * u;
-                    if (t < m) {
-                        const double a = ar[u] - (vcur[0] * wv[t] + w0 * vcur[t]);
-                        if (t >= 2) ss += a * a;
-                        else if (t == 1) bc[1] = a;
-                        else if (wg == 0) d[k + 1] = a;
-                        if (t >= 1) vnext[t - 1] = a;   // scaled below
-                    }
+            for (int u = 0; u < XP; ++u) {
+                const int t = tid + TT * u;
+                if (t < m) {
+                    const double a = ar[u] - (vcur[0] * wv[t] + w0 * vcur[t]);
+                    if (t >= 2) ss += a * a;
+                    else if (t == 1) bc[1] = a;
+                    else if (wg == 0) d[k + 1] = a;
+                    if (t >= 1) vnext[t - 1] = a;   // scaled below
                 }
             }
             ss = block_sum(ss, red, tid, TW);   // its barriers also publish bc[1] and vnext
@@ -690,15 +692,21 @@ __global__ void __launch_bounds__(TT) k_xtri_blk(const JobDev* __restrict__ jobs
                     }
                 }
             }
+            // this thread's p and x' elements (t = tid + TT u, m <= TT XP), loaded
+            // before the wait for the partials so the loads overlap
+            double pr[XP], xr[XP];
+#pragma unroll
+            for (int u = 0; u < XP; ++u) {
+                const int t = tid + TT * u;
+                pr[u] = t < m ? pb[cur][t] : 0.0;
+                xr[u] = t < m ? xq[cur][k + 1 + t] : 0.0;
+            }
             const double pv = grid_partials_sum(qb[cur], G, &bc[0]);   // its barrier publishes the above
             const double K = -0.5 * tk * pv;   // w = p + K v
-            for (int t0 = tid; t0 < m; t0 += TT * XP) {
-                double pr[XP];
 #pragma unroll
-                for (int u = 0; u < XP; ++u) pr[u] = t0 + TT * u < m ? pb[cur][t0 + TT * u] : 0.0;
-#pragma unroll
-                for (int u = 0; u < XP; ++u)
-                    if (t0 + TT * u < m) wv[t0 + TT * u] = pr[u] + K * vcur[t0 + TT * u];
+            for (int u = 0; u < XP; ++u) {
+                const int t = tid + TT * u;
+                if (t < m) wv[t] = pr[u] + K * vcur[t];
             }
             __syncthreads();
             if (wg == 0)   // w_k for the later steps of the panel (read after the next grid barrier)
@@ -723,27 +731,22 @@ __global__ void __launch_bounds__(TT) k_xtri_blk(const JobDev* __restrict__ jobs
             // its tail norm and its dots with v_k and w_k past index k+2
             const double v0 = vcur[0], w0 = wv[0];
             double ss = 0.0, dv = 0.0, dw = 0.0;
-            for (int t0 = tid; t0 < m; t0 += TT * XP) {
-                double xr[XP];
 #pragma unroll
-                for (int u = 0; u < XP; ++u) xr[u] = t0 + TT * u < m ? xq[cur][k + 1 + t0 + TT * u] : 0.0;
-#pragma unroll
-                for (int u = 0; u < XP; ++u) {
-                    const int t = t0 + TT * u;
-                    if (t < m) {
-                        const double vt = vcur[t], wt = wv[t];
-                        const double x = xr[u] - (v0 * wt + w0 * vt);
-                        if (t >= 2) {
-                            ss += x * x;
-                            dv += vt * x;
-                            dw += wt * x;
-                        } else if (t == 1) {
-                            bc[1] = x;
-                        } else if (wg == 0) {
-                            d[k + 1] = x;
-                        }
-                        if (t >= 1) vnext[t - 1] = x;   // scaled below
+            for (int u = 0; u < XP; ++u) {
+                const int t = tid + TT * u;
+                if (t < m) {
+                    const double vt = vcur[t], wt = wv[t];
+                    const double x = xr[u] - (v0 * wt + w0 * vt);
+                    if (t >= 2) {
+                        ss += x * x;
+                        dv += vt * x;
+                        dw += wt * x;
+                    } else if (t == 1) {
+                        bc[1] = x;
+                    } else if (wg == 0) {
+                        d[k + 1] = x;
                     }
+                    if (t >= 1) vnext[t - 1] = x;   // scaled below
                 }
             }
             block_sum3(ss, dv, dw, red3, sum3, tid);   // its barriers also publish bc[1] and vnext
