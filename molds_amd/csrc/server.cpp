@@ -114,6 +114,14 @@ double secs_since(Clock::time_point t) { return std::chrono::duration<double>(Cl
 constexpr int kMaxCohorts = 8;
 constexpr int kSmallReserve = 4;
 constexpr int kFirstLargeClass = 2;
+// HIP gives a new stream the least-used hardware queue, so the slot streams,
+// made one after another, take the queues in turn: slots k and k + 4 share
+// one, slots k .. k + 3 are on four.  Cohorts of the large size classes take
+// the slots of two queues ({0, 1, 4, 5}), those of the small classes the
+// other two ({2, 3, 6, 7}), so a small cohort's kernels never queue behind a
+// large cohort's device-wide launches; a small cohort borrows a large slot
+// only while no large cohort is resident.
+inline bool large_slot(int k) { return (k & 2) == 0; }
 // AO count <= 64, <= 137 (eigensolver matrix in LDS), <= 256, <= 512, up to
 // the device-wide eigensolver's threshold (one workgroup per molecule; empty
 // in served mode, whose threshold is 513), larger (device-wide eigensolver,
@@ -376,7 +384,7 @@ void Engine::start_server() {
                 if (cohorts.empty()) scv.wait(lk, [&] { return stopping || !queue.empty(); });
                 if (stopping) break;
                 std::unique_ptr<Cohort> bucket[kSizeClasses];
-                size_t nb = 0, admitted = 0;
+                size_t admitted = 0;
                 // the budget, the device memory free right now (a synchronous
                 // batch's workspace or other users may hold some) and the bytes
                 // of retired results not yet fetched, less 1 GiB for the runtime
@@ -389,16 +397,26 @@ void Engine::start_server() {
                 const size_t room = std::min(room_dev, room_budget);
                 int nlarge = 0;   // resident and new cohorts of the large size classes
                 for (auto& c : cohorts) nlarge += c->cls >= kFirstLargeClass;
+                int free_l = 0, free_s = 0;   // free slots of the large / small classes' queues
+                for (int k : free_slots) (large_slot(k) ? free_l : free_s) += 1;
                 for (auto it = queue.begin(); it != queue.end();) {
                     ServedJob* j = *it;
                     const int k = size_class(j->res.nao, served_xl_min(this));
                     if (!bucket[k]) {
-                        if (nb >= free_slots.size()) { ++it; continue; }
-                        if (k >= kFirstLargeClass && nlarge >= kMaxCohorts - kSmallReserve) { ++it; continue; }
+                        if (k >= kFirstLargeClass) {
+                            if (free_l == 0 || nlarge >= kMaxCohorts - kSmallReserve) { ++it; continue; }
+                            --free_l;
+                        } else if (free_s > 0) {
+                            --free_s;
+                        } else if (free_l > 0 && nlarge == 0) {
+                            --free_l;
+                        } else {
+                            ++it;
+                            continue;
+                        }
                         bucket[k].reset(new Cohort());
                         bucket[k]->cls = k;
                         nlarge += k >= kFirstLargeClass;
-                        ++nb;
                     }
                     Cohort* c = bucket[k].get();
                     const int kcap = k == kSizeClasses - 1 ? std::min(cap, xl_max_jobs()) : cap;
@@ -424,9 +442,20 @@ void Engine::start_server() {
                 std::this_thread::sleep_for(std::chrono::milliseconds(1));
                 continue;
             }
+            // large cohorts first on their queues' slots, then the small ones on
+            // theirs and, when admission allowed it, on a large queue's slot
+            std::stable_sort(fresh.begin(), fresh.end(), [](const std::unique_ptr<Cohort>& a,
+                                                            const std::unique_ptr<Cohort>& b) {
+                return (a->cls >= kFirstLargeClass) > (b->cls >= kFirstLargeClass);
+            });
             for (auto& f : fresh) {
-                const int k = free_slots.back();
-                free_slots.pop_back();
+                const bool lg = f->cls >= kFirstLargeClass;
+                auto pick = free_slots.end();
+                for (auto sl = free_slots.begin(); sl != free_slots.end(); ++sl)
+                    if (large_slot(*sl) == lg) { pick = sl; break; }
+                if (pick == free_slots.end()) pick = free_slots.begin();   // a small cohort on a large slot
+                const int k = *pick;
+                free_slots.erase(pick);
                 f->slot = k;
                 f->st = spool[k];
                 f->ev = epool[k];
