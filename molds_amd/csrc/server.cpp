@@ -105,9 +105,10 @@ double secs_since(Clock::time_point t) { return std::chrono::duration<double>(Cl
 // hardware queues (GPU_MAX_HW_QUEUES, 4 by default), where a kernel waits for
 // every kernel queued before it. A cohort of large molecules holds its slot for
 // seconds (up to max_iter iterations of tens of ms), one of small molecules
-// for milliseconds, so at most kMaxCohorts - kSmallReserve (one per hardware
-// queue) resident cohorts may hold molecules above 137 AOs, and the rest of the
-// slots are kept for the small size classes. Measured on the bench's Poisson
+// for milliseconds, so at most kMaxCohorts - kSmallReserve resident cohorts
+// may hold molecules above 137 AOs, the rest of the slots are kept for the
+// small size classes, and the two kinds run on different hardware queues
+// (large_slot). Measured on the bench's Poisson
 // mix at 500 requests/s (mixed precision): 16 slots, no reserve: 33.8 solves/s,
 // p50 2.26 s; 12 slots, 4 reserved: 34.6, 0.87 s; 8 slots, 4 reserved: 38.3,
 // 0.40 s (requests up to 137 AOs 0.14 s); 32 slots, 8 reserved: 27.5, 3.2 s.
