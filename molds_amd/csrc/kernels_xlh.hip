@@ -148,9 +148,9 @@ constexpr int XNB = 8;              // reflectors per panel of the blocked k_xtr
 constexpr int XSR = 6 * XNB;        // k_xtri_blk's per-workgroup record of panel dot partials
 // Size from which k_xtri_blk is the default.  Measured on MI355X (ms per
 // tridiagonalization, per-reflector / blocked): n = 4608: 149 / 165; n = 5200:
-// 198 / 203; n = 6000: 293 / 273.  The row passes of both are bound by the
-// loads one wavefront keeps in flight on its one or two rows, not by HBM
-// bytes, so halving the bytes pays only where the trailing blocks are large.
+// 198 / 203; n = 6000: 293 / 273.  The blocked row pass reads half the bytes
+// (1.75x faster at n = 6000), but the panel update and the larger pivot phase
+// add a fixed cost per column that smaller trailing blocks do not repay.
 constexpr int kXtriBlkMinN = 5600;
 
 // Workspace of one job in xlws (doubles): p[2][n], per-wave partials [2][XWAVE_MAX],
