@@ -463,8 +463,8 @@ __device__ inline void block_sum3(double a, double b, double c, double* red3, do
 
 // A[r][c] -= sum_{l < XNB} (V_l[r] W_l[c] + W_l[r] V_l[c]) for r, c in [r0, n):
 // V_l = row k0 + l of Wm, W_l = row l of Wp (both by global index), the last
-// W_l from LDS (wl[t] <-> index r0 - 1 + t).  One wavefront per 16 x 16 tile,
-// K = 2 XNB = 16 in four 16x16x4 FP64 MFMAs: A operand row rb + lane % 16,
+// W_l from LDS (wl[t] <-> index r0 - 1 + t).  A wavefront takes two 16 x 16
+// tiles side by side, K = 2 XNB = 16 in four 16x16x4 FP64 MFMAs each: A operand row rb + lane % 16,
 // k = lane / 16 of the step (V, then W); B operand column cb + lane % 16 (W,
 // then V); the accumulator holds rows rb + 4 v + lane / 16.
 __device__ inline void xtri_panel_update(double* __restrict__ A, const double* __restrict__ Wm,
@@ -473,52 +473,53 @@ __device__ inline void xtri_panel_update(double* __restrict__ A, const double* _
     static_assert(2 * XNB == 16, "four MFMA k-steps");
     const long long ln = n;
     const int nt = (n - r0 + 15) >> 4;
-    const int ntiles = nt * nt;
     const int lr = lane & 15, lk = lane >> 4;
-    // two tiles per pass (tile and tile + GW), their loads in flight together
-    for (int tile0 = gw; tile0 < ntiles; tile0 += 2 * GW) {
-        int rbs[2], cbs[2];
-        bool on[2];
+    // two side-by-side tiles (I, 2 J2) and (I, 2 J2 + 1) per pass: 256-byte row
+    // segments, the row operands loaded once for both, all loads in flight together
+    const int nt2 = (nt + 1) >> 1, npairs = nt * nt2;
+    for (int p = gw; p < npairs; p += GW) {
+        const int I = p / nt2, J2 = p - I * nt2;
+        const int rb = r0 + 16 * I, ra = rb + lr;
+        int cs[2];
         dbl4 acc[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int tile = tile0 + h * GW;
-            on[h] = tile < ntiles;
-            const int I = on[h] ? tile / nt : 0, J = on[h] ? tile - I * nt : 0;
-            rbs[h] = r0 + 16 * I;
-            cbs[h] = r0 + 16 * J;
-            const int c = cbs[h] + lr;
+            cs[h] = r0 + 16 * (2 * J2 + h) + lr;
+            const int c = cs[h];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = rbs[h] + 4 * v + lk;
-                acc[h][v] = (on[h] && r < n && c < n) ? A[(long long)r * ln + c] : 0.0;
+                const int r = rb + 4 * v + lk;
+                acc[h][v] = (r < n && c < n) ? A[(long long)r * ln + c] : 0.0;
             }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int kk = 4 * q + lk;
+            double a = 0.0;
+            if (q < 2) {   // V_kk (rows) against W_kk (columns)
+                if (ra < n) a = Wm[(long long)(k0 + kk) * ln + ra];
+            } else {       // W_l (rows) against V_l (columns)
+                const int l = kk - XNB;
+                if (ra < n) a = l == XNB - 1 ? wl[ra - r0 + 1] : Wp[(long long)l * ln + ra];
+            }
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int ra = rbs[h] + lr, c = cbs[h] + lr;
-                double a = 0.0, b = 0.0;
-                if (q < 2) {   // V_kk (rows) against W_kk (columns)
-                    if (ra < n) a = Wm[(long long)(k0 + kk) * ln + ra];
-                    if (c < n) b = kk == XNB - 1 ? wl[c - r0 + 1] : Wp[(long long)kk * ln + c];
-                } else {       // W_l (rows) against V_l (columns)
-                    const int l = kk - XNB;
-                    if (ra < n) a = l == XNB - 1 ? wl[ra - r0 + 1] : Wp[(long long)l * ln + ra];
-                    if (c < n) b = Wm[(long long)(k0 + l) * ln + c];
+                const int c = cs[h];
+                double b = 0.0;
+                if (c < n) {
+                    if (q < 2) b = kk == XNB - 1 ? wl[c - r0 + 1] : Wp[(long long)kk * ln + c];
+                    else b = Wm[(long long)(k0 + kk - XNB) * ln + c];
                 }
                 acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a, b, acc[h], 0, 0, 0);
             }
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int c = cbs[h] + lr;
+            const int c = cs[h];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = rbs[h] + 4 * v + lk;
-                if (on[h] && r < n && c < n) A[(long long)r * ln + c] = acc[h][v];
+                const int r = rb + 4 * v + lk;
+                if (r < n && c < n) A[(long long)r * ln + c] = acc[h][v];
             }
         }
     }
