@@ -127,6 +127,26 @@ def test_scf_device_wide_blocked_equals_unblocked(dw_tri):
     assert np.max(np.abs(a["eps"] - b["eps"])) * EV <= 1e-9
 
 
+def test_device_wide_blocked_side_by_side(dw_tri):
+    """The blocked tridiagonalization with several molecules of different sizes
+    in one launch (a workgroup group and a workspace each, different iteration
+    counts), forced below its size threshold: each equals its alone solve."""
+    print("device-wide blocked, molecules side by side", flush=True)
+    from molds_amd import workloads as W
+    from molds_amd.nddo import Job
+    dw_tri.set_xl_tridiag(True)
+    Zc, xc = W.alkane(90, seed=33)
+    big = [Job("pm3", *_water(95, 12), rms_density=1e-8), Job("pm3", Zc, W.to_bohr(xc), rms_density=1e-8),
+           Job("am1", *_water(88, 13), rms_density=1e-8)]
+    mix = dw_tri.scf_batch(big)
+    assert all(r["status"] == 0 for r in mix)
+    for j, r in zip(big, mix):
+        alone = dw_tri.scf_batch([j])[0]
+        assert alone["status"] == 0
+        assert abs(alone["energy"] - r["energy"]) * EV <= 1e-9
+        assert alone["iterations"] == r["iterations"]
+
+
 # the block-Jacobi alternative (NDDO_ENGINE_BLOCK_JACOBI) stays available
 @pytest.mark.parametrize("n", [600, 1536])
 def test_eigh_block_jacobi(engine_bj, n):
