@@ -58,21 +58,22 @@ struct SecSums {
 // sum |terms|, as four interleaved partial sums so that four terms' reciprocal
 // chains are in flight at once.
 __device__ inline void dc_sums_range(const double* dl, const double* zl, int t0, int t1, double dO, double tau,
-                                     double& sum, double& dsum, double& asum) {
+                                     double& sum, double& dsum, double& asum, int sub = 0, int g = 1) {
+    // poles t0 + sub, t0 + sub + g, ... (a lane's share when g lanes split the sum)
     double s[4] = {0.0, 0.0, 0.0, 0.0}, ds[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
-    int t = t0;
-    for (; t + 3 < t1; t += 4) {
+    int t = t0 + sub;
+    for (; t + 3 * g < t1; t += 4 * g) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const double z = zl[t + u];
-            const double inv = rcp_nr((dl[t + u] - dO) - tau);
+            const double z = zl[t + u * g];
+            const double inv = rcp_nr((dl[t + u * g] - dO) - tau);
             const double term = z * z * inv;
             s[u] += term;
             ds[u] += term * inv;
             as[u] += fabs(term);
         }
     }
-    for (; t < t1; ++t) {
+    for (; t < t1; t += g) {
         const double z = zl[t];
         const double inv = rcp_nr((dl[t] - dO) - tau);
         const double term = z * z * inv;
@@ -85,11 +86,28 @@ __device__ inline void dc_sums_range(const double* dl, const double* zl, int t0,
     asum = (as[0] + as[1]) + (as[2] + as[3]);
 }
 
+// Sum over the g (a power of two, at most 64) aligned lanes of a group by an
+// xor butterfly: every lane of the group gets bitwise the same sum.
+__device__ inline double group_allsum(double v, int g) {
+    for (int off = g >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// With g > 1 the g lanes of a group split the poles (lane sub takes every
+// g-th) and all get the same sums.
 __device__ inline SecSums dc_sums(const double* dl, const double* zl, int kk, double rho, int j, double dO,
-                                  double tau) {
+                                  double tau, int sub = 0, int g = 1) {
     double psi, dpsi, phi, dphi, a1, a2;
-    dc_sums_range(dl, zl, 0, j + 1, dO, tau, psi, dpsi, a1);
-    dc_sums_range(dl, zl, j + 1, kk, dO, tau, phi, dphi, a2);
+    dc_sums_range(dl, zl, 0, j + 1, dO, tau, psi, dpsi, a1, sub, g);
+    dc_sums_range(dl, zl, j + 1, kk, dO, tau, phi, dphi, a2, sub, g);
+    if (g > 1) {
+        psi = group_allsum(psi, g);
+        dpsi = group_allsum(dpsi, g);
+        phi = group_allsum(phi, g);
+        dphi = group_allsum(dphi, g);
+        a1 = group_allsum(a1, g);
+        a2 = group_allsum(a2, g);
+    }
     return SecSums{rho * psi, rho * dpsi, rho * phi, rho * dphi, a1 + a2};
 }
 
@@ -100,8 +118,9 @@ __device__ inline SecSums dc_sums(const double* dl, const double* zl, int kk, do
 // the iteration.
 // ue: the unit roundoff the root is resolved to (FP64, or FP32 for a job in
 // its FP32 iterations, whose tridiagonal is only that accurate).
+// sub, g: as dc_sums (g lanes solve the root together and all return it).
 __device__ inline void dc_secular(const double* dl, const double* zl, int kk, double rho, int j, int* orig,
-                           double* taup, double ue) {
+                           double* taup, double ue, int sub = 0, int g = 1) {
     int o = kk - 1;
     double lo = 0.0, hi = 0.0, tau = 0.0, dO = dl[kk - 1];
     // the first evaluation point: the middle of (dl_j, dl_j+1) seen from dl_j,
@@ -111,13 +130,14 @@ __device__ inline void dc_secular(const double* dl, const double* zl, int kk, do
     double evO = inner ? dl[j] : dl[kk - 1], evT = mid;
     if (!inner) {
         double zz = 0.0;
-        for (int t = 0; t < kk; ++t) zz += zl[t] * zl[t];
+        for (int t = sub; t < kk; t += g) zz += zl[t] * zl[t];
+        if (g > 1) zz = group_allsum(zz, g);
         hi = rho * zz;
         tau = 0.5 * hi;
         evT = tau;
     }
     for (int it = 0; it < 100; ++it) {
-        const SecSums S = dc_sums(dl, zl, kk, rho, j, evO, evT);   // the one evaluation site
+        const SecSums S = dc_sums(dl, zl, kk, rho, j, evO, evT, sub, g);   // the one evaluation site
         if (it == 0 && inner) {   // the first value picks the nearer pole as origin
             if (1.0 + S.psi + S.phi >= 0.0) { o = j; lo = 0.0; hi = mid; tau = mid; }
             else { o = j + 1; lo = -mid; hi = 0.0; tau = -mid; }

@@ -950,6 +950,10 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
         }
         int L = 0;
         while ((1 << L) < n) ++L;
+        // lanes per secular root: the largest power of two up to 64 with n of
+        // them in the group's threads
+        int sg = 1;
+        while (sg < 64 && (long long)n * sg * 2 <= GT) sg *= 2;
         {
             double* Q0 = (L & 1) ? QX : QC;
             for (int i = gt; i < n; i += GT) {
@@ -1111,7 +1115,7 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
             }
             alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
-            // C: deflation rotations on the rows of QO (thread = column); D: secular roots
+            // C: deflation rotations on the rows of QO (thread = column)
             for (int i = gt; i < n; i += GT) {
                 const int q = i >> l, a = q << l;
                 const int kk = kkq[q];
@@ -1123,15 +1127,26 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                     QO[p1] = C * x + S * y;
                     QO[p2] = C * y - S * x;
                 }
+            }
+            // D: secular roots, sg aligned lanes per root splitting its pole sums,
+            // so the roots of the top merges spread over the whole grid instead
+            // of the first n / XT workgroups
+            for (int i = gt / sg; i < n; i += GT / sg) {
+                const int sub = gt & (sg - 1);
+                const int q = i >> l, a = q << l;
+                const int kk = kkq[q];
+                if (kk < 0) continue;
                 const int j = i - a;
                 if (j < kk) {
                     int o;
                     double tau;
-                    dc_secular(dl + a, zl + a, kk, rhoq[q], j, &o, &tau, ue);
-                    rorig[i] = a + o;
-                    rtau[i] = tau;
-                    vals[i] = dl[a + o] + tau;
-                } else {
+                    dc_secular(dl + a, zl + a, kk, rhoq[q], j, &o, &tau, ue, sub, sg);
+                    if (sub == 0) {
+                        rorig[i] = a + o;
+                        rtau[i] = tau;
+                        vals[i] = dl[a + o] + tau;
+                    }
+                } else if (sub == 0) {
                     vals[i] = dl[i];
                 }
             }
