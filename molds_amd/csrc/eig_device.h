@@ -93,6 +93,15 @@ __device__ inline double group_allsum(double v, int g) {
     return v;
 }
 
+__device__ inline int group_allsum(int v, int g) {
+    for (int off = g >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ inline double group_allprod(double v, int g) {
+    for (int off = g >> 1; off > 0; off >>= 1) v *= __shfl_xor(v, off);
+    return v;
+}
+
 // With g > 1 the g lanes of a group split the poles (lane sub takes every
 // g-th) and all get the same sums.
 __device__ inline SecSums dc_sums(const double* dl, const double* zl, int kk, double rho, int j, double dO,

@@ -967,23 +967,28 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
             const int s = 1 << l, h = s >> 1, nm = (n + s - 1) >> l;
             double* QO = ((L - l + 1) & 1) ? QX : QC;
             double* QN = ((L - l) & 1) ? QX : QC;
-            // A: z and the stable merge order of the two sorted blocks
-            for (int i = gt; i < n; i += GT) {
+            // A: z and the stable merge order of the two sorted blocks (sg lanes
+            // per element split the count)
+            for (int i = gt / sg; i < n; i += GT / sg) {
+                const int sub = gt & (sg - 1);
                 const int a = (i >> l) << l, m = min(a + h, n), b = min(a + s, n);
                 const long long ri = (long long)i * n;
                 if (m == b) {
-                    for (int c = a; c < b; ++c) QN[ri + c] = QO[ri + c];
+                    for (int c = a + sub; c < b; c += sg) QN[ri + c] = QO[ri + c];
                     continue;
                 }
-                double z = i < m ? QO[ri + m - 1] : QO[ri + m];
-                if (i >= m && e[m - 1] < 0.0) z = -z;
-                z *= 0.70710678118654752440;
                 const double di = Dc[i];
                 int pos = 0;
-                for (int t = a; t < b; ++t) pos += dc_before(Dc[t], t, di, i);
-                ds[a + pos] = di;
-                zs[a + pos] = z;
-                col[a + pos] = i;
+                for (int t = a + sub; t < b; t += sg) pos += dc_before(Dc[t], t, di, i);
+                pos = group_allsum(pos, sg);
+                if (sub == 0) {
+                    double z = i < m ? QO[ri + m - 1] : QO[ri + m];
+                    if (i >= m && e[m - 1] < 0.0) z = -z;
+                    z *= 0.70710678118654752440;
+                    ds[a + pos] = di;
+                    zs[a + pos] = z;
+                    col[a + pos] = i;
+                }
             }
             // z comes from the diagonal blocks of QO; only its off-diagonal blocks are cleared here
             for (long long x = gt; x < ((long long)n << (l - 1)); x += GT) {
@@ -1153,40 +1158,46 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
             alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
             // E: Loewner z-hat and the rank of every new eigenvalue in its block
-            for (int i = gt; i < n; i += GT) {
+            // (sg lanes per element split the product and the count)
+            for (int i = gt / sg; i < n; i += GT / sg) {
+                const int sub = gt & (sg - 1);
                 const int q = i >> l, a = q << l;
                 const int kk = kkq[q];
                 if (kk < 0) continue;
                 const int b = min(a + s, n), j = i - a;
                 if (j < kk) {
                     const double di = dl[i];
-                    double wv = (di - dl[rorig[i]]) - rtau[i];
-                    for (int t = 0; t < kk; ++t) {
+                    double wv = sub == 0 ? (di - dl[rorig[i]]) - rtau[i] : 1.0;
+                    for (int t = sub; t < kk; t += sg) {
                         if (t == j) continue;
                         const int ct = a + t;
                         wv *= ((di - dl[rorig[ct]]) - rtau[ct]) / (di - dl[ct]);
                     }
-                    zhat[i] = copysign(sqrt(fmax(-wv, 0.0)), zl[i]);
+                    wv = group_allprod(wv, sg);
+                    if (sub == 0) zhat[i] = copysign(sqrt(fmax(-wv, 0.0)), zl[i]);
                 }
                 const double v = vals[i];
                 int rk = 0;
-                for (int t = a; t < b; ++t) rk += dc_before(vals[t], t, v, i);
-                rank[i] = rk;
+                for (int t = a + sub; t < b; t += sg) rk += dc_before(vals[t], t, v, i);
+                rk = group_allsum(rk, sg);
+                if (sub == 0) rank[i] = rk;
             }
             alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
-            // F: inverse norms of the columns of U
-            for (int i = gt; i < n; i += GT) {
+            // F: inverse norms of the columns of U (sg lanes per column)
+            for (int i = gt / sg; i < n; i += GT / sg) {
+                const int sub = gt & (sg - 1);
                 const int q = i >> l, a = q << l;
                 const int kk = kkq[q];
                 if (i - a >= kk) continue;
                 const double dO = dl[rorig[i]], tj = rtau[i];
                 double s2 = 0.0;
-                for (int t = 0; t < kk; ++t) {
+                for (int t = sub; t < kk; t += sg) {
                     const double u = zhat[a + t] / ((dl[a + t] - dO) - tj);
                     s2 += u * u;
                 }
-                cnorm[i] = 1.0 / sqrt(s2);
+                s2 = group_allsum(s2, sg);
+                if (sub == 0) cnorm[i] = 1.0 / sqrt(s2);
             }
             alive = xsync(xg.bar, G, wg, &gen, &sh_ok);
             if (!alive) break;
