@@ -883,6 +883,14 @@ __global__ void __launch_bounds__(TT) k_xtri_blk(const JobDev* __restrict__ jobs
 }
 
 // ------------------------------------------------------------------ divide and conquer
+// Lane t's value of v, as a wave-uniform value (v_readlane on both halves).
+__device__ inline double lane_value(double v, int t) {
+    const long long x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffLL), t);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), t);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // The one-workgroup k_dc (kernels_eig.hip) over the whole grid: gt / GT index
 // the grid's threads, gwv / GWV its wavefronts, and every __syncthreads of
 // k_dc between phases is a grid barrier.  Tables live in device memory.
@@ -1018,44 +1026,60 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
                 if (rho * zmax <= tol) {
                     for (int jj = a + lane; jj < b; jj += 64) flag[jj] = 1;
                     if (lane == 0) nrot[q] = 0;
-                } else if (lane == 0) {
-                    int pj = -1, nr = 0, ps = 0;
+                } else {
+                    // the scan, wave-uniform: each chunk of 64 poles is loaded by
+                    // the wavefront at once and read lane by lane (readlane), so
+                    // no load waits behind the scan's own stores; lane 0 stores
+                    int pj = -1, nr = 0, ps = 0, cp = 0;
                     double zp = 0.0, dp = 0.0;
-                    double zn = zs[a], dn = ds[a];
-                    int cn = col[a];
-                    for (int jj = a; jj < b; ++jj) {
-                        double zj = zn, dj = dn;
-                        int js = cn < m ? 1 : 2;
-                        if (jj + 1 < b) { zn = zs[jj + 1]; dn = ds[jj + 1]; cn = col[jj + 1]; }
-                        if (rho * fabs(zj) <= tol) { flag[jj] = 1; continue; }
-                        flag[jj] = 0;
-                        if (pj >= 0) {
-                            const double t = dj - dp, zz = zj * zj + zp * zp;
-                            if (fabs(t * zj * zp) <= tol * zz) {
-                                const double tz = sqrt(zz);
-                                const double C = zj / tz, S = -zp / tz;
-                                zs[jj] = tz;
-                                zs[pj] = 0.0;
-                                r1[a + nr] = col[pj];
-                                r2[a + nr] = col[jj];
-                                rc[a + nr] = C;
-                                rs[a + nr] = S;
-                                ++nr;
-                                ds[pj] = dp * C * C + dj * S * S;
-                                dj = dp * S * S + dj * C * C;
-                                ds[jj] = dj;
-                                zj = tz;
-                                flag[pj] = 1;
-                                js |= ps;
-                                if (js == 3) flag[jj] = 2;
+                    for (int c0 = a; c0 < b; c0 += 64) {
+                        const int jl = c0 + lane;
+                        const double zc = jl < b ? zs[jl] : 0.0, dc = jl < b ? ds[jl] : 0.0;
+                        const int cc = jl < b ? col[jl] : 0;
+                        const int cnt = min(64, b - c0);
+                        for (int t = 0; t < cnt; ++t) {
+                            const int jj = c0 + t;
+                            double zj = lane_value(zc, t), dj = lane_value(dc, t);
+                            const int cj = __builtin_amdgcn_readlane(cc, t);
+                            int js = cj < m ? 1 : 2;
+                            if (rho * fabs(zj) <= tol) {
+                                if (lane == 0) flag[jj] = 1;
+                                continue;
                             }
+                            int fj = 0;
+                            if (pj >= 0) {
+                                const double tt = dj - dp, zz = zj * zj + zp * zp;
+                                if (fabs(tt * zj * zp) <= tol * zz) {
+                                    const double tz = sqrt(zz);
+                                    const double C = zj / tz, S = -zp / tz;
+                                    const double dpn = dp * C * C + dj * S * S;
+                                    dj = dp * S * S + dj * C * C;
+                                    if (lane == 0) {
+                                        zs[jj] = tz;
+                                        zs[pj] = 0.0;
+                                        r1[a + nr] = cp;
+                                        r2[a + nr] = cj;
+                                        rc[a + nr] = C;
+                                        rs[a + nr] = S;
+                                        ds[pj] = dpn;
+                                        ds[jj] = dj;
+                                        flag[pj] = 1;
+                                    }
+                                    ++nr;
+                                    zj = tz;
+                                    js |= ps;
+                                    if (js == 3) fj = 2;
+                                }
+                            }
+                            if (lane == 0) flag[jj] = fj;
+                            pj = jj;
+                            ps = js;
+                            zp = zj;
+                            dp = dj;
+                            cp = cj;
                         }
-                        pj = jj;
-                        ps = js;
-                        zp = zj;
-                        dp = dj;
                     }
-                    nrot[q] = nr;
+                    if (lane == 0) nrot[q] = nr;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
                 __builtin_amdgcn_wave_barrier();
