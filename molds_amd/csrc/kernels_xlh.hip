@@ -1339,12 +1339,17 @@ __global__ void __launch_bounds__(XT) k_xdc(const JobDev* __restrict__ jobs, Job
 // column tiles Z[:, j] += Y2 V^T on MFMA, read and written in place.
 // V[j][c] = W[k0 + c][j] for j > k0 + c (v[0] = 1 is stored).
 constexpr int kXbt = 16;
+// Two compact-WY blocks per pass over the workgroup's 16 rows: with block a
+// (applied first) and block b = a - 1, Z B_a B_b = Z - Y_a T_a^T V_a^T -
+// Y_b T_b^T V_b^T where Y_a = Z V_a and Y_b = Z V_b - Y_a T_a^T (V_a^T V_b), so
+// one read pass forms Z V_a, Z V_b and V_a^T V_b, and one K = 32 update pass
+// writes Z: half the row traffic of a pass per block.
 __global__ void __launch_bounds__(XT) k_xbt(const JobDev* __restrict__ jobs, JobState* __restrict__ state,
                                             const int* __restrict__ joblist, double* __restrict__ arena,
                                             const double* __restrict__ tri) {
-    __shared__ double Yp[XW][16 * 16];
-    __shared__ double Ys[16 * 16];
-    __shared__ double Ts[16 * 16];
+    __shared__ double Yp[3][XW][16 * 16];   // per-wavefront partials of Z V_a, Z V_b, V_a^T V_b
+    __shared__ double Ya[16 * 16], Xb[16 * 16], Gs[16 * 16], Ta[16 * 16], Tb[16 * 16];
+    __shared__ double Us[16 * 32];          // [Y2_a | Y2_b], row-major 16 x 32
     const int job = joblist[blockIdx.y];
     const JobDev jb = jobs[job];
     const JobState& st = state[job];
@@ -1361,55 +1366,92 @@ __global__ void __launch_bounds__(XT) k_xbt(const JobDev* __restrict__ jobs, Job
     const int nblk = (nref + kXbt - 1) / kXbt;
     const bool rowok = r0 + li < nrow;
     const long long zrow = (long long)(r0 + li) * ln;
-    for (int bi = nblk - 1; bi >= 0; --bi) {
-        const int k0 = bi * kXbt, nb = min(kXbt, nref - k0);
-        const int jlo = k0 + 1;
-        if (tid < 256) Ts[tid] = Tall[(long long)bi * 256 + tid];
-        // Y[r][c] = sum_j Z[r][j] V[j][c]: A = Z (i = row li, k = column), B = V (k, j = reflector li)
-        dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int ba = nblk - 1; ba >= 0; ba -= 2) {
+        const int bb = ba - 1;   // < 0: block a alone
+        const bool pair = bb >= 0;
+        const int ka = ba * kXbt, nba = min(kXbt, nref - ka);
+        const int kb = pair ? bb * kXbt : ka;
+        const int jlo = kb + 1;
+        if (tid < 256) {
+            Ta[tid] = Tall[(long long)ba * 256 + tid];
+            Tb[tid] = pair ? Tall[(long long)bb * 256 + tid] : 0.0;
+        }
+        // read pass: Y_a = Z V_a, X_b = Z V_b (A = Z: i = row li, k = column;
+        // B = V: k = column, j = reflector li) and G = V_a^T V_b (A = V_a^T, the
+        // same register as V_a's B operand)
+        dbl4 accA = dbl4{0.0, 0.0, 0.0, 0.0}, accB = accA, accG = accA;
         for (int j0 = jlo + 4 * wave; j0 < n; j0 += 4 * XW) {
             const int j = j0 + lk;
-            const double a = (rowok && j < n) ? Z[zrow + j] : 0.0;
-            const double b = (li < nb && j < n && j > k0 + li) ? Wm[(long long)(k0 + li) * ln + j] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            const double z = (rowok && j < n) ? Z[zrow + j] : 0.0;
+            const double va = (li < nba && j < n && j > ka + li) ? Wm[(long long)(ka + li) * ln + j] : 0.0;
+            const double vb = (pair && j < n && j > kb + li) ? Wm[(long long)(kb + li) * ln + j] : 0.0;
+            accA = __builtin_amdgcn_mfma_f64_16x16x4f64(z, va, accA, 0, 0, 0);
+            accB = __builtin_amdgcn_mfma_f64_16x16x4f64(z, vb, accB, 0, 0, 0);
+            accG = __builtin_amdgcn_mfma_f64_16x16x4f64(va, vb, accG, 0, 0, 0);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Yp[wave][(4 * v + lk) * 16 + li] = acc[v];
-        __syncthreads();
-        {   // Y (row-major 16 x 16) and Y2[r][c] = -sum_{c2 >= c} T[c][c2] Y[r][c2]
-            const int r = tid >> 4, c = tid & 15;
-            double y = 0.0;
-#pragma unroll
-            for (int w = 0; w < XW; ++w) y += Yp[w][r * 16 + c];
-            Ys[r * 16 + c] = y;
+        for (int v = 0; v < 4; ++v) {
+            Yp[0][wave][(4 * v + lk) * 16 + li] = accA[v];
+            Yp[1][wave][(4 * v + lk) * 16 + li] = accB[v];
+            Yp[2][wave][(4 * v + lk) * 16 + li] = accG[v];
         }
         __syncthreads();
-        double y2 = 0.0;
+        const int r = tid >> 4, c = tid & 15;   // XT = 256: one (r, c) per thread
         {
-            const int r = tid >> 4, c = tid & 15;
-            for (int c2 = c; c2 < kXbt; ++c2) y2 -= Ts[c * 16 + c2] * Ys[r * 16 + c2];
+            double y = 0.0, x = 0.0, g = 0.0;
+#pragma unroll
+            for (int w = 0; w < XW; ++w) {
+                y += Yp[0][w][r * 16 + c];
+                x += Yp[1][w][r * 16 + c];
+                g += Yp[2][w][r * 16 + c];
+            }
+            Ya[r * 16 + c] = y;
+            Xb[r * 16 + c] = x;
+            Gs[r * 16 + c] = g;
         }
         __syncthreads();
-        Ys[tid] = y2;
+        // Y2_a[r][c] = -sum_{c2 >= c} T_a[c][c2] Y_a[r][c2]
+        double y2a = 0.0;
+        for (int c2 = c; c2 < kXbt; ++c2) y2a -= Ta[c * 16 + c2] * Ya[r * 16 + c2];
+        Us[r * 32 + c] = y2a;
         __syncthreads();
-        // Z[r][j] += sum_c Y2[r][c] V[j][c] over the column tiles j >= jlo
+        // Y_b = X_b + Y2_a G, then Y2_b[r][c] = -sum_{c2 >= c} T_b[c][c2] Y_b[r][c2]
+        {
+            double yb = Xb[r * 16 + c];
+            for (int c2 = 0; c2 < kXbt; ++c2) yb += Us[r * 32 + c2] * Gs[c2 * 16 + c];
+            Ya[r * 16 + c] = yb;   // Y_a is no longer needed
+        }
+        __syncthreads();
+        {
+            double y2b = 0.0;
+            for (int c2 = c; c2 < kXbt; ++c2) y2b -= Tb[c * 16 + c2] * Ya[r * 16 + c2];
+            Us[r * 32 + 16 + c] = pair ? y2b : 0.0;
+        }
+        __syncthreads();
+        // Z[r][j] += sum_c U[r][c] [V_a | V_b][j][c] over the column tiles j >= jlo
         const int tlo = jlo >> 4, ntj = (n + 15) >> 4;
         for (int tj = tlo + wave; tj < ntj; tj += XW) {
             dbl4 u = dbl4{0.0, 0.0, 0.0, 0.0};
             const int j = 16 * tj + li;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int c = 4 * q + lk;
-                const double a = Ys[li * 16 + c];   // A[i = row li][k = c]
-                const double b = (c < nb && j < n && j > k0 + c) ? Wm[(long long)(k0 + c) * ln + j] : 0.0;
-                u = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, u, 0, 0, 0);
+            for (int q = 0; q < 8; ++q) {
+                const int cc = 4 * q + lk;
+                const double aa = Us[li * 32 + cc];   // A[i = row li][k = cc]
+                double bv = 0.0;
+                if (q < 4) {
+                    if (cc < nba && j < n && j > ka + cc) bv = Wm[(long long)(ka + cc) * ln + j];
+                } else {
+                    const int cb = cc - 16;
+                    if (pair && j < n && j > kb + cb) bv = Wm[(long long)(kb + cb) * ln + j];
+                }
+                u = __builtin_amdgcn_mfma_f64_16x16x4f64(aa, bv, u, 0, 0, 0);
             }
             // D[i][j] sits in lane j + 16 (i % 4), element i / 4
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = r0 + 4 * v + lk;
-                if (r < nrow && j < n && j >= jlo) {
-                    double* zp = Z + (long long)r * ln + j;
+                const int rr = r0 + 4 * v + lk;
+                if (rr < nrow && j < n && j >= jlo) {
+                    double* zp = Z + (long long)rr * ln + j;
                     *zp += u[v];
                 }
             }
