@@ -13,16 +13,20 @@
 //            (so nobody writes it while others read it), with the same
 //            fixed-order reductions everywhere, hence bitwise the same v in
 //            every workgroup.  One grid barrier per step.
+//   k_xtri_blk  the same reduction in panels of 8 reflectors (from 5600 AOs):
+//            read-only row passes and one rank-16 FP64 MFMA update per panel.
 //   k_xdc    Cuppen divide and conquer on T, bottom-up from 1 x 1 leaves, all
 //            merges of a level at once, as the one-workgroup k_dc
 //            (kernels_eig.hip) with its per-row tables in device memory and
 //            every phase spread over the grid: deflation (Dongarra-Sorensen),
 //            secular roots (rational model with bisection safeguard), Loewner
 //            eigenvector components (Gu-Eisenstat) and the merge products on
-//            FP64 MFMA.  One grid barrier per phase.
+//            FP64 MFMA; the per-element loops over a merge block give each
+//            element a group of lanes.  One grid barrier per phase.
 //   k_xbt    eigenvector rows <- rows * H_{n-3} ... H_0 in compact WY blocks of
-//            16 reflectors (T from k_wy_t), one workgroup per 16 rows streaming
-//            them through FP64 MFMA; no synchronisation between workgroups.
+//            16 reflectors (T from k_wy_t), two blocks per pass, one workgroup
+//            per 16 rows streaming them through FP64 MFMA; no synchronisation
+//            between workgroups.
 //
 // Output as the one-workgroup path: rows of AR_C are the eigenvectors in
 // ascending eigenvalue order, eps[] ascending, perm[] = identity.
