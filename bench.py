@@ -488,6 +488,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched legs (profiling runs of the other legs; the line then has no value)")
+    ap.add_argument("--served-only", action="store_true",
+                    help="only the mixed-precision served legs of --served-rates (profiling runs; the line has no value)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -517,6 +519,16 @@ def main():
 
     from molds_amd.nddo import Engine
     engine = Engine(0)
+    if args.served_only:   # profiling run of the served legs only
+        legs = []
+        for rate in [float(r) for r in args.served_rates.split(",") if r]:
+            count = min(args.served_max, max(50, int(round(rate * args.served_seconds))))
+            legs.append(served_leg(engine, rate, count, "mixed"))
+            log(f"served mixed {rate:g}/s: {legs[-1]}")
+        engine.close()
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1,
+                          "note": "served legs only", "served_mixed": legs}), flush=True)
+        return
     if args.no_batched:   # profiling run of the single-molecule legs only
         out = {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "note": "single-molecule legs only",
                "large_single": large_leg(engine), "water1000_pm3": water1000_leg(engine),
